@@ -1,0 +1,168 @@
+"""Encoded Mpix/s of the NNTC encoder on one MI355X.
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--form exact|fast] [--no-cpu-baseline]
+
+A step is one whole encode (the reference's nntc_encode schedule: init, residual seed, 8 rounds, grid freeze, BC pack,
+refinement and outer repacks, the shipped objective and the base decode) of the plain GPU workload - 4 RGB8 textures
+1024x1024 from the seeded generator, --c0 2 --c1 4, mips on, K = 4 - with the source chain already resident in HBM
+(device_create happens before the timed region). value = W*H*T / seconds per step, in Mpix/s.
+
+The roofline object describes the step's dominant kernel: its algorithmic bytes (or flops) per launch over its
+average launch time, both taken live from HIP events on the arm's own stream over the timed steps. The cpu_baseline
+object times the reference CPU encoder (oracle/_ref/nntc_encode, built from the reference's own sources) on a bounded
+sample of the same workload: the top-left 256x256 of the same four textures, same options.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+WORKLOAD = "plain_gpu_4x1024"
+
+
+def kernel_stats(hip):
+    hip.nntc_hip_kernel_stats.argtypes = [C.c_int, C.c_char_p, C.c_size_t, C.POINTER(C.c_longlong),
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    out = {}
+    i = 0
+    while True:
+        name = C.create_string_buffer(64)
+        n = C.c_longlong(0)
+        ms, by, fl = C.c_double(0), C.c_double(0), C.c_double(0)
+        if hip.nntc_hip_kernel_stats(i, name, 64, C.byref(n), C.byref(ms), C.byref(by), C.byref(fl)) != 0:
+            break
+        out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": by.value, "flops": fl.value}
+        i += 1
+    return out
+
+
+def roofline(stats):
+    name, s = max(stats.items(), key=lambda kv: kv[1]["ms"])
+    avg_s = s["ms"] / 1e3 / max(1, s["launches"])
+    bytes_per = s["bytes"] / max(1, s["launches"])
+    flops_per = s["flops"] / max(1, s["launches"])
+    t_hbm = bytes_per / (HBM_PEAK_GBS * 1e9)
+    t_fp = flops_per / (FP64_PEAK_TFLOPS * 1e12)
+    if t_hbm >= t_fp:
+        achieved = bytes_per / avg_s / 1e9
+        r = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s"}
+    else:
+        achieved = flops_per / avg_s / 1e12
+        r = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s"}
+    r["frac"] = r["achieved"] / r["peak"]
+    r["traffic"] = None
+    r["kernel"] = name
+    r["launches_per_step"] = None
+    r["avg_launch_ms"] = avg_s * 1e3
+    r["share_of_kernel_time"] = s["ms"] / sum(v["ms"] for v in stats.values())
+    return r
+
+
+def cpu_baseline(textures, spec):
+    ref = os.path.join(ROOT, "oracle", "_ref", "nntc_encode")
+    if not os.path.exists(ref):
+        return None
+    from nntc_amd import synth
+
+    cores = min(os.cpu_count() or 1, 16)
+    crop = [t[:256, :256].copy() for t in textures]
+    with tempfile.TemporaryDirectory() as tmp:
+        paths = synth.write_pngs(crop, os.path.join(tmp, "m"))
+        cmd = [ref, *paths, "--c0", str(spec["c0"]), "--c1", str(spec["c1"]), "--rounds", "8", "--tol", "0",
+               "--png", "0", "--backend", "cpu", "-j", str(cores), "--quiet", "-o", os.path.join(tmp, "out") + os.sep]
+        t0 = time.time()
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+        wall = time.time() - t0
+    if r.returncode != 0:
+        return None
+    mpix = 256 * 256 * len(crop) / 1e6
+    return {"value": mpix / wall, "unit": "Mpix/s", "cores": cores, "kind": "reference",
+            "sample": f"reference CPU encoder (oracle/_ref/nntc_encode -j {cores}) on the top-left 256x256 of the "
+                      f"same 4 textures, same options (--c0 2 --c1 4 --rounds 8 --tol 0); {wall:.2f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--form", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: the encoder shards by material across GPUs with no exchange, and this "
+              f"build runs and reports the one-GPU share only (asked for --gpus {a.gpus})", file=sys.stderr)
+        sys.exit(2)
+
+    import torch  # device plumbing only: the synchronisation around the timed region
+
+    from nntc_amd import abi, encoder, synth
+
+    spec = synth.WORKLOADS[WORKLOAD]
+    textures = synth.workload(WORKLOAD)
+    form = abi.FORM_EXACT if a.form == "exact" else abi.FORM_FAST
+    enc = encoder.Encoder(textures, c0=spec["c0"], c1=spec["c1"], rounds=8, tol=0.0, quiet=1, form=form)
+    hip = C.CDLL(abi.HIP_LIB, mode=C.RTLD_GLOBAL)
+    for _ in range(a.warmup):
+        enc.run()
+    hip.nntc_hip_profile_reset()
+    hip.nntc_hip_profile_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rep = enc.run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    hip.nntc_hip_profile_enable(0)
+    stats = kernel_stats(hip)
+    ms_per_step = elapsed / a.steps * 1e3
+    mpix = rep.mpix
+    roof = roofline(stats)
+    roof["launches_per_step"] = stats[roof["kernel"]]["launches"] / a.steps
+
+    gap = None
+    gpath = os.path.join(ROOT, "tests", "golden", WORKLOAD, "report.json")
+    if os.path.exists(gpath):
+        g = json.load(open(gpath))
+        gap = max(abs(rep.psnr[t] - g["psnr"][t]) for t in range(len(g["psnr"])))
+
+    line = {
+        "metric": "encoded Mpix/s at <=0.05 dB PSNR gap to reference; chosen vs plain form, 1 GPU",
+        "value": mpix / (elapsed / a.steps),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: seeded value-noise material (nntc_amd/synth.py)",
+        "config": {"workload": f"{WORKLOAD}: 4 RGB8 textures 1024x1024, --c0 2 --c1 4, mips on, K 4, 8 rounds, "
+                               f"tol 0, {a.form} form", "form": a.form, "textures": 4, "width": 1024,
+                   "height": 1024, "rounds": 8, "parallelism": "single GPU"},
+        "psnr": [round(rep.psnr[t], 4) for t in range(4)],
+        "psnr_gap_db": gap,
+        "roofline": roof,
+        "kernels": {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in stats.items()},
+        "cpu_baseline": None if a.no_cpu_baseline else cpu_baseline(textures, spec),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,109 @@
+/* nntc_encode.h: the encode driver over the gfx950 arm - the reference's `nntc_encode` schedule as a C ABI.
+ *
+ * The reference's encoder is one function, encode() (src/main.cpp:1753-3300): load the textures, build the source
+ * chain and the mip weights, create the device, initialise both latents, seed level 0 from the residual, run the
+ * alternating rounds, freeze the grids, pack level 0 to BC4/BC5 and refine the pack, and report. This header exposes
+ * the same run in two halves so that a caller can keep the device resident between runs:
+ *
+ *   nntc_encoder_create   the host half before the first seam call: the packed source, the chain, the plane layout,
+ *                         the mip weights, the output weights, and device_create (the upload of the chain);
+ *   nntc_encoder_run      everything from init_level0 to the shipped objective and the decode of the base plane,
+ *                         restarted from scratch on every call (the device's latents, decoder and grids are reset);
+ *
+ * and the run's outputs through accessors. The arm is the gfx950 library by default; a different library that exports
+ * the same entry points under another prefix may be named instead (the tests bind the reference CPU backend this way
+ * to check the driver against the reference binary).
+ */
+#ifndef NNTC_ENCODE_H
+#define NNTC_ENCODE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "nntc_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The options of Options (src/options.h:67-191) that the encode path reads, with the reference's defaults. */
+typedef struct nntc_encode_options
+{
+    int32_t c0, c1;              /* --c0 2, --c1 4 */
+    int32_t bits1;               /* --bits1 8 */
+    int32_t k;                   /* --k 4 */
+    int32_t rounds;              /* --rounds 20 */
+    double tol;                  /* --tol 1e-4 */
+    int32_t q1_start;            /* --q1-start 3 */
+    int32_t q1_sweeps;           /* --q1-sweeps 4 */
+    int32_t q1_range;            /* --q1-range pct (NNTC_RANGE_PCT) */
+    double ridge;                /* --ridge 1e-4 */
+    int32_t mips;                /* --mips 1 */
+    int32_t mip_min;             /* --mip-min 8 */
+    int32_t mip_weight;          /* --mip-weight pixels: 0 pixels, 1 sqrt, 2 uniform */
+    double mip_mix;              /* --mip-mix 0.5 */
+    int32_t bc_refine;           /* --bc-refine 4 */
+    int32_t bc_refine_after;     /* --bc-refine-after 0 */
+    int32_t bc_outer;            /* --bc-outer 2 */
+    int32_t init0_chain;         /* --init0-scope chain */
+    int32_t form;                /* NNTC_FORM_EXACT or NNTC_FORM_FAST (not a reference option) */
+    int32_t quiet;               /* --quiet */
+    double weights[6];           /* --weights (per texture; 1 when unset) */
+    double rgb_weights[3];       /* --rgb-weights 1,1,1 */
+} nntc_encode_options;
+
+void nntc_encode_default_options(nntc_encode_options* o);
+
+typedef struct nntc_encoder nntc_encoder;
+
+/* The host half and the device's creation. rgb[t] is texture t as width * height * 3 bytes, row-major RGB.
+ * arm_lib / arm_prefix name the seam's implementation; NULL for both is the gfx950 arm. Returns NULL on failure
+ * (nntc_encode_last_error says why). */
+nntc_encoder* nntc_encoder_create(const nntc_encode_options* o, int textures, int width, int height,
+                                  const uint8_t* const* rgb, const char* arm_lib, const char* arm_prefix);
+
+/* The same from image files (PNG, 8-bit RGB / RGBA / grey), as `nntc_encode a.png b.png ...` reads them. */
+nntc_encoder* nntc_encoder_create_files(const nntc_encode_options* o, int textures, const char* const* paths,
+                                        const char* arm_lib, const char* arm_prefix);
+
+/* One whole encode on the resident device. Returns 0, or nonzero with the reason in nntc_encode_last_error. */
+int nntc_encoder_run(nntc_encoder* e);
+
+void nntc_encoder_destroy(nntc_encoder* e);
+
+const char* nntc_encode_last_error(void);
+
+/* ---- the last run's outputs */
+typedef struct nntc_encode_report
+{
+    int32_t textures, nplanes, rounds_used;
+    int32_t outer_taken, outer_tried;
+    double psnr[6];              /* per texture, 8-bit, base plane, original extent (src/image.h:387-407) */
+    double centre_psnr, sampled_psnr;
+    double e_shipped;
+    double e_round[64][3];       /* per round: E after blocks (a), (b), (c') */
+    double pack_psnr, pack_psnr_seed;
+    double seconds;              /* the run's wall clock */
+    double ms_a, ms_b, ms_c, ms_e, ms_pack, ms_refine;
+    long long ridge_tally[4];
+    double mpix;                 /* width * height * textures / 1e6 */
+} nntc_encode_report;
+
+int nntc_encoder_report(const nntc_encoder* e, nntc_encode_report* r);
+int nntc_encoder_state(const nntc_encoder* e, nntc_model_state* s);
+/* Plane `plane`'s level-1 indices (w1*h1*c1 bytes) and level-0 .dds blocks of file `file`, as the asset stores them. */
+int nntc_encoder_level1_indices(const nntc_encoder* e, int plane, uint8_t* out, size_t n, size_t* count);
+int nntc_encoder_level0_blocks(const nntc_encoder* e, int plane, int file, uint8_t* out, size_t n, size_t* count);
+/* The base plane decoded to bytes (w0*h0*3T). */
+int nntc_encoder_recon(const nntc_encoder* e, uint8_t* out, size_t n, size_t* count);
+/* The model descriptor the encoder built (plane layout, channels, output weights). */
+int nntc_encoder_desc(const nntc_encoder* e, nntc_model_desc* d);
+
+/* Write the asset: PREFIX_lat0.dds (or _lat0a/_lat0b), PREFIX_lat1.dds and PREFIX_nntc.json (src/export.cpp:470). */
+int nntc_encoder_write(const nntc_encoder* e, const char* prefix);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNTC_ENCODE_H */

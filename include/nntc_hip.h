@@ -1,0 +1,272 @@
+/* nntc_hip.h: the C ABI of the MI355X (gfx950) arm of the NNTC encoder.
+ *
+ * The reference encoder reaches its device work through one seam of 46 C++ functions in namespace `be`
+ * (src/backend.h:86-150), each taking `be::Device*`, `Model&` and `std::vector` arguments. This header is the same seam
+ * as a C ABI: an opaque context in place of `be::Device*`, a plain model descriptor and a plain mutable model state in
+ * place of `Model&`, and pointer + length pairs in place of vectors. Every entry point names the `be::` function it
+ * replaces. A caller in the reference's own tree binds these from a `nntc_hip::` namespace that converts the plain
+ * structs to and from `Model` (INTEGRATION.md shows that binding).
+ *
+ * Conventions.
+ *   - Every entry point that can fail returns int: 0 on success, nonzero on failure, with the reason readable from
+ *     nntc_hip_last_error(). The reference prints one `ERROR:` line and exits (src/device.cuh:45-76); the C ABI
+ *     reports instead, and the C++ layer above it keeps the reference's exit behaviour.
+ *   - Block functions write their elapsed milliseconds to *ms (may be NULL), as the reference returns them.
+ *   - `model` is the layout the encode was created with; `state` is the part of the reference's Model that the seam
+ *     mutates (decoder, grids, palette) and it is read and written by the calls that read or write those fields.
+ *   - All arrays are host memory. Plane buffers are interleaved [y][x][channel], exactly the reference's layout.
+ *   - One context is driven from one host thread. Results never depend on scheduling: every reduction is folded in a
+ *     fixed order, so repeated runs are byte-identical.
+ */
+#ifndef NNTC_HIP_H
+#define NNTC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NNTC_MAX_PLANES 13   /* MAX_MIP_LEVELS + 1 (src/model.h:32) */
+#define NNTC_MAX_CHANNELS 4  /* per-level channel cap (src/cpu/cpu_constants.h:24) */
+#define NNTC_MAX_NIN 24      /* |phi| cap (src/cpu/cpu_constants.h:25) */
+#define NNTC_MAX_NOUT 18     /* 3 * MAX_TEXTURES (src/model.h:39) */
+
+/* One stored mip level: level 0's plane and level 1's plane (src/model.h:42-46). */
+typedef struct nntc_plane_size
+{
+    int32_t w0, h0, w1, h1;
+} nntc_plane_size;
+
+/* The fixed layout of one encode: the fields of Model (src/model.h:56-114) that no seam function changes. */
+typedef struct nntc_model_desc
+{
+    int32_t textures;                       /* T */
+    int32_t nout;                           /* 3 T */
+    int32_t c0, c1;                         /* latent channels */
+    int32_t bits0[NNTC_MAX_CHANNELS];       /* level-0 bits per channel (8 under --l0 bc8) */
+    int32_t bits1;                          /* level-1 bits */
+    int32_t bc0;                            /* level 0 shipped block-compressed */
+    int32_t l0_bc8;                         /* --l0 bc8 */
+    int32_t init0_residual, init0_texture, init0_chain;
+    int32_t nplanes;
+    nntc_plane_size planes[NNTC_MAX_PLANES];
+    float cw[NNTC_MAX_NOUT];                /* per-output objective weight (src/main.cpp:1856-1862) */
+} nntc_model_desc;
+
+/* The mutable part of Model: the decoder (Model::dec), both grids and the palette. */
+typedef struct nntc_model_state
+{
+    int32_t has_decoder;                    /* Model::dec.w non-empty */
+    float w[NNTC_MAX_NOUT * NNTC_MAX_NIN];  /* nout x nin, row-major */
+    float b[NNTC_MAX_NOUT];
+    int32_t has_grid0, has_grid1;           /* Model::lo0 / lo1 non-empty */
+    float lo0[NNTC_MAX_CHANNELS], hi0[NNTC_MAX_CHANNELS];
+    float lo1[NNTC_MAX_CHANNELS], hi1[NNTC_MAX_CHANNELS];
+    int32_t has_palette0;                   /* palette mode only */
+    float palette0[NNTC_MAX_CHANNELS][16];
+} nntc_model_state;
+
+/* DeviceInfo (src/model.h:255-261). */
+typedef struct nntc_device_info
+{
+    char name[256];
+    int32_t major, minor;
+    char error[256];
+} nntc_device_info;
+
+/* Level1InitReport (src/model.h:281-289). */
+typedef struct nntc_level1_init_report
+{
+    int32_t pca, components;
+    double eigenvalue[4], peak[4], total_variance;
+} nntc_level1_init_report;
+
+/* Init0ChannelReport (src/model.h:314-326). */
+typedef struct nntc_init0_report
+{
+    double eigenvalue, total_variance, share, peak, divisor;
+    double dir[NNTC_MAX_NOUT];
+    int32_t nout, texture;
+} nntc_init0_report;
+
+/* Level1Report (src/model.h:374-398); one per plane. */
+typedef struct nntc_level1_report
+{
+    int32_t iterations;
+    double residual, lambda, min_diag, max_diag;
+    double ms_assemble, ms_precond, ms_cg;
+    int64_t moved_values;
+    double delta2, plane2;
+    float lo[NNTC_MAX_CHANNELS], hi[NNTC_MAX_CHANNELS];
+    int32_t quantised, sweeps;
+    int64_t moved;
+    double ms_sweeps;
+} nntc_level1_report;
+
+/* Level0Report (src/model.h:440-447); one per plane. */
+typedef struct nntc_level0_report
+{
+    int64_t moved[4], moved_total;
+    double ms;
+    int32_t joint;
+    int64_t states;
+} nntc_level0_report;
+
+/* Objective (src/model.h:467-478), with its per-plane vectors as fixed arrays. */
+typedef struct nntc_objective
+{
+    double e, centre_mse, sampled_mse, ms;
+    int32_t nplanes;
+    double e_plane[NNTC_MAX_PLANES];
+    double centre_plane[NNTC_MAX_PLANES];
+} nntc_objective;
+
+/* BcRefineReport (src/model.h:512-519); one per pass. */
+typedef struct nntc_bc_refine_report
+{
+    int32_t pass;
+    int64_t blocks, improved;
+    double delta_e, ms;
+} nntc_bc_refine_report;
+
+/* Range policies of the grid fits (src/model.h:334-345): "minmax" or "pct". */
+#define NNTC_RANGE_MINMAX 0
+#define NNTC_RANGE_PCT 1
+
+/* Execution forms. EXACT replays the reference CPU backend's reduction orders (fixed 4096-element chunks folded in
+ * chunk order, the 256-site decoder tiles, the covariance grid) so that every result is the reference's bits. FAST
+ * keeps every per-texel kernel identical but folds the dot products, the objective and the diagonal with wavefront
+ * tree reductions, which changes last bits only. */
+#define NNTC_FORM_EXACT 0
+#define NNTC_FORM_FAST 1
+
+typedef struct nntc_hip_ctx nntc_hip_ctx;
+
+/* The reason the last failing call on this thread gave; empty when none has failed. */
+const char* nntc_hip_last_error(void);
+
+/* Replaces be::device_select (src/backend.h:86). Returns 1 when device `index` is usable, 0 otherwise. */
+int nntc_hip_device_select(int index, nntc_device_info* info);
+
+/* Replaces be::device_create (src/backend.h:87): allocates every plane buffer and uploads the source chain.
+ * source_chain[p] holds planes[p].w0 * planes[p].h0 * nout floats. Returns NULL on failure. */
+nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* model, const float* const* source_chain);
+
+/* Replaces be::device_destroy (src/backend.h:88); NULL is ignored. */
+void nntc_hip_device_destroy(nntc_hip_ctx* ctx);
+
+/* Replaces be::device_memory (src/backend.h:89): bytes of device memory the context holds. */
+size_t nntc_hip_device_memory(const nntc_hip_ctx* ctx);
+
+/* Selects the execution form (NNTC_FORM_EXACT by default). Not in the reference's seam: the reference has one form. */
+int nntc_hip_set_form(nntc_hip_ctx* ctx, int form);
+
+/* Replaces be::init_level0 (src/backend.h:91). k0 mirrors are read with nntc_hip_level0_download. */
+int nntc_hip_init_level0(nntc_hip_ctx* ctx, nntc_model_state* state, const float luma_weights[3], int luma_channel0);
+
+/* Replaces be::init_level1 (src/backend.h:92). Fits and writes state->lo1/hi1. */
+int nntc_hip_init_level1(nntc_hip_ctx* ctx, nntc_model_state* state, int pca, nntc_level1_init_report* rep);
+
+/* Replaces be::init0_residual_channel (src/backend.h:93). plane_weight has nplanes entries. */
+int nntc_hip_init0_residual_channel(nntc_hip_ctx* ctx, nntc_model_state* state, int channel,
+                                    const double* plane_weight, nntc_init0_report* rep);
+
+/* Replace be::quantise_level1, freeze_level1_grid, quantise_level0, freeze_level0_grid, snap_level0_on_grid
+ * (src/backend.h:96-100). policy is NNTC_RANGE_MINMAX or NNTC_RANGE_PCT. */
+int nntc_hip_quantise_level1(nntc_hip_ctx* ctx, nntc_model_state* state, int policy);
+int nntc_hip_freeze_level1_grid(nntc_hip_ctx* ctx, nntc_model_state* state, int policy);
+int nntc_hip_quantise_level0(nntc_hip_ctx* ctx, nntc_model_state* state, int policy);
+int nntc_hip_freeze_level0_grid(nntc_hip_ctx* ctx, nntc_model_state* state, int policy);
+int nntc_hip_snap_level0_on_grid(nntc_hip_ctx* ctx, nntc_model_state* state);
+
+/* Replaces be::stencil_monomials (src/backend.h:102): 1 + c + c (c + 1) / 2. */
+int nntc_hip_stencil_monomials(int c0);
+
+/* Replaces be::solve_decoder (src/backend.h:104): block (a). per_site has nplanes entries. Writes state's decoder. */
+int nntc_hip_solve_decoder(nntc_hip_ctx* ctx, nntc_model_state* state, int k, const double* per_site, double* ms);
+
+/* Replaces be::decoder_ridge_tally (src/backend.h:105): the rungs the process's block (a) calls ended on. */
+void nntc_hip_decoder_ridge_tally(long long out[4]);
+
+/* Replace be::solve_level1_all, sweep_level1_all, solve_level0_cont_all, sweep_level0_cont_all
+ * (src/backend.h:107-111): blocks (b) and (c'). rep has nplanes entries. */
+int nntc_hip_solve_level1_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge,
+                              nntc_level1_report* rep, double* ms);
+int nntc_hip_sweep_level1_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge, int sweeps,
+                              nntc_level1_report* rep, double* ms);
+int nntc_hip_solve_level0_cont_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge,
+                                   nntc_level1_report* rep, double* ms);
+int nntc_hip_sweep_level0_cont_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge, int sweeps,
+                                   nntc_level1_report* rep, double* ms);
+
+/* Replaces be::assemble_level0_for_refine (src/backend.h:112). */
+int nntc_hip_assemble_level0_for_refine(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double* ms);
+
+/* The transfers (src/backend.h:114-123, 128-130). `n` is the element count the caller's buffer holds; a download
+ * writes min(n, plane size) elements and returns the plane size through *count when count is not NULL. An upload of
+ * a size other than the plane's is ignored, as the reference ignores it (src/cpu/cpu_model.cpp:191-197). */
+int nntc_hip_level1_download_indices(nntc_hip_ctx* ctx, int plane, uint8_t* k1, size_t n, size_t* count);
+int nntc_hip_level1_upload(nntc_hip_ctx* ctx, int plane, const float* v1, size_t n);
+int nntc_hip_level1_upload_indices(nntc_hip_ctx* ctx, int plane, const uint8_t* k1, size_t n);
+int nntc_hip_level0_upload_indices(nntc_hip_ctx* ctx, int plane, const uint8_t* k0, size_t n);
+int nntc_hip_upload_decoder_to_device(nntc_hip_ctx* ctx, const nntc_model_state* state);
+int nntc_hip_upload_grids_to_device(nntc_hip_ctx* ctx, const nntc_model_state* state);
+int nntc_hip_level1_download(nntc_hip_ctx* ctx, int plane, float* v1, size_t n, size_t* count);
+int nntc_hip_level1_delta(nntc_hip_ctx* ctx, int plane, double* delta, size_t n, size_t* count);
+int nntc_hip_level1_poke(nntc_hip_ctx* ctx, int plane, size_t index, float value);
+int nntc_hip_level1_range(nntc_hip_ctx* ctx, int plane, float* lo, float* hi);
+int nntc_hip_level0_download(nntc_hip_ctx* ctx, int plane, uint8_t* k0, size_t n, size_t* count);
+int nntc_hip_level0_upload(nntc_hip_ctx* ctx, int plane, const float* v0, size_t n);
+int nntc_hip_level0_download_values(nntc_hip_ctx* ctx, int plane, float* v0, size_t n, size_t* count);
+
+/* Replaces be::decode_plane (src/backend.h:132): out = saturate(W phi + b) rounded to bytes, w0*h0*nout bytes. */
+int nntc_hip_decode_plane(nntc_hip_ctx* ctx, int plane, uint8_t* rgb8, size_t n);
+
+/* Replaces be::objective_eval (src/backend.h:133). per_site has nplanes entries. */
+int nntc_hip_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* r);
+
+/* Replace be::objective_total_ms and be::objective_passes (src/backend.h:134-135). */
+double nntc_hip_objective_total_ms(void);
+long long nntc_hip_objective_passes(void);
+
+/* Replace be::bc_pack_prepare, bc_refine, bc_blocks_from_device (src/backend.h:138-141). The refined blocks are read
+ * with nntc_hip_bc_blocks_download; rep has `passes` entries.
+ *
+ * The reference reads the host's index mirror Model::k0 in two places: the seed pack packs it
+ * (src/export.cpp:294-331) and the packing error is measured against it (src/export.cpp:283-292). Model::k0 is the
+ * level-0 index plane as the encode last downloaded it, so here it is an argument: `k0_mirror[plane]` points at the
+ * caller's copy of each plane's indices, or k0_mirror is NULL, which means "the device's level-0 indices as they stand"
+ * for bc_pack_prepare and "the mirror the previous bc_* call used" for the other two. */
+int nntc_hip_bc_pack_prepare(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, int seed,
+                             const uint8_t* const* k0_mirror, double* pack_psnr, size_t* pack_texels, double* ms);
+int nntc_hip_bc_refine(nntc_hip_ctx* ctx, int passes, const uint8_t* const* k0_mirror, nntc_bc_refine_report* rep,
+                       double* pack_psnr, size_t* pack_texels, double* ms);
+int nntc_hip_bc_blocks_from_device(nntc_hip_ctx* ctx, const uint8_t* const* k0_mirror, double* pack_psnr,
+                                   size_t* pack_texels);
+
+/* The BC4/BC5 blocks of one plane's level-0 file `file` (0, or 1 when c0 > 2), as the .dds stores them: the model's
+ * bc0_blocks[plane][file] after a bc_* call (src/model.h:108). */
+int nntc_hip_bc_blocks_download(nntc_hip_ctx* ctx, int plane, int file, uint8_t* blocks, size_t n, size_t* count);
+
+/* Replace be::bc_state_save / bc_state_restore (src/backend.h:142-145): one plane's endpoints (blocks*c0*2 bytes) and
+ * selectors (blocks*c0*16 bytes). */
+int nntc_hip_bc_state_save(nntc_hip_ctx* ctx, int plane, uint8_t* ep, size_t n_ep, uint8_t* sel, size_t n_sel);
+int nntc_hip_bc_state_restore(nntc_hip_ctx* ctx, int plane, const uint8_t* ep, size_t n_ep, const uint8_t* sel,
+                              size_t n_sel);
+
+/* Per-kernel timing (not in the reference's seam; the benchmark's instrument). While enabled, every launch of the
+ * arm's named kernels is bracketed by HIP events on the arm's stream and its algorithmic bytes and flops are added
+ * up. nntc_hip_kernel_stats reads one kernel's totals (index-based enumeration: name_out receives the name); it
+ * returns 0, or nonzero past the last kernel. */
+void nntc_hip_profile_enable(int on);
+void nntc_hip_profile_reset(void);
+int nntc_hip_kernel_stats(int index, char* name_out, size_t name_len, long long* launches, double* ms, double* bytes,
+                          double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NNTC_HIP_H */

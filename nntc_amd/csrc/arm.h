@@ -1,0 +1,203 @@
+// arm.h: the gfx950 arm's device model and the launchers each kernel file provides.
+//
+// The device model is the MI355X counterpart of the reference's DeviceModel / CpuModel (src/device.cuh:193-375,
+// src/cpu/cpu_model.h:76-174), laid out for batched launches: every per-plane buffer of the chain is ONE allocation
+// with the planes concatenated (the base first), and every kernel covers every plane of the chain in one launch,
+// mapping its global index to (plane, local index) through the plane table. The deep mip planes are a few hundred
+// texels each, so one launch per plane (the reference's stream-per-plane fan-out) would be launch-bound.
+
+#pragma once
+
+#include "nntc_common.h"
+#include "nntc_hip.h"
+
+#include <string>
+#include <vector>
+
+namespace nntc
+{
+
+#define HIP_OK(expr)                                                                                                   \
+    do                                                                                                                 \
+    {                                                                                                                  \
+        hipError_t e_ = (expr);                                                                                        \
+        if (e_ != hipSuccess)                                                                                          \
+            throw ::nntc::HipFailure(#expr, e_, __FILE__, __LINE__);                                                   \
+    } while (0)
+
+struct HipFailure
+{
+    std::string what;
+    HipFailure(const char* expr, hipError_t e, const char* file, int line);
+};
+
+struct Refused
+{
+    std::string what;
+};
+
+// A typed device allocation that frees itself.
+template <class T>
+struct DevBuf
+{
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count);
+    void release();
+    ~DevBuf() { release(); }
+    size_t bytes() const { return n * sizeof(T); }
+};
+
+// Scalar row per plane (the reference's PlaneScalar slots, src/cpu/cpu_constants.h:82-101).
+enum Scalar
+{
+    SC_DIAG_SUM = 0,
+    SC_DIAG_MIN,
+    SC_DIAG_MAX,
+    SC_LAMBDA,
+    SC_RHS2,
+    SC_RZ,
+    SC_RZ_NEW,
+    SC_PAP,
+    SC_RR,
+    SC_RESID,
+    SC_ALPHA,
+    SC_BETA,
+    SC_MOVED,
+    SC_DELTA2,
+    SC_PLANE2,
+    SC_LO,
+    SC_HI = SC_LO + MAX_CH,
+    SC_ACTIVE = SC_HI + MAX_CH,   // 1 while the plane's conjugate gradients still iterate
+    SC_COUNT = SC_ACTIVE + 1
+};
+
+// One latent's solve workspace: level 1 (block (b)) or level 0 under --l0 bc8 (block (c')).
+struct Latent
+{
+    bool level0 = false;
+    int c = 0;              // channels
+    int bits = 8;
+    DevBuf<double> stencil; // texels * 5 * c * c
+    DevBuf<double> grad;    // texels * c
+    DevBuf<float> precond;  // texels * c * c
+    DevBuf<double> x, r, z, p, ap;
+    DevBuf<float> prev;
+    DevBuf<double> mono;    // nmono * c * c
+    int nmono = 0;
+};
+
+// Per-kernel timing for the benchmark: when on, every timed launch is bracketed by a pair of events on the arm's
+// stream, and its algorithmic bytes and flops (computed from the shapes, not measured) are added up by kernel name.
+struct KernelStat
+{
+    long long launches = 0;
+    double bytes = 0.0, flops = 0.0, ms = 0.0;
+    std::vector<hipEvent_t> pending;   // begin/end pairs not yet read
+};
+
+struct Profiler
+{
+    bool on = false;
+    std::vector<std::pair<std::string, KernelStat>> stats;
+    std::vector<hipEvent_t> pool;
+    KernelStat& get(const char* name);
+    hipEvent_t event();
+    void settle();                      // read every pending pair (synchronizes on the last)
+    void reset();
+    ~Profiler();
+};
+
+Profiler& profiler();
+
+// RAII bracket around one launch: `KernelTimer t(c.stream, "k_matvec", bytes, flops); hipLaunchKernelGGL(...);`
+struct KernelTimer
+{
+    KernelStat* st = nullptr;
+    hipStream_t stream;
+    hipEvent_t b = nullptr;
+    KernelTimer(hipStream_t s, const char* name, double bytes, double flops);
+    ~KernelTimer();
+};
+
+struct Ctx
+{
+    nntc_model_desc md {};
+    int nin = 0, nout = 0, c0 = 0, c1 = 0;
+    int form = NNTC_FORM_EXACT;
+    Planes planes {};
+    long long n0_total = 0, n1_total = 0, nb_total = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    DevBuf<float> src, v0, v1;
+    DevBuf<uint8_t> k0, k1;
+    DevBuf<float> weights, bias, cw;       // nout * nin, nout, nout
+    DevBuf<float> grid;                    // lo0[4] hi0[4] lo1[4] hi1[4] palette[4*16]
+    Latent lat1, lat0;
+    DevBuf<double> scalars;                // nplanes * SC_COUNT
+    DevBuf<double> partial;                // chunk partials of the fixed-order reductions
+    DevBuf<double> ls_partial;             // nplanes * entries * SITE_BLOCKS
+    DevBuf<double> ls_sum;                 // entries
+    DevBuf<double> obj;                    // nplanes * 3
+    DevBuf<float> resid;                   // base texels * nout
+    DevBuf<double> cov_partial;            // SITE_BLOCKS * entries
+    DevBuf<unsigned int> counters;         // per plane
+    DevBuf<uint8_t> bc_ep, bc_sel;         // blocks * c0 * 2, blocks * c0 * 16
+    DevBuf<uint8_t> bc_blocks;             // blocks * files * 16 (file f of block b at (b * files + f) * 16)
+    DevBuf<uint8_t> k0_mirror;             // the host's Model::k0 as the bc calls read it
+    DevBuf<double> bc_taken;               // per block: the accepted decrease
+    DevBuf<double> bc_se;                  // per plane: the packing error
+    DevBuf<float> work_f;                  // staging for grid fits (values of one channel)
+    DevBuf<unsigned int> work_u;           // radix-select histograms
+    DevBuf<uint8_t> rgb8;                  // one plane's decode
+
+    // Pinned host staging for the small per-call readbacks.
+    double* host_scalars = nullptr;
+
+    size_t bytes = 0;
+
+    // Host-side view of the decoder and grids as last uploaded.
+    std::vector<float> h_w, h_b;
+    float h_lo0[4] = {}, h_hi0[4] = {}, h_lo1[4] = {}, h_hi1[4] = {};
+
+    Latent& latent(bool level0) { return level0 ? lat0 : lat1; }
+    const PlaneGeom& g(int p) const { return planes.p[p]; }
+    float ms_since(hipEvent_t a);
+};
+
+// ---- launchers (kernel files)
+// objective.hip
+void objective_pass(Ctx& c, int k, double* sums_out /* nplanes * 3 */);
+void decode_pass(Ctx& c, int plane, uint8_t* out);
+void residual_pass(Ctx& c, int plane, float* resid);
+// decoder.hip
+void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc);
+// latent.hip
+void build_block_matrices(Ctx& c, bool level0);
+void assemble(Ctx& c, bool level0, int k, double ridge);
+void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations);
+void correction_and_probe(Ctx& c, bool level0);
+void quant_sweeps(Ctx& c, bool level0, int sweeps);
+void copy_prev_zero_delta(Ctx& c, bool level0);
+// init.cu counterparts (init.hip)
+void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0);
+void init_level1_box(Ctx& c);
+void quantise_values(Ctx& c, bool level0, const float* in_device, int plane);
+void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi);
+void cov_partials(Ctx& c, const float* data, long long texels, std::vector<double>& raw, double weight);
+double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, const double* mu);
+void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const double* mu, double inv, int channel);
+// refine.hip
+void bc_decode(Ctx& c);
+void bc_seed(Ctx& c);
+void bc_refine_pass(Ctx& c, double* taken, double* improved);
+void bc_pack_blocks(Ctx& c, double& se, size_t& n);
+
+// ---- shared launch helpers
+inline unsigned int blocks_for(long long n, int threads)
+{
+    return (unsigned int)((n + threads - 1) / threads);
+}
+
+}   // namespace nntc

@@ -1,0 +1,1367 @@
+// arm.hip: the gfx950 arm's lifecycle, transfers and host-side numerics, and the C ABI of include/nntc_hip.h.
+//
+// Host-side numerics kept here, each the reference's arithmetic on the reference's operands:
+//   - block (a)'s solve: mean diagonal, the incoming decoder as x, Q's normaliser, qform and its rounding bound, the
+//     ridge ladder {1e-9, 1e-12, 0} x mean diag with Gauss-Jordan on [A + ridge I | rhs^T], the float round trip before
+//     judging, the acceptance bar, the tally and the constant-fit fallback (src/solve_decoder.cu:184-458,
+//     src/cpu/cpu_solve_decoder.cpp:199-361);
+//   - the residual seed's covariance, cyclic Jacobi eigensolver, sign rule and divisor (src/init.cu:813-1400,
+//     src/cpu/cpu_init.cpp:372-419, 729-910);
+//   - the grid fits' fallback to [-1, 1] on an empty range (src/cpu/cpu_init.cpp:428-507).
+// The host compiler runs with -ffp-contract=off, as the reference's host build does.
+
+#include "arm.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+
+namespace nntc
+{
+
+HipFailure::HipFailure(const char* expr, hipError_t e, const char* file, int line)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "HIP error %d (%s) at %s:%d in %s", (int)e, hipGetErrorString(e), file, line, expr);
+    what = buf;
+}
+
+template <class T>
+void DevBuf<T>::alloc(size_t count)
+{
+    release();
+    if (count == 0)
+        count = 1;
+    void* ptr = nullptr;
+    HIP_OK(hipMalloc(&ptr, count * sizeof(T)));
+    p = static_cast<T*>(ptr);
+    n = count;
+}
+
+template <class T>
+void DevBuf<T>::release()
+{
+    if (p)
+        (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+}
+
+template struct DevBuf<float>;
+template struct DevBuf<double>;
+template struct DevBuf<uint8_t>;
+template struct DevBuf<unsigned int>;
+
+Profiler& profiler()
+{
+    static Profiler p;
+    return p;
+}
+
+KernelStat& Profiler::get(const char* name)
+{
+    for (auto& kv : stats)
+        if (kv.first == name)
+            return kv.second;
+    stats.emplace_back(name, KernelStat());
+    return stats.back().second;
+}
+
+hipEvent_t Profiler::event()
+{
+    if (!pool.empty())
+    {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    HIP_OK(hipEventCreate(&e));
+    return e;
+}
+
+void Profiler::settle()
+{
+    for (auto& kv : stats)
+    {
+        KernelStat& st = kv.second;
+        for (size_t i = 0; i + 1 < st.pending.size(); i += 2)
+        {
+            HIP_OK(hipEventSynchronize(st.pending[i + 1]));
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, st.pending[i], st.pending[i + 1]));
+            st.ms += ms;
+            pool.push_back(st.pending[i]);
+            pool.push_back(st.pending[i + 1]);
+        }
+        st.pending.clear();
+    }
+}
+
+void Profiler::reset()
+{
+    settle();
+    stats.clear();
+}
+
+Profiler::~Profiler()
+{
+    for (hipEvent_t e : pool)
+        (void)hipEventDestroy(e);
+}
+
+KernelTimer::KernelTimer(hipStream_t s, const char* name, double bytes, double flops) : stream(s)
+{
+    Profiler& p = profiler();
+    if (!p.on)
+        return;
+    st = &p.get(name);
+    st->launches++;
+    st->bytes += bytes;
+    st->flops += flops;
+    b = p.event();
+    HIP_OK(hipEventRecord(b, stream));
+}
+
+KernelTimer::~KernelTimer()
+{
+    if (!st)
+        return;
+    Profiler& p = profiler();
+    hipEvent_t e = p.event();
+    if (hipEventRecord(e, stream) == hipSuccess)
+    {
+        st->pending.push_back(b);
+        st->pending.push_back(e);
+    }
+}
+
+float Ctx::ms_since(hipEvent_t a)
+{
+    HIP_OK(hipEventRecord(ev1, stream));
+    HIP_OK(hipEventSynchronize(ev1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, a, ev1));
+    return ms;
+}
+
+namespace
+{
+
+thread_local std::string g_error;
+double g_objective_ms = 0.0;
+long long g_objective_calls = 0;
+long long g_ridge_tally[4] = { 0, 0, 0, 0 };
+
+const double RIDGE_LADDER[] = { 1e-9, 1e-12, 0.0 };
+const double LS_ACCEPT_RISE = 1e-12;
+const size_t RIDGE_REFUSED = 3;
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Run a body, turning HIP failures and refusals into a status and a message.
+template <class F>
+int guarded(F&& f)
+{
+    try
+    {
+        f();
+        return 0;
+    }
+    catch (const HipFailure& e)
+    {
+        g_error = e.what;
+    }
+    catch (const Refused& e)
+    {
+        g_error = e.what;
+    }
+    catch (const std::bad_alloc&)
+    {
+        g_error = "out of host memory";
+    }
+    return 1;
+}
+
+inline int tri_index(int i, int j, int mm)
+{
+    return i * mm - i * (i - 1) / 2 + (j - i);
+}
+
+bool gauss_jordan(std::vector<double>& g, int m, int nrhs)
+{
+    const int cols = m + nrhs;
+    for (int k = 0; k < m; k++)
+    {
+        int pivot = k;
+        double best = std::fabs(g[(size_t)k * cols + k]);
+        for (int i = k + 1; i < m; i++)
+        {
+            const double a = std::fabs(g[(size_t)i * cols + k]);
+            if (a > best)
+            {
+                best = a;
+                pivot = i;
+            }
+        }
+        if (best == 0.0)
+            return false;
+        if (pivot != k)
+            for (int j = 0; j < cols; j++)
+                std::swap(g[(size_t)k * cols + j], g[(size_t)pivot * cols + j]);
+        const double inv = 1.0 / g[(size_t)k * cols + k];
+        for (int j = 0; j < cols; j++)
+            g[(size_t)k * cols + j] *= inv;
+        for (int i = 0; i < m; i++)
+        {
+            if (i == k)
+                continue;
+            const double f = g[(size_t)i * cols + k];
+            if (f == 0.0)
+                continue;
+            for (int j = 0; j < cols; j++)
+                g[(size_t)i * cols + j] -= f * g[(size_t)k * cols + j];
+        }
+    }
+    return true;
+}
+
+// The cyclic Jacobi eigendecomposition of a small symmetric matrix; vec's COLUMNS are the eigenvectors.
+void jacobi_eigen(std::vector<double>& a, int n, std::vector<double>& vec, std::vector<double>& val)
+{
+    vec.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        vec[(size_t)i * n + i] = 1.0;
+    for (int sweep = 0; sweep < 100; sweep++)
+    {
+        double off = 0.0;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++)
+                off += a[(size_t)p * n + q] * a[(size_t)p * n + q];
+        if (off <= 1e-30)
+            break;
+        for (int p = 0; p < n; p++)
+            for (int q = p + 1; q < n; q++)
+            {
+                const double apq = a[(size_t)p * n + q];
+                if (std::fabs(apq) <= 1e-300)
+                    continue;
+                const double theta = (a[(size_t)q * n + q] - a[(size_t)p * n + p]) / (2.0 * apq);
+                const double t = (theta >= 0.0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < n; k++)
+                {
+                    const double akp = a[(size_t)k * n + p], akq = a[(size_t)k * n + q];
+                    a[(size_t)k * n + p] = c * akp - s * akq;
+                    a[(size_t)k * n + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++)
+                {
+                    const double apk = a[(size_t)p * n + k], aqk = a[(size_t)q * n + k];
+                    a[(size_t)p * n + k] = c * apk - s * aqk;
+                    a[(size_t)q * n + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++)
+                {
+                    const double vkp = vec[(size_t)k * n + p], vkq = vec[(size_t)k * n + q];
+                    vec[(size_t)k * n + p] = c * vkp - s * vkq;
+                    vec[(size_t)k * n + q] = s * vkp + c * vkq;
+                }
+            }
+    }
+    val.assign((size_t)n, 0.0);
+    for (int i = 0; i < n; i++)
+        val[(size_t)i] = a[(size_t)i * n + i];
+}
+
+void upload_decoder(Ctx& c, const nntc_model_state* st)
+{
+    const size_t nw = (size_t)c.nout * c.nin;
+    if (st && st->has_decoder)
+    {
+        c.h_w.assign(st->w, st->w + nw);
+        c.h_b.assign(st->b, st->b + c.nout);
+        HIP_OK(hipMemcpyAsync(c.weights.p, st->w, nw * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(c.bias.p, st->b, c.nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+    }
+}
+
+// Both grids written to the device, zero-padded (src/cpu/cpu_model.cpp:385-413).
+void upload_grids(Ctx& c, const nntc_model_state* st)
+{
+    float g[16 + 64] = {};
+    if (st)
+    {
+        if (st->has_grid0)
+            for (int q = 0; q < c.c0; q++)
+            {
+                g[q] = st->lo0[q];
+                g[4 + q] = st->hi0[q];
+            }
+        if (st->has_grid1)
+            for (int q = 0; q < c.c1; q++)
+            {
+                g[8 + q] = st->lo1[q];
+                g[12 + q] = st->hi1[q];
+            }
+        if (st->has_palette0)
+            for (int q = 0; q < 4; q++)
+                for (int k = 0; k < 16; k++)
+                    g[16 + q * 16 + k] = st->palette0[q][k];
+    }
+    std::memcpy(c.h_lo0, g, sizeof(c.h_lo0));
+    std::memcpy(c.h_hi0, g + 4, sizeof(c.h_hi0));
+    std::memcpy(c.h_lo1, g + 8, sizeof(c.h_lo1));
+    std::memcpy(c.h_hi1, g + 12, sizeof(c.h_hi1));
+    HIP_OK(hipMemcpyAsync(c.grid.p, g, sizeof(g), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+// A plane-sized download (or, with n smaller, its head).
+template <class T>
+void download(Ctx& c, const T* src, size_t count, T* out, size_t n, size_t* total)
+{
+    if (total)
+        *total = count;
+    const size_t m = count < n ? count : n;
+    if (out && m)
+    {
+        HIP_OK(hipMemcpyAsync(out, src, m * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+    }
+}
+
+template <class T>
+void upload(Ctx& c, T* dst, size_t count, const T* in, size_t n)
+{
+    if (n != count)
+        return;   // the reference ignores an upload of the wrong size (src/cpu/cpu_model.cpp:191-197)
+    HIP_OK(hipMemcpyAsync(dst, in, n * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+void check_plane(const Ctx& c, int plane)
+{
+    if (plane < 0 || plane >= c.planes.n)
+        throw Refused { "plane index out of range" };
+}
+
+// Fit one latent's grid over every plane, write it into the state and the device, snap every plane onto it.
+void fit_and_snap(Ctx& c, nntc_model_state* st, bool level0, int policy)
+{
+    float lo[4], hi[4];
+    fit_range(c, level0, policy, lo, hi);
+    const int ch = level0 ? c.c0 : c.c1;
+    for (int q = 0; q < ch; q++)
+        if (!(hi[q] > lo[q]))
+        {
+            lo[q] = -1.0f;
+            hi[q] = 1.0f;
+        }
+    for (int q = 0; q < ch; q++)
+    {
+        (level0 ? st->lo0 : st->lo1)[q] = lo[q];
+        (level0 ? st->hi0 : st->hi1)[q] = hi[q];
+    }
+    (level0 ? st->has_grid0 : st->has_grid1) = 1;
+    upload_grids(c, st);
+    quantise_values(c, level0, level0 ? c.v0.p : c.v1.p, -1);
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+void read_scalars(Ctx& c, int channels, nntc_level1_report* rep)
+{
+    HIP_OK(hipMemcpyAsync(c.host_scalars, c.scalars.p, sizeof(double) * SC_COUNT * c.planes.n,
+                          hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        const double* s = c.host_scalars + (size_t)p * SC_COUNT;
+        rep[p].lambda = s[SC_LAMBDA];
+        rep[p].min_diag = s[SC_DIAG_MIN];
+        rep[p].max_diag = s[SC_DIAG_MAX];
+        rep[p].residual = s[SC_RESID];
+        rep[p].moved_values = (long long)s[SC_MOVED];
+        rep[p].delta2 = s[SC_DELTA2];
+        rep[p].plane2 = s[SC_PLANE2];
+        for (int q = 0; q < channels; q++)
+        {
+            rep[p].lo[q] = (float)s[SC_LO + q];
+            rep[p].hi[q] = (float)s[SC_HI + q];
+        }
+    }
+}
+
+// Blocks (b) and (c'), continuous: assemble, solve, correct, probe (src/cpu/cpu_solve_level1.cpp:969-986).
+double solve_block(Ctx& c, const nntc_model_state* st, int k, double ridge, bool level0, nntc_level1_report* rep)
+{
+    const double t0 = now_ms();
+    upload_decoder(c, st);
+    for (int p = 0; p < c.planes.n; p++)
+        rep[p] = nntc_level1_report {};
+    build_block_matrices(c, level0);
+    HIP_OK(hipEventRecord(c.ev0, c.stream));
+    assemble(c, level0, k, ridge);
+    const float ms_asm = c.ms_since(c.ev0);
+    HIP_OK(hipEventRecord(c.ev0, c.stream));
+    std::vector<int> iterations;
+    cg_solve(c, level0, iterations);
+    correction_and_probe(c, level0);
+    const float ms_cg = c.ms_since(c.ev0);
+    const int ch = level0 ? c.c0 : c.c1;
+    read_scalars(c, ch, rep);
+    // Every plane of the chain runs in the same launches, so each plane's report carries the block's kernel time.
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        rep[p].iterations = iterations[(size_t)p];
+        rep[p].quantised = 0;
+        rep[p].ms_assemble = ms_asm;
+        rep[p].ms_cg = ms_cg;
+    }
+    return now_ms() - t0;
+}
+
+// The same block on a frozen grid: assemble, then `sweeps` four-colour Gauss-Seidel passes
+// (src/cpu/cpu_solve_level1.cpp:991-1054).
+double sweep_block(Ctx& c, const nntc_model_state* st, int k, double ridge, int sweeps, bool level0,
+                   nntc_level1_report* rep)
+{
+    const double t0 = now_ms();
+    upload_decoder(c, st);
+    for (int p = 0; p < c.planes.n; p++)
+        rep[p] = nntc_level1_report {};
+    build_block_matrices(c, level0);
+    HIP_OK(hipEventRecord(c.ev0, c.stream));
+    assemble(c, level0, k, ridge);
+    const float ms_asm = c.ms_since(c.ev0);
+    HIP_OK(hipEventRecord(c.ev0, c.stream));
+    copy_prev_zero_delta(c, level0);
+    quant_sweeps(c, level0, sweeps);
+    const float ms_sw = c.ms_since(c.ev0);
+    const int ch = level0 ? c.c0 : c.c1;
+    read_scalars(c, ch, rep);
+    std::vector<unsigned int> moved((size_t)c.planes.n);
+    HIP_OK(hipMemcpyAsync(moved.data(), c.counters.p, moved.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                          c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        rep[p].quantised = 1;
+        rep[p].sweeps = sweeps;
+        rep[p].ms_assemble = ms_asm;
+        rep[p].ms_sweeps = ms_sw;
+        // The scalar row persists between blocks, so the continuous-only fields are cleared.
+        rep[p].iterations = 0;
+        rep[p].residual = 0.0;
+        rep[p].moved_values = 0;
+        rep[p].delta2 = 0.0;
+        rep[p].plane2 = 0.0;
+        rep[p].moved = (long long)moved[(size_t)p];
+    }
+    return now_ms() - t0;
+}
+
+// Block (c')'s assembly alone at zero ridge, prev = v0, delta = 0 (src/cpu/cpu_solve_level1.cpp:1082-1097).
+double assemble_for_refine(Ctx& c, const nntc_model_state* st, int k)
+{
+    const double t0 = now_ms();
+    upload_decoder(c, st);
+    build_block_matrices(c, true);
+    assemble(c, true, k, 0.0);
+    copy_prev_zero_delta(c, true);
+    HIP_OK(hipStreamSynchronize(c.stream));
+    return now_ms() - t0;
+}
+
+void set_k0_mirror(Ctx& c, const uint8_t* const* k0, bool snapshot)
+{
+    if (k0)
+    {
+        for (int p = 0; p < c.planes.n; p++)
+            HIP_OK(hipMemcpyAsync(c.k0_mirror.p + c.g(p).off0 * c.c0, k0[p], (size_t)c.g(p).n0 * c.c0,
+                                  hipMemcpyHostToDevice, c.stream));
+    }
+    else if (snapshot)
+        HIP_OK(hipMemcpyAsync(c.k0_mirror.p, c.k0.p, (size_t)c.n0_total * c.c0, hipMemcpyDeviceToDevice, c.stream));
+}
+
+void blocks_from_device(Ctx& c, double* pack_psnr, size_t* pack_texels)
+{
+    double se = 0.0;
+    size_t n = 0;
+    bc_pack_blocks(c, se, n);
+    if (pack_texels)
+        *pack_texels = n;
+    if (pack_psnr)
+        *pack_psnr = n > 0 && se > 0.0 ? 10.0 * std::log10(255.0 * 255.0 / (se / (double)n)) : 100.0;
+}
+
+}   // namespace
+
+}   // namespace nntc
+
+using namespace nntc;
+
+struct nntc_hip_ctx
+{
+    Ctx c;
+};
+
+extern "C"
+{
+
+const char* nntc_hip_last_error(void)
+{
+    return g_error.c_str();
+}
+
+int nntc_hip_device_select(int index, nntc_device_info* info)
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || index < 0 || index >= count)
+    {
+        if (info)
+            std::snprintf(info->error, sizeof(info->error), "%s",
+                          e != hipSuccess ? hipGetErrorString(e) : "no such device");
+        return 0;
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, index) != hipSuccess || hipSetDevice(index) != hipSuccess)
+        return 0;
+    if (info)
+    {
+        std::snprintf(info->name, sizeof(info->name), "%s (%s)", prop.name, prop.gcnArchName);
+        info->major = prop.major;
+        info->minor = prop.minor;
+        info->error[0] = 0;
+    }
+    return 1;
+}
+
+nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* const* source_chain)
+{
+    nntc_hip_ctx* h = new nntc_hip_ctx();
+    Ctx& c = h->c;
+    const int st = guarded([&] {
+        if (!md || md->nplanes < 1 || md->nplanes > MAX_PLANES || md->c0 < 1 || md->c0 > MAX_CH || md->c1 < 1 ||
+            md->c1 > MAX_CH || md->nout < 3 || md->nout > MAX_NOUT)
+            throw Refused { "model descriptor out of range" };
+        if (!md->l0_bc8)
+            throw Refused { "the gfx950 arm encodes --l0 bc8; the palette mode is not built" };
+        c.md = *md;
+        c.nout = md->nout;
+        c.c0 = md->c0;
+        c.c1 = md->c1;
+        c.nin = c.c1 + c.c0 + c.c0 * c.c1;
+        c.planes.n = md->nplanes;
+        long long o0 = 0, o1 = 0, ob = 0;
+        for (int p = 0; p < md->nplanes; p++)
+        {
+            PlaneGeom& g = c.planes.p[p];
+            g.w0 = md->planes[p].w0;
+            g.h0 = md->planes[p].h0;
+            g.w1 = md->planes[p].w1;
+            g.h1 = md->planes[p].h1;
+            if (g.w0 < 1 || g.h0 < 1 || g.w1 < 1 || g.h1 < 1)
+                throw Refused { "empty plane" };
+            g.n0 = (long long)g.w0 * g.h0;
+            g.n1 = (long long)g.w1 * g.h1;
+            g.nb = (long long)((g.w0 + 3) / 4) * ((g.h0 + 3) / 4);
+            g.off0 = o0;
+            g.off1 = o1;
+            g.offb = ob;
+            o0 += g.n0;
+            o1 += g.n1;
+            ob += g.nb;
+        }
+        c.n0_total = o0;
+        c.n1_total = o1;
+        c.nb_total = ob;
+        HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+        HIP_OK(hipEventCreate(&c.ev0));
+        HIP_OK(hipEventCreate(&c.ev1));
+
+        const int nout = c.nout, c0 = c.c0, c1 = c.c1;
+        c.src.alloc((size_t)o0 * nout);
+        c.v0.alloc((size_t)o0 * c0);
+        c.v1.alloc((size_t)o1 * c1);
+        c.k0.alloc((size_t)o0 * c0);
+        c.k1.alloc((size_t)o1 * c1);
+        c.weights.alloc((size_t)nout * c.nin);
+        c.bias.alloc((size_t)nout);
+        c.cw.alloc((size_t)nout);
+        c.grid.alloc(16 + 64);
+        for (int lvl = 0; lvl < 2; lvl++)
+        {
+            const bool level0 = lvl == 0;
+            Latent& L = c.latent(level0);
+            const long long t = level0 ? o0 : o1;
+            L.level0 = level0;
+            L.c = level0 ? c0 : c1;
+            L.bits = level0 ? 8 : md->bits1;
+            L.nmono = stencil_monomials(level0 ? c1 : c0);
+            L.stencil.alloc((size_t)t * STENCIL_BLOCKS * L.c * L.c);
+            L.grad.alloc((size_t)t * L.c);
+            L.precond.alloc((size_t)t * L.c * L.c);
+            L.x.alloc((size_t)t * L.c);
+            L.r.alloc((size_t)t * L.c);
+            L.z.alloc((size_t)t * L.c);
+            L.p.alloc((size_t)t * L.c);
+            L.ap.alloc((size_t)t * L.c);
+            L.prev.alloc((size_t)t * L.c);
+            L.mono.alloc((size_t)L.nmono * L.c * L.c);
+        }
+        c.scalars.alloc((size_t)SC_COUNT * md->nplanes);
+        long long chunks_obj = 0;
+        for (int p = 0; p < md->nplanes; p++)
+            chunks_obj += (c.planes.p[p].n0 * 5 + CHUNK - 1) / CHUNK + 1;
+        c.partial.alloc((size_t)std::max<long long>(chunks_obj * 3, (o0 / CHUNK + md->nplanes + 1) * 8));
+        const int mm = c.nin + 1;
+        const int entries = mm * (mm + 1) / 2 + nout * mm;
+        c.ls_partial.alloc((size_t)md->nplanes * entries * SITE_BLOCKS);
+        c.ls_sum.alloc((size_t)entries + MAX_PLANES);
+        c.obj.alloc((size_t)md->nplanes * 3);
+        c.resid.alloc((size_t)o0 * nout);
+        c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout));
+        c.counters.alloc((size_t)md->nplanes);
+        c.bc_ep.alloc((size_t)ob * c0 * 2);
+        c.bc_sel.alloc((size_t)ob * c0 * 16);
+        c.bc_blocks.alloc((size_t)ob * 32);
+        c.k0_mirror.alloc((size_t)o0 * c0);
+        c.bc_taken.alloc((size_t)ob);
+        c.bc_se.alloc((size_t)ob);
+        c.work_f.alloc(256 * 2 * MAX_CH);
+        c.work_u.alloc(MAX_CH * 2 * 256 + 64);
+        c.rgb8.alloc((size_t)c.planes.p[0].n0 * nout);
+        HIP_OK(hipHostMalloc((void**)&c.host_scalars, sizeof(double) * SC_COUNT * MAX_PLANES, hipHostMallocDefault));
+
+        c.bytes = c.src.bytes() + c.v0.bytes() + c.v1.bytes() + c.k0.bytes() + c.k1.bytes() + c.weights.bytes() +
+                  c.bias.bytes() + c.cw.bytes() + c.grid.bytes() + c.scalars.bytes() + c.partial.bytes() +
+                  c.ls_partial.bytes() + c.ls_sum.bytes() + c.obj.bytes() + c.resid.bytes() + c.cov_partial.bytes() +
+                  c.counters.bytes() + c.bc_ep.bytes() + c.bc_sel.bytes() + c.bc_blocks.bytes() +
+                  c.k0_mirror.bytes() + c.bc_taken.bytes() + c.bc_se.bytes() + c.work_f.bytes() + c.work_u.bytes() +
+                  c.rgb8.bytes();
+        for (int lvl = 0; lvl < 2; lvl++)
+        {
+            Latent& L = c.latent(lvl == 0);
+            c.bytes += L.stencil.bytes() + L.grad.bytes() + L.precond.bytes() + L.x.bytes() + L.r.bytes() +
+                       L.z.bytes() + L.p.bytes() + L.ap.bytes() + L.prev.bytes() + L.mono.bytes();
+        }
+
+        // The source chain, the one upload before any block; the decoder starts at zero, the grids at the state
+        // device_create is handed (src/cpu/cpu_model.cpp:208-237).
+        for (int p = 0; p < md->nplanes; p++)
+            HIP_OK(hipMemcpyAsync(c.src.p + c.g(p).off0 * nout, source_chain[p],
+                                  (size_t)c.g(p).n0 * nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(c.cw.p, md->cw, nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemsetAsync(c.weights.p, 0, c.weights.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.bias.p, 0, c.bias.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.v0.p, 0, c.v0.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.v1.p, 0, c.v1.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.k0.p, 0, c.k0.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.k1.p, 0, c.k1.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.scalars.p, 0, c.scalars.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.bc_ep.p, 0, c.bc_ep.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.bc_sel.p, 0, c.bc_sel.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.lat0.x.p, 0, c.lat0.x.bytes(), c.stream));
+        HIP_OK(hipMemsetAsync(c.lat1.x.p, 0, c.lat1.x.bytes(), c.stream));
+        c.h_w.assign((size_t)nout * c.nin, 0.0f);
+        c.h_b.assign((size_t)nout, 0.0f);
+        nntc_model_state s {};
+        s.has_grid0 = 1;
+        for (int q = 0; q < c0; q++)
+        {
+            s.lo0[q] = -1.0f;
+            s.hi0[q] = 1.0f;
+        }
+        upload_grids(c, &s);
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+    if (st)
+    {
+        nntc_hip_device_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+void nntc_hip_device_destroy(nntc_hip_ctx* h)
+{
+    if (!h)
+        return;
+    Ctx& c = h->c;
+    if (c.stream)
+        (void)hipStreamSynchronize(c.stream);
+    if (c.host_scalars)
+        (void)hipHostFree(c.host_scalars);
+    if (c.ev0)
+        (void)hipEventDestroy(c.ev0);
+    if (c.ev1)
+        (void)hipEventDestroy(c.ev1);
+    hipStream_t s = c.stream;
+    delete h;
+    if (s)
+        (void)hipStreamDestroy(s);
+}
+
+size_t nntc_hip_device_memory(const nntc_hip_ctx* h)
+{
+    return h ? h->c.bytes : 0;
+}
+
+int nntc_hip_set_form(nntc_hip_ctx* h, int form)
+{
+    return guarded([&] {
+        if (form != NNTC_FORM_EXACT && form != NNTC_FORM_FAST)
+            throw Refused { "unknown form" };
+        h->c.form = form;
+    });
+}
+
+int nntc_hip_stencil_monomials(int c0)
+{
+    return stencil_monomials(c0);
+}
+
+int nntc_hip_init_level0(nntc_hip_ctx* h, nntc_model_state* st, const float luma[3], int luma_channel0)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        upload_grids(c, st);
+        init_level0_kernel(c, luma, luma_channel0);
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int nntc_hip_init_level1(nntc_hip_ctx* h, nntc_model_state* st, int pca, nntc_level1_init_report* rep)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        if (pca)
+            throw Refused { "--init pca is not built in the gfx950 arm; the default box init is" };
+        if (rep)
+        {
+            *rep = nntc_level1_init_report {};
+            rep->pca = 0;
+        }
+        init_level1_box(c);
+        fit_and_snap(c, st, false, NNTC_RANGE_MINMAX);
+    });
+}
+
+int nntc_hip_init0_residual_channel(nntc_hip_ctx* h, nntc_model_state* st, int channel, const double* plane_weight,
+                                    nntc_init0_report* rep)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        const int nout = c.nout;
+        if (c.md.init0_texture)
+            throw Refused { "--init0 texture is not built in the gfx950 arm; the default --init0 residual is" };
+        if (channel < 0 || channel >= c.c0)
+            throw Refused { "channel out of range" };
+        upload_decoder(c, st);
+        nntc_init0_report r {};
+        r.nout = nout;
+        r.texture = -1;
+        // (1) the residual of every plane at once: each plane's depends on that plane alone, and the seed below
+        // writes a plane only after its residual is read.
+        for (int p = 0; p < c.planes.n; p++)
+            residual_pass(c, p, c.resid.p + c.g(p).off0 * nout);
+        // (2) the covariance over the base plane, or every plane weighted under --init0-scope chain.
+        const int entries = nout + nout * nout;
+        std::vector<double> raw((size_t)entries, 0.0);
+        double count = 0.0;
+        const int nplanes = c.md.init0_chain ? c.planes.n : 1;
+        for (int p = 0; p < nplanes; p++)
+        {
+            const double w = c.md.init0_chain ? plane_weight[p] : 1.0;
+            if (!(w > 0.0))
+                continue;
+            cov_partials(c, c.resid.p + c.g(p).off0 * nout, c.g(p).n0, raw, w);
+            count += w * (double)c.g(p).n0;
+        }
+        std::vector<double> mu((size_t)nout, 0.0), cov((size_t)nout * nout, 0.0);
+        for (int rr = 0; rr < nout; rr++)
+            mu[(size_t)rr] = count > 0.0 ? raw[(size_t)rr] / count : 0.0;
+        for (int rr = 0; rr < nout; rr++)
+            for (int s = 0; s < nout; s++)
+                cov[(size_t)rr * nout + s] = (count > 0.0 ? raw[(size_t)(nout + rr * nout + s)] / count : 0.0) -
+                                             mu[(size_t)rr] * mu[(size_t)s];
+        for (int rr = 0; rr < nout; rr++)
+            r.total_variance += cov[(size_t)rr * nout + rr];
+        // (3) the leading eigenvector, its largest component made positive.
+        std::vector<double> work = cov, vec, val;
+        jacobi_eigen(work, nout, vec, val);
+        int top = 0;
+        for (int i = 1; i < nout; i++)
+            if (val[(size_t)i] > val[(size_t)top])
+                top = i;
+        r.eigenvalue = val[(size_t)top];
+        std::vector<double> e((size_t)nout);
+        for (int rr = 0; rr < nout; rr++)
+            e[(size_t)rr] = vec[(size_t)rr * nout + top];
+        r.share = r.total_variance > 0.0 ? r.eigenvalue / r.total_variance : 0.0;
+        int biggest = 0;
+        for (int rr = 1; rr < nout; rr++)
+            if (std::fabs(e[(size_t)rr]) > std::fabs(e[(size_t)biggest]))
+                biggest = rr;
+        if (e[(size_t)biggest] < 0.0)
+            for (int rr = 0; rr < nout; rr++)
+                e[(size_t)rr] = -e[(size_t)rr];
+        for (int rr = 0; rr < nout && rr < MAX_NOUT; rr++)
+            r.dir[rr] = e[(size_t)rr];
+        // (4a) the divisor: the largest |projection| over the whole chain.
+        double peak = 0.0;
+        for (int p = 0; p < c.planes.n; p++)
+            peak = std::max(peak, proj_peak(c, c.resid.p + c.g(p).off0 * nout, c.g(p).n0, e.data(), mu.data()));
+        r.peak = peak;
+        r.divisor = peak;
+        // (4b) the channel on every plane, continuous under --l0 bc8.
+        const double inv = peak > 0.0 ? 1.0 / peak : 0.0;
+        for (int p = 0; p < c.planes.n; p++)
+            seed_channel(c, p, c.resid.p + c.g(p).off0 * nout, e.data(), mu.data(), inv, channel);
+        HIP_OK(hipStreamSynchronize(c.stream));
+        if (rep)
+            *rep = r;
+    });
+}
+
+int nntc_hip_quantise_level1(nntc_hip_ctx* h, nntc_model_state* st, int policy)
+{
+    return guarded([&] { fit_and_snap(h->c, st, false, policy); });
+}
+
+int nntc_hip_freeze_level1_grid(nntc_hip_ctx* h, nntc_model_state* st, int policy)
+{
+    return guarded([&] { fit_and_snap(h->c, st, false, policy); });
+}
+
+int nntc_hip_quantise_level0(nntc_hip_ctx* h, nntc_model_state* st, int policy)
+{
+    return guarded([&] { fit_and_snap(h->c, st, true, policy); });
+}
+
+int nntc_hip_freeze_level0_grid(nntc_hip_ctx* h, nntc_model_state* st, int policy)
+{
+    return guarded([&] { fit_and_snap(h->c, st, true, policy); });
+}
+
+int nntc_hip_snap_level0_on_grid(nntc_hip_ctx* h, nntc_model_state* st)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        upload_grids(c, st);
+        quantise_values(c, true, c.v0.p, -1);
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int nntc_hip_solve_decoder(nntc_hip_ctx* h, nntc_model_state* st, int k, const double* per_site, double* ms_out)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        const int nin = c.nin, nout = c.nout, mm = nin + 1;
+        const int tri = mm * (mm + 1) / 2;
+        const int planes = c.planes.n;
+        HIP_OK(hipEventRecord(c.ev0, c.stream));
+        std::vector<double> acc;
+        ls_accumulate(c, k, per_site, acc);
+        const double ms = c.ms_since(c.ev0);
+
+        double trace = 0.0;
+        for (int i = 0; i < mm; i++)
+            trace += acc[(size_t)tri_index(i, i, mm)];
+        const double mean_diag = trace > 0.0 ? trace / mm : 1.0;
+        const bool had_previous = st->has_decoder != 0;
+        std::vector<double> x_prev((size_t)nout * mm, 0.0);
+        if (had_previous)
+            for (int cc = 0; cc < nout; cc++)
+            {
+                for (int i = 0; i < nin; i++)
+                    x_prev[(size_t)cc * mm + i] = (double)st->w[(size_t)cc * nin + i];
+                x_prev[(size_t)cc * mm + nin] = (double)st->b[cc];
+            }
+        double sum_cw = 0.0;
+        for (int cc = 0; cc < nout; cc++)
+            sum_cw += (double)c.md.cw[cc];
+        double den = 0.0;
+        for (int i = 0; i < planes; i++)
+            den += per_site[i] * (double)c.g(i).w0 * (double)c.g(i).h0 * (double)(1 + k);
+        const bool norm_ok = sum_cw * den > 0.0;
+        const double inv_norm = norm_ok ? 1.0 / (sum_cw * den) : 0.0;
+        auto qform = [&](const std::vector<double>& x, double& err) {
+            double q = 0.0, e = 0.0;
+            for (int cc = 0; cc < nout; cc++)
+            {
+                double qc = 0.0, ec = 0.0;
+                for (int i = 0; i < mm; i++)
+                {
+                    for (int j = 0; j < mm; j++)
+                    {
+                        const double a = i <= j ? acc[(size_t)tri_index(i, j, mm)] : acc[(size_t)tri_index(j, i, mm)];
+                        qc += x[(size_t)cc * mm + i] * a * x[(size_t)cc * mm + j];
+                        ec += std::fabs(x[(size_t)cc * mm + i]) * std::fabs(a) * std::fabs(x[(size_t)cc * mm + j]);
+                    }
+                    qc -= 2.0 * x[(size_t)cc * mm + i] * acc[(size_t)tri + (size_t)cc * mm + i];
+                    ec += 2.0 * std::fabs(x[(size_t)cc * mm + i]) * std::fabs(acc[(size_t)tri + (size_t)cc * mm + i]);
+                }
+                q += (double)c.md.cw[cc] * qc;
+                e += (double)c.md.cw[cc] * ec;
+            }
+            err = DBL_EPSILON * (double)mm * e * inv_norm;
+            return q;
+        };
+        double err_prev = 0.0;
+        const double q_prev = had_previous ? qform(x_prev, err_prev) : 0.0;
+        const int cols = mm + nout;
+        std::vector<double> g((size_t)mm * cols, 0.0);
+        std::vector<double> x_new((size_t)nout * mm, 0.0);
+        bool accepted = false;
+        size_t taken = 0;
+        for (size_t step = 0; step < RIDGE_REFUSED && !accepted; step++)
+        {
+            const double ridge = RIDGE_LADDER[step] * mean_diag;
+            std::fill(g.begin(), g.end(), 0.0);
+            for (int i = 0; i < mm; i++)
+            {
+                for (int j = i; j < mm; j++)
+                {
+                    const double a = acc[(size_t)tri_index(i, j, mm)];
+                    g[(size_t)i * cols + j] = a;
+                    g[(size_t)j * cols + i] = a;
+                }
+                g[(size_t)i * cols + i] += ridge;
+                for (int cc = 0; cc < nout; cc++)
+                    g[(size_t)i * cols + mm + cc] = acc[(size_t)tri + (size_t)cc * mm + i];
+            }
+            if (!gauss_jordan(g, mm, nout))
+                continue;
+            bool finite = true;
+            for (int cc = 0; cc < nout; cc++)
+                for (int i = 0; i < mm; i++)
+                {
+                    const double v = (double)(float)g[(size_t)i * cols + mm + cc];
+                    x_new[(size_t)cc * mm + i] = v;
+                    finite = finite && std::isfinite(v);
+                }
+            if (!finite)
+                continue;
+            double err_cand = 0.0;
+            const double q_cand = qform(x_new, err_cand);
+            if (!had_previous || (norm_ok && (q_cand - q_prev) * inv_norm + err_prev + err_cand <= LS_ACCEPT_RISE))
+            {
+                accepted = true;
+                taken = step;
+            }
+        }
+        g_ridge_tally[accepted ? taken : RIDGE_REFUSED]++;
+        if (accepted)
+        {
+            st->has_decoder = 1;
+            for (int cc = 0; cc < nout; cc++)
+            {
+                for (int i = 0; i < nin; i++)
+                    st->w[(size_t)cc * nin + i] = (float)x_new[(size_t)cc * mm + i];
+                st->b[cc] = (float)x_new[(size_t)cc * mm + nin];
+            }
+        }
+        else if (!had_previous)
+        {
+            st->has_decoder = 1;
+            const double count = acc[(size_t)tri_index(nin, nin, mm)];
+            for (int cc = 0; cc < nout; cc++)
+            {
+                for (int i = 0; i < nin; i++)
+                    st->w[(size_t)cc * nin + i] = 0.0f;
+                st->b[cc] = count > 0.0 ? (float)(acc[(size_t)tri + (size_t)cc * mm + nin] / count) : 0.0f;
+            }
+        }
+        upload_decoder(c, st);
+        if (ms_out)
+            *ms_out = ms;
+    });
+}
+
+void nntc_hip_decoder_ridge_tally(long long out[4])
+{
+    for (int i = 0; i < 4; i++)
+        out[i] = g_ridge_tally[i];
+}
+
+int nntc_hip_solve_level1_all(nntc_hip_ctx* h, const nntc_model_state* st, int k, double ridge,
+                              nntc_level1_report* rep, double* ms)
+{
+    return guarded([&] {
+        const double t = solve_block(h->c, st, k, ridge, false, rep);
+        if (ms)
+            *ms = t;
+    });
+}
+
+int nntc_hip_sweep_level1_all(nntc_hip_ctx* h, const nntc_model_state* st, int k, double ridge, int sweeps,
+                              nntc_level1_report* rep, double* ms)
+{
+    return guarded([&] {
+        const double t = sweep_block(h->c, st, k, ridge, sweeps, false, rep);
+        if (ms)
+            *ms = t;
+    });
+}
+
+int nntc_hip_solve_level0_cont_all(nntc_hip_ctx* h, const nntc_model_state* st, int k, double ridge,
+                                   nntc_level1_report* rep, double* ms)
+{
+    return guarded([&] {
+        const double t = solve_block(h->c, st, k, ridge, true, rep);
+        if (ms)
+            *ms = t;
+    });
+}
+
+int nntc_hip_sweep_level0_cont_all(nntc_hip_ctx* h, const nntc_model_state* st, int k, double ridge, int sweeps,
+                                   nntc_level1_report* rep, double* ms)
+{
+    return guarded([&] {
+        const double t = sweep_block(h->c, st, k, ridge, sweeps, true, rep);
+        if (ms)
+            *ms = t;
+    });
+}
+
+int nntc_hip_assemble_level0_for_refine(nntc_hip_ctx* h, const nntc_model_state* st, int k, double* ms)
+{
+    return guarded([&] {
+        const double t = assemble_for_refine(h->c, st, k);
+        if (ms)
+            *ms = t;
+    });
+}
+
+int nntc_hip_level1_download_indices(nntc_hip_ctx* h, int plane, uint8_t* k1, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        download(c, c.k1.p + c.g(plane).off1 * c.c1, (size_t)c.g(plane).n1 * c.c1, k1, n, count);
+    });
+}
+
+int nntc_hip_level1_upload(nntc_hip_ctx* h, int plane, const float* v1, size_t n)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        upload(c, c.v1.p + c.g(plane).off1 * c.c1, (size_t)c.g(plane).n1 * c.c1, v1, n);
+    });
+}
+
+int nntc_hip_level1_upload_indices(nntc_hip_ctx* h, int plane, const uint8_t* k1, size_t n)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        upload(c, c.k1.p + c.g(plane).off1 * c.c1, (size_t)c.g(plane).n1 * c.c1, k1, n);
+    });
+}
+
+int nntc_hip_level0_upload_indices(nntc_hip_ctx* h, int plane, const uint8_t* k0, size_t n)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        upload(c, c.k0.p + c.g(plane).off0 * c.c0, (size_t)c.g(plane).n0 * c.c0, k0, n);
+    });
+}
+
+int nntc_hip_upload_decoder_to_device(nntc_hip_ctx* h, const nntc_model_state* st)
+{
+    return guarded([&] { upload_decoder(h->c, st); });
+}
+
+int nntc_hip_upload_grids_to_device(nntc_hip_ctx* h, const nntc_model_state* st)
+{
+    return guarded([&] { upload_grids(h->c, st); });
+}
+
+int nntc_hip_level1_download(nntc_hip_ctx* h, int plane, float* v1, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        download(c, c.v1.p + c.g(plane).off1 * c.c1, (size_t)c.g(plane).n1 * c.c1, v1, n, count);
+    });
+}
+
+int nntc_hip_level1_delta(nntc_hip_ctx* h, int plane, double* delta, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        download(c, c.lat1.x.p + c.g(plane).off1 * c.c1, (size_t)c.g(plane).n1 * c.c1, delta, n, count);
+    });
+}
+
+int nntc_hip_level1_poke(nntc_hip_ctx* h, int plane, size_t index, float value)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        if (index >= (size_t)c.g(plane).n1 * c.c1)
+            throw Refused { "level1_poke past the end of the plane" };
+        HIP_OK(hipMemcpyAsync(c.v1.p + c.g(plane).off1 * c.c1 + index, &value, sizeof(float), hipMemcpyHostToDevice,
+                              c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int nntc_hip_level1_range(nntc_hip_ctx* h, int plane, float* lo, float* hi)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        std::vector<float> v((size_t)c.g(plane).n1 * c.c1);
+        download(c, c.v1.p + c.g(plane).off1 * c.c1, v.size(), v.data(), v.size(), nullptr);
+        for (int q = 0; q < c.c1; q++)
+        {
+            lo[q] = 1e30f;
+            hi[q] = -1e30f;
+        }
+        for (size_t i = 0; i < v.size(); i++)
+        {
+            const int q = (int)(i % (size_t)c.c1);
+            lo[q] = v[i] < lo[q] ? v[i] : lo[q];
+            hi[q] = v[i] > hi[q] ? v[i] : hi[q];
+        }
+    });
+}
+
+int nntc_hip_level0_download(nntc_hip_ctx* h, int plane, uint8_t* k0, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        download(c, c.k0.p + c.g(plane).off0 * c.c0, (size_t)c.g(plane).n0 * c.c0, k0, n, count);
+    });
+}
+
+int nntc_hip_level0_upload(nntc_hip_ctx* h, int plane, const float* v0, size_t n)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        upload(c, c.v0.p + c.g(plane).off0 * c.c0, (size_t)c.g(plane).n0 * c.c0, v0, n);
+    });
+}
+
+int nntc_hip_level0_download_values(nntc_hip_ctx* h, int plane, float* v0, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        download(c, c.v0.p + c.g(plane).off0 * c.c0, (size_t)c.g(plane).n0 * c.c0, v0, n, count);
+    });
+}
+
+int nntc_hip_decode_plane(nntc_hip_ctx* h, int plane, uint8_t* rgb8, size_t n)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        decode_pass(c, plane, c.rgb8.p);
+        download(c, (const uint8_t*)c.rgb8.p, (size_t)c.g(plane).n0 * c.nout, rgb8, n, nullptr);
+    });
+}
+
+int nntc_hip_objective_eval(nntc_hip_ctx* h, int k, const double* per_site, nntc_objective* r)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        const double t0 = now_ms();
+        std::vector<double> acc((size_t)c.planes.n * 3);
+        objective_pass(c, k, acc.data());
+        const double ms = now_ms() - t0;
+        double sum_cw = 0.0;
+        for (int q = 0; q < c.nout; q++)
+            sum_cw += (double)c.md.cw[q];
+        double num = 0.0, den = 0.0;
+        *r = nntc_objective {};
+        r->nplanes = c.planes.n;
+        for (int p = 0; p < c.planes.n; p++)
+        {
+            const double pixels = (double)c.g(p).w0 * (double)c.g(p).h0;
+            const double sites = pixels * (double)(1 + k);
+            r->e_plane[p] = acc[(size_t)3 * p] / (sum_cw * sites);
+            r->centre_plane[p] = acc[(size_t)3 * p + 1] / (pixels * (double)c.nout);
+            num += per_site[p] * acc[(size_t)3 * p];
+            den += per_site[p] * sites;
+        }
+        r->e = den > 0.0 ? num / (sum_cw * den) : 0.0;
+        const double base = (double)c.g(0).w0 * (double)c.g(0).h0 * (double)c.nout;
+        r->centre_mse = acc[1] / base;
+        r->sampled_mse = k > 0 ? acc[2] / (base * (double)k) : 0.0;
+        r->ms = ms;
+        g_objective_ms += ms;
+        g_objective_calls++;
+    });
+}
+
+double nntc_hip_objective_total_ms(void)
+{
+    return g_objective_ms;
+}
+
+long long nntc_hip_objective_passes(void)
+{
+    return g_objective_calls;
+}
+
+int nntc_hip_bc_pack_prepare(nntc_hip_ctx* h, const nntc_model_state* st, int k, int seed,
+                             const uint8_t* const* k0_mirror, double* pack_psnr, size_t* pack_texels, double* ms_out)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        double ms = assemble_for_refine(c, st, k);
+        const double t0 = now_ms();
+        set_k0_mirror(c, k0_mirror, true);
+        if (seed)
+            bc_seed(c);
+        bc_decode(c);
+        HIP_OK(hipStreamSynchronize(c.stream));
+        ms += now_ms() - t0;
+        blocks_from_device(c, pack_psnr, pack_texels);
+        if (ms_out)
+            *ms_out = ms;
+    });
+}
+
+int nntc_hip_bc_refine(nntc_hip_ctx* h, int passes, const uint8_t* const* k0_mirror, nntc_bc_refine_report* rep,
+                       double* pack_psnr, size_t* pack_texels, double* ms_out)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        double ms = 0.0;
+        set_k0_mirror(c, k0_mirror, false);
+        if (passes <= 0)
+        {
+            if (ms_out)
+                *ms_out = 0.0;
+            return;
+        }
+        std::vector<double> taken((size_t)c.planes.n), improved((size_t)c.planes.n);
+        for (int pass = 0; pass < passes; pass++)
+        {
+            const double t0 = now_ms();
+            bc_refine_pass(c, taken.data(), improved.data());
+            const double pass_ms = now_ms() - t0;
+            ms += pass_ms;
+            nntc_bc_refine_report r {};
+            r.pass = pass + 1;
+            r.ms = pass_ms;
+            for (int p = 0; p < c.planes.n; p++)
+            {
+                r.delta_e += taken[(size_t)p];
+                r.improved += (long long)improved[(size_t)p];
+                r.blocks += c.g(p).nb;
+            }
+            if (rep)
+                rep[pass] = r;
+        }
+        blocks_from_device(c, pack_psnr, pack_texels);
+        if (ms_out)
+            *ms_out = ms;
+    });
+}
+
+int nntc_hip_bc_blocks_from_device(nntc_hip_ctx* h, const uint8_t* const* k0_mirror, double* pack_psnr,
+                                   size_t* pack_texels)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        set_k0_mirror(c, k0_mirror, false);
+        blocks_from_device(c, pack_psnr, pack_texels);
+    });
+}
+
+int nntc_hip_bc_blocks_download(nntc_hip_ctx* h, int plane, int file, uint8_t* blocks, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        const int files = c.c0 <= 2 ? 1 : 2;
+        if (file < 0 || file >= files)
+            throw Refused { "no such level-0 file" };
+        const int nc = file == 0 ? (c.c0 == 1 ? 1 : 2) : c.c0 - 2;
+        const int bpb = nc == 1 ? 8 : 16;
+        const PlaneGeom& g = c.g(plane);
+        download(c, (const uint8_t*)(c.bc_blocks.p + g.offb * 32 + (long long)file * g.nb * 16), (size_t)g.nb * bpb,
+                 blocks, n, count);
+    });
+}
+
+int nntc_hip_bc_state_save(nntc_hip_ctx* h, int plane, uint8_t* ep, size_t n_ep, uint8_t* sel, size_t n_sel)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        const PlaneGeom& g = c.g(plane);
+        download(c, (const uint8_t*)(c.bc_ep.p + g.offb * c.c0 * 2), (size_t)g.nb * c.c0 * 2, ep, n_ep, nullptr);
+        download(c, (const uint8_t*)(c.bc_sel.p + g.offb * c.c0 * 16), (size_t)g.nb * c.c0 * 16, sel, n_sel, nullptr);
+    });
+}
+
+void nntc_hip_profile_enable(int on)
+{
+    profiler().on = on != 0;
+}
+
+void nntc_hip_profile_reset(void)
+{
+    (void)guarded([] { profiler().reset(); });
+}
+
+int nntc_hip_kernel_stats(int index, char* name_out, size_t name_len, long long* launches, double* ms, double* bytes,
+                          double* flops)
+{
+    Profiler& p = profiler();
+    if (guarded([&] { p.settle(); }) != 0)
+        return 1;
+    if (index < 0 || (size_t)index >= p.stats.size())
+        return 1;
+    const auto& kv = p.stats[(size_t)index];
+    if (name_out && name_len)
+        std::snprintf(name_out, name_len, "%s", kv.first.c_str());
+    if (launches)
+        *launches = kv.second.launches;
+    if (ms)
+        *ms = kv.second.ms;
+    if (bytes)
+        *bytes = kv.second.bytes;
+    if (flops)
+        *flops = kv.second.flops;
+    return 0;
+}
+
+int nntc_hip_bc_state_restore(nntc_hip_ctx* h, int plane, const uint8_t* ep, size_t n_ep, const uint8_t* sel,
+                              size_t n_sel)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        const PlaneGeom& g = c.g(plane);
+        if (n_ep > (size_t)g.nb * c.c0 * 2 || n_sel > (size_t)g.nb * c.c0 * 16)
+            throw Refused { "BC state restore was handed more blocks than the plane holds" };
+        HIP_OK(hipMemcpyAsync(c.bc_ep.p + g.offb * c.c0 * 2, ep, n_ep, hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(c.bc_sel.p + g.offb * c.c0 * 16, sel, n_sel, hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+}
+
+}   // extern "C"

@@ -1,0 +1,433 @@
+// init.hip: the latents' initial state and their grids, on gfx950 (src/init.cu, src/cpu/cpu_init.cpp).
+//
+//   - level 0 starts at zero, or at 2 luma - 1 on channel 0 under --init0 luma (k_init_level0_cont);
+//   - level 1 starts at the block means of source channel j mapped by 2 mean - 1 (k_init_level1_box), with a float
+//     accumulator as the reference's;
+//   - a grid is fitted per channel over the base and every chain plane together, by min/max or by the 0.1 % / 99.9 %
+//     order statistics of the values, and every value is snapped to the nearest grid value (level1_index/value).
+//     The order statistics come from a radix select on the device: four passes of 8-bit digit histograms over the
+//     order-preserving unsigned image of each float, which lands on exactly the element a full sort would put at
+//     that rank;
+//   - the residual seed: the centre residual of the whole chain in fp32, its covariance over the base plane summed in
+//     the reference's k_cov_partial order (SITE_BLOCKS workgroups of 256 lanes, grid-stride, then a halving tree -
+//     the order the reference CPU backend replays because the leading eigenvector amplifies it, src/cpu/cpu_init.cpp
+//     :9-15), the host's Jacobi eigensolver, and the projection scaled by its peak over the chain.
+
+#include "arm.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace nntc
+{
+
+namespace
+{
+
+__global__ __launch_bounds__(256) void k_init_level0_cont(const float* src, float* v0, uint8_t* k0, long long n, int nout,
+                                                          int c0, float lr, float lg, float lb, int luma0)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float* s = src + i * nout;
+    const float luma = lr * s[0] + lg * s[1] + lb * s[2];
+    for (int c = 0; c < c0; c++)
+    {
+        v0[i * c0 + c] = (c == 0 && luma0) ? 2.0f * luma - 1.0f : 0.0f;
+        k0[i * c0 + c] = 0;
+    }
+}
+
+// The footprint of level-1 texel (x, y): [x w0 / w1, (x + 1) w0 / w1) widened to one pixel and cut at the edge.
+NNTC_HD void box_footprint(int x, int y, int w0, int h0, int w1, int h1, int& xa, int& xb, int& ya, int& yb)
+{
+    xa = (int)((long long)x * w0 / w1);
+    xb = (int)((long long)(x + 1) * w0 / w1);
+    ya = (int)((long long)y * h0 / h1);
+    yb = (int)((long long)(y + 1) * h0 / h1);
+    if (xb <= xa)
+        xb = xa + 1;
+    if (yb <= ya)
+        yb = ya + 1;
+    xb = xb > w0 ? w0 : xb;
+    yb = yb > h0 ? h0 : yb;
+}
+
+__global__ __launch_bounds__(256) void k_init_level1_box(Planes planes, const float* src_all, float* out_all, int nout,
+                                                         int c1, long long total)
+{
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= total)
+        return;
+    int p = 0;
+    while (p + 1 < planes.n && gi >= planes.p[p + 1].off1)
+        p++;
+    const PlaneGeom& g = planes.p[p];
+    const long long i = gi - g.off1;
+    const float* src = src_all + g.off0 * nout;
+    const int x = (int)(i % g.w1), y = (int)(i / g.w1);
+    int xa, xb, ya, yb;
+    box_footprint(x, y, g.w0, g.h0, g.w1, g.h1, xa, xb, ya, yb);
+    float acc[MAX_CH];
+    for (int c = 0; c < c1; c++)
+        acc[c] = 0.0f;
+    int n = 0;
+    for (int sy = ya; sy < yb; sy++)
+        for (int sx = xa; sx < xb; sx++)
+        {
+            const float* s = src + ((size_t)sy * g.w0 + sx) * nout;
+            for (int c = 0; c < c1; c++)
+                acc[c] += c < nout ? s[c] : 0.0f;
+            n++;
+        }
+    const float inv = n > 0 ? 1.0f / (float)n : 0.0f;
+    for (int c = 0; c < c1; c++)
+        out_all[gi * c1 + c] = c < nout ? 2.0f * acc[c] * inv - 1.0f : 0.0f;
+}
+
+// Value i snapped onto its channel's grid: v = level1_value(level1_index(in)), k = the index. `in` may be `v`.
+__global__ __launch_bounds__(256) void k_quantise(const float* in, float* v, uint8_t* k, long long n, int channels,
+                                                  int bits, const float* lo, const float* hi)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int c = (int)(i % channels);
+    const int q = level1_index(lo[c], hi[c], bits, in[i]);
+    v[i] = level1_value(lo[c], hi[c], bits, q);
+    k[i] = (uint8_t)q;
+}
+
+// Per-channel min/max over n interleaved values: partial per workgroup.
+__global__ __launch_bounds__(256) void k_minmax(const float* v, long long n, int channels, float* out)
+{
+    __shared__ float slo[256 * MAX_CH], shi[256 * MAX_CH];
+    float lo[MAX_CH], hi[MAX_CH];
+    for (int c = 0; c < MAX_CH; c++)
+    {
+        lo[c] = 1e30f;
+        hi[c] = -1e30f;
+    }
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t * channels < n; t += stride)
+        for (int c = 0; c < channels; c++)
+        {
+            const float x = v[t * channels + c];
+            lo[c] = x < lo[c] ? x : lo[c];
+            hi[c] = hi[c] < x ? x : hi[c];
+        }
+    for (int c = 0; c < MAX_CH; c++)
+    {
+        slo[threadIdx.x * MAX_CH + c] = lo[c];
+        shi[threadIdx.x * MAX_CH + c] = hi[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < MAX_CH)
+    {
+        const int c = threadIdx.x;
+        float l = 1e30f, h = -1e30f;
+        for (int t = 0; t < 256; t++)
+        {
+            const float a = slo[t * MAX_CH + c], b = shi[t * MAX_CH + c];
+            l = a < l ? a : l;
+            h = h < b ? b : h;
+        }
+        out[blockIdx.x * 2 * MAX_CH + c] = l;
+        out[blockIdx.x * 2 * MAX_CH + MAX_CH + c] = h;
+    }
+}
+
+// The order-preserving unsigned image of a float: negative floats flip every bit, non-negative ones the sign bit.
+__device__ __forceinline__ unsigned int order_key(float f)
+{
+    const unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// One radix-select pass: for every channel and both ranks, the histogram of digit `shift` over the keys whose higher
+// digits equal that rank's prefix so far. hist is channels x 2 x 256 counters.
+__global__ __launch_bounds__(256) void k_radix_hist(const float* v, long long n, int channels, int shift,
+                                                    const unsigned int* prefix, unsigned int mask, unsigned int* hist)
+{
+    __shared__ unsigned int h[MAX_CH * 2 * 256];
+    for (int i = threadIdx.x; i < MAX_CH * 2 * 256; i += blockDim.x)
+        h[i] = 0;
+    __syncthreads();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    {
+        const int c = (int)(i % channels);
+        const unsigned int key = order_key(v[i]);
+        const unsigned int digit = (key >> shift) & 0xFFu;
+        for (int r = 0; r < 2; r++)
+            if ((key & mask) == prefix[c * 2 + r])
+                atomicAdd(&h[(c * 2 + r) * 256 + digit], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < channels * 2 * 256; i += blockDim.x)
+        if (h[i])
+            atomicAdd(&hist[i], h[i]);
+}
+
+// k_cov_partial: entry e of the nout + nout^2 sums over `texels` rows of `nout` floats, lane x of workgroup b taking
+// rows b 256 + x, then every SITE_BLOCKS 256 further, and the 256 lanes halved into one.
+__global__ __launch_bounds__(256) void k_cov_partial(const float* data, long long texels, int nout, double* partial)
+{
+    __shared__ double lane[256];
+    const int b = blockIdx.x, e = blockIdx.y, x = threadIdx.x;
+    const int entries = nout + nout * nout;
+    const int r = e < nout ? e : (e - nout) / nout;
+    const int s = e < nout ? -1 : (e - nout) % nout;
+    const long long stride = (long long)SITE_BLOCKS * 256;
+    double acc = 0.0;
+    for (long long row = (long long)b * 256; row < texels; row += stride)
+    {
+        const long long t = row + x;
+        if (t < texels)
+        {
+            const double a = (double)data[t * nout + r];
+            acc += s < 0 ? a : a * (double)data[t * nout + s];
+        }
+    }
+    lane[x] = acc;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1)
+    {
+        if (x < half)
+            lane[x] += lane[x + half];
+        __syncthreads();
+    }
+    if (x == 0)
+        partial[(size_t)b * entries + e] = lane[0];
+}
+
+struct ProjDir
+{
+    double e[MAX_NOUT];
+    double mu[MAX_NOUT];
+};
+
+__device__ __forceinline__ double proj_value(const float* resid, long long t, int nout, const ProjDir& dir)
+{
+    double acc = 0.0;
+    for (int r = 0; r < nout; r++)
+        acc += dir.e[r] * ((double)resid[t * nout + r] - dir.mu[r]);
+    return acc;
+}
+
+__global__ __launch_bounds__(256) void k_proj_peak(const float* resid, long long texels, int nout, ProjDir dir,
+                                                   double* out)
+{
+    __shared__ double red[256];
+    double acc = 0.0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < texels; t += stride)
+    {
+        const double v = fabs(proj_value(resid, t, nout, dir));
+        acc = v > acc ? v : acc;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int half = 128; half > 0; half >>= 1)
+    {
+        if (threadIdx.x < half)
+        {
+            const double o = red[threadIdx.x + half];
+            red[threadIdx.x] = o > red[threadIdx.x] ? o : red[threadIdx.x];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_seed_channel(const float* resid, long long texels, int nout, ProjDir dir,
+                                                      double inv, int c0, int channel, float* v0)
+{
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= texels)
+        return;
+    const double p = proj_value(resid, t, nout, dir) * inv;
+    v0[t * c0 + channel] = (float)(p < -1.0 ? -1.0 : (p > 1.0 ? 1.0 : p));
+}
+
+}   // namespace
+
+void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0)
+{
+    hipLaunchKernelGGL(k_init_level0_cont, dim3(blocks_for(c.n0_total, 256)), dim3(256), 0, c.stream,
+                       (const float*)c.src.p, c.v0.p, c.k0.p, c.n0_total, c.nout, c.c0, luma[0], luma[1], luma[2],
+                       luma_channel0);
+    HIP_OK(hipGetLastError());
+}
+
+void init_level1_box(Ctx& c)
+{
+    hipLaunchKernelGGL(k_init_level1_box, dim3(blocks_for(c.n1_total, 256)), dim3(256), 0, c.stream, c.planes,
+                       (const float*)c.src.p, c.v1.p, c.nout, c.c1, c.n1_total);
+    HIP_OK(hipGetLastError());
+}
+
+// Snap every value of a latent onto the grid the device holds; `in_device` is the continuous plane (may be the
+// plane itself). plane < 0 means every plane of the chain.
+void quantise_values(Ctx& c, bool level0, const float* in_device, int plane)
+{
+    const int ch = level0 ? c.c0 : c.c1;
+    const int bits = level0 ? 8 : c.md.bits1;
+    float* v = level0 ? c.v0.p : c.v1.p;
+    uint8_t* k = level0 ? c.k0.p : c.k1.p;
+    long long begin = 0, n = level0 ? c.n0_total : c.n1_total;
+    if (plane >= 0)
+    {
+        begin = level0 ? c.g(plane).off0 : c.g(plane).off1;
+        n = level0 ? c.g(plane).n0 : c.g(plane).n1;
+    }
+    begin *= ch;
+    n *= ch;
+    const float* lo = c.grid.p + (level0 ? 0 : 8);
+    const float* hi = c.grid.p + (level0 ? 4 : 12);
+    hipLaunchKernelGGL(k_quantise, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, in_device + begin, v + begin,
+                       k + begin, n, ch, bits, lo, hi);
+    HIP_OK(hipGetLastError());
+}
+
+// The per-channel range of a latent over every plane (src/cpu/cpu_init.cpp:338-370).
+void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi)
+{
+    const int ch = level0 ? c.c0 : c.c1;
+    const float* v = level0 ? c.v0.p : c.v1.p;
+    const long long n = (level0 ? c.n0_total : c.n1_total) * ch;
+    for (int q = 0; q < ch; q++)
+    {
+        lo[q] = 1e30f;
+        hi[q] = -1e30f;
+    }
+    if (n == 0)
+        return;
+    if (policy == NNTC_RANGE_MINMAX)
+    {
+        const int blocks = 256;
+        float* out = c.work_f.p;
+        hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(256), 0, c.stream, v, n, ch, out);
+        HIP_OK(hipGetLastError());
+        std::vector<float> h((size_t)blocks * 2 * MAX_CH);
+        HIP_OK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+        for (int b = 0; b < blocks; b++)
+            for (int q = 0; q < ch; q++)
+            {
+                lo[q] = std::min(lo[q], h[(size_t)b * 2 * MAX_CH + q]);
+                hi[q] = std::max(hi[q], h[(size_t)b * 2 * MAX_CH + MAX_CH + q]);
+            }
+        return;
+    }
+    // pct: ranks a = round(0.001 last), b = round(0.999 last) of each channel's sorted values.
+    const long long per = n / ch;
+    const long long last = per - 1;
+    const unsigned long long rank0 = (unsigned long long)(0.001 * (double)last + 0.5);
+    const unsigned long long rank1 = (unsigned long long)(0.999 * (double)last + 0.5);
+    unsigned int prefix[MAX_CH * 2] = {};
+    unsigned long long want[MAX_CH * 2];
+    for (int q = 0; q < ch; q++)
+    {
+        want[q * 2] = rank0;
+        want[q * 2 + 1] = rank1;
+    }
+    unsigned int mask = 0;
+    unsigned int* d_hist = c.work_u.p;
+    unsigned int* d_prefix = c.work_u.p + MAX_CH * 2 * 256;
+    std::vector<unsigned int> hist((size_t)ch * 2 * 256);
+    for (int shift = 24; shift >= 0; shift -= 8)
+    {
+        HIP_OK(hipMemsetAsync(d_hist, 0, sizeof(unsigned int) * MAX_CH * 2 * 256, c.stream));
+        HIP_OK(hipMemcpyAsync(d_prefix, prefix, sizeof(prefix), hipMemcpyHostToDevice, c.stream));
+        const unsigned blocks = (unsigned)std::min<long long>(1024, (n + 255) / 256);
+        hipLaunchKernelGGL(k_radix_hist, dim3(blocks), dim3(256), 0, c.stream, v, n, ch, shift,
+                           (const unsigned int*)d_prefix, mask, d_hist);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(hist.data(), d_hist, hist.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
+                              c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < ch * 2; i++)
+        {
+            unsigned long long cum = 0;
+            int digit = 255;
+            for (int d = 0; d < 256; d++)
+            {
+                const unsigned long long cnt = hist[(size_t)i * 256 + d];
+                if (want[i] < cum + cnt)
+                {
+                    digit = d;
+                    break;
+                }
+                cum += cnt;
+            }
+            want[i] -= cum;
+            prefix[i] |= (unsigned int)digit << shift;
+        }
+        mask |= 0xFFu << shift;
+    }
+    for (int q = 0; q < ch; q++)
+        for (int r = 0; r < 2; r++)
+        {
+            const unsigned int key = prefix[q * 2 + r];
+            const unsigned int u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+            float f;
+            std::memcpy(&f, &u, sizeof(f));
+            (r == 0 ? lo : hi)[q] = f;
+        }
+}
+
+void cov_partials(Ctx& c, const float* data, long long texels, std::vector<double>& raw, double weight)
+{
+    const int entries = c.nout + c.nout * c.nout;
+    hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, entries), dim3(256), 0, c.stream, data, texels, c.nout,
+                       c.cov_partial.p);
+    HIP_OK(hipGetLastError());
+    std::vector<double> part((size_t)SITE_BLOCKS * entries);
+    HIP_OK(hipMemcpyAsync(part.data(), c.cov_partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    for (int e = 0; e < entries; e++)
+        for (int b = 0; b < SITE_BLOCKS; b++)
+            raw[(size_t)e] += weight * part[(size_t)b * entries + e];
+}
+
+double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, const double* mu)
+{
+    ProjDir dir;
+    for (int r = 0; r < MAX_NOUT; r++)
+    {
+        dir.e[r] = r < c.nout ? e[r] : 0.0;
+        dir.mu[r] = r < c.nout ? mu[r] : 0.0;
+    }
+    const unsigned blocks = (unsigned)std::max<long long>(1, std::min<long long>(256, (texels + 255) / 256));
+    double* out = c.cov_partial.p;
+    hipLaunchKernelGGL(k_proj_peak, dim3(blocks), dim3(256), 0, c.stream, resid, texels, c.nout, dir, out);
+    HIP_OK(hipGetLastError());
+    std::vector<double> h(blocks);
+    HIP_OK(hipMemcpyAsync(h.data(), out, blocks * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    double peak = 0.0;
+    for (double v : h)
+        peak = std::max(peak, v);
+    return peak;
+}
+
+void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const double* mu, double inv, int channel)
+{
+    ProjDir dir;
+    for (int r = 0; r < MAX_NOUT; r++)
+    {
+        dir.e[r] = r < c.nout ? e[r] : 0.0;
+        dir.mu[r] = r < c.nout ? mu[r] : 0.0;
+    }
+    const PlaneGeom& g = c.g(plane);
+    hipLaunchKernelGGL(k_seed_channel, dim3(blocks_for(g.n0, 256)), dim3(256), 0, c.stream, resid, g.n0, c.nout, dir,
+                       inv, c.c0, channel, c.v0.p + g.off0 * c.c0);
+    HIP_OK(hipGetLastError());
+}
+
+}   // namespace nntc

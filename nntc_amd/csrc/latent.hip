@@ -1,0 +1,1348 @@
+// latent.hip: blocks (b) and (c') - one latent plane as one sparse weighted least squares per stored plane, on gfx950
+// (src/solve_level1.cu, src/cpu/cpu_solve_level1.cpp).
+//
+// With the other latent and the decoder held, E is an exact quadratic in the unknown plane. Its 9-point block stencil
+// is assembled by a GATHER, one texel per thread, from monomial sums of the other latent's sample expanded against
+// fixed C x C matrices built once per call on the host (build_form_matrices). The proximal step
+// (H + lambda I) delta = -grad, lambda = ridge * mean diag H, is solved by conjugate gradients preconditioned by the
+// block Jacobi of the stencil, every plane of the chain in lockstep with a plane dropping out at the probe where its
+// relative residual is under the bar. Once the grids are frozen the same stencil drives four-colour Gauss-Seidel sweeps
+// whose per-value step is the exact argmin over the grid.
+//
+// Every per-texel routine here - the gather, the block inverse, the mat-vec (neighbours tested, not clamped), the
+// sweep's channel-by-channel argmin - is the reference's arithmetic in the reference's order, one texel per thread.
+// The reductions are the fixed-order folds of the reference CPU backend in the exact form: one wavefront per chunk of
+// CHUNK texels folds the chunk's values serially (lane order through readlane), and one thread per plane folds the
+// chunk sums in chunk order. The fast form folds each chunk with a wavefront tree.
+//
+// Roofline (HBM): assembly writes 8 (5 c^2 + c) bytes per texel and reads the three planes; a CG iteration streams
+// the stencil (40 c^2 B per texel), the preconditioner (4 c^2) and about six fp64 vectors (48 c); a sweep colour reads
+// the stencil of its texels and their neighbours' corrections.
+
+#include "arm.h"
+
+#include <cmath>
+
+namespace nntc
+{
+
+namespace
+{
+
+constexpr int CG_PROBE = 4;
+constexpr int CG_MAX_ITERATIONS = 200;
+constexpr double CG_TOLERANCE = 1e-10;
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Host: the fixed matrices of the quadratic form (src/cpu/cpu_solve_level1.cpp:103-187)
+// ---------------------------------------------------------------------------------------------------------------------
+
+// X^T C X = P_0 + sum_i y_i P_i + sum_{i <= j} y_i y_j P_ij for X(y) = X0 + sum_i y_i Y_i.
+void build_form_matrices(const Ctx& c, const std::vector<double>& x0, const std::vector<double>& y, int terms, int dim,
+                         std::vector<double>& p, const std::vector<float>& cw)
+{
+    const int nout = c.nout;
+    const int nmono = 1 + terms + terms * (terms + 1) / 2;
+    p.assign((size_t)nmono * dim * dim, 0.0);
+    auto product = [&](const double* x, const double* z, double* out) {
+        for (int u = 0; u < dim; u++)
+            for (int v = 0; v < dim; v++)
+            {
+                double s = 0.0;
+                for (int r = 0; r < nout; r++)
+                    s += (double)cw[(size_t)r] * x[(size_t)r * dim + u] * z[(size_t)r * dim + v];
+                out[u * dim + v] += s;
+            }
+    };
+    int slot = 0;
+    product(x0.data(), x0.data(), p.data());
+    slot++;
+    for (int i = 0; i < terms; i++, slot++)
+    {
+        const double* y_i = y.data() + (size_t)i * nout * dim;
+        product(y_i, x0.data(), p.data() + (size_t)slot * dim * dim);
+        product(x0.data(), y_i, p.data() + (size_t)slot * dim * dim);
+    }
+    for (int i = 0; i < terms; i++)
+        for (int j = i; j < terms; j++, slot++)
+        {
+            const double* y_i = y.data() + (size_t)i * nout * dim;
+            const double* y_j = y.data() + (size_t)j * nout * dim;
+            product(y_i, y_j, p.data() + (size_t)slot * dim * dim);
+            if (i != j)
+                product(y_j, y_i, p.data() + (size_t)slot * dim * dim);
+        }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------------------------------------------------
+
+__device__ __forceinline__ double lane_value(double v, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), l);
+    return __hiloint2double(hi, lo);
+}
+
+// acc plus the values of lanes 0 .. count-1 in lane order: a serial fold that every lane computes identically.
+__device__ __forceinline__ double fold_lanes(double acc, double v, int count)
+{
+    if (count == 64)
+    {
+#pragma unroll
+        for (int l = 0; l < 64; l++)
+            acc += lane_value(v, l);
+    }
+    else
+        for (int l = 0; l < count; l++)
+            acc += lane_value(v, l);
+    return acc;
+}
+
+// The fast form's wavefront tree: every lane ends holding the sum of all 64.
+__device__ __forceinline__ double tree_lanes(double v)
+{
+    for (int off = 32; off > 0; off >>= 1)
+        v += __shfl_xor(v, off);
+    return v;
+}
+
+struct LatentView
+{
+    Planes planes;
+    Ranges texels;        // per plane, this latent's texels
+    bool level0;
+    int c;                // channels
+    int bits;
+    int levels;
+    double* stencil;
+    double* grad;
+    float* precond;
+    double* x;
+    double* r;
+    double* z;
+    double* p;
+    double* ap;
+    float* prev;
+    float* v;             // the latent's plane values (chain-concatenated)
+    uint8_t* k;           // its indices
+    const float* lo;      // grid lo/hi of this latent, 4 each
+    const float* hi;
+    double* scalars;      // nplanes * SC_COUNT
+    int pw[MAX_PLANES];   // per plane latent width
+    int ph[MAX_PLANES];   // per plane latent height
+};
+
+__device__ __forceinline__ long long view_offset(const LatentView& L, int p)
+{
+    return L.texels.start[p];
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Assembly (src/cpu/cpu_solve_level1.cpp:203-468)
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct AssembleArgs
+{
+    LatentView L;
+    const float* src;
+    const float* v0;
+    const float* v1;
+    const float* weights;
+    const float* bias;
+    const float* cw;
+    const double* mono_p;
+    int nmono, c0, c1, nin, nout, k;
+};
+
+NNTC_HD void footprint_range(int ti, int n0, int n1, int& lo, int& hi)
+{
+    const double r = (double)n0 / (double)n1;
+    lo = (int)ceil(((double)ti - 0.5) * r - 1.0);
+    hi = (int)ceil(((double)ti + 1.5) * r) - 1;
+    lo = lo < 0 ? 0 : lo;
+    hi = hi > n0 - 1 ? n0 - 1 : hi;
+}
+
+__device__ __forceinline__ void stencil_monomial_values(const float* s, int c, double* mono)
+{
+    int n = 0;
+    mono[n++] = 1.0;
+    for (int i = 0; i < c; i++)
+        mono[n++] = (double)s[i];
+    for (int i = 0; i < c; i++)
+        for (int j = i; j < c; j++)
+            mono[n++] = (double)s[i] * (double)s[j];
+}
+
+// The five owned blocks expanded from the monomial sums; a zero sum is skipped.
+__device__ __forceinline__ void expand_blocks(const double* sums, int nmono, const double* mono_p, int c,
+                                              double* out_h)
+{
+    for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
+    {
+        double blk[MAX_CH * MAX_CH];
+        for (int i = 0; i < c * c; i++)
+            blk[i] = 0.0;
+        for (int i = 0; i < nmono; i++)
+        {
+            const double s = sums[slot * MAX_MONO + i];
+            if (s == 0.0)
+                continue;
+            const double* mat = mono_p + (size_t)i * c * c;
+            for (int e = 0; e < c * c; e++)
+                blk[e] += s * mat[e];
+        }
+        for (int i = 0; i < c * c; i++)
+            out_h[(size_t)slot * c * c + i] = blk[i];
+    }
+}
+
+// Level-1 texel ti's five blocks and gradient: every site whose bilinear footprint reaches it, in the reference's
+// pixel, then site order.
+__global__ __launch_bounds__(128) void k_assemble1(AssembleArgs a)
+{
+    const LatentView& L = a.L;
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gi);
+    const PlaneGeom& g = L.planes.p[p];
+    const long long ti = gi - L.texels.start[p];
+    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
+    const int c0 = a.c0, c1 = a.c1, nout = a.nout, nin = a.nin, nmono = a.nmono;
+    const float* src = a.src + g.off0 * nout;
+    const float* v0 = a.v0 + g.off0 * c0;
+    const float* v1 = a.v1 + g.off1 * c1;
+    const int tx = (int)(ti % w1), ty = (int)(ti / w1);
+
+    double sums[STENCIL_BLOCKS * MAX_MONO];
+    double gacc[MAX_CH];
+    for (int i = 0; i < STENCIL_BLOCKS * MAX_MONO; i++)
+        sums[i] = 0.0;
+    for (int i = 0; i < c1; i++)
+        gacc[i] = 0.0;
+
+    int px_lo, px_hi, py_lo, py_hi;
+    footprint_range(tx, w0, w1, px_lo, px_hi);
+    footprint_range(ty, h0, h1, py_lo, py_hi);
+
+    for (int py = py_lo; py <= py_hi; py++)
+        for (int px = px_lo; px <= px_hi; px++)
+            for (int j = 0; j <= a.k; j++)
+            {
+                float dx, dy;
+                site_delta(a.k, j, px, py, dx, dy);
+                const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+                int nx[4], ny[4];
+                float tw[4];
+                int taps = 0;
+                const int cx[4] = { tp.x0, tp.x1, tp.x0, tp.x1 };
+                const int cy[4] = { tp.y0, tp.y0, tp.y1, tp.y1 };
+                const float cwt[4] = { (1.0f - tp.fx) * (1.0f - tp.fy), tp.fx * (1.0f - tp.fy),
+                                       (1.0f - tp.fx) * tp.fy, tp.fx * tp.fy };
+                for (int q = 0; q < 4; q++)
+                {
+                    int found = -1;
+                    for (int e = 0; e < taps; e++)
+                        if (nx[e] == cx[q] && ny[e] == cy[q])
+                            found = e;
+                    if (found >= 0)
+                        tw[found] += cwt[q];
+                    else
+                    {
+                        nx[taps] = cx[q];
+                        ny[taps] = cy[q];
+                        tw[taps] = cwt[q];
+                        taps++;
+                    }
+                }
+                int self = -1;
+                for (int e = 0; e < taps; e++)
+                    if (nx[e] == tx && ny[e] == ty)
+                        self = e;
+                if (self < 0)
+                    continue;
+                const float w_self = tw[self];
+
+                float sbar[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
+                sample_bilinear(v1, w1, c1, tp, cbar);
+                if (j == 0)
+                {
+                    const size_t pi = (size_t)py * w0 + px;
+                    for (int i = 0; i < c0; i++)
+                        sbar[i] = v0[pi * c0 + i];
+                    for (int r = 0; r < nout; r++)
+                        target[r] = src[pi * nout + r];
+                }
+                else
+                {
+                    const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+                    sample_bilinear(v0, w0, c0, t0, sbar);
+                    sample_bilinear(src, w0, nout, t0, target);
+                }
+
+                // G = A + sum_i sbar_i M_i and the residual r = b + B sbar + G cbar - target, in fp32.
+                float gmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
+                for (int r = 0; r < nout; r++)
+                {
+                    const float* row = a.weights + (size_t)r * nin;
+                    float out = a.bias[r];
+                    for (int i = 0; i < c0; i++)
+                        out += row[c1 + i] * sbar[i];
+                    for (int q = 0; q < c1; q++)
+                    {
+                        float gg = row[q];
+                        for (int i = 0; i < c0; i++)
+                            gg += sbar[i] * row[c1 + c0 + i * c1 + q];
+                        gmat[r * c1 + q] = gg;
+                        out += gg * cbar[q];
+                    }
+                    res[r] = out - target[r];
+                }
+                double mono[MAX_MONO];
+                stencil_monomial_values(sbar, c0, mono);
+                for (int aa = 0; aa < c1; aa++)
+                {
+                    double acc = 0.0;
+                    for (int r = 0; r < nout; r++)
+                        acc += (double)a.cw[r] * (double)gmat[r * c1 + aa] * (double)res[r];
+                    gacc[aa] += (double)w_self * acc;
+                }
+                for (int e = 0; e < taps; e++)
+                {
+                    const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
+                    if (slot < 0)
+                        continue;
+                    const double scale = (double)w_self * (double)tw[e];
+                    for (int i = 0; i < nmono; i++)
+                        sums[slot * MAX_MONO + i] += scale * mono[i];
+                }
+            }
+
+    const long long gt = L.texels.start[p] + ti;
+    expand_blocks(sums, nmono, a.mono_p, c1, L.stencil + gt * (STENCIL_BLOCKS * c1 * c1));
+    for (int i = 0; i < c1; i++)
+        L.grad[gt * c1 + i] = gacc[i];
+}
+
+// Level-0 texel ti's five blocks and gradient (block (c')): the 3x3-pixel neighbourhood, the centre site read NEAREST
+// as one tap of weight 1.
+__global__ __launch_bounds__(128) void k_assemble0(AssembleArgs a)
+{
+    const LatentView& L = a.L;
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gi);
+    const PlaneGeom& g = L.planes.p[p];
+    const long long ti = gi - L.texels.start[p];
+    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
+    const int c0 = a.c0, c1 = a.c1, nout = a.nout, nin = a.nin, nmono = a.nmono;
+    const float* src = a.src + g.off0 * nout;
+    const float* v0 = a.v0 + g.off0 * c0;
+    const float* v1 = a.v1 + g.off1 * c1;
+    const int tx = (int)(ti % w0), ty = (int)(ti / w0);
+
+    double sums[STENCIL_BLOCKS * MAX_MONO];
+    double gacc[MAX_CH];
+    for (int i = 0; i < STENCIL_BLOCKS * MAX_MONO; i++)
+        sums[i] = 0.0;
+    for (int i = 0; i < c0; i++)
+        gacc[i] = 0.0;
+
+    int px_lo, px_hi, py_lo, py_hi;
+    footprint_range(tx, w0, w0, px_lo, px_hi);
+    footprint_range(ty, h0, h0, py_lo, py_hi);
+
+    for (int py = py_lo; py <= py_hi; py++)
+        for (int px = px_lo; px <= px_hi; px++)
+            for (int j = 0; j <= a.k; j++)
+            {
+                float dx, dy;
+                site_delta(a.k, j, px, py, dx, dy);
+                int nx[4], ny[4];
+                float tw[4];
+                int taps = 0;
+                float sbar[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
+                const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+                if (j == 0)
+                {
+                    nx[0] = px;
+                    ny[0] = py;
+                    tw[0] = 1.0f;
+                    taps = 1;
+                }
+                else
+                {
+                    const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+                    const int cx[4] = { t0.x0, t0.x1, t0.x0, t0.x1 };
+                    const int cy[4] = { t0.y0, t0.y0, t0.y1, t0.y1 };
+                    const float cwt[4] = { (1.0f - t0.fx) * (1.0f - t0.fy), t0.fx * (1.0f - t0.fy),
+                                           (1.0f - t0.fx) * t0.fy, t0.fx * t0.fy };
+                    for (int q = 0; q < 4; q++)
+                    {
+                        int found = -1;
+                        for (int e = 0; e < taps; e++)
+                            if (nx[e] == cx[q] && ny[e] == cy[q])
+                                found = e;
+                        if (found >= 0)
+                            tw[found] += cwt[q];
+                        else
+                        {
+                            nx[taps] = cx[q];
+                            ny[taps] = cy[q];
+                            tw[taps] = cwt[q];
+                            taps++;
+                        }
+                    }
+                }
+                int self = -1;
+                for (int e = 0; e < taps; e++)
+                    if (nx[e] == tx && ny[e] == ty)
+                        self = e;
+                if (self < 0)
+                    continue;
+                const float w_self = tw[self];
+
+                sample_bilinear(v1, w1, c1, tp, cbar);
+                if (j == 0)
+                {
+                    const size_t pi = (size_t)py * w0 + px;
+                    for (int i = 0; i < c0; i++)
+                        sbar[i] = v0[pi * c0 + i];
+                    for (int r = 0; r < nout; r++)
+                        target[r] = src[pi * nout + r];
+                }
+                else
+                {
+                    const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+                    sample_bilinear(v0, w0, c0, t0, sbar);
+                    sample_bilinear(src, w0, nout, t0, target);
+                }
+
+                // Q = B + sum_q cbar_q N_q and the residual r = b + A cbar + Q sbar - target, in fp32.
+                float qmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
+                for (int r = 0; r < nout; r++)
+                {
+                    const float* row = a.weights + (size_t)r * nin;
+                    float out = a.bias[r];
+                    for (int q = 0; q < c1; q++)
+                        out += row[q] * cbar[q];
+                    for (int i = 0; i < c0; i++)
+                    {
+                        float qq = row[c1 + i];
+                        for (int q = 0; q < c1; q++)
+                            qq += cbar[q] * row[c1 + c0 + i * c1 + q];
+                        qmat[r * c0 + i] = qq;
+                        out += qq * sbar[i];
+                    }
+                    res[r] = out - target[r];
+                }
+                double mono[MAX_MONO];
+                stencil_monomial_values(cbar, c1, mono);
+                for (int aa = 0; aa < c0; aa++)
+                {
+                    double acc = 0.0;
+                    for (int r = 0; r < nout; r++)
+                        acc += (double)a.cw[r] * (double)qmat[r * c0 + aa] * (double)res[r];
+                    gacc[aa] += (double)w_self * acc;
+                }
+                for (int e = 0; e < taps; e++)
+                {
+                    const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
+                    if (slot < 0)
+                        continue;
+                    const double scale = (double)w_self * (double)tw[e];
+                    for (int i = 0; i < nmono; i++)
+                        sums[slot * MAX_MONO + i] += scale * mono[i];
+                }
+            }
+
+    const long long gt = L.texels.start[p] + ti;
+    expand_blocks(sums, nmono, a.mono_p, c0, L.stencil + gt * (STENCIL_BLOCKS * c0 * c0));
+    for (int i = 0; i < c0; i++)
+        L.grad[gt * c0 + i] = gacc[i];
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Fixed-order reductions
+// ---------------------------------------------------------------------------------------------------------------------
+
+// The chunks of CHUNK texels of every plane, one wavefront each. Chunk index space: plane p owns
+// [cstart[p], cstart[p+1]).
+struct ChunkArgs
+{
+    LatentView L;
+    Ranges chunks;
+    double* partial;
+    int exact;
+};
+
+__device__ __forceinline__ void chunk_bounds(const ChunkArgs& a, long long chunk, int& p, long long& vb, long long& ve)
+{
+    p = range_plane(a.chunks, chunk);
+    const long long texels = a.L.texels.start[p + 1] - a.L.texels.start[p];
+    const long long tb = (chunk - a.chunks.start[p]) * CHUNK;
+    const long long te = tb + CHUNK < texels ? tb + CHUNK : texels;
+    vb = (a.L.texels.start[p] + tb) * a.L.c;
+    ve = (a.L.texels.start[p] + te) * a.L.c;
+}
+
+// The diagonal of H over the chunk's texels: its sum (serial, texel then channel), smallest and largest.
+__global__ __launch_bounds__(64) void k_diag_chunks(ChunkArgs a)
+{
+    int p;
+    long long vb, ve;
+    chunk_bounds(a, blockIdx.x, p, vb, ve);
+    const int c = a.L.c;
+    const long long tb = vb / c, te = ve / c;
+    const int lane = threadIdx.x;
+    double sum = 0.0, lo = 1e300, hi = -1e300;
+    for (long long base = tb * c; base < te * c; base += 64)
+    {
+        const long long i = base + lane;
+        const int count = (int)(te * c - base < 64 ? te * c - base : 64);
+        double x = 0.0;
+        if (lane < count)
+        {
+            const long long t = i / c;
+            const int q = (int)(i % c);
+            x = a.L.stencil[t * (STENCIL_BLOCKS * c * c) + q * c + q];
+        }
+        sum = a.exact ? fold_lanes(sum, x, count) : sum + tree_lanes(lane < count ? x : 0.0);
+        for (int l = 0; l < count; l++)
+        {
+            const double y = lane_value(x, l);
+            lo = y < lo ? y : lo;
+            hi = y > hi ? y : hi;
+        }
+    }
+    if (lane == 0)
+    {
+        a.partial[blockIdx.x * 3] = sum;
+        a.partial[blockIdx.x * 3 + 1] = lo;
+        a.partial[blockIdx.x * 3 + 2] = hi;
+    }
+}
+
+// Per plane: the chunk sums in chunk order, then lambda = ridge * sum / unknowns.
+__global__ void k_diag_fold(ChunkArgs a, double ridge)
+{
+    const int p = threadIdx.x;
+    if (p >= a.chunks.n)
+        return;
+    double sum = 0.0, lo = 1e300, hi = -1e300;
+    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+    {
+        sum += a.partial[ch * 3];
+        const double l = a.partial[ch * 3 + 1], h = a.partial[ch * 3 + 2];
+        lo = l < lo ? l : lo;
+        hi = h > hi ? h : hi;
+    }
+    double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    const long long unknowns = (a.L.texels.start[p + 1] - a.L.texels.start[p]) * a.L.c;
+    s[SC_DIAG_SUM] = sum;
+    s[SC_DIAG_MIN] = lo;
+    s[SC_DIAG_MAX] = hi;
+    s[SC_LAMBDA] = unknowns > 0 ? ridge * sum / (double)unknowns : 0.0;
+}
+
+// Two inner products over the chunk's values, each folded serially (exact) or by tree (fast): sum a_i b_i and
+// sum c_i d_i.
+__global__ __launch_bounds__(64) void k_dot_chunks(ChunkArgs a, const double* x0, const double* y0, const double* x1,
+                                                   const double* y1)
+{
+    int p;
+    long long vb, ve;
+    chunk_bounds(a, blockIdx.x, p, vb, ve);
+    if (a.L.scalars[(size_t)p * SC_COUNT + SC_ACTIVE] == 0.0)
+        return;
+    const int lane = threadIdx.x;
+    double s0 = 0.0, s1 = 0.0;
+    for (long long base = vb; base < ve; base += 64)
+    {
+        const long long i = base + lane;
+        const int count = (int)(ve - base < 64 ? ve - base : 64);
+        double t0 = 0.0, t1 = 0.0;
+        if (lane < count)
+        {
+            t0 = x0[i] * y0[i];
+            if (x1)
+                t1 = x1[i] * y1[i];
+        }
+        if (a.exact)
+        {
+            s0 = fold_lanes(s0, t0, count);
+            if (x1)
+                s1 = fold_lanes(s1, t1, count);
+        }
+        else
+        {
+            s0 += tree_lanes(t0);
+            if (x1)
+                s1 += tree_lanes(t1);
+        }
+    }
+    if (lane == 0)
+    {
+        a.partial[blockIdx.x * 2] = s0;
+        a.partial[blockIdx.x * 2 + 1] = s1;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Preconditioner, mat-vec and the CG vector updates (src/cpu/cpu_solve_level1.cpp:476-604, 777-894)
+// ---------------------------------------------------------------------------------------------------------------------
+
+__device__ bool invert_block(double* a, int n, double* inv)
+{
+    for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++)
+            inv[i * n + j] = i == j ? 1.0 : 0.0;
+    for (int k = 0; k < n; k++)
+    {
+        int pivot = k;
+        double best = fabs(a[k * n + k]);
+        for (int i = k + 1; i < n; i++)
+            if (fabs(a[i * n + k]) > best)
+            {
+                best = fabs(a[i * n + k]);
+                pivot = i;
+            }
+        if (!(best > 0.0))
+            return false;
+        if (pivot != k)
+            for (int j = 0; j < n; j++)
+            {
+                double t = a[k * n + j];
+                a[k * n + j] = a[pivot * n + j];
+                a[pivot * n + j] = t;
+                t = inv[k * n + j];
+                inv[k * n + j] = inv[pivot * n + j];
+                inv[pivot * n + j] = t;
+            }
+        const double s = 1.0 / a[k * n + k];
+        for (int j = 0; j < n; j++)
+        {
+            a[k * n + j] *= s;
+            inv[k * n + j] *= s;
+        }
+        for (int i = 0; i < n; i++)
+        {
+            if (i == k)
+                continue;
+            const double f = a[i * n + k];
+            if (f == 0.0)
+                continue;
+            for (int j = 0; j < n; j++)
+            {
+                a[i * n + j] -= f * a[k * n + j];
+                inv[i * n + j] -= f * inv[k * n + j];
+            }
+        }
+    }
+    return true;
+}
+
+__device__ __forceinline__ void apply_precond(const float* minv, const double* r, int c, double* z, long long t)
+{
+    const float* a = minv + t * (c * c);
+    const double* x = r + t * c;
+    double* o = z + t * c;
+    for (int i = 0; i < c; i++)
+    {
+        double acc = 0.0;
+        for (int j = 0; j < c; j++)
+            acc += (double)a[i * c + j] * x[j];
+        o[i] = acc;
+    }
+}
+
+// The start of the iteration, per texel: the block inverse of (H_tt + lambda I) kept as floats (the identity where
+// it is singular), delta = 0, r = -grad, z = Minv r, p = z.
+__global__ __launch_bounds__(256) void k_cg_start(LatentView L)
+{
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gt);
+    const double lambda = L.scalars[(size_t)p * SC_COUNT + SC_LAMBDA];
+    const int c = L.c;
+    const double* dg = L.stencil + gt * (STENCIL_BLOCKS * c * c);
+    double a[MAX_CH * MAX_CH], inv[MAX_CH * MAX_CH];
+    for (int i = 0; i < c; i++)
+        for (int j = 0; j < c; j++)
+            a[i * c + j] = dg[i * c + j] + (i == j ? lambda : 0.0);
+    if (!invert_block(a, c, inv))
+        for (int i = 0; i < c; i++)
+            for (int j = 0; j < c; j++)
+                inv[i * c + j] = i == j ? 1.0 : 0.0;
+    float* o = L.precond + gt * (c * c);
+    for (int i = 0; i < c * c; i++)
+        o[i] = (float)inv[i];
+    for (int i = 0; i < c; i++)
+    {
+        L.x[gt * c + i] = 0.0;
+        L.r[gt * c + i] = -L.grad[gt * c + i];
+    }
+    apply_precond(L.precond, L.r, c, L.z, gt);
+    for (int i = 0; i < c; i++)
+        L.p[gt * c + i] = L.z[gt * c + i];
+}
+
+// y_t = (H_tt + lambda I) x_t + the eight neighbours' blocks, the four the texel does not own read as the transpose of
+// the neighbour's; a neighbour outside the plane is absent, not clamped.
+__global__ __launch_bounds__(256) void k_matvec(LatentView L)
+{
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gt);
+    const double* s = L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] == 0.0)
+        return;
+    const double lambda = s[SC_LAMBDA];
+    const long long t = gt - L.texels.start[p];
+    const int w = L.pw[p], h = L.ph[p], c = L.c;
+    const int tx = (int)(t % w), ty = (int)(t / w);
+    const long long base = L.texels.start[p];
+    const size_t bstride = (size_t)(STENCIL_BLOCKS * c * c);
+    const double* x = L.p;
+    double acc[MAX_CH];
+    const double* own = L.stencil + gt * bstride;
+    const double* xt = x + gt * c;
+    for (int i = 0; i < c; i++)
+    {
+        double sacc = lambda * xt[i];
+        for (int j = 0; j < c; j++)
+            sacc += own[i * c + j] * xt[j];
+        acc[i] = sacc;
+    }
+    for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
+    {
+        int dx, dy;
+        stencil_offset(slot, dx, dy);
+        const int fx = tx + dx, fy = ty + dy;
+        if (fx >= 0 && fx < w && fy >= 0 && fy < h)
+        {
+            const double* blk = own + (size_t)slot * c * c;
+            const double* xn = x + (base + (long long)fy * w + fx) * c;
+            for (int i = 0; i < c; i++)
+                for (int j = 0; j < c; j++)
+                    acc[i] += blk[i * c + j] * xn[j];
+        }
+        const int bx = tx - dx, by = ty - dy;
+        if (bx >= 0 && bx < w && by >= 0 && by < h)
+        {
+            const long long nt = base + (long long)by * w + bx;
+            const double* blk = L.stencil + nt * bstride + (size_t)slot * c * c;
+            const double* xn = x + nt * c;
+            for (int i = 0; i < c; i++)
+                for (int j = 0; j < c; j++)
+                    acc[i] += blk[j * c + i] * xn[j];
+        }
+    }
+    for (int i = 0; i < c; i++)
+        L.ap[gt * c + i] = acc[i];
+}
+
+// Per plane after the curvature fold: p.Ap, then alpha = rz / pAp (zero on non-positive curvature).
+__global__ void k_fold_alpha(ChunkArgs a)
+{
+    const int p = threadIdx.x;
+    if (p >= a.chunks.n)
+        return;
+    double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] == 0.0)
+        return;
+    double pap = 0.0;
+    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+        pap += a.partial[ch * 2];
+    s[SC_PAP] = pap;
+    s[SC_ALPHA] = pap > 0.0 ? s[SC_RZ] / pap : 0.0;
+}
+
+// x += alpha p, r += -alpha Ap, z = Minv r, per texel.
+__global__ __launch_bounds__(256) void k_cg_update(LatentView L)
+{
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gt);
+    const double* s = L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] == 0.0)
+        return;
+    const double alpha = s[SC_ALPHA];
+    const double plus = 1.0, minus = -1.0;
+    const int c = L.c;
+    for (int i = 0; i < c; i++)
+    {
+        const long long e = gt * c + i;
+        L.x[e] += plus * alpha * L.p[e];
+        L.r[e] += minus * alpha * L.ap[e];
+    }
+    apply_precond(L.precond, L.r, c, L.z, gt);
+}
+
+// Per plane: |r|^2 and r.z, beta, the relative residual; at a probe iteration a plane under the bar stops.
+__global__ void k_fold_beta(ChunkArgs a, int probe)
+{
+    const int p = threadIdx.x;
+    if (p >= a.chunks.n)
+        return;
+    double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] == 0.0)
+        return;
+    double rr = 0.0, rz_new = 0.0;
+    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+    {
+        rr += a.partial[ch * 2];
+        rz_new += a.partial[ch * 2 + 1];
+    }
+    s[SC_RR] = rr;
+    s[SC_RZ_NEW] = rz_new;
+    const double rz = s[SC_RZ];
+    s[SC_BETA] = rz != 0.0 ? rz_new / rz : 0.0;
+    s[SC_RZ] = rz_new;
+    const double rhs2 = s[SC_RHS2];
+    s[SC_RESID] = rhs2 > 0.0 ? sqrt((rr > 0.0 ? rr : 0.0) / rhs2) : 0.0;
+}
+
+// p = z + beta p; then, at a probe iteration, a plane whose residual is under the bar (or NaN) stops.
+__global__ __launch_bounds__(256) void k_xpby(LatentView L)
+{
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gt);
+    const double* s = L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] == 0.0)
+        return;
+    const double beta = s[SC_BETA];
+    for (int i = 0; i < L.c; i++)
+    {
+        const long long e = gt * L.c + i;
+        L.p[e] = L.z[e] + beta * L.p[e];
+    }
+}
+
+__global__ void k_probe(LatentView L, int nplanes)
+{
+    const int p = threadIdx.x;
+    if (p >= nplanes)
+        return;
+    double* s = L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] != 0.0 && !(s[SC_RESID] >= CG_TOLERANCE))
+        s[SC_ACTIVE] = 0.0;
+}
+
+// The start's two products, folded per plane: |r|^2 -> rhs2, r.z -> rz; every plane becomes active.
+__global__ void k_fold_start(ChunkArgs a)
+{
+    const int p = threadIdx.x;
+    if (p >= a.chunks.n)
+        return;
+    double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    double rr = 0.0, rz = 0.0;
+    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+    {
+        rr += a.partial[ch * 2];
+        rz += a.partial[ch * 2 + 1];
+    }
+    s[SC_RHS2] = rr;
+    s[SC_RZ] = rz;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// The correction and the movement probe (src/cpu/cpu_solve_level1.cpp:896-965)
+// ---------------------------------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_add_correction(LatentView L)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.texels.start[L.texels.n] * L.c)
+        return;
+    L.v[e] = (float)((double)L.v[e] + L.x[e]);
+}
+
+// Per chunk: each channel's smallest and largest value (exact in any order).
+__global__ __launch_bounds__(64) void k_range_chunks(ChunkArgs a)
+{
+    int p;
+    long long vb, ve;
+    chunk_bounds(a, blockIdx.x, p, vb, ve);
+    const int c = a.L.c, lane = threadIdx.x;
+    double lo[MAX_CH], hi[MAX_CH];
+    for (int q = 0; q < MAX_CH; q++)
+    {
+        lo[q] = 1e300;
+        hi[q] = -1e300;
+    }
+    for (long long i = vb + lane; i < ve; i += 64)
+    {
+        const int q = (int)(i % c);
+        const double val = (double)a.L.v[i];
+        lo[q] = val < lo[q] ? val : lo[q];
+        hi[q] = val > hi[q] ? val : hi[q];
+    }
+    for (int q = 0; q < c; q++)
+        for (int off = 32; off > 0; off >>= 1)
+        {
+            const double l = __shfl_xor(lo[q], off), h = __shfl_xor(hi[q], off);
+            lo[q] = l < lo[q] ? l : lo[q];
+            hi[q] = h > hi[q] ? h : hi[q];
+        }
+    if (lane == 0)
+        for (int q = 0; q < MAX_CH; q++)
+        {
+            a.partial[blockIdx.x * 8 + q] = lo[q];
+            a.partial[blockIdx.x * 8 + 4 + q] = hi[q];
+        }
+}
+
+__global__ void k_range_fold(ChunkArgs a)
+{
+    const int p = threadIdx.x;
+    if (p >= a.chunks.n)
+        return;
+    double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    for (int q = 0; q < a.L.c; q++)
+    {
+        double lo = 1e300, hi = -1e300;
+        for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+        {
+            const double l = a.partial[ch * 8 + q], h = a.partial[ch * 8 + 4 + q];
+            lo = l < lo ? l : lo;
+            hi = h > hi ? h : hi;
+        }
+        s[SC_LO + q] = lo;
+        s[SC_HI + q] = hi;
+    }
+}
+
+// Per chunk, serially over its values: the count that moved by more than half a grid step of the plane's range, the
+// correction's square norm and the plane's.
+__global__ __launch_bounds__(64) void k_movement_chunks(ChunkArgs a)
+{
+    int p;
+    long long vb, ve;
+    chunk_bounds(a, blockIdx.x, p, vb, ve);
+    const int c = a.L.c, lane = threadIdx.x;
+    const double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    double moved = 0.0, d2 = 0.0, p2 = 0.0;
+    for (long long base = vb; base < ve; base += 64)
+    {
+        const long long i = base + lane;
+        const int count = (int)(ve - base < 64 ? ve - base : 64);
+        double m = 0.0, xx = 0.0, vv = 0.0;
+        if (lane < count)
+        {
+            const int q = (int)(i % c);
+            const double step = (s[SC_HI + q] - s[SC_LO + q]) / (double)a.L.levels;
+            const double x = a.L.x[i];
+            m = fabs(x) > 0.5 * step ? 1.0 : 0.0;
+            xx = x * x;
+            vv = (double)a.L.v[i] * (double)a.L.v[i];
+        }
+        if (a.exact)
+        {
+            moved = fold_lanes(moved, m, count);
+            d2 = fold_lanes(d2, xx, count);
+            p2 = fold_lanes(p2, vv, count);
+        }
+        else
+        {
+            moved += tree_lanes(m);
+            d2 += tree_lanes(xx);
+            p2 += tree_lanes(vv);
+        }
+    }
+    if (lane == 0)
+    {
+        a.partial[blockIdx.x * 3] = moved;
+        a.partial[blockIdx.x * 3 + 1] = d2;
+        a.partial[blockIdx.x * 3 + 2] = p2;
+    }
+}
+
+__global__ void k_movement_fold(ChunkArgs a)
+{
+    const int p = threadIdx.x;
+    if (p >= a.chunks.n)
+        return;
+    double* s = a.L.scalars + (size_t)p * SC_COUNT;
+    double moved = 0.0, d2 = 0.0, p2 = 0.0;
+    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+    {
+        moved += a.partial[ch * 3];
+        d2 += a.partial[ch * 3 + 1];
+        p2 += a.partial[ch * 3 + 2];
+    }
+    s[SC_MOVED] = moved;
+    s[SC_DELTA2] = d2;
+    s[SC_PLANE2] = p2;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// The quantised sweeps (src/cpu/cpu_solve_level1.cpp:610-674, 988-1054)
+// ---------------------------------------------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_copy_prev(LatentView L)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= L.texels.start[L.texels.n] * L.c)
+        return;
+    L.prev[e] = L.v[e];
+    L.x[e] = 0.0;
+}
+
+// One colour of one sweep: every texel (tx, ty) of the colour, the neighbours' share of (H delta)_t once, then its
+// channels in turn, each one's exact grid argmin using the values the earlier channels have just taken.
+__global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour, int colour_index,
+                                                     unsigned int* changed)
+{
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= colour.start[colour.n])
+        return;
+    const int p = range_plane(colour, gi);
+    const long long ci = gi - colour.start[p];
+    const int w = L.pw[p], h = L.ph[p], c = L.c;
+    const int cx = colour_index & 1, cy = colour_index >> 1;
+    const int cw1 = (w - cx + 1) / 2;
+    const int tx = cx + 2 * (int)(ci % cw1);
+    const int ty = cy + 2 * (int)(ci / cw1);
+    const long long base = L.texels.start[p];
+    const long long t = base + (long long)ty * w + tx;
+    const double lambda = L.scalars[(size_t)p * SC_COUNT + SC_LAMBDA];
+    const size_t bstride = (size_t)(STENCIL_BLOCKS * c * c);
+    const double* own = L.stencil + t * bstride;
+    double* delta = L.x;
+
+    double nb[MAX_CH];
+    for (int i = 0; i < c; i++)
+        nb[i] = 0.0;
+    for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
+    {
+        int dx, dy;
+        stencil_offset(slot, dx, dy);
+        const int fx = tx + dx, fy = ty + dy;
+        if (fx >= 0 && fx < w && fy >= 0 && fy < h)
+        {
+            const double* blk = own + (size_t)slot * c * c;
+            const double* xn = delta + (base + (long long)fy * w + fx) * c;
+            for (int i = 0; i < c; i++)
+                for (int j = 0; j < c; j++)
+                    nb[i] += blk[i * c + j] * xn[j];
+        }
+        const int bx = tx - dx, by = ty - dy;
+        if (bx >= 0 && bx < w && by >= 0 && by < h)
+        {
+            const long long nt = base + (long long)by * w + bx;
+            const double* blk = L.stencil + nt * bstride + (size_t)slot * c * c;
+            const double* xn = delta + nt * c;
+            for (int i = 0; i < c; i++)
+                for (int j = 0; j < c; j++)
+                    nb[i] += blk[j * c + i] * xn[j];
+        }
+    }
+
+    unsigned int moved = 0;
+    for (int q = 0; q < c; q++)
+    {
+        const long long idx = t * c + q;
+        const float l = L.lo[q], span = L.hi[q] - L.lo[q];
+        const double aq = own[q * c + q] + lambda;
+        if (!(aq > 0.0) || !(span > 0.0f))
+            continue;
+        double rest = nb[q];
+        for (int j = 0; j < c; j++)
+            if (j != q)
+                rest += own[q * c + j] * delta[t * c + j];
+        const double bq = L.grad[idx] + rest;
+        const double want = (double)L.prev[idx] - bq / aq;
+        const int kk = level1_index(l, L.hi[q], L.bits, (float)want);
+        const float value = level1_value(l, L.hi[q], L.bits, kk);
+        if ((int)L.k[idx] != kk)
+        {
+            L.k[idx] = (uint8_t)kk;
+            moved++;
+        }
+        L.v[idx] = value;
+        delta[idx] = (double)value - (double)L.prev[idx];
+    }
+    if (moved)
+        atomicAdd(changed + p, moved);
+}
+
+}   // namespace
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------------------------------------------
+
+static LatentView make_view(Ctx& c, bool level0)
+{
+    Latent& lt = c.latent(level0);
+    LatentView L;
+    L.planes = c.planes;
+    L.texels.n = c.planes.n;
+    long long acc = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        L.texels.start[p] = acc;
+        acc += level0 ? c.g(p).n0 : c.g(p).n1;
+        L.pw[p] = level0 ? c.g(p).w0 : c.g(p).w1;
+        L.ph[p] = level0 ? c.g(p).h0 : c.g(p).h1;
+    }
+    L.texels.start[c.planes.n] = acc;
+    L.level0 = level0;
+    L.c = lt.c;
+    L.bits = lt.bits;
+    L.levels = (1 << lt.bits) - 1;
+    L.stencil = lt.stencil.p;
+    L.grad = lt.grad.p;
+    L.precond = lt.precond.p;
+    L.x = lt.x.p;
+    L.r = lt.r.p;
+    L.z = lt.z.p;
+    L.p = lt.p.p;
+    L.ap = lt.ap.p;
+    L.prev = lt.prev.p;
+    L.v = level0 ? c.v0.p : c.v1.p;
+    L.k = level0 ? c.k0.p : c.k1.p;
+    L.lo = c.grid.p + (level0 ? 0 : 8);
+    L.hi = c.grid.p + (level0 ? 4 : 12);
+    L.scalars = c.scalars.p;
+    return L;
+}
+
+static ChunkArgs make_chunks(Ctx& c, const LatentView& L)
+{
+    ChunkArgs a;
+    a.L = L;
+    a.chunks.n = c.planes.n;
+    long long acc = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        a.chunks.start[p] = acc;
+        const long long texels = L.texels.start[p + 1] - L.texels.start[p];
+        acc += (texels + CHUNK - 1) / CHUNK;
+    }
+    a.chunks.start[c.planes.n] = acc;
+    a.partial = c.partial.p;
+    a.exact = c.form == NNTC_FORM_EXACT ? 1 : 0;
+    if ((size_t)acc * 8 > c.partial.n)
+        throw Refused { "latent: chunk partials exceed the workspace" };
+    return a;
+}
+
+void build_block_matrices(Ctx& c, bool level0)
+{
+    const int c0 = c.c0, c1 = c.c1, nout = c.nout, nin = c.nin;
+    std::vector<float> cw(c.md.cw, c.md.cw + nout);
+    std::vector<double> mono;
+    if (!level0)
+    {
+        // Block (b): A and the sc-blocks M_i of the decoder, G(s) = A + sum_i s_i M_i.
+        std::vector<double> a((size_t)nout * c1), mi((size_t)MAX_CH * nout * c1, 0.0);
+        for (int r = 0; r < nout; r++)
+        {
+            const float* row = c.h_w.data() + (size_t)r * nin;
+            for (int q = 0; q < c1; q++)
+                a[(size_t)r * c1 + q] = (double)row[q];
+            for (int i = 0; i < c0; i++)
+                for (int q = 0; q < c1; q++)
+                    mi[((size_t)i * nout + r) * c1 + q] = (double)row[c1 + c0 + i * c1 + q];
+        }
+        build_form_matrices(c, a, mi, c0, c1, mono, cw);
+    }
+    else
+    {
+        // Block (c'): B and the sc-columns N_q, Q(c) = B + sum_q c_q N_q.
+        std::vector<double> b((size_t)nout * c0), nq((size_t)MAX_CH * nout * c0, 0.0);
+        for (int r = 0; r < nout; r++)
+        {
+            const float* row = c.h_w.data() + (size_t)r * nin;
+            for (int i = 0; i < c0; i++)
+                b[(size_t)r * c0 + i] = (double)row[c1 + i];
+            for (int q = 0; q < c1; q++)
+                for (int i = 0; i < c0; i++)
+                    nq[((size_t)q * nout + r) * c0 + i] = (double)row[c1 + c0 + i * c1 + q];
+        }
+        build_form_matrices(c, b, nq, c1, c0, mono, cw);
+    }
+    Latent& lt = c.latent(level0);
+    const size_t n = mono.size() < lt.mono.n ? mono.size() : lt.mono.n;
+    HIP_OK(hipMemcpyAsync(lt.mono.p, mono.data(), n * sizeof(double), hipMemcpyHostToDevice, c.stream));
+}
+
+void assemble(Ctx& c, bool level0, int k, double ridge)
+{
+    LatentView L = make_view(c, level0);
+    AssembleArgs a;
+    a.L = L;
+    a.src = c.src.p;
+    a.v0 = c.v0.p;
+    a.v1 = c.v1.p;
+    a.weights = c.weights.p;
+    a.bias = c.bias.p;
+    a.cw = c.cw.p;
+    a.mono_p = c.latent(level0).mono.p;
+    a.nmono = c.latent(level0).nmono;
+    a.c0 = c.c0;
+    a.c1 = c.c1;
+    a.nin = c.nin;
+    a.nout = c.nout;
+    a.k = k;
+    const long long n = L.texels.start[L.texels.n];
+    {
+        // The three planes read once and the stencil and gradient written: 8 (5 c^2 + c) bytes per texel.
+        const double in = 4.0 * c.n0_total * (c.nout + c.c0) + 4.0 * c.n1_total * c.c1;
+        const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
+        KernelTimer kt(c.stream, level0 ? "k_assemble0" : "k_assemble1", in + out, 0.0);
+        if (level0)
+            hipLaunchKernelGGL(k_assemble0, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, a);
+        else
+            hipLaunchKernelGGL(k_assemble1, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, a);
+    }
+    HIP_OK(hipGetLastError());
+    ChunkArgs ch = make_chunks(c, L);
+    const long long nch = ch.chunks.start[ch.chunks.n];
+    hipLaunchKernelGGL(k_diag_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+    hipLaunchKernelGGL(k_diag_fold, dim3(1), dim3(64), 0, c.stream, ch, ridge);
+    HIP_OK(hipGetLastError());
+}
+
+void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
+{
+    LatentView L = make_view(c, level0);
+    ChunkArgs ch = make_chunks(c, L);
+    const long long n = L.texels.start[L.texels.n];
+    const long long nch = ch.chunks.start[ch.chunks.n];
+    const int np = c.planes.n;
+    const unsigned tb = blocks_for(n, 256);
+    // Every plane starts active.
+    std::vector<double> one((size_t)np, 1.0);
+    for (int p = 0; p < np; p++)
+        HIP_OK(hipMemcpyAsync(c.scalars.p + (size_t)p * SC_COUNT + SC_ACTIVE, &one[(size_t)p], sizeof(double),
+                              hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_cg_start, dim3(tb), dim3(256), 0, c.stream, L);
+    hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, (const double*)L.r,
+                       (const double*)L.r, (const double*)L.r, (const double*)L.z);
+    hipLaunchKernelGGL(k_fold_start, dim3(1), dim3(64), 0, c.stream, ch);
+    HIP_OK(hipGetLastError());
+
+    iterations.assign((size_t)np, CG_MAX_ITERATIONS);
+    std::vector<int> done((size_t)np, 0);
+    // Algorithmic bytes per launch at full activity: the stencil (40 c^2 per texel) and the vectors each kernel streams.
+    const double cc = L.c, nv = (double)n * cc;
+    const double b_matvec = (double)n * 40.0 * cc * cc + 16.0 * nv;
+    const double b_update = (double)n * 4.0 * cc * cc + 56.0 * nv;
+    for (int it = 0; it < CG_MAX_ITERATIONS; it++)
+    {
+        {
+            KernelTimer kt(c.stream, "k_matvec", b_matvec, (double)n * (2.0 * 9.0 * cc * cc));
+            hipLaunchKernelGGL(k_matvec, dim3(tb), dim3(256), 0, c.stream, L);
+        }
+        {
+            KernelTimer kt(c.stream, "k_dot_chunks", 16.0 * nv, 2.0 * nv);
+            hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, (const double*)L.p,
+                               (const double*)L.ap, (const double*)nullptr, (const double*)nullptr);
+        }
+        hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(64), 0, c.stream, ch);
+        {
+            KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
+            hipLaunchKernelGGL(k_cg_update, dim3(tb), dim3(256), 0, c.stream, L);
+        }
+        {
+            KernelTimer kt(c.stream, "k_dot_chunks", 16.0 * nv, 4.0 * nv);
+            hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, (const double*)L.r,
+                               (const double*)L.r, (const double*)L.r, (const double*)L.z);
+        }
+        hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(64), 0, c.stream, ch, 0);
+        {
+            KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);
+            hipLaunchKernelGGL(k_xpby, dim3(tb), dim3(256), 0, c.stream, L);
+        }
+        HIP_OK(hipGetLastError());
+        if ((it + 1) % CG_PROBE != 0)
+            continue;
+        hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, c.stream, L, np);
+        HIP_OK(hipGetLastError());
+        HIP_OK(hipMemcpyAsync(c.host_scalars, c.scalars.p, sizeof(double) * SC_COUNT * np, hipMemcpyDeviceToHost,
+                              c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+        bool any = false;
+        for (int p = 0; p < np; p++)
+        {
+            if (!done[(size_t)p] && c.host_scalars[(size_t)p * SC_COUNT + SC_ACTIVE] == 0.0)
+            {
+                done[(size_t)p] = 1;
+                iterations[(size_t)p] = it + 1;
+            }
+            any = any || !done[(size_t)p];
+        }
+        if (!any)
+            break;
+    }
+}
+
+void correction_and_probe(Ctx& c, bool level0)
+{
+    LatentView L = make_view(c, level0);
+    ChunkArgs ch = make_chunks(c, L);
+    const long long n = L.texels.start[L.texels.n];
+    const long long nch = ch.chunks.start[ch.chunks.n];
+    hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(n * L.c, 256)), dim3(256), 0, c.stream, L);
+    hipLaunchKernelGGL(k_range_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+    hipLaunchKernelGGL(k_range_fold, dim3(1), dim3(64), 0, c.stream, ch);
+    hipLaunchKernelGGL(k_movement_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+    hipLaunchKernelGGL(k_movement_fold, dim3(1), dim3(64), 0, c.stream, ch);
+    HIP_OK(hipGetLastError());
+}
+
+void copy_prev_zero_delta(Ctx& c, bool level0)
+{
+    LatentView L = make_view(c, level0);
+    const long long n = L.texels.start[L.texels.n];
+    hipLaunchKernelGGL(k_copy_prev, dim3(blocks_for(n * L.c, 256)), dim3(256), 0, c.stream, L);
+    HIP_OK(hipGetLastError());
+}
+
+void quant_sweeps(Ctx& c, bool level0, int sweeps)
+{
+    LatentView L = make_view(c, level0);
+    HIP_OK(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned int) * c.planes.n, c.stream));
+    Ranges colour[4];
+    for (int col = 0; col < 4; col++)
+    {
+        const int cx = col & 1, cy = col >> 1;
+        colour[col].n = c.planes.n;
+        long long acc = 0;
+        for (int p = 0; p < c.planes.n; p++)
+        {
+            colour[col].start[p] = acc;
+            const int cw1 = (L.pw[p] - cx + 1) / 2, ch1 = (L.ph[p] - cy + 1) / 2;
+            acc += (long long)cw1 * ch1;
+        }
+        colour[col].start[c.planes.n] = acc;
+    }
+    for (int sw = 0; sw < sweeps; sw++)
+        for (int col = 0; col < 4; col++)
+        {
+            const long long n = colour[col].start[colour[col].n];
+            if (n == 0)
+                continue;
+            // A colour's texels read their own stencil, gradient and plane and their neighbours' corrections.
+            const double cc = L.c;
+            KernelTimer kt(c.stream, "k_quant_sweep", (double)n * (40.0 * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc),
+                           0.0);
+            hipLaunchKernelGGL(k_quant_sweep, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, L, colour[col], col,
+                               c.counters.p);
+        }
+    HIP_OK(hipGetLastError());
+}
+
+}   // namespace nntc

@@ -1,0 +1,335 @@
+// nntc_common.h: the sampling rule, the feature order, the grid math and the BC4 palette, for the gfx950 kernels and
+// the arm's host code alike.
+//
+// Every function here restates a rule the reference fixes, operation for operation, because the encode's promise is
+// that the value the solver measured and the value the consumer samples are the same number:
+//   - the bilinear clamp-to-edge addressing from pixel coordinates and the four-tap blend (src/sample.cuh:43-72,
+//     src/cpu/cpu_sample.h:36-66);
+//   - phi = [c_j][s_i][s_i c_j] (src/sample.cuh:74-85);
+//   - the K = 1, 2, 4 subtexel offsets turned a quarter per texel of a 2x2 cell (src/sample.cuh:87-128);
+//   - level1_value / level1_index / replicate_index (src/model.h:141-199);
+//   - the BC4 eight-value palette entry and the plane value of a decoded byte (src/refine_bc.cu:102-131).
+// All of it is compiled with -ffp-contract=off: the reference's host build and its CUDA build both forbid fused
+// multiply-adds (CMakeLists.txt:183-187, 206-208), and a fused add here would move last bits against it.
+
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NNTC_HD __host__ __device__ __forceinline__
+
+namespace nntc
+{
+
+constexpr int MAX_PLANES = 13;
+constexpr int MAX_CH = 4;
+constexpr int MAX_NIN = 24;
+constexpr int MAX_NOUT = 18;
+constexpr int STENCIL_BLOCKS = 5;
+constexpr int MAX_MONO = 15;          // 1 + 4 + 4*5/2, the quadratic form's monomials at the channel cap
+constexpr int CHUNK = 4096;           // the reference CPU backend's fixed reduction chunk (src/cpu/cpu_constants.h:103)
+constexpr int SITE_BLOCKS = 512;      // the decoder accumulation's fixed block count (src/cpu/cpu_constants.h:41)
+constexpr int LS_TILE = 256;          // the decoder accumulation's tile (src/cpu/cpu_solve_decoder.cpp:55)
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Sampling (src/cpu/cpu_sample.h:27-141)
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct Tap
+{
+    int x0, x1, y0, y1;
+    float fx, fy;
+};
+
+NNTC_HD int clamp_index(int i, int n)
+{
+    return i < 0 ? 0 : (i >= n ? n - 1 : i);
+}
+
+// The hardware's bilinear addressing with clamp-to-edge, for a site given as a pixel of the w0 x h0 plane being
+// fitted, into a w x h plane: exact in float when the two extents are equal.
+NNTC_HD Tap tap_pixel(int px, int py, float dx, float dy, int w0, int h0, int w, int h)
+{
+    const float x = w == w0 ? (float)px + dx : (((float)px + 0.5f + dx) * (float)w) / (float)w0 - 0.5f;
+    const float y = h == h0 ? (float)py + dy : (((float)py + 0.5f + dy) * (float)h) / (float)h0 - 0.5f;
+    const int x0 = (int)floorf(x);
+    const int y0 = (int)floorf(y);
+    Tap t;
+    t.fx = x - (float)x0;
+    t.fy = y - (float)y0;
+    t.x0 = clamp_index(x0, w);
+    t.x1 = clamp_index(x0 + 1, w);
+    t.y0 = clamp_index(y0, h);
+    t.y1 = clamp_index(y0 + 1, h);
+    return t;
+}
+
+// The four clamped taps blended channel by channel, in fp32, in the order w00 a + w10 b + w01 e + w11 f.
+NNTC_HD void sample_bilinear(const float* plane, int w, int c, const Tap& t, float* out)
+{
+    const float* a = plane + ((size_t)t.y0 * w + t.x0) * c;
+    const float* b = plane + ((size_t)t.y0 * w + t.x1) * c;
+    const float* e = plane + ((size_t)t.y1 * w + t.x0) * c;
+    const float* f = plane + ((size_t)t.y1 * w + t.x1) * c;
+    const float w00 = (1.0f - t.fx) * (1.0f - t.fy), w10 = t.fx * (1.0f - t.fy);
+    const float w01 = (1.0f - t.fx) * t.fy, w11 = t.fx * t.fy;
+    for (int k = 0; k < c; k++)
+        out[k] = w00 * a[k] + w10 * b[k] + w01 * e[k] + w11 * f[k];
+}
+
+// phi(s, c): [c_j][s_i][s_i c_j], i outer and j inner in the product block.
+NNTC_HD int build_phi(const float* s, int c0, const float* c, int c1, float* phi)
+{
+    int k = 0;
+    for (int j = 0; j < c1; j++)
+        phi[k++] = c[j];
+    for (int i = 0; i < c0; i++)
+        phi[k++] = s[i];
+    for (int i = 0; i < c0; i++)
+        for (int j = 0; j < c1; j++)
+            phi[k++] = s[i] * c[j];
+    return k;
+}
+
+// Fractional site j of pixel (tx, ty): the K = 4 rotated grid {(-3/8,-1/8),(-1/8,3/8),(1/8,-3/8),(3/8,1/8)} (or the
+// K = 2 pair, or K = 1's (1/4, 1/4)), rotated a quarter turn per texel of the 2x2 cell, n = 2 (ty & 1) + (tx & 1).
+NNTC_HD void subtexel_offset(int k_count, int j, int tx, int ty, float& dx, float& dy)
+{
+    if (k_count == 2)
+    {
+        dx = j == 0 ? -0.25f : 0.25f;
+        dy = j == 0 ? 0.25f : -0.25f;
+        return;
+    }
+    float ox = 0.25f, oy = 0.25f;
+    if (k_count == 4)
+    {
+        ox = j == 0 ? -0.375f : (j == 1 ? -0.125f : (j == 2 ? 0.125f : 0.375f));
+        oy = j == 0 ? -0.125f : (j == 1 ? 0.375f : (j == 2 ? -0.375f : 0.125f));
+    }
+    const int n = (ty & 1) * 2 + (tx & 1);
+    for (int r = 0; r < n; r++)
+    {
+        const float t = ox;
+        ox = -oy;
+        oy = t;
+    }
+    dx = ox;
+    dy = oy;
+}
+
+// Site j of pixel (px, py): j = 0 the centre, 1..K the fractional sites.
+NNTC_HD void site_delta(int k_count, int j, int px, int py, float& dx, float& dy)
+{
+    dx = 0.0f;
+    dy = 0.0f;
+    if (j > 0)
+        subtexel_offset(k_count, j - 1, px, py, dx, dy);
+}
+
+// The decoder's input phi and the target at one site, in fp32: level 1 sampled bilinearly; level 0 and the source
+// read at the texel for the centre and sampled bilinearly at a fractional site (src/cpu/cpu_sample.h:115-139).
+NNTC_HD void site_row(const float* v0, const float* v1, const float* src, int w0, int h0, int w1, int h1, int c0,
+                      int c1, int nout, int k_count, int j, int px, int py, float* phi, float* target)
+{
+    float dx, dy;
+    site_delta(k_count, j, px, py, dx, dy);
+    float s[MAX_CH], c[MAX_CH];
+    const Tap t1 = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+    sample_bilinear(v1, w1, c1, t1, c);
+    if (j == 0)
+    {
+        const size_t idx = (size_t)py * w0 + px;
+        for (int i = 0; i < c0; i++)
+            s[i] = v0[idx * c0 + i];
+        for (int i = 0; i < nout; i++)
+            target[i] = src[idx * nout + i];
+    }
+    else
+    {
+        const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+        sample_bilinear(v0, w0, c0, t0, s);
+        sample_bilinear(src, w0, nout, t0, target);
+    }
+    build_phi(s, c0, c, c1, phi);
+}
+
+// The objective's own bilinear rule in double, from normalised coordinates (src/objective.cu:104-118).
+NNTC_HD void obj_sample(const float* plane, int w, int h, int c, double u, double v, double* out)
+{
+    const double x = u * (double)w - 0.5, y = v * (double)h - 0.5;
+    const double xf = floor(x), yf = floor(y);
+    const double fx = x - xf, fy = y - yf;
+    const int x0 = clamp_index((int)xf, w), x1 = clamp_index((int)xf + 1, w);
+    const int y0 = clamp_index((int)yf, h), y1 = clamp_index((int)yf + 1, h);
+    const float* a = plane + ((size_t)y0 * w + x0) * c;
+    const float* b = plane + ((size_t)y0 * w + x1) * c;
+    const float* e = plane + ((size_t)y1 * w + x0) * c;
+    const float* f = plane + ((size_t)y1 * w + x1) * c;
+    const double w00 = (1.0 - fx) * (1.0 - fy), w10 = fx * (1.0 - fy), w01 = (1.0 - fx) * fy, w11 = fx * fy;
+    for (int k = 0; k < c; k++)
+        out[k] = w00 * (double)a[k] + w10 * (double)b[k] + w01 * (double)e[k] + w11 * (double)f[k];
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// The grid (src/model.h:141-199)
+// ---------------------------------------------------------------------------------------------------------------------
+
+// The index in the byte's top bits replicated into the low bits.
+NNTC_HD int replicate_index(int k, int bits)
+{
+    unsigned b8 = 0;
+    for (int sh = 8 - bits; sh > -bits; sh -= bits)
+        b8 |= sh >= 0 ? ((unsigned)k << sh) : ((unsigned)k >> -sh);
+    return (int)(b8 & 0xFFu);
+}
+
+// value = lo + rep(k) / 255 (hi - lo), in double with the multiply and the add rounded separately, rounded to float
+// once.
+NNTC_HD float level1_value(float lo, float hi, int bits, int k)
+{
+    const double span = (double)hi - (double)lo;
+    const double frac = (double)replicate_index(k, bits) / 255.0;
+    return (float)((double)lo + frac * span);
+}
+
+// The index of the nearest grid value; a tie keeps the lower index.
+NNTC_HD int level1_index(float lo, float hi, int bits, float value)
+{
+    const double span = (double)hi - (double)lo;
+    const int levels = (1 << bits) - 1;
+    if (!(span > 0.0))
+        return 0;
+    double x = ((double)value - (double)lo) / span * (double)levels;
+    x = x < 0.0 ? 0.0 : (x > (double)levels ? (double)levels : x);
+    int k = (int)(x + 0.5);
+    k = k < 0 ? 0 : (k > levels ? levels : k);
+    int best = k;
+    double best_d = fabs((double)value - (double)level1_value(lo, hi, bits, k));
+    for (int d = -1; d <= 1; d += 2)
+    {
+        const int n = k + d;
+        if (n < 0 || n > levels)
+            continue;
+        const double dist = fabs((double)value - (double)level1_value(lo, hi, bits, n));
+        if (dist < best_d || (dist == best_d && n < best))
+        {
+            best_d = dist;
+            best = n;
+        }
+    }
+    return best;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// BC4 (shared/bc_pack.h:62-79, src/refine_bc.cu:102-131)
+// ---------------------------------------------------------------------------------------------------------------------
+
+// The eight-value palette (r0 > r1) as the weight on the high endpoint: p = a r0 + (1 - a) r1.
+NNTC_HD double bc4_weight8(int sel)
+{
+    return sel == 0 ? 1.0 : (sel == 1 ? 0.0 : (double)(8 - sel) / 7.0);
+}
+
+// Palette entry `sel` of a block with endpoints r0, r1, in float, in either mode.
+NNTC_HD float bc4_entry(int r0, int r1, int sel)
+{
+    if (sel == 0)
+        return (float)r0;
+    if (sel == 1)
+        return (float)r1;
+    if (r0 > r1)
+        return (float)((8 - sel) * r0 + (sel - 1) * r1) / 7.0f;
+    if (sel >= 6)
+        return sel == 6 ? 0.0f : 255.0f;
+    return (float)((6 - sel) * r0 + (sel - 1) * r1) / 5.0f;
+}
+
+// The eight-value-mode entry the refinement's own arithmetic uses (src/refine_bc.cu:111-118).
+NNTC_HD float bc4_entry8(int r0, int r1, int sel)
+{
+    if (sel == 0)
+        return (float)r0;
+    if (sel == 1)
+        return (float)r1;
+    return (float)((8 - sel) * r0 + (sel - 1) * r1) / 7.0f;
+}
+
+// The plane value a decoded byte stands for: lo + byte / 255 (hi - lo), multiply and add rounded apart.
+NNTC_HD float bc0_value(float lo, float hi, float byte)
+{
+    const double frac = (double)byte / 255.0;
+    const double span = (double)hi - (double)lo;
+    return (float)((double)lo + frac * span);
+}
+
+// The byte's nearest 8-bit index, with the float round trip (int)(byte + 0.5f).
+NNTC_HD uint8_t bc0_index(float byte)
+{
+    return (uint8_t)(byte < 0.0f ? 0 : (byte > 255.0f ? 255 : (int)(byte + 0.5f)));
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// Stencil geometry (src/cpu/cpu_constants.h:47-68)
+// ---------------------------------------------------------------------------------------------------------------------
+
+// The five blocks a texel owns: itself, right, down-left, down, down-right.
+NNTC_HD void stencil_offset(int slot, int& dx, int& dy)
+{
+    dx = slot == 1 ? 1 : (slot == 2 ? -1 : (slot == 4 ? 1 : 0));
+    dy = slot >= 2 ? 1 : 0;
+}
+
+NNTC_HD int stencil_slot(int dx, int dy)
+{
+    for (int s = 0; s < STENCIL_BLOCKS; s++)
+    {
+        int ox, oy;
+        stencil_offset(s, ox, oy);
+        if (ox == dx && oy == dy)
+            return s;
+    }
+    return -1;
+}
+
+NNTC_HD int stencil_monomials(int c)
+{
+    return 1 + c + c * (c + 1) / 2;
+}
+
+// ---------------------------------------------------------------------------------------------------------------------
+// The plane table every batched launch carries: one launch covers every stored plane of the chain.
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct PlaneGeom
+{
+    int w0, h0, w1, h1;
+    long long n0, n1;       // texels of level 0 and of level 1
+    long long off0, off1;   // this plane's first texel in the chain-concatenated level-0 / level-1 arrays
+    long long nb, offb;     // BC blocks ((w0 + 3) / 4) ((h0 + 3) / 4) and the first one
+};
+
+struct Planes
+{
+    int n;
+    PlaneGeom p[MAX_PLANES];
+};
+
+// A 1-D index space that is the concatenation of one range per plane: `count[p]` items starting at `start[p]`.
+struct Ranges
+{
+    int n;
+    long long start[MAX_PLANES + 1];
+};
+
+// The plane whose range holds global index g (ranges are contiguous and increasing).
+NNTC_HD int range_plane(const Ranges& r, long long g)
+{
+    int p = 0;
+    while (p + 1 < r.n && g >= r.start[p + 1])
+        p++;
+    return p;
+}
+
+}   // namespace nntc

@@ -1,0 +1,107 @@
+// nntc_host.h: the encode driver's internals - the seam's function table, the encoder state, the host helpers.
+#pragma once
+
+#include "nntc_encode.h"
+#include "nntc_hip.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace nntc_host
+{
+
+extern thread_local std::string g_error;
+
+struct Failure
+{
+    std::string what;
+};
+
+// Every seam entry point the driver calls, by name (the C ABI of include/nntc_hip.h minus the prefix).
+#define NNTC_SEAM_FUNCTIONS(X)                                                                                         \
+    X(last_error)                                                                                                      \
+    X(device_create)                                                                                                   \
+    X(device_destroy)                                                                                                  \
+    X(device_memory)                                                                                                   \
+    X(set_form)                                                                                                        \
+    X(init_level0)                                                                                                     \
+    X(init_level1)                                                                                                     \
+    X(init0_residual_channel)                                                                                          \
+    X(quantise_level1)                                                                                                 \
+    X(freeze_level1_grid)                                                                                              \
+    X(quantise_level0)                                                                                                 \
+    X(freeze_level0_grid)                                                                                              \
+    X(snap_level0_on_grid)                                                                                             \
+    X(solve_decoder)                                                                                                   \
+    X(decoder_ridge_tally)                                                                                             \
+    X(solve_level1_all)                                                                                                \
+    X(sweep_level1_all)                                                                                                \
+    X(solve_level0_cont_all)                                                                                           \
+    X(sweep_level0_cont_all)                                                                                           \
+    X(level1_download_indices)                                                                                         \
+    X(level1_upload)                                                                                                   \
+    X(level1_upload_indices)                                                                                           \
+    X(level0_upload_indices)                                                                                           \
+    X(upload_decoder_to_device)                                                                                        \
+    X(upload_grids_to_device)                                                                                          \
+    X(level1_download)                                                                                                 \
+    X(level0_download)                                                                                                 \
+    X(level0_upload)                                                                                                   \
+    X(level0_download_values)                                                                                          \
+    X(decode_plane)                                                                                                    \
+    X(objective_eval)                                                                                                  \
+    X(objective_total_ms)                                                                                              \
+    X(objective_passes)                                                                                                \
+    X(bc_pack_prepare)                                                                                                 \
+    X(bc_refine)                                                                                                       \
+    X(bc_blocks_from_device)                                                                                           \
+    X(bc_blocks_download)                                                                                              \
+    X(bc_state_save)                                                                                                   \
+    X(bc_state_restore)
+
+struct Seam
+{
+#define NNTC_MEMBER(n) decltype(&nntc_hip_##n) n = nullptr;
+    NNTC_SEAM_FUNCTIONS(NNTC_MEMBER)
+#undef NNTC_MEMBER
+    std::string name;
+    bool load(const char* lib_path, const char* prefix);
+};
+
+struct Encoder
+{
+    Seam seam;
+    nntc_encode_options opts {};
+    nntc_model_desc desc {};
+    nntc_model_state state {};
+    nntc_hip_ctx* ctx = nullptr;
+    std::vector<std::vector<float>> chain;   // the source at every stored level, interleaved 3T channels
+    std::vector<double> per_site;            // the mip weights' per-site weight of every plane
+    std::vector<uint8_t> recon;              // the base plane decoded to bytes
+    std::vector<std::string> inputs;
+    int source_w = 0, source_h = 0;
+    nntc_encode_report report {};
+    bool ran = false;
+
+    void check(int status, const char* what);
+    void setup(int textures, int width, int height, const uint8_t* const* rgb);
+    void box_reorder();
+    void verify_level1_on_grid();
+    void decode_base();
+    double texture_psnr(int t) const;
+    ~Encoder();
+};
+
+// image.cpp
+bool load_png_rgb(const char* path, std::vector<uint8_t>& rgb, int& w, int& h, bool& alpha_ignored, std::string& why);
+bool save_png_rgb(const char* path, const uint8_t* rgb, int w, int h);
+void plane_layout(int width, int height, bool mips, int mip_min, std::vector<nntc_plane_size>& planes);
+void box_halve(const std::vector<float>& in, int w, int h, int nc, std::vector<float>& out, int ow, int oh);
+void mip_weights(const std::vector<nntc_plane_size>& planes, int kind, double mix, int k, std::vector<double>& per_site);
+
+// level1_value (src/model.h:159-168), for the host's on-grid check.
+int replicate_index(int k, int bits);
+float level1_value(float lo, float hi, int bits, int k);
+
+}   // namespace nntc_host

@@ -1,0 +1,239 @@
+// writer.cpp: the asset - PREFIX_lat0.dds (PREFIX_lat0a/b.dds for three or four level-0 channels), PREFIX_lat1.dds
+// and PREFIX_nntc.json - in the reference's format (src/export.cpp:89-147, 470-812, docs/FORMAT.md).
+//
+// .dds: the DX10 header (magic + 31-dword DDS_HEADER + 5-dword DDS_HEADER_DXT10 = 148 bytes, little-endian), then
+// every mip level base first, tightly packed. Level 0 is BC4_UNORM (dxgi 80) or BC5_UNORM (dxgi 83) holding the blocks
+// the refinement chose; level 1 is R8 / R8G8 / R8G8B8A8_UNORM (61 / 49 / 28) holding each index replicated over its
+// byte, a three-channel plane padded to four with 255. The header fields are the reference's: the flags for a chain
+// or a single level, the base level's linear size (compressed) or row pitch (uncompressed).
+//
+// JSON: the same keys and values a consumer reads (sizes, block, LOD bias, per-level files, formats, bits, mip sizes,
+// the dequantisation lo/hi/levels, and the decoder's C0, C1, nin, nout, weights and bias), every float as %.9g.
+
+#include "nntc_host.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace nntc_host;
+
+namespace
+{
+
+bool dds_write(const std::string& name, int w, int h, int dxgi, bool compressed, uint32_t size_field,
+               const std::vector<std::vector<uint8_t>>& levels)
+{
+    const bool chain = levels.size() > 1;
+    uint32_t hdr[37] = { 0 };
+    hdr[0] = 0x20534444u;                                                    // "DDS "
+    hdr[1] = 124;                                                            // dwSize
+    hdr[2] = 0x00001007u | (chain ? 0x00020000u : 0u) | (compressed ? 0x00080000u : 0x8u);
+    hdr[3] = (uint32_t)h;
+    hdr[4] = (uint32_t)w;
+    hdr[5] = size_field;
+    hdr[6] = 1;
+    hdr[7] = (uint32_t)levels.size();
+    hdr[19] = 32;
+    hdr[20] = 0x4u;                                                          // DDPF_FOURCC
+    hdr[21] = 0x30315844u;                                                   // "DX10"
+    hdr[27] = chain ? 0x00401008u : 0x00001000u;
+    hdr[32] = (uint32_t)dxgi;
+    hdr[33] = 3;                                                             // TEXTURE2D
+    hdr[35] = 1;                                                             // arraySize
+    FILE* f = std::fopen(name.c_str(), "wb");
+    if (!f)
+    {
+        g_error = "cannot write '" + name + "'";
+        return false;
+    }
+    bool ok = std::fwrite(hdr, 4, 37, f) == 37;
+    for (const std::vector<uint8_t>& level : levels)
+        ok = ok && std::fwrite(level.data(), 1, level.size(), f) == level.size();
+    if (std::fclose(f) != 0)
+        ok = false;
+    if (!ok)
+        g_error = "writing '" + name + "' failed";
+    return ok;
+}
+
+std::string num(double x)
+{
+    char b[32];
+    std::snprintf(b, sizeof(b), "%.9g", std::isfinite(x) ? x : 0.0);
+    return b;
+}
+
+std::string nums(const float* v, size_t n)
+{
+    std::string s = "[";
+    for (size_t i = 0; i < n; i++)
+        s += num(v[i]) + (i + 1 < n ? "," : "");
+    return s + "]";
+}
+
+std::string base_name(const std::string& p)
+{
+    const size_t slash = p.find_last_of("/\\");
+    return slash == std::string::npos ? p : p.substr(slash + 1);
+}
+
+}   // namespace
+
+extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
+{
+    const Encoder& e = *reinterpret_cast<const Encoder*>(h);
+    if (!e.ran)
+    {
+        g_error = "nothing to write: the encoder has not run";
+        return 1;
+    }
+    const nntc_model_desc& d = e.desc;
+    const nntc_model_state& st = e.state;
+    const int nin = d.c1 + d.c0 + d.c0 * d.c1;
+    for (int i = 0; i < d.nout * nin; i++)
+        if (!std::isfinite(st.w[i]))
+        {
+            g_error = "the decoder carries a value that is not finite; the asset is not written";
+            return 1;
+        }
+    const std::string prefix = prefix_c;
+    const int np = d.nplanes;
+    const int files = d.c0 <= 2 ? 1 : 2;
+    std::string js = "{\n  \"format\": \"nntc-dds-1\",\n";
+    js += "  \"source\": { \"width\": " + std::to_string(e.source_w) + ", \"height\": " + std::to_string(e.source_h);
+    if (e.source_w != d.planes[0].w0 || e.source_h != d.planes[0].h0)
+        js += ", \"padded\": true, \"padded_width\": " + std::to_string(d.planes[0].w0) + ", \"padded_height\": " +
+              std::to_string(d.planes[0].h0);
+    if (!e.inputs.empty())
+    {
+        js += ", \"inputs\": [";
+        for (size_t t = 0; t < e.inputs.size(); t++)
+            js += std::string(t ? ", " : "") + "{ \"file\": \"" + base_name(e.inputs[t]) + "\" }";
+        js += "]";
+    }
+    js += " },\n";
+    js += "  \"decode\": { \"width\": " + std::to_string(d.planes[0].w0) + ", \"height\": " +
+          std::to_string(d.planes[0].h0) + ", \"textures_out\": " + std::to_string(d.textures) + " },\n";
+    js += "  \"block\": 4,\n  \"lod_bias_level1\": 2,\n";
+    js += "  \"sampling\": \"both latents are sampled bilinearly at the same coordinate, level 1 with MipLODBias = "
+          "lod_bias_level1; output mip m reads mip m of both; dequantise after sampling: value = lo + sample * "
+          "(hi - lo) per channel\",\n";
+    js += "  \"textures\": [\n";
+
+    // ---- level 0: the refined BC4 / BC5 blocks, as the .dds holds them
+    std::string files_js;
+    for (int f = 0; f < files; f++)
+    {
+        const int nc = f == 0 ? (d.c0 == 1 ? 1 : 2) : d.c0 - 2;
+        const int dxgi = nc == 1 ? 80 : 83;
+        const int bpb = nc == 1 ? 8 : 16;
+        std::vector<std::vector<uint8_t>> levels((size_t)np);
+        for (int i = 0; i < np; i++)
+        {
+            const size_t nb = (size_t)((d.planes[i].w0 + 3) / 4) * ((d.planes[i].h0 + 3) / 4);
+            levels[(size_t)i].resize(nb * bpb);
+            if (e.seam.bc_blocks_download(e.ctx, i, f, levels[(size_t)i].data(), levels[(size_t)i].size(),
+                                          nullptr) != 0)
+            {
+                g_error = std::string("bc_blocks_download: ") + e.seam.last_error();
+                return 1;
+            }
+        }
+        const std::string file = prefix + "_lat0" + (files == 2 ? (f == 0 ? "a" : "b") : "") + ".dds";
+        if (!dds_write(file, d.planes[0].w0, d.planes[0].h0, dxgi, true, (uint32_t)levels[0].size(), levels))
+            return 1;
+        const std::string fname = nc == 1 ? "BC4_UNORM" : "BC5_UNORM";
+        if (files == 2)
+            files_js += std::string(f ? ", " : "") + "{ \"file\": \"" + base_name(file) + "\", \"dxgi_format\": \"" +
+                        fname + "\", \"dxgi_format_id\": " + std::to_string(dxgi) + ", \"channels_stored\": " +
+                        std::to_string(nc) + " }";
+        else
+            files_js += "\"" + base_name(file) + "\"";
+    }
+    std::string sizes0 = "[", sizes1 = "[";
+    for (int i = 0; i < np; i++)
+    {
+        sizes0 += "[" + std::to_string(d.planes[i].w0) + "," + std::to_string(d.planes[i].h0) + "]" +
+                  (i + 1 < np ? "," : "");
+        sizes1 += "[" + std::to_string(d.planes[i].w1) + "," + std::to_string(d.planes[i].h1) + "]" +
+                  (i + 1 < np ? "," : "");
+    }
+    sizes0 += "]";
+    sizes1 += "]";
+    const int nc0 = d.c0 == 1 ? 1 : 2;
+    js += "    {\n      " + std::string(files == 2 ? "\"files\": [" + files_js + "]" : "\"file\": " + files_js) +
+          ", \"level\": 0, \"width\": " + std::to_string(d.planes[0].w0) + ", \"height\": " +
+          std::to_string(d.planes[0].h0) + ",\n";
+    js += "      \"dxgi_format\": \"" + std::string(nc0 == 1 ? "BC4_UNORM" : "BC5_UNORM") +
+          "\", \"dxgi_format_id\": " + std::to_string(nc0 == 1 ? 80 : 83) + ", \"channels_used\": " +
+          std::to_string(d.c0) + ", \"channels_stored\": " + std::to_string(nc0) + ",\n";
+    js += "      \"bc_palette\": \"standard\",\n      \"bits_per_channel\": [";
+    for (int c = 0; c < d.c0; c++)
+        js += std::string("8") + (c + 1 < d.c0 ? "," : "");
+    js += "],\n      \"mip_count\": " + std::to_string(np) + ", \"mip_sizes\": " + sizes0 + ",\n";
+    js += "      \"texel\": \"an 8-bit continuous value, BC4 / BC5 encoded\",\n";
+    js += "      \"dequantise\": { \"kind\": \"range\", \"lo\": " + nums(st.lo0, (size_t)d.c0) + ", \"hi\": " +
+          nums(st.hi0, (size_t)d.c0) + ", \"levels\": 255 }\n    },\n";
+
+    // ---- level 1: the indices replicated over their bytes, three channels padded to four
+    {
+        const int ch = d.c1, stored = ch == 3 ? 4 : ch;
+        const int dxgi = stored == 1 ? 61 : (stored == 2 ? 49 : 28);
+        std::vector<std::vector<uint8_t>> levels((size_t)np);
+        for (int i = 0; i < np; i++)
+        {
+            const size_t n = (size_t)d.planes[i].w1 * d.planes[i].h1;
+            std::vector<uint8_t> k(n * ch);
+            if (e.seam.level1_download_indices(e.ctx, i, k.data(), k.size(), nullptr) != 0)
+            {
+                g_error = std::string("level1_download_indices: ") + e.seam.last_error();
+                return 1;
+            }
+            levels[(size_t)i].assign(n * stored, 255);
+            for (size_t t = 0; t < n; t++)
+                for (int c = 0; c < ch; c++)
+                    levels[(size_t)i][t * stored + c] = (uint8_t)replicate_index(k[t * ch + c], d.bits1);
+        }
+        const std::string file = prefix + "_lat1.dds";
+        if (!dds_write(file, d.planes[0].w1, d.planes[0].h1, dxgi, false, (uint32_t)(d.planes[0].w1 * stored),
+                       levels))
+            return 1;
+        const std::string fname = stored == 1 ? "R8_UNORM" : (stored == 2 ? "R8G8_UNORM" : "R8G8B8A8_UNORM");
+        js += "    {\n      \"file\": \"" + base_name(file) + "\", \"level\": 1, \"width\": " +
+              std::to_string(d.planes[0].w1) + ", \"height\": " + std::to_string(d.planes[0].h1) + ",\n";
+        js += "      \"dxgi_format\": \"" + fname + "\", \"dxgi_format_id\": " + std::to_string(dxgi) +
+              ", \"channels_used\": " + std::to_string(ch) + ", \"channels_stored\": " + std::to_string(stored) +
+              ",\n      \"bits_per_channel\": [";
+        for (int c = 0; c < ch; c++)
+            js += std::to_string(d.bits1) + (c + 1 < ch ? "," : "");
+        js += "],\n      \"mip_count\": " + std::to_string(np) + ", \"mip_sizes\": " + sizes1 + ",\n";
+        js += "      \"texel\": \"the quantisation index in the byte's top bits, replicated\",\n";
+        js += "      \"dequantise\": { \"kind\": \"range\", \"lo\": " + nums(st.lo1, (size_t)ch) + ", \"hi\": " +
+              nums(st.hi1, (size_t)ch) + ", \"levels\": " + std::to_string((1 << d.bits1) - 1) + " }\n    }\n";
+    }
+    js += "  ],\n  \"decoder\": {\n    \"type\": \"bilinear\",\n    \"terms\": \"a b sc\", \"C0\": " +
+          std::to_string(d.c0) + ", \"C1\": " + std::to_string(d.c1) + ",\n";
+    js += "    \"features\": \"[c_j for j < C1][s_i for i < C0][s_i c_j, i outer, j inner]\",\n";
+    js += "    \"nin\": " + std::to_string(nin) + ", \"nout\": " + std::to_string(d.nout) +
+          ", \"hidden\": [], \"activation\": \"leaky_relu\", \"leak\": 0,\n    \"output\": \"identity\",\n";
+    js += "    \"layers\": [\n      { \"rows\": " + std::to_string(d.nout) + ", \"cols\": " + std::to_string(nin) +
+          ", \"weights\": " + nums(st.w, (size_t)d.nout * nin) + ", \"bias\": " + nums(st.b, (size_t)d.nout) +
+          " }\n    ]\n  }\n}\n";
+    const std::string jname = prefix + "_nntc.json";
+    FILE* f = std::fopen(jname.c_str(), "wb");
+    if (!f)
+    {
+        g_error = "cannot write '" + jname + "'";
+        return 1;
+    }
+    const bool ok = std::fwrite(js.data(), 1, js.size(), f) == js.size();
+    if (std::fclose(f) != 0 || !ok)
+    {
+        g_error = "writing '" + jname + "' failed";
+        return 1;
+    }
+    return 0;
+}

@@ -1,0 +1,72 @@
+"""The whole encode on the gfx950 arm against the reference encoder's own outputs.
+
+The driver (include/nntc_encode.h) runs the reference's schedule over the HIP arm on a seeded workload and writes the
+asset; the fixtures in tests/golden/<workload>/report.json are the reference CPU encoder's outputs on the same
+workload (tests/golden/make_golden.py). The exact form must reproduce them: the same .dds bytes, the same decoder
+coefficients and grids, the same per-texture psnr, the same E in every round.
+"""
+
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+from nntc_amd import encoder, synth
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def golden(name):
+    p = os.path.join(GOLDEN, name, "report.json")
+    if not os.path.exists(p):
+        pytest.skip(f"no golden fixture for {name}")
+    return json.load(open(p))
+
+
+def encode(name, tmp_path, **kw):
+    spec = synth.WORKLOADS[name]
+    tex = synth.workload(name)
+    e = encoder.Encoder(tex, c0=spec["c0"], c1=spec["c1"], rounds=8, tol=0.0, quiet=1, **kw)
+    rep = e.run()
+    prefix = str(tmp_path / "m")
+    e.write(prefix)
+    return e, rep, prefix
+
+
+@pytest.mark.parametrize("name", ["plain_cpu_4x256", "plain_gpu_4x1024"])
+def test_encode_exact_matches_reference(name, tmp_path):
+    g = golden(name)
+    print(f"encode {name}", flush=True)
+    e, rep, prefix = encode(name, tmp_path)
+    print(f"  {rep.seconds:.3f} s, {rep.mpix / rep.seconds:.3f} Mpix/s", flush=True)
+    assert rep.rounds_used == g["rounds_used"]
+    for r, (ea, eb, ec) in enumerate(g["rounds"]):
+        # The reference prints E to 7 significant digits; the exact form's E is the reference's double.
+        for x, y in zip((rep.e_round[r][0], rep.e_round[r][1], rep.e_round[r][2]), (ea, eb, ec)):
+            assert float(f"{x:.6e}") == y, f"round {r + 1}: {x!r} vs {y!r}"
+    assert float(f"{rep.e_shipped:.9e}") == g["e_shipped"]
+    for t, p in enumerate(g["psnr"]):
+        assert round(rep.psnr[t], 2) == p
+    w = np.array(e.state().w[: len(g["decoder"]["weights"])], dtype=np.float32)
+    np.testing.assert_array_equal(w, np.array(g["decoder"]["weights"], dtype=np.float32))
+    for suffix, digest in g["dds"].items():
+        data = open(prefix + suffix, "rb").read()
+        assert hashlib.sha256(data).hexdigest() == digest, f".dds {suffix} differs from the reference's"
+
+
+def test_encode_repeatable(tmp_path):
+    """Two runs on one resident device give the same bytes (the driver resets every piece of state)."""
+    spec = synth.WORKLOADS["plain_cpu_4x256"]
+    e = encoder.Encoder(synth.workload("plain_cpu_4x256"), c0=spec["c0"], c1=spec["c1"], rounds=3, tol=0.0, quiet=1)
+    a = e.run()
+    ka = [e.level1_indices(p).copy() for p in range(e.desc.nplanes)]
+    ba = [e.level0_blocks(p, 0).copy() for p in range(e.desc.nplanes)]
+    b = e.run()
+    assert a.e_shipped == b.e_shipped
+    for p in range(e.desc.nplanes):
+        np.testing.assert_array_equal(ka[p], e.level1_indices(p))
+        np.testing.assert_array_equal(ba[p], e.level0_blocks(p, 0))

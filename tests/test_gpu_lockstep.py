@@ -1,0 +1,80 @@
+"""Parity of the gfx950 arm with the reference CPU backend, call by call, over the whole encode schedule.
+
+Both arms are created from the same seeded material and driven through the same schedule (tests/schedule.py); after
+every seam call the two arms' state is compared: the objective, the decoder, both grids, both latents' values and
+indices on every plane, and the BC blocks. The exact form is held to the reference's bits.
+"""
+
+import numpy as np
+import pytest
+
+from nntc_amd import abi, chain, synth
+from schedule import Options, steps
+
+pytestmark = pytest.mark.gpu
+
+
+def material(name):
+    if name in synth.WORKLOADS:
+        spec = synth.WORKLOADS[name]
+        return synth.workload(name), spec["c0"], spec["c1"]
+    textures, w, h, c0, c1 = {
+        "small_2x100x76_c3c3": (2, 100, 76, 3, 3),
+        "small_1x64x40_c1c2": (1, 64, 40, 1, 2),
+    }[name]
+    tex = synth.make_material(textures, (w, h), 32, 3, 1.0, config=9)
+    return tex, c0, c1
+
+
+def sessions(name, hip_arm, ref_arm):
+    tex, c0, c1 = material(name)
+    planes, ch = chain.source_chain(tex)
+    _, per_site = chain.mip_weights(planes)
+    desc = abi.make_desc(len(tex), c0, c1, planes, chain.output_weights(len(tex)))
+    flat = [c.reshape(-1) for c in ch]
+    return abi.Session(hip_arm, desc, flat), abi.Session(ref_arm, desc, flat), per_site
+
+
+def compare(a, b, step, i):
+    where = f"step {i} ({step})"
+    assert a.state.has_decoder == b.state.has_decoder, where
+    wa, ba = a.decoder()
+    wb, bb = b.decoder()
+    np.testing.assert_array_equal(wa, wb, err_msg=f"decoder W after {where}")
+    np.testing.assert_array_equal(ba, bb, err_msg=f"decoder b after {where}")
+    for f in ("lo0", "hi0", "lo1", "hi1"):
+        np.testing.assert_array_equal(np.array(getattr(a.state, f)), np.array(getattr(b.state, f)),
+                                      err_msg=f"{f} after {where}")
+    for p in range(a.nplanes):
+        for fn in ("v0", "k0", "v1", "k1"):
+            x, y = getattr(a, fn)(p), getattr(b, fn)(p)
+            if not np.array_equal(x, y):
+                bad = np.flatnonzero(x != y)
+                raise AssertionError(f"{fn} plane {p} after {where}: {bad.size} of {x.size} differ, first at "
+                                     f"{bad[0]}: {x[bad[0]]!r} vs {y[bad[0]]!r}")
+
+
+@pytest.mark.parametrize("name", ["small_1x64x40_c1c2", "small_2x100x76_c3c3", "plain_cpu_4x256"])
+def test_lockstep_exact(name, hip_arm, ref_arm):
+    print(f"lockstep {name}", flush=True)
+    h, r, per_site = sessions(name, hip_arm, ref_arm)
+    o = Options()
+    gh, gr = steps(h, o, per_site), steps(r, o, per_site)
+    i = 0
+    for (nh, ih), (nr, ir) in zip(gh, gr):
+        assert nh == nr
+        for key in ih:
+            if key == "pack_psnr":
+                # A report figure: the packing error is summed per block on the device, not in the reference's
+                # row-major order, so only its last bits may differ. No decision reads it.
+                assert abs(ih[key] - ir[key]) <= 1e-12 * abs(ir[key]), f"{key} at step {i}"
+                continue
+            assert ih[key] == ir[key], f"{key} at step {i} ({nh}): {ih[key]!r} vs {ir[key]!r}"
+        compare(h, r, nh, i)
+        i += 1
+    assert i > 40
+    for p in range(h.nplanes):
+        for f in range(1 if h.desc.c0 <= 2 else 2):
+            np.testing.assert_array_equal(h.bc_blocks(p, f), r.download_blocks(p, f, h.bc_blocks(p, f).size * 2),
+                                          err_msg=f"blocks plane {p} file {f}")
+        np.testing.assert_array_equal(h.decode_plane(p), r.decode_plane(p), err_msg=f"decode plane {p}")
