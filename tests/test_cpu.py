@@ -1,0 +1,118 @@
+"""CPU-side checks of the boundary and the host path (no GPU calls).
+
+- the built libraries export every entry point include/*.h declares;
+- the encode driver (include/nntc_encode.h) run over the reference CPU backend reproduces the reference binary's
+  assets byte for byte (the driver and the writer are thereby pinned independently of the gfx950 arm);
+- the PNG path and the array path give the same encode;
+- the synthetic generator is deterministic;
+- bench.py refuses a multi-GPU run with a message.
+"""
+
+import ctypes
+import glob
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from nntc_amd import abi, encoder, synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+REF_LIB = os.path.join(ROOT, "oracle", "_ref", "libnntc_ref.so")
+
+
+def header_symbols(name):
+    return abi.declared_symbols(os.path.join(ROOT, "include", name))
+
+
+@pytest.mark.parametrize("header,lib", [("nntc_hip.h", abi.HIP_LIB), ("nntc_encode.h", encoder.HOST_LIB)])
+def test_library_exports_every_declared_symbol(header, lib):
+    names = header_symbols(header)
+    assert len(names) > 10
+    so = ctypes.CDLL(lib, mode=ctypes.RTLD_GLOBAL)
+    missing = [n for n in names if not hasattr(so, n)]
+    assert not missing, f"{os.path.basename(lib)} lacks {missing}"
+
+
+def test_c_abi_struct_layouts_match_header():
+    """The ctypes mirrors have the sizes the C compiler gives the header's structs."""
+    src = os.path.join(ROOT, "_build", "layout_probe.c")
+    os.makedirs(os.path.dirname(src), exist_ok=True)
+    structs = ["nntc_model_desc", "nntc_model_state", "nntc_level1_report", "nntc_objective", "nntc_init0_report",
+               "nntc_level1_init_report", "nntc_bc_refine_report", "nntc_device_info", "nntc_encode_options",
+               "nntc_encode_report"]
+    with open(src, "w") as f:
+        f.write('#include <stdio.h>\n#include "nntc_encode.h"\nint main(void){\n')
+        for s in structs:
+            f.write(f'printf("%zu\\n", sizeof({s}));\n')
+        f.write("return 0;}\n")
+    exe = src[:-2]
+    subprocess.run(["gcc", "-I" + os.path.join(ROOT, "include"), src, "-o", exe], check=True)
+    sizes = [int(x) for x in subprocess.run([exe], stdout=subprocess.PIPE, text=True).stdout.split()]
+    mirrors = [abi.ModelDesc, abi.ModelState, abi.Level1Report, abi.Objective, abi.Init0Report,
+               abi.Level1InitReport, abi.BcRefineReport, abi.DeviceInfo, encoder.Options, encoder.Report]
+    for s, m, n in zip(structs, mirrors, sizes):
+        assert ctypes.sizeof(m) == n, f"{s}: C {n} bytes, ctypes {ctypes.sizeof(m)}"
+
+
+def ref_encode(name, tmp_path, rounds):
+    if not os.path.exists(REF_LIB):
+        pytest.skip("oracle/_ref is not built")
+    ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_GLOBAL).nntc_ref_set_threads(min(8, os.cpu_count() or 1))
+    spec = synth.WORKLOADS[name]
+    e = encoder.Encoder(synth.workload(name), arm_lib=REF_LIB, arm_prefix="nntc_ref_", c0=spec["c0"],
+                        c1=spec["c1"], rounds=rounds, tol=0.0, quiet=1)
+    rep = e.run()
+    e.write(str(tmp_path / "m"))
+    return e, rep
+
+
+@pytest.mark.parametrize("name,rounds", [("smoke_1x64x40", 4), ("small_2x100x76", 8)])
+def test_driver_over_reference_reproduces_reference_binary(name, rounds, tmp_path):
+    g = json.load(open(os.path.join(GOLDEN, name, "report.json")))
+    e, rep = ref_encode(name, tmp_path, rounds)
+    assert f"{rep.e_shipped:.9e}" == f"{g['e_shipped']:.9e}"
+    assert [round(rep.psnr[t], 2) for t in range(len(g["psnr"]))] == g["psnr"]
+    assert rep.rounds_used == g["rounds_used"]
+    for suffix, digest in g["dds"].items():
+        data = open(str(tmp_path / "m") + suffix, "rb").read()
+        assert hashlib.sha256(data).hexdigest() == digest, suffix
+    js = json.load(open(str(tmp_path / "m") + "_nntc.json"))
+    np.testing.assert_array_equal(np.array(js["decoder"]["layers"][0]["weights"], np.float32),
+                                  np.array(g["decoder"]["weights"], np.float32))
+    for lvl in range(2):
+        assert js["textures"][lvl]["dequantise"]["lo"] == g["grids"][lvl]["lo"]
+        assert js["textures"][lvl]["dequantise"]["hi"] == g["grids"][lvl]["hi"]
+
+
+def test_png_path_matches_array_path(tmp_path):
+    if not os.path.exists(REF_LIB):
+        pytest.skip("oracle/_ref is not built")
+    ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_GLOBAL).nntc_ref_set_threads(4)
+    tex = synth.workload("smoke_1x64x40")
+    paths = synth.write_pngs(tex, str(tmp_path / "in"))
+    kw = dict(arm_lib=REF_LIB, arm_prefix="nntc_ref_", c0=1, c1=2, rounds=2, tol=0.0, quiet=1)
+    a = encoder.Encoder(tex, **kw).run()
+    b = encoder.Encoder(files=paths, **kw).run()
+    assert a.e_shipped == b.e_shipped
+
+
+def test_synthetic_generator_is_deterministic():
+    a = synth.workload("smoke_1x64x40")
+    b = synth.workload("smoke_1x64x40")
+    assert all(np.array_equal(x, y) for x, y in zip(a, b))
+    h = hashlib.sha256(b"".join(x.tobytes() for x in synth.workload("plain_cpu_4x256"))).hexdigest()
+    g = json.load(open(os.path.join(GOLDEN, "synth_digests.json")))
+    assert h == g["plain_cpu_4x256"]
+
+
+def test_bench_refuses_more_than_one_gpu():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], stdout=subprocess.PIPE,
+                       stderr=subprocess.PIPE, text=True)
+    assert r.returncode != 0
+    assert "one GPU" in r.stderr
