@@ -151,6 +151,8 @@ struct Ctx
     DevBuf<float> work_f;                  // staging for grid fits (values of one channel)
     DevBuf<unsigned int> work_u;           // radix-select histograms
     DevBuf<uint8_t> rgb8;                  // one plane's decode
+    DevBuf<float> site_f;                  // per site of the chain: the other latent's sample (assembly pass 1)
+    DevBuf<double> site_d;                 // per site: the residual pulled back through the decoder
 
     // Pinned host staging for the small per-call readbacks.
     double* host_scalars = nullptr;

@@ -641,6 +641,9 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.work_f.alloc(256 * 2 * MAX_CH);
         c.work_u.alloc(MAX_CH * 2 * 256 + 64);
         c.rgb8.alloc((size_t)c.planes.p[0].n0 * nout);
+        // The assembly's site buffer: every site of the chain at the largest K the encoder takes (4).
+        c.site_f.alloc((size_t)o0 * 5 * std::max(c0, c1));
+        c.site_d.alloc((size_t)o0 * 5 * std::max(c0, c1));
         HIP_OK(hipHostMalloc((void**)&c.host_scalars, sizeof(double) * SC_COUNT * MAX_PLANES, hipHostMallocDefault));
 
         c.bytes = c.src.bytes() + c.v0.bytes() + c.v1.bytes() + c.k0.bytes() + c.k1.bytes() + c.weights.bytes() +
@@ -648,7 +651,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
                   c.ls_partial.bytes() + c.ls_sum.bytes() + c.obj.bytes() + c.resid.bytes() + c.cov_partial.bytes() +
                   c.counters.bytes() + c.bc_ep.bytes() + c.bc_sel.bytes() + c.bc_blocks.bytes() +
                   c.k0_mirror.bytes() + c.bc_taken.bytes() + c.bc_se.bytes() + c.work_f.bytes() + c.work_u.bytes() +
-                  c.rgb8.bytes();
+                  c.rgb8.bytes() + c.site_f.bytes() + c.site_d.bytes();
         for (int lvl = 0; lvl < 2; lvl++)
         {
             Latent& L = c.latent(lvl == 0);

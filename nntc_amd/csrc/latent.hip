@@ -100,6 +100,26 @@ __device__ __forceinline__ double fold_lanes(double acc, double v, int count)
     return acc;
 }
 
+// acc plus t[0] + t[1] + ... + t[n-1], added in index order; sixteen values are read ahead of their sixteen dependent
+// additions so the chain waits on the adds alone.
+__device__ __forceinline__ double fold_serial16(const double* t, int n, double acc)
+{
+    int i = 0;
+    for (; i + 16 <= n; i += 16)
+    {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            v[u] = t[i + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            acc += v[u];
+    }
+    for (; i < n; i++)
+        acc += t[i];
+    return acc;
+}
+
 // The fast form's wavefront tree: every lane ends holding the sum of all 64.
 __device__ __forceinline__ double tree_lanes(double v)
 {
@@ -143,6 +163,20 @@ __device__ __forceinline__ long long view_offset(const LatentView& L, int p)
 // Assembly (src/cpu/cpu_solve_level1.cpp:203-468)
 // ---------------------------------------------------------------------------------------------------------------------
 
+// The assembly in two passes. A site's contribution to a texel's sums is w_self * tw_e * mono(site) into the owned
+// slots and w_self * acc(site) into the gradient, where mono is the monomials of the OTHER latent's sample at the site
+// and acc = sum_r cw_r G_ra res_r is the site's residual pulled back through the decoder - neither depends on the
+// texel. So:
+//   1. k_site_terms: one thread per site computes the other latent's fp32 sample (from which the gather rebuilds the
+//      monomials exactly: a product of two floats is exact in double) and the fp64 acc vector, once, into the site
+//      buffer;
+//   2. k_gather: one thread per texel walks its footprint's sites in the reference's order (pixel row, pixel, site),
+//      recomputes each site's taps (fp32 addressing only) and adds the stored terms. The sums and the gradient receive
+//      the same operands in the same order as the reference's per-texel loop, so the stencil is its bits. The kernel
+//      is instantiated per (channels, sample channels), so its five-slot accumulators live in registers.
+// The reference's gather evaluated every site about four times over (each site reaches four level-1 texels) and
+// carried its accumulators in scratch; this form reads each site's 8 c_other + 8 c bytes instead.
+
 struct AssembleArgs
 {
     LatentView L;
@@ -153,6 +187,9 @@ struct AssembleArgs
     const float* bias;
     const float* cw;
     const double* mono_p;
+    float* site_f;          // per site: the other latent's sample (cs floats)
+    double* site_d;         // per site: acc (c doubles)
+    Ranges sites;           // per plane: its sites, w0 h0 (1 + K)
     int nmono, c0, c1, nin, nout, k;
 };
 
@@ -165,172 +202,152 @@ NNTC_HD void footprint_range(int ti, int n0, int n1, int& lo, int& hi)
     hi = hi > n0 - 1 ? n0 - 1 : hi;
 }
 
-__device__ __forceinline__ void stencil_monomial_values(const float* s, int c, double* mono)
+// The monomials of the form in the one order the matrices and the sums use: 1, s_i, then s_i s_j (i <= j).
+template <int CS>
+__device__ __forceinline__ void monomials(const float* s, double* mono)
 {
     int n = 0;
     mono[n++] = 1.0;
-    for (int i = 0; i < c; i++)
+#pragma unroll
+    for (int i = 0; i < CS; i++)
         mono[n++] = (double)s[i];
-    for (int i = 0; i < c; i++)
-        for (int j = i; j < c; j++)
+#pragma unroll
+    for (int i = 0; i < CS; i++)
+#pragma unroll
+        for (int j = i; j < CS; j++)
             mono[n++] = (double)s[i] * (double)s[j];
 }
 
-// The five owned blocks expanded from the monomial sums; a zero sum is skipped.
-__device__ __forceinline__ void expand_blocks(const double* sums, int nmono, const double* mono_p, int c,
-                                              double* out_h)
+// One site's terms for the level-1 (LEVEL0 false) or the level-0 (LEVEL0 true) assembly.
+template <bool LEVEL0>
+__global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 {
-    for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
+    const long long gs = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gs >= a.sites.start[a.sites.n])
+        return;
+    const int p = range_plane(a.sites, gs);
+    const PlaneGeom& g = a.L.planes.p[p];
+    const long long si = gs - a.sites.start[p];
+    const int per_pixel = 1 + a.k;
+    const long long pixel = si / per_pixel;
+    const int j = (int)(si % per_pixel);
+    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
+    const int px = (int)(pixel % w0), py = (int)(pixel / w0);
+    const int c0 = a.c0, c1 = a.c1, nout = a.nout, nin = a.nin;
+    const float* src = a.src + g.off0 * nout;
+    const float* v0 = a.v0 + g.off0 * c0;
+    const float* v1 = a.v1 + g.off1 * c1;
+    float dx, dy;
+    site_delta(a.k, j, px, py, dx, dy);
+    const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+    float sbar[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
+    sample_bilinear(v1, w1, c1, tp, cbar);
+    if (j == 0)
     {
-        double blk[MAX_CH * MAX_CH];
-        for (int i = 0; i < c * c; i++)
-            blk[i] = 0.0;
-        for (int i = 0; i < nmono; i++)
+        const size_t pi = (size_t)py * w0 + px;
+        for (int i = 0; i < c0; i++)
+            sbar[i] = v0[pi * c0 + i];
+        for (int r = 0; r < nout; r++)
+            target[r] = src[pi * nout + r];
+    }
+    else
+    {
+        const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+        sample_bilinear(v0, w0, c0, t0, sbar);
+        sample_bilinear(src, w0, nout, t0, target);
+    }
+    if (!LEVEL0)
+    {
+        // G = A + sum_i sbar_i M_i and the residual r = b + B sbar + G cbar - target, in fp32.
+        float gmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
+        for (int r = 0; r < nout; r++)
         {
-            const double s = sums[slot * MAX_MONO + i];
-            if (s == 0.0)
-                continue;
-            const double* mat = mono_p + (size_t)i * c * c;
-            for (int e = 0; e < c * c; e++)
-                blk[e] += s * mat[e];
+            const float* row = a.weights + (size_t)r * nin;
+            float out = a.bias[r];
+            for (int i = 0; i < c0; i++)
+                out += row[c1 + i] * sbar[i];
+            for (int q = 0; q < c1; q++)
+            {
+                float gg = row[q];
+                for (int i = 0; i < c0; i++)
+                    gg += sbar[i] * row[c1 + c0 + i * c1 + q];
+                gmat[r * c1 + q] = gg;
+                out += gg * cbar[q];
+            }
+            res[r] = out - target[r];
         }
-        for (int i = 0; i < c * c; i++)
-            out_h[(size_t)slot * c * c + i] = blk[i];
+        for (int i = 0; i < c0; i++)
+            a.site_f[gs * c0 + i] = sbar[i];
+        for (int aa = 0; aa < c1; aa++)
+        {
+            double acc = 0.0;
+            for (int r = 0; r < nout; r++)
+                acc += (double)a.cw[r] * (double)gmat[r * c1 + aa] * (double)res[r];
+            a.site_d[gs * c1 + aa] = acc;
+        }
+    }
+    else
+    {
+        // Q = B + sum_q cbar_q N_q and the residual r = b + A cbar + Q sbar - target, in fp32.
+        float qmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
+        for (int r = 0; r < nout; r++)
+        {
+            const float* row = a.weights + (size_t)r * nin;
+            float out = a.bias[r];
+            for (int q = 0; q < c1; q++)
+                out += row[q] * cbar[q];
+            for (int i = 0; i < c0; i++)
+            {
+                float qq = row[c1 + i];
+                for (int q = 0; q < c1; q++)
+                    qq += cbar[q] * row[c1 + c0 + i * c1 + q];
+                qmat[r * c0 + i] = qq;
+                out += qq * sbar[i];
+            }
+            res[r] = out - target[r];
+        }
+        for (int q = 0; q < c1; q++)
+            a.site_f[gs * c1 + q] = cbar[q];
+        for (int aa = 0; aa < c0; aa++)
+        {
+            double acc = 0.0;
+            for (int r = 0; r < nout; r++)
+                acc += (double)a.cw[r] * (double)qmat[r * c0 + aa] * (double)res[r];
+            a.site_d[gs * c0 + aa] = acc;
+        }
     }
 }
 
-// Level-1 texel ti's five blocks and gradient: every site whose bilinear footprint reaches it, in the reference's
-// pixel, then site order.
-__global__ __launch_bounds__(128) void k_assemble1(AssembleArgs a)
+// Merge four taps by texel index, so a clamped duplicate adds its weight to the one texel.
+__device__ __forceinline__ int merge_taps(const int* cx, const int* cy, const float* cwt, int* nx, int* ny, float* tw)
 {
-    const LatentView& L = a.L;
-    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= L.texels.start[L.texels.n])
-        return;
-    const int p = range_plane(L.texels, gi);
-    const PlaneGeom& g = L.planes.p[p];
-    const long long ti = gi - L.texels.start[p];
-    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
-    const int c0 = a.c0, c1 = a.c1, nout = a.nout, nin = a.nin, nmono = a.nmono;
-    const float* src = a.src + g.off0 * nout;
-    const float* v0 = a.v0 + g.off0 * c0;
-    const float* v1 = a.v1 + g.off1 * c1;
-    const int tx = (int)(ti % w1), ty = (int)(ti / w1);
-
-    double sums[STENCIL_BLOCKS * MAX_MONO];
-    double gacc[MAX_CH];
-    for (int i = 0; i < STENCIL_BLOCKS * MAX_MONO; i++)
-        sums[i] = 0.0;
-    for (int i = 0; i < c1; i++)
-        gacc[i] = 0.0;
-
-    int px_lo, px_hi, py_lo, py_hi;
-    footprint_range(tx, w0, w1, px_lo, px_hi);
-    footprint_range(ty, h0, h1, py_lo, py_hi);
-
-    for (int py = py_lo; py <= py_hi; py++)
-        for (int px = px_lo; px <= px_hi; px++)
-            for (int j = 0; j <= a.k; j++)
-            {
-                float dx, dy;
-                site_delta(a.k, j, px, py, dx, dy);
-                const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
-                int nx[4], ny[4];
-                float tw[4];
-                int taps = 0;
-                const int cx[4] = { tp.x0, tp.x1, tp.x0, tp.x1 };
-                const int cy[4] = { tp.y0, tp.y0, tp.y1, tp.y1 };
-                const float cwt[4] = { (1.0f - tp.fx) * (1.0f - tp.fy), tp.fx * (1.0f - tp.fy),
-                                       (1.0f - tp.fx) * tp.fy, tp.fx * tp.fy };
-                for (int q = 0; q < 4; q++)
-                {
-                    int found = -1;
-                    for (int e = 0; e < taps; e++)
-                        if (nx[e] == cx[q] && ny[e] == cy[q])
-                            found = e;
-                    if (found >= 0)
-                        tw[found] += cwt[q];
-                    else
-                    {
-                        nx[taps] = cx[q];
-                        ny[taps] = cy[q];
-                        tw[taps] = cwt[q];
-                        taps++;
-                    }
-                }
-                int self = -1;
-                for (int e = 0; e < taps; e++)
-                    if (nx[e] == tx && ny[e] == ty)
-                        self = e;
-                if (self < 0)
-                    continue;
-                const float w_self = tw[self];
-
-                float sbar[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
-                sample_bilinear(v1, w1, c1, tp, cbar);
-                if (j == 0)
-                {
-                    const size_t pi = (size_t)py * w0 + px;
-                    for (int i = 0; i < c0; i++)
-                        sbar[i] = v0[pi * c0 + i];
-                    for (int r = 0; r < nout; r++)
-                        target[r] = src[pi * nout + r];
-                }
-                else
-                {
-                    const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-                    sample_bilinear(v0, w0, c0, t0, sbar);
-                    sample_bilinear(src, w0, nout, t0, target);
-                }
-
-                // G = A + sum_i sbar_i M_i and the residual r = b + B sbar + G cbar - target, in fp32.
-                float gmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
-                for (int r = 0; r < nout; r++)
-                {
-                    const float* row = a.weights + (size_t)r * nin;
-                    float out = a.bias[r];
-                    for (int i = 0; i < c0; i++)
-                        out += row[c1 + i] * sbar[i];
-                    for (int q = 0; q < c1; q++)
-                    {
-                        float gg = row[q];
-                        for (int i = 0; i < c0; i++)
-                            gg += sbar[i] * row[c1 + c0 + i * c1 + q];
-                        gmat[r * c1 + q] = gg;
-                        out += gg * cbar[q];
-                    }
-                    res[r] = out - target[r];
-                }
-                double mono[MAX_MONO];
-                stencil_monomial_values(sbar, c0, mono);
-                for (int aa = 0; aa < c1; aa++)
-                {
-                    double acc = 0.0;
-                    for (int r = 0; r < nout; r++)
-                        acc += (double)a.cw[r] * (double)gmat[r * c1 + aa] * (double)res[r];
-                    gacc[aa] += (double)w_self * acc;
-                }
-                for (int e = 0; e < taps; e++)
-                {
-                    const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
-                    if (slot < 0)
-                        continue;
-                    const double scale = (double)w_self * (double)tw[e];
-                    for (int i = 0; i < nmono; i++)
-                        sums[slot * MAX_MONO + i] += scale * mono[i];
-                }
-            }
-
-    const long long gt = L.texels.start[p] + ti;
-    expand_blocks(sums, nmono, a.mono_p, c1, L.stencil + gt * (STENCIL_BLOCKS * c1 * c1));
-    for (int i = 0; i < c1; i++)
-        L.grad[gt * c1 + i] = gacc[i];
+    int taps = 0;
+    for (int q = 0; q < 4; q++)
+    {
+        int found = -1;
+        for (int e = 0; e < taps; e++)
+            if (nx[e] == cx[q] && ny[e] == cy[q])
+                found = e;
+        if (found >= 0)
+            tw[found] += cwt[q];
+        else
+        {
+            nx[taps] = cx[q];
+            ny[taps] = cy[q];
+            tw[taps] = cwt[q];
+            taps++;
+        }
+    }
+    return taps;
 }
 
-// Level-0 texel ti's five blocks and gradient (block (c')): the 3x3-pixel neighbourhood, the centre site read NEAREST
-// as one tap of weight 1.
-__global__ __launch_bounds__(128) void k_assemble0(AssembleArgs a)
+// Texel ti's five blocks and gradient. LEVEL0 false: a level-1 texel, every site whose level-1 taps reach it, C = c1,
+// CS = c0. LEVEL0 true: a level-0 texel (block (c')), the 3x3-pixel neighbourhood with the centre site read nearest as
+// one tap of weight 1, C = c0, CS = c1.
+template <bool LEVEL0, int C, int CS>
+__global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
 {
+    constexpr int NM = 1 + CS + CS * (CS + 1) / 2;
     const LatentView& L = a.L;
     const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (gi >= L.texels.start[L.texels.n])
@@ -339,22 +356,33 @@ __global__ __launch_bounds__(128) void k_assemble0(AssembleArgs a)
     const PlaneGeom& g = L.planes.p[p];
     const long long ti = gi - L.texels.start[p];
     const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
-    const int c0 = a.c0, c1 = a.c1, nout = a.nout, nin = a.nin, nmono = a.nmono;
-    const float* src = a.src + g.off0 * nout;
-    const float* v0 = a.v0 + g.off0 * c0;
-    const float* v1 = a.v1 + g.off1 * c1;
-    const int tx = (int)(ti % w0), ty = (int)(ti / w0);
+    const int tw_ = LEVEL0 ? w0 : w1;
+    const int tx = (int)(ti % tw_), ty = (int)(ti / tw_);
+    const int per_pixel = 1 + a.k;
+    const long long site0 = a.sites.start[p];
 
-    double sums[STENCIL_BLOCKS * MAX_MONO];
-    double gacc[MAX_CH];
-    for (int i = 0; i < STENCIL_BLOCKS * MAX_MONO; i++)
-        sums[i] = 0.0;
-    for (int i = 0; i < c0; i++)
+    double sums[STENCIL_BLOCKS][NM];
+    double gacc[C];
+#pragma unroll
+    for (int s = 0; s < STENCIL_BLOCKS; s++)
+#pragma unroll
+        for (int i = 0; i < NM; i++)
+            sums[s][i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < C; i++)
         gacc[i] = 0.0;
 
     int px_lo, px_hi, py_lo, py_hi;
-    footprint_range(tx, w0, w0, px_lo, px_hi);
-    footprint_range(ty, h0, h0, py_lo, py_hi);
+    if (LEVEL0)
+    {
+        footprint_range(tx, w0, w0, px_lo, px_hi);
+        footprint_range(ty, h0, h0, py_lo, py_hi);
+    }
+    else
+    {
+        footprint_range(tx, w0, w1, px_lo, px_hi);
+        footprint_range(ty, h0, h1, py_lo, py_hi);
+    }
 
     for (int py = py_lo; py <= py_hi; py++)
         for (int px = px_lo; px <= px_hi; px++)
@@ -364,10 +392,8 @@ __global__ __launch_bounds__(128) void k_assemble0(AssembleArgs a)
                 site_delta(a.k, j, px, py, dx, dy);
                 int nx[4], ny[4];
                 float tw[4];
-                int taps = 0;
-                float sbar[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
-                const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
-                if (j == 0)
+                int taps;
+                if (LEVEL0 && j == 0)
                 {
                     nx[0] = px;
                     ny[0] = py;
@@ -376,27 +402,13 @@ __global__ __launch_bounds__(128) void k_assemble0(AssembleArgs a)
                 }
                 else
                 {
-                    const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-                    const int cx[4] = { t0.x0, t0.x1, t0.x0, t0.x1 };
-                    const int cy[4] = { t0.y0, t0.y0, t0.y1, t0.y1 };
-                    const float cwt[4] = { (1.0f - t0.fx) * (1.0f - t0.fy), t0.fx * (1.0f - t0.fy),
-                                           (1.0f - t0.fx) * t0.fy, t0.fx * t0.fy };
-                    for (int q = 0; q < 4; q++)
-                    {
-                        int found = -1;
-                        for (int e = 0; e < taps; e++)
-                            if (nx[e] == cx[q] && ny[e] == cy[q])
-                                found = e;
-                        if (found >= 0)
-                            tw[found] += cwt[q];
-                        else
-                        {
-                            nx[taps] = cx[q];
-                            ny[taps] = cy[q];
-                            tw[taps] = cwt[q];
-                            taps++;
-                        }
-                    }
+                    const Tap t = LEVEL0 ? tap_pixel(px, py, dx, dy, w0, h0, w0, h0)
+                                         : tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+                    const int cx[4] = { t.x0, t.x1, t.x0, t.x1 };
+                    const int cy[4] = { t.y0, t.y0, t.y1, t.y1 };
+                    const float cwt[4] = { (1.0f - t.fx) * (1.0f - t.fy), t.fx * (1.0f - t.fy),
+                                           (1.0f - t.fx) * t.fy, t.fx * t.fy };
+                    taps = merge_taps(cx, cy, cwt, nx, ny, tw);
                 }
                 int self = -1;
                 for (int e = 0; e < taps; e++)
@@ -405,65 +417,83 @@ __global__ __launch_bounds__(128) void k_assemble0(AssembleArgs a)
                 if (self < 0)
                     continue;
                 const float w_self = tw[self];
-
-                sample_bilinear(v1, w1, c1, tp, cbar);
-                if (j == 0)
-                {
-                    const size_t pi = (size_t)py * w0 + px;
-                    for (int i = 0; i < c0; i++)
-                        sbar[i] = v0[pi * c0 + i];
-                    for (int r = 0; r < nout; r++)
-                        target[r] = src[pi * nout + r];
-                }
-                else
-                {
-                    const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-                    sample_bilinear(v0, w0, c0, t0, sbar);
-                    sample_bilinear(src, w0, nout, t0, target);
-                }
-
-                // Q = B + sum_q cbar_q N_q and the residual r = b + A cbar + Q sbar - target, in fp32.
-                float qmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
-                for (int r = 0; r < nout; r++)
-                {
-                    const float* row = a.weights + (size_t)r * nin;
-                    float out = a.bias[r];
-                    for (int q = 0; q < c1; q++)
-                        out += row[q] * cbar[q];
-                    for (int i = 0; i < c0; i++)
-                    {
-                        float qq = row[c1 + i];
-                        for (int q = 0; q < c1; q++)
-                            qq += cbar[q] * row[c1 + c0 + i * c1 + q];
-                        qmat[r * c0 + i] = qq;
-                        out += qq * sbar[i];
-                    }
-                    res[r] = out - target[r];
-                }
-                double mono[MAX_MONO];
-                stencil_monomial_values(cbar, c1, mono);
-                for (int aa = 0; aa < c0; aa++)
-                {
-                    double acc = 0.0;
-                    for (int r = 0; r < nout; r++)
-                        acc += (double)a.cw[r] * (double)qmat[r * c0 + aa] * (double)res[r];
-                    gacc[aa] += (double)w_self * acc;
-                }
+                const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
+                float sv[CS];
+#pragma unroll
+                for (int i = 0; i < CS; i++)
+                    sv[i] = a.site_f[s * CS + i];
+                double mono[NM];
+                monomials<CS>(sv, mono);
+#pragma unroll
+                for (int aa = 0; aa < C; aa++)
+                    gacc[aa] += (double)w_self * a.site_d[s * C + aa];
                 for (int e = 0; e < taps; e++)
                 {
                     const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
                     if (slot < 0)
                         continue;
                     const double scale = (double)w_self * (double)tw[e];
-                    for (int i = 0; i < nmono; i++)
-                        sums[slot * MAX_MONO + i] += scale * mono[i];
+#pragma unroll
+                    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+                        if (sl == slot)
+#pragma unroll
+                            for (int i = 0; i < NM; i++)
+                                sums[sl][i] += scale * mono[i];
                 }
             }
 
+    // The five owned blocks expanded from the monomial sums; a zero sum is skipped.
     const long long gt = L.texels.start[p] + ti;
-    expand_blocks(sums, nmono, a.mono_p, c0, L.stencil + gt * (STENCIL_BLOCKS * c0 * c0));
-    for (int i = 0; i < c0; i++)
-        L.grad[gt * c0 + i] = gacc[i];
+    double* out_h = L.stencil + gt * (STENCIL_BLOCKS * C * C);
+#pragma unroll
+    for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
+    {
+        double blk[C * C];
+#pragma unroll
+        for (int i = 0; i < C * C; i++)
+            blk[i] = 0.0;
+#pragma unroll
+        for (int i = 0; i < NM; i++)
+        {
+            const double sv = sums[slot][i];
+            if (sv == 0.0)
+                continue;
+            const double* mat = a.mono_p + (size_t)i * C * C;
+#pragma unroll
+            for (int e = 0; e < C * C; e++)
+                blk[e] += sv * mat[e];
+        }
+#pragma unroll
+        for (int i = 0; i < C * C; i++)
+            out_h[slot * C * C + i] = blk[i];
+    }
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        L.grad[gt * C + i] = gacc[i];
+}
+
+template <bool LEVEL0, int C>
+void launch_gather_cs(int cs, dim3 grid, hipStream_t s, const AssembleArgs& a)
+{
+    switch (cs)
+    {
+    case 1: hipLaunchKernelGGL((k_gather<LEVEL0, C, 1>), grid, dim3(128), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_gather<LEVEL0, C, 2>), grid, dim3(128), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_gather<LEVEL0, C, 3>), grid, dim3(128), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_gather<LEVEL0, C, 4>), grid, dim3(128), 0, s, a); break;
+    }
+}
+
+template <bool LEVEL0>
+void launch_gather(int c, int cs, dim3 grid, hipStream_t s, const AssembleArgs& a)
+{
+    switch (c)
+    {
+    case 1: launch_gather_cs<LEVEL0, 1>(cs, grid, s, a); break;
+    case 2: launch_gather_cs<LEVEL0, 2>(cs, grid, s, a); break;
+    case 3: launch_gather_cs<LEVEL0, 3>(cs, grid, s, a); break;
+    default: launch_gather_cs<LEVEL0, 4>(cs, grid, s, a); break;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -500,23 +530,55 @@ __global__ __launch_bounds__(64) void k_diag_chunks(ChunkArgs a)
     const long long tb = vb / c, te = ve / c;
     const int lane = threadIdx.x;
     double sum = 0.0, lo = 1e300, hi = -1e300;
-    for (long long base = tb * c; base < te * c; base += 64)
+    constexpr int ROWS = 8;
+    __shared__ double buf[64 * ROWS];
+    const long long end = te * c;
+    for (long long base = tb * c; base < end; base += 64 * ROWS)
     {
-        const long long i = base + lane;
-        const int count = (int)(te * c - base < 64 ? te * c - base : 64);
-        double x = 0.0;
-        if (lane < count)
+        const int n = (int)(end - base < 64 * ROWS ? end - base : 64 * ROWS);
+        double x[ROWS];
+#pragma unroll
+        for (int u = 0; u < ROWS; u++)
         {
-            const long long t = i / c;
-            const int q = (int)(i % c);
-            x = a.L.stencil[t * (STENCIL_BLOCKS * c * c) + q * c + q];
+            const long long i = base + u * 64 + lane;
+            x[u] = 0.0;
+            if (i < end)
+            {
+                const long long t = i / c;
+                const int q = (int)(i % c);
+                x[u] = a.L.stencil[t * (STENCIL_BLOCKS * c * c) + q * c + q];
+            }
         }
-        sum = a.exact ? fold_lanes(sum, x, count) : sum + tree_lanes(lane < count ? x : 0.0);
-        for (int l = 0; l < count; l++)
+#pragma unroll
+        for (int u = 0; u < ROWS; u++)
         {
-            const double y = lane_value(x, l);
-            lo = y < lo ? y : lo;
-            hi = y > hi ? y : hi;
+            const long long row = base + u * 64;
+            const int count = row >= end ? 0 : (int)(end - row < 64 ? end - row : 64);
+            if (count == 0)
+                continue;
+            // Minimum and maximum are exact in any order; the sum is folded serially in the exact form.
+            double l = lane < count ? x[u] : 1e300, h = lane < count ? x[u] : -1e300;
+            for (int off = 32; off > 0; off >>= 1)
+            {
+                const double ol = __shfl_xor(l, off), oh = __shfl_xor(h, off);
+                l = ol < l ? ol : l;
+                h = oh > h ? oh : h;
+            }
+            lo = l < lo ? l : lo;
+            hi = h > hi ? h : hi;
+            if (!a.exact)
+                sum += tree_lanes(lane < count ? x[u] : 0.0);
+        }
+        if (a.exact)
+        {
+            // Staged through LDS so lane 0 folds the batch in index order (texel then channel).
+#pragma unroll
+            for (int u = 0; u < ROWS; u++)
+                buf[u * 64 + lane] = x[u];
+            __syncthreads();
+            if (lane == 0)
+                sum = fold_serial16(buf, n, sum);
+            __syncthreads();
         }
     }
     if (lane == 0)
@@ -560,30 +622,46 @@ __global__ __launch_bounds__(64) void k_dot_chunks(ChunkArgs a, const double* x0
     if (a.L.scalars[(size_t)p * SC_COUNT + SC_ACTIVE] == 0.0)
         return;
     const int lane = threadIdx.x;
+    double acc = 0.0;   // the exact form: lane 0 folds the first product, lane 1 the second
     double s0 = 0.0, s1 = 0.0;
-    for (long long base = vb; base < ve; base += 64)
+    constexpr int ROWS = 8;   // 512 values staged per batch: one memory latency per batch
+    __shared__ double buf[2][64 * ROWS];
+    for (long long base = vb; base < ve; base += 64 * ROWS)
     {
-        const long long i = base + lane;
-        const int count = (int)(ve - base < 64 ? ve - base : 64);
-        double t0 = 0.0, t1 = 0.0;
-        if (lane < count)
+        const int n = (int)(ve - base < 64 * ROWS ? ve - base : 64 * ROWS);
+        double t0[ROWS], t1[ROWS];
+#pragma unroll
+        for (int u = 0; u < ROWS; u++)
         {
-            t0 = x0[i] * y0[i];
-            if (x1)
-                t1 = x1[i] * y1[i];
+            const long long i = base + u * 64 + lane;
+            t0[u] = i < ve ? x0[i] * y0[i] : 0.0;
+            t1[u] = (x1 && i < ve) ? x1[i] * y1[i] : 0.0;
         }
-        if (a.exact)
+        if (!a.exact)
         {
-            s0 = fold_lanes(s0, t0, count);
-            if (x1)
-                s1 = fold_lanes(s1, t1, count);
+#pragma unroll
+            for (int u = 0; u < ROWS; u++)
+            {
+                s0 += tree_lanes(t0[u]);
+                s1 += tree_lanes(t1[u]);
+            }
+            continue;
         }
-        else
+#pragma unroll
+        for (int u = 0; u < ROWS; u++)
         {
-            s0 += tree_lanes(t0);
-            if (x1)
-                s1 += tree_lanes(t1);
+            buf[0][u * 64 + lane] = t0[u];
+            buf[1][u * 64 + lane] = t1[u];
         }
+        __syncthreads();
+        if (lane < 2)
+            acc = fold_serial16(buf[lane], n, acc);
+        __syncthreads();
+    }
+    if (a.exact)
+    {
+        s0 = __shfl(acc, 0);
+        s1 = __shfl(acc, 1);
     }
     if (lane == 0)
     {
@@ -1196,21 +1274,49 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     a.nin = c.nin;
     a.nout = c.nout;
     a.k = k;
+    a.site_f = c.site_f.p;
+    a.site_d = c.site_d.p;
+    a.sites.n = c.planes.n;
+    long long acc = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        a.sites.start[p] = acc;
+        acc += c.g(p).n0 * (1 + k);
+    }
+    a.sites.start[c.planes.n] = acc;
+    const int cs = level0 ? c.c1 : c.c0;   // the other latent's channels: the sample each site stores
+    if (k > 4 || (size_t)acc * (size_t)std::max(cs, L.c) > c.site_f.n)
+        throw Refused { "assembly: the site buffer holds K <= 4" };
     const long long n = L.texels.start[L.texels.n];
     {
-        // The three planes read once and the stencil and gradient written: 8 (5 c^2 + c) bytes per texel.
+        // Pass 1 reads the three planes once and writes (4 cs + 8 c) bytes per site.
         const double in = 4.0 * c.n0_total * (c.nout + c.c0) + 4.0 * c.n1_total * c.c1;
-        const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
-        KernelTimer kt(c.stream, level0 ? "k_assemble0" : "k_assemble1", in + out, 0.0);
+        KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
+                       0.0);
         if (level0)
-            hipLaunchKernelGGL(k_assemble0, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, a);
+            hipLaunchKernelGGL(k_site_terms<true>, dim3(blocks_for(acc, 256)), dim3(256), 0, c.stream, a);
         else
-            hipLaunchKernelGGL(k_assemble1, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, a);
+            hipLaunchKernelGGL(k_site_terms<false>, dim3(blocks_for(acc, 256)), dim3(256), 0, c.stream, a);
+    }
+    HIP_OK(hipGetLastError());
+    {
+        // Pass 2 reads every site's terms once per texel it reaches (4 for level 1's sites, about 3.4 for level 0's)
+        // and writes the stencil and gradient, 8 (5 c^2 + c) bytes per texel; the algorithmic traffic counts each
+        // site once.
+        const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
+        KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
+        if (level0)
+            launch_gather<true>(L.c, cs, dim3(blocks_for(n, 128)), c.stream, a);
+        else
+            launch_gather<false>(L.c, cs, dim3(blocks_for(n, 128)), c.stream, a);
     }
     HIP_OK(hipGetLastError());
     ChunkArgs ch = make_chunks(c, L);
     const long long nch = ch.chunks.start[ch.chunks.n];
-    hipLaunchKernelGGL(k_diag_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+    {
+        KernelTimer kt(c.stream, "k_diag", 8.0 * (double)n * L.c, 0.0);
+        hipLaunchKernelGGL(k_diag_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+    }
     hipLaunchKernelGGL(k_diag_fold, dim3(1), dim3(64), 0, c.stream, ch, ridge);
     HIP_OK(hipGetLastError());
 }
@@ -1228,9 +1334,14 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     for (int p = 0; p < np; p++)
         HIP_OK(hipMemcpyAsync(c.scalars.p + (size_t)p * SC_COUNT + SC_ACTIVE, &one[(size_t)p], sizeof(double),
                               hipMemcpyHostToDevice, c.stream));
+    // Two inner products per chunk, into the chunk partials: one wavefront per chunk, folded serially in index order
+    // (the exact form) or by a wavefront tree (the fast form).
+    auto dot = [&](const double* x0, const double* y0, const double* x1, const double* y1, double bytes) {
+        KernelTimer kt(c.stream, "k_dot", bytes, 0.0);
+        hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, x0, y0, x1, y1);
+    };
     hipLaunchKernelGGL(k_cg_start, dim3(tb), dim3(256), 0, c.stream, L);
-    hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, (const double*)L.r,
-                       (const double*)L.r, (const double*)L.r, (const double*)L.z);
+    dot(L.r, L.r, L.r, L.z, 16.0 * (double)n * L.c);
     hipLaunchKernelGGL(k_fold_start, dim3(1), dim3(64), 0, c.stream, ch);
     HIP_OK(hipGetLastError());
 
@@ -1246,21 +1357,13 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             KernelTimer kt(c.stream, "k_matvec", b_matvec, (double)n * (2.0 * 9.0 * cc * cc));
             hipLaunchKernelGGL(k_matvec, dim3(tb), dim3(256), 0, c.stream, L);
         }
-        {
-            KernelTimer kt(c.stream, "k_dot_chunks", 16.0 * nv, 2.0 * nv);
-            hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, (const double*)L.p,
-                               (const double*)L.ap, (const double*)nullptr, (const double*)nullptr);
-        }
+        dot(L.p, L.ap, nullptr, nullptr, 16.0 * nv);
         hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(64), 0, c.stream, ch);
         {
             KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
             hipLaunchKernelGGL(k_cg_update, dim3(tb), dim3(256), 0, c.stream, L);
         }
-        {
-            KernelTimer kt(c.stream, "k_dot_chunks", 16.0 * nv, 4.0 * nv);
-            hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, (const double*)L.r,
-                               (const double*)L.r, (const double*)L.r, (const double*)L.z);
-        }
+        dot(L.r, L.r, L.r, L.z, 16.0 * nv);
         hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(64), 0, c.stream, ch, 0);
         {
             KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);

@@ -172,6 +172,65 @@ NNTC_HD void obj_sample(const float* plane, int w, int h, int c, double u, doubl
         out[k] = w00 * (double)a[k] + w10 * (double)b[k] + w01 * (double)e[k] + w11 * (double)f[k];
 }
 
+// The same two blends with the channel loop unrolled to a compile-time bound and guarded by the runtime count, so the
+// output array is indexed by constants and stays in registers. The arithmetic per channel is unchanged.
+template <int MAXC>
+NNTC_HD void sample_bilinear_r(const float* plane, int w, int c, const Tap& t, float* out)
+{
+    const float* a = plane + ((size_t)t.y0 * w + t.x0) * c;
+    const float* b = plane + ((size_t)t.y0 * w + t.x1) * c;
+    const float* e = plane + ((size_t)t.y1 * w + t.x0) * c;
+    const float* f = plane + ((size_t)t.y1 * w + t.x1) * c;
+    const float w00 = (1.0f - t.fx) * (1.0f - t.fy), w10 = t.fx * (1.0f - t.fy);
+    const float w01 = (1.0f - t.fx) * t.fy, w11 = t.fx * t.fy;
+#pragma unroll
+    for (int k = 0; k < MAXC; k++)
+        if (k < c)
+            out[k] = w00 * a[k] + w10 * b[k] + w01 * e[k] + w11 * f[k];
+}
+
+template <int MAXC>
+NNTC_HD void obj_sample_r(const float* plane, int w, int h, int c, double u, double v, double* out)
+{
+    const double x = u * (double)w - 0.5, y = v * (double)h - 0.5;
+    const double xf = floor(x), yf = floor(y);
+    const double fx = x - xf, fy = y - yf;
+    const int x0 = clamp_index((int)xf, w), x1 = clamp_index((int)xf + 1, w);
+    const int y0 = clamp_index((int)yf, h), y1 = clamp_index((int)yf + 1, h);
+    const float* a = plane + ((size_t)y0 * w + x0) * c;
+    const float* b = plane + ((size_t)y0 * w + x1) * c;
+    const float* e = plane + ((size_t)y1 * w + x0) * c;
+    const float* f = plane + ((size_t)y1 * w + x1) * c;
+    const double w00 = (1.0 - fx) * (1.0 - fy), w10 = fx * (1.0 - fy), w01 = (1.0 - fx) * fy, w11 = fx * fy;
+#pragma unroll
+    for (int k = 0; k < MAXC; k++)
+        if (k < c)
+            out[k] = w00 * (double)a[k] + w10 * (double)b[k] + w01 * (double)e[k] + w11 * (double)f[k];
+}
+
+// One output of the decoder, b_r + W_r phi, with phi = [c_j][s_i][s_i c_j] formed on the fly (in double when T is
+// double, in float when float), the terms added in phi's order.
+template <class T>
+NNTC_HD T decode_row(const float* row, float bias, const T* s, int c0, const T* c, int c1)
+{
+    T acc = (T)bias;
+#pragma unroll
+    for (int q = 0; q < MAX_CH; q++)
+        if (q < c1)
+            acc += (T)row[q] * c[q];
+#pragma unroll
+    for (int i = 0; i < MAX_CH; i++)
+        if (i < c0)
+            acc += (T)row[c1 + i] * s[i];
+#pragma unroll
+    for (int i = 0; i < MAX_CH; i++)
+#pragma unroll
+        for (int q = 0; q < MAX_CH; q++)
+            if (i < c0 && q < c1)
+                acc += (T)row[c1 + c0 + i * c1 + q] * (s[i] * c[q]);
+    return acc;
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // The grid (src/model.h:141-199)
 // ---------------------------------------------------------------------------------------------------------------------

@@ -41,37 +41,30 @@ __device__ __forceinline__ void objective_site(const float* v0, const float* v1,
         subtexel_offset(k_count, j - 1, px, py, dx, dy);
     const double u = ((double)px + 0.5 + (double)dx) / (double)w0;
     const double v = ((double)py + 0.5 + (double)dy) / (double)h0;
+    // Every array below is indexed by compile-time constants (loops unrolled to the caps, guarded by the runtime
+    // counts), so it lives in registers; the operations and their order are the reference's.
     double sv[MAX_CH], c[MAX_CH], target[MAX_NOUT];
-    obj_sample(v1, w1, h1, c1, u, v, c);
+    obj_sample_r<MAX_CH>(v1, w1, h1, c1, u, v, c);
     if (j == 0)
     {
-        for (int i = 0; i < c0; i++)
-            sv[i] = (double)v0[pixel * c0 + i];
-        for (int i = 0; i < nout; i++)
-            target[i] = (double)src[pixel * nout + i];
+#pragma unroll
+        for (int i = 0; i < MAX_CH; i++)
+            if (i < c0)
+                sv[i] = (double)v0[pixel * c0 + i];
+#pragma unroll
+        for (int i = 0; i < MAX_NOUT; i++)
+            if (i < nout)
+                target[i] = (double)src[pixel * nout + i];
     }
     else
     {
-        obj_sample(v0, w0, h0, c0, u, v, sv);
-        obj_sample(src, w0, h0, nout, u, v, target);
+        obj_sample_r<MAX_CH>(v0, w0, h0, c0, u, v, sv);
+        obj_sample_r<MAX_NOUT>(src, w0, h0, nout, u, v, target);
     }
-    double phi[MAX_NIN];
-    int n = 0;
-    for (int q = 0; q < c1; q++)
-        phi[n++] = c[q];
-    for (int i = 0; i < c0; i++)
-        phi[n++] = sv[i];
-    for (int i = 0; i < c0; i++)
-        for (int q = 0; q < c1; q++)
-            phi[n++] = sv[i] * c[q];
-    for (int r = 0; r < nout; r++)
-    {
-        double acc = (double)bias[r];
-        const float* row = weights + (size_t)r * nin;
-        for (int i = 0; i < nin; i++)
-            acc += (double)row[i] * phi[i];
-        e[r] = acc - target[r];
-    }
+#pragma unroll
+    for (int r = 0; r < MAX_NOUT; r++)
+        if (r < nout)
+            e[r] = decode_row<double>(weights + (size_t)r * nin, bias[r], sv, c0, c, c1) - target[r];
     j_out = j;
 }
 
@@ -110,48 +103,68 @@ __global__ __launch_bounds__(OBJ_WAVE) void k_objective(ObjArgs a)
     {
         const long long si = base + lane;
         const int count = (int)(end - base < OBJ_WAVE ? end - base : OBJ_WAVE);
+        double lane_sum[3] = { 0.0, 0.0, 0.0 };
         if (lane < count)
         {
             double e[MAX_NOUT];
             int j;
             objective_site(v0, v1, src, g.w0, g.h0, g.w1, g.h1, a.c0, a.c1, a.nin, nout, a.weights, a.bias, a.k, si,
                            e, j);
-            for (int r = 0; r < nout; r++)
-            {
-                const double ee = e[r] * e[r];
-                term[0][lane * nout + r] = (double)a.cw[r] * e[r] * e[r];
-                term[1][lane * nout + r] = j == 0 ? ee : 0.0;
-                term[2][lane * nout + r] = j == 0 ? 0.0 : ee;
-            }
+#pragma unroll
+            for (int r = 0; r < MAX_NOUT; r++)
+                if (r < nout)
+                {
+                    const double ee = e[r] * e[r];
+                    const double we = (double)a.cw[r] * e[r] * e[r];
+                    if (EXACT)
+                    {
+                        term[0][lane * nout + r] = we;
+                        term[1][lane * nout + r] = j == 0 ? ee : 0.0;
+                        term[2][lane * nout + r] = j == 0 ? 0.0 : ee;
+                    }
+                    else
+                    {
+                        lane_sum[0] += we;
+                        if (j == 0)
+                            lane_sum[1] += ee;
+                        else
+                            lane_sum[2] += ee;
+                    }
+                }
+        }
+        if (!EXACT)
+        {
+            // The fast form: each lane's own outputs, then a wavefront tree.
+            for (int q = 0; q < 3; q++)
+                for (int off = 32; off > 0; off >>= 1)
+                    lane_sum[q] += __shfl_xor(lane_sum[q], off);
+            if (lane < 3)
+                acc += lane_sum[lane == 0 ? 0 : (lane == 1 ? 1 : 2)];
+            continue;
         }
         __syncthreads();
-        if (EXACT)
         {
             // The reference's chunk loop in its own order: site by site, output by output. Adding the 0.0 a lane's
             // array holds where the reference skips the add leaves a non-negative sum's bits unchanged.
             if (lane < 3)
             {
+                // Sixteen terms read ahead of their sixteen dependent additions, so the chain waits on adds only.
                 const double* t = term[lane];
                 const int n = count * nout;
-                for (int i = 0; i < n; i++)
+                int i = 0;
+                for (; i + 16 <= n; i += 16)
+                {
+                    double v[16];
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        v[u] = t[i + u];
+#pragma unroll
+                    for (int u = 0; u < 16; u++)
+                        acc += v[u];
+                }
+                for (; i < n; i++)
                     acc += t[i];
             }
-        }
-        else
-        {
-            // A per-lane partial over this lane's outputs, reduced across the wave at the end.
-            if (lane < count)
-                for (int q = 0; q < 3; q++)
-                {
-                    double s = 0.0;
-                    for (int r = 0; r < nout; r++)
-                        s += term[q][lane * nout + r];
-                    term[q][lane * nout] = s;
-                }
-            __syncthreads();
-            if (lane < 3)
-                for (int s = 0; s < count; s++)
-                    acc += term[lane][s * nout];
         }
         __syncthreads();
     }
