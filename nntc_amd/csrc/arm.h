@@ -168,6 +168,25 @@ struct Ctx
     float ms_since(hipEvent_t a);
 };
 
+// The stencil layout of one latent's chain (see StencilMap).
+inline StencilMap stencil_map(const Ctx& c, bool level0)
+{
+    StencilMap m {};
+    long long acc = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        const PlaneGeom& g = c.planes.p[p];
+        const int w = level0 ? g.w0 : g.w1, h = level0 ? g.h0 : g.h1;
+        for (int col = 0; col < 4; col++)
+        {
+            m.cpos[p][col] = acc;
+            acc += colour_count(w, h, col);
+        }
+    }
+    m.sn = acc;
+    return m;
+}
+
 // ---- launchers (kernel files)
 // objective.hip
 void objective_pass(Ctx& c, int k, double* sums_out /* nplanes * 3 */);

@@ -136,7 +136,8 @@ struct LatentView
     int c;                // channels
     int bits;
     int levels;
-    double* stencil;
+    double* stencil;      // entry e of the texel at position s: stencil[e * sm.sn + s]
+    StencilMap sm;
     double* grad;
     float* precond;
     double* x;
@@ -444,7 +445,8 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
 
     // The five owned blocks expanded from the monomial sums; a zero sum is skipped.
     const long long gt = L.texels.start[p] + ti;
-    double* out_h = L.stencil + gt * (STENCIL_BLOCKS * C * C);
+    const long long sn = L.sm.sn;
+    double* out_h = L.stencil + stencil_pos(L.sm, p, tw_, tx, ty);
 #pragma unroll
     for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
     {
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
         }
 #pragma unroll
         for (int i = 0; i < C * C; i++)
-            out_h[slot * C * C + i] = blk[i];
+            out_h[(slot * C * C + i) * sn] = blk[i];
     }
 #pragma unroll
     for (int i = 0; i < C; i++)
@@ -544,9 +546,11 @@ __global__ __launch_bounds__(64) void k_diag_chunks(ChunkArgs a)
             x[u] = 0.0;
             if (i < end)
             {
-                const long long t = i / c;
+                const long long t = i / c - a.L.texels.start[p];
                 const int q = (int)(i % c);
-                x[u] = a.L.stencil[t * (STENCIL_BLOCKS * c * c) + q * c + q];
+                const int w = a.L.pw[p];
+                const long long s = stencil_pos(a.L.sm, p, w, (int)(t % w), (int)(t / w));
+                x[u] = a.L.stencil[(q * c + q) * a.L.sm.sn + s];
             }
         }
 #pragma unroll
@@ -748,11 +752,14 @@ __global__ __launch_bounds__(256) void k_cg_start(LatentView L)
     const int p = range_plane(L.texels, gt);
     const double lambda = L.scalars[(size_t)p * SC_COUNT + SC_LAMBDA];
     const int c = L.c;
-    const double* dg = L.stencil + gt * (STENCIL_BLOCKS * c * c);
+    const long long t = gt - L.texels.start[p];
+    const int w = L.pw[p];
+    const double* dg = L.stencil + stencil_pos(L.sm, p, w, (int)(t % w), (int)(t / w));
+    const long long sn = L.sm.sn;
     double a[MAX_CH * MAX_CH], inv[MAX_CH * MAX_CH];
     for (int i = 0; i < c; i++)
         for (int j = 0; j < c; j++)
-            a[i * c + j] = dg[i * c + j] + (i == j ? lambda : 0.0);
+            a[i * c + j] = dg[(i * c + j) * sn] + (i == j ? lambda : 0.0);
     if (!invert_block(a, c, inv))
         for (int i = 0; i < c; i++)
             for (int j = 0; j < c; j++)
@@ -772,6 +779,7 @@ __global__ __launch_bounds__(256) void k_cg_start(LatentView L)
 
 // y_t = (H_tt + lambda I) x_t + the eight neighbours' blocks, the four the texel does not own read as the transpose of
 // the neighbour's; a neighbour outside the plane is absent, not clamped.
+template <int C>
 __global__ __launch_bounds__(256) void k_matvec(LatentView L)
 {
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -783,21 +791,27 @@ __global__ __launch_bounds__(256) void k_matvec(LatentView L)
         return;
     const double lambda = s[SC_LAMBDA];
     const long long t = gt - L.texels.start[p];
-    const int w = L.pw[p], h = L.ph[p], c = L.c;
+    const int w = L.pw[p], h = L.ph[p];
     const int tx = (int)(t % w), ty = (int)(t / w);
     const long long base = L.texels.start[p];
-    const size_t bstride = (size_t)(STENCIL_BLOCKS * c * c);
+    const long long sn = L.sm.sn;
+    const double* S = L.stencil;
+    const long long pos = stencil_pos(L.sm, p, w, tx, ty);
     const double* x = L.p;
-    double acc[MAX_CH];
-    const double* own = L.stencil + gt * bstride;
-    const double* xt = x + gt * c;
-    for (int i = 0; i < c; i++)
+    double acc[C], xt[C];
+#pragma unroll
+    for (int j = 0; j < C; j++)
+        xt[j] = x[gt * C + j];
+#pragma unroll
+    for (int i = 0; i < C; i++)
     {
         double sacc = lambda * xt[i];
-        for (int j = 0; j < c; j++)
-            sacc += own[i * c + j] * xt[j];
+#pragma unroll
+        for (int j = 0; j < C; j++)
+            sacc += S[(i * C + j) * sn + pos] * xt[j];
         acc[i] = sacc;
     }
+#pragma unroll
     for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
     {
         int dx, dy;
@@ -805,25 +819,36 @@ __global__ __launch_bounds__(256) void k_matvec(LatentView L)
         const int fx = tx + dx, fy = ty + dy;
         if (fx >= 0 && fx < w && fy >= 0 && fy < h)
         {
-            const double* blk = own + (size_t)slot * c * c;
-            const double* xn = x + (base + (long long)fy * w + fx) * c;
-            for (int i = 0; i < c; i++)
-                for (int j = 0; j < c; j++)
-                    acc[i] += blk[i * c + j] * xn[j];
+            const double* xn = x + (base + (long long)fy * w + fx) * C;
+            double xv[C];
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                xv[j] = xn[j];
+#pragma unroll
+            for (int i = 0; i < C; i++)
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    acc[i] += S[(slot * C * C + i * C + j) * sn + pos] * xv[j];
         }
         const int bx = tx - dx, by = ty - dy;
         if (bx >= 0 && bx < w && by >= 0 && by < h)
         {
-            const long long nt = base + (long long)by * w + bx;
-            const double* blk = L.stencil + nt * bstride + (size_t)slot * c * c;
-            const double* xn = x + nt * c;
-            for (int i = 0; i < c; i++)
-                for (int j = 0; j < c; j++)
-                    acc[i] += blk[j * c + i] * xn[j];
+            const long long npos = stencil_pos(L.sm, p, w, bx, by);
+            const double* xn = x + (base + (long long)by * w + bx) * C;
+            double xv[C];
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                xv[j] = xn[j];
+#pragma unroll
+            for (int i = 0; i < C; i++)
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    acc[i] += S[(slot * C * C + j * C + i) * sn + npos] * xv[j];
         }
     }
-    for (int i = 0; i < c; i++)
-        L.ap[gt * c + i] = acc[i];
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        L.ap[gt * C + i] = acc[i];
 }
 
 // Per plane after the curvature fold: p.Ap, then alpha = rz / pAp (zero on non-positive curvature).
@@ -1076,17 +1101,39 @@ __global__ __launch_bounds__(256) void k_copy_prev(LatentView L)
     L.x[e] = 0.0;
 }
 
+template <int C>
+__device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const Ranges& colour, int colour_index,
+                                                    long long gi, int p);
+
 // One colour of one sweep: every texel (tx, ty) of the colour, the neighbours' share of (H delta)_t once, then its
 // channels in turn, each one's exact grid argmin using the values the earlier channels have just taken.
+template <int C>
 __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour, int colour_index,
                                                      unsigned int* changed)
 {
     const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= colour.start[colour.n])
-        return;
-    const int p = range_plane(colour, gi);
+    const bool live = gi < colour.start[colour.n];
+    const int p = range_plane(colour, live ? gi : colour.start[colour.n] - 1);
+    const unsigned int moved = live ? sweep_texel<C>(L, colour, colour_index, gi, p) : 0u;
+    // One atomic per wavefront and plane: the wavefront's lanes of its first lane's plane are summed by shuffles; a
+    // lane of another plane (a wavefront across a plane boundary) adds its own.
+    const int p0 = __shfl(p, 0);
+    unsigned int same = p == p0 ? moved : 0u;
+    for (int off = 32; off > 0; off >>= 1)
+        same += __shfl_xor(same, off);
+    if (threadIdx.x % 64 == 0 && same)
+        atomicAdd(changed + p0, same);
+    if (p != p0 && moved)
+        atomicAdd(changed + p, moved);
+}
+
+// One texel of the sweep's colour: returns 1 per channel whose index changed.
+template <int C>
+__device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const Ranges& colour, int colour_index,
+                                                    long long gi, int p)
+{
     const long long ci = gi - colour.start[p];
-    const int w = L.pw[p], h = L.ph[p], c = L.c;
+    const int w = L.pw[p], h = L.ph[p];
     const int cx = colour_index & 1, cy = colour_index >> 1;
     const int cw1 = (w - cx + 1) / 2;
     const int tx = cx + 2 * (int)(ci % cw1);
@@ -1094,13 +1141,16 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
     const long long base = L.texels.start[p];
     const long long t = base + (long long)ty * w + tx;
     const double lambda = L.scalars[(size_t)p * SC_COUNT + SC_LAMBDA];
-    const size_t bstride = (size_t)(STENCIL_BLOCKS * c * c);
-    const double* own = L.stencil + t * bstride;
+    const long long sn = L.sm.sn;
+    const double* S = L.stencil;
+    const long long pos = L.sm.cpos[p][colour_index] + ci;
     double* delta = L.x;
 
-    double nb[MAX_CH];
-    for (int i = 0; i < c; i++)
+    double nb[C];
+#pragma unroll
+    for (int i = 0; i < C; i++)
         nb[i] = 0.0;
+#pragma unroll
     for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
     {
         int dx, dy;
@@ -1108,38 +1158,57 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
         const int fx = tx + dx, fy = ty + dy;
         if (fx >= 0 && fx < w && fy >= 0 && fy < h)
         {
-            const double* blk = own + (size_t)slot * c * c;
-            const double* xn = delta + (base + (long long)fy * w + fx) * c;
-            for (int i = 0; i < c; i++)
-                for (int j = 0; j < c; j++)
-                    nb[i] += blk[i * c + j] * xn[j];
+            const double* xn = delta + (base + (long long)fy * w + fx) * C;
+            double xv[C];
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                xv[j] = xn[j];
+#pragma unroll
+            for (int i = 0; i < C; i++)
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    nb[i] += S[(slot * C * C + i * C + j) * sn + pos] * xv[j];
         }
         const int bx = tx - dx, by = ty - dy;
         if (bx >= 0 && bx < w && by >= 0 && by < h)
         {
-            const long long nt = base + (long long)by * w + bx;
-            const double* blk = L.stencil + nt * bstride + (size_t)slot * c * c;
-            const double* xn = delta + nt * c;
-            for (int i = 0; i < c; i++)
-                for (int j = 0; j < c; j++)
-                    nb[i] += blk[j * c + i] * xn[j];
+            const long long npos = stencil_pos(L.sm, p, w, bx, by);
+            const double* xn = delta + (base + (long long)by * w + bx) * C;
+            double xv[C];
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                xv[j] = xn[j];
+#pragma unroll
+            for (int i = 0; i < C; i++)
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    nb[i] += S[(slot * C * C + j * C + i) * sn + npos] * xv[j];
         }
     }
 
+    // The texel's own channels in order, each seeing the ones before it already moved (the deltas kept in registers
+    // are the values the reference reads back from its plane).
+    double dl[C];
+#pragma unroll
+    for (int j = 0; j < C; j++)
+        dl[j] = delta[t * C + j];
     unsigned int moved = 0;
-    for (int q = 0; q < c; q++)
+#pragma unroll
+    for (int q = 0; q < C; q++)
     {
-        const long long idx = t * c + q;
+        const long long idx = t * C + q;
         const float l = L.lo[q], span = L.hi[q] - L.lo[q];
-        const double aq = own[q * c + q] + lambda;
+        const double aq = S[(q * C + q) * sn + pos] + lambda;
         if (!(aq > 0.0) || !(span > 0.0f))
             continue;
         double rest = nb[q];
-        for (int j = 0; j < c; j++)
+#pragma unroll
+        for (int j = 0; j < C; j++)
             if (j != q)
-                rest += own[q * c + j] * delta[t * c + j];
+                rest += S[(q * C + j) * sn + pos] * dl[j];
         const double bq = L.grad[idx] + rest;
-        const double want = (double)L.prev[idx] - bq / aq;
+        const float pv = L.prev[idx];
+        const double want = (double)pv - bq / aq;
         const int kk = level1_index(l, L.hi[q], L.bits, (float)want);
         const float value = level1_value(l, L.hi[q], L.bits, kk);
         if ((int)L.k[idx] != kk)
@@ -1148,10 +1217,10 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
             moved++;
         }
         L.v[idx] = value;
-        delta[idx] = (double)value - (double)L.prev[idx];
+        dl[q] = (double)value - (double)pv;
+        delta[idx] = dl[q];
     }
-    if (moved)
-        atomicAdd(changed + p, moved);
+    return moved;
 }
 
 }   // namespace
@@ -1180,6 +1249,7 @@ static LatentView make_view(Ctx& c, bool level0)
     L.bits = lt.bits;
     L.levels = (1 << lt.bits) - 1;
     L.stencil = lt.stencil.p;
+    L.sm = stencil_map(c, level0);
     L.grad = lt.grad.p;
     L.precond = lt.precond.p;
     L.x = lt.x.p;
@@ -1337,7 +1407,7 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     // Two inner products per chunk, into the chunk partials: one wavefront per chunk, folded serially in index order
     // (the exact form) or by a wavefront tree (the fast form).
     auto dot = [&](const double* x0, const double* y0, const double* x1, const double* y1, double bytes) {
-        KernelTimer kt(c.stream, "k_dot", bytes, 0.0);
+        KernelTimer kt(c.stream, level0 ? "k_dot0" : "k_dot1", bytes, 0.0);
         hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, x0, y0, x1, y1);
     };
     hipLaunchKernelGGL(k_cg_start, dim3(tb), dim3(256), 0, c.stream, L);
@@ -1354,8 +1424,14 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     for (int it = 0; it < CG_MAX_ITERATIONS; it++)
     {
         {
-            KernelTimer kt(c.stream, "k_matvec", b_matvec, (double)n * (2.0 * 9.0 * cc * cc));
-            hipLaunchKernelGGL(k_matvec, dim3(tb), dim3(256), 0, c.stream, L);
+            KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", b_matvec, (double)n * (2.0 * 9.0 * cc * cc));
+            switch (L.c)
+            {
+            case 1: hipLaunchKernelGGL(k_matvec<1>, dim3(tb), dim3(256), 0, c.stream, L); break;
+            case 2: hipLaunchKernelGGL(k_matvec<2>, dim3(tb), dim3(256), 0, c.stream, L); break;
+            case 3: hipLaunchKernelGGL(k_matvec<3>, dim3(tb), dim3(256), 0, c.stream, L); break;
+            default: hipLaunchKernelGGL(k_matvec<4>, dim3(tb), dim3(256), 0, c.stream, L); break;
+            }
         }
         dot(L.p, L.ap, nullptr, nullptr, 16.0 * nv);
         hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(64), 0, c.stream, ch);
@@ -1440,10 +1516,16 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
                 continue;
             // A colour's texels read their own stencil, gradient and plane and their neighbours' corrections.
             const double cc = L.c;
-            KernelTimer kt(c.stream, "k_quant_sweep", (double)n * (40.0 * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc),
-                           0.0);
-            hipLaunchKernelGGL(k_quant_sweep, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, L, colour[col], col,
-                               c.counters.p);
+            KernelTimer kt(c.stream, level0 ? "k_quant_sweep0" : "k_quant_sweep1",
+                           (double)n * (40.0 * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc), 0.0);
+            const dim3 grid(blocks_for(n, 256));
+            switch (L.c)
+            {
+            case 1: hipLaunchKernelGGL(k_quant_sweep<1>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
+            case 2: hipLaunchKernelGGL(k_quant_sweep<2>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
+            case 3: hipLaunchKernelGGL(k_quant_sweep<3>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
+            default: hipLaunchKernelGGL(k_quant_sweep<4>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
+            }
         }
     HIP_OK(hipGetLastError());
 }

@@ -391,4 +391,33 @@ NNTC_HD int range_plane(const Ranges& r, long long g)
     return p;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// The stencil's layout in device memory. Entry e (slot * c * c + i * c + j of a texel's five c x c blocks) of every
+// texel of the chain is one run of `sn` doubles, so a wavefront reading one entry of 64 texels reads one contiguous
+// span. Within a run the texels are grouped by plane, then by colour (x & 1, y & 1) in the order the quantised sweeps
+// visit them, then row-major inside the colour: a sweep over one colour reads each entry as a contiguous run, and
+// the neighbours a texel reads (all of other colours) sit at consecutive positions for consecutive texels.
+// ---------------------------------------------------------------------------------------------------------------------
+
+struct StencilMap
+{
+    long long sn;                       // texels of the chain: the stride between entries
+    long long cpos[MAX_PLANES][4];      // the first position of colour (cy * 2 + cx) of each plane
+};
+
+// Texels of colour (cx, cy) in a w x h plane.
+NNTC_HD long long colour_count(int w, int h, int colour)
+{
+    const int cx = colour & 1, cy = colour >> 1;
+    return (long long)((w - cx + 1) / 2) * ((h - cy + 1) / 2);
+}
+
+// The position of texel (tx, ty) of plane p (width w) within each entry's run.
+NNTC_HD long long stencil_pos(const StencilMap& m, int p, int w, int tx, int ty)
+{
+    const int cx = tx & 1, cy = ty & 1;
+    const int cw = (w - cx + 1) >> 1;
+    return m.cpos[p][cy * 2 + cx] + (long long)(ty >> 1) * cw + (tx >> 1);
+}
+
 }   // namespace nntc

@@ -5,12 +5,11 @@
 //           smallest byte as the endpoints (eight-value mode), each texel's nearest palette entry as its selector, one
 //           thread per block-channel (integer, bit-exact);
 //   decode  every texel's plane value, index and correction from its block's bytes;
-//   refine  one workgroup per 4x4 block of one colour of the block grid (blocks two apart share no site): the block's
-//           exact quadratic gathered from block (c')'s stencil with every texel outside it held, each channel's
+//   refine  one lane per 4x4 block of one colour of the block grid (blocks two apart share no site): the block's
+//           exact quadratic taken from block (c')'s stencil with every texel outside it held, each channel's
 //           endpoints by a least-squares step and nine integer candidates, then two Gauss-Seidel passes of exact
-//           8-way selector argmins, every step taken only on a strict decrease. The parallel parts (the gather, the
-//           row products, the updates) are one thread per texel or row; every sum that feeds a decision is folded by
-//           one thread over the sixteen texels in increasing order, as the reference CPU backend folds it;
+//           8-way selector argmins, every step taken only on a strict decrease; every sum that feeds a decision is
+//           folded over the sixteen texels in increasing order, as the reference CPU backend folds it;
 //   pack    the endpoints and selectors written as the .dds's 8-byte BC4 blocks, and the pack's squared error in byte
 //           units against the index plane the host mirrors (a report figure).
 
@@ -35,7 +34,8 @@ struct BcArgs
     uint8_t* k0;
     double* delta;
     const float* prev;
-    const double* stencil;
+    const double* stencil;  // level 0's, entry e of position s at stencil[e * sm.sn + s]
+    StencilMap sm;
     const double* grad;
     const float* lo0;
     const float* hi0;
@@ -127,23 +127,130 @@ __global__ __launch_bounds__(256) void k_bc_decode(BcArgs a, long long total)
     }
 }
 
-constexpr int RB_THREADS = 64;
-constexpr int RB_MAXN = 16 * MAX_CH;
+// ---------------------------------------------------------------------------------------------------------------------
+// The refinement, one LANE per 4x4 block (src/cpu/cpu_refine_bc.cpp:138-384)
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// The reference refines a block with a chain of small serial decisions (a least-squares endpoint step, nine candidates,
+// then 2 x 16 x c0 selector argmins, each one seeing the moves before it). Spread over a workgroup, every decision is
+// one thread's work between two barriers; here a lane carries one block through the whole chain and a wavefront
+// advances 64 blocks' chains at once, with no barrier. The lane keeps the block's two n-vectors (a1: the linear term
+// from the texels around the block; uv: the quadratic applied to the block's own corrections, kept current as the block
+// moves) in its own column of LDS, and reads the block's quadratic a2 straight from block (c')'s stencil when
+// a step needs an entry: entry (texel u, channel i; texel v, channel j) of a2 is one entry of a stencil block of u or
+// of v (the transpose of v's when v owns the pair), zero unless u and v are neighbours in the plane.
+//
+// Every sum folds its terms in the reference's order. The reference's sums run over all sixteen texels of the block;
+// the terms skipped here are products with a zero entry of a2, which add a zero to a sum that starts at +0.0 - a sum
+// such a zero cannot change.
 
-// k_bc_refine: one block of one colour per workgroup (src/cpu/cpu_refine_bc.cpp:138-384).
-__global__ __launch_bounds__(RB_THREADS) void k_bc_refine(BcArgs a, Ranges colour, int colour_index)
+// Texel (lx, ly) of block (gx, gy), lx and ly in -1..4, in the stencil's runs: base[col] + (ly >> 1) cw[col] + (lx >> 1).
+struct BlockPos
 {
-    __shared__ double a2[RB_MAXN * RB_MAXN];
-    __shared__ double a1[RB_MAXN], dv[RB_MAXN], uv[RB_MAXN], dlt[16];
-    __shared__ double t5[5][16];
-    __shared__ int s_ep[2 * MAX_CH];
-    __shared__ unsigned char s_sel[16 * MAX_CH];
-    __shared__ int s_new[3];          // new0, new1, accept
-    __shared__ double s_move;
-    __shared__ int s_which;
-    __shared__ int s_i;
+    // The colour's base as b0 + ox d1 + oy d2 + ox oy d3 and its row length as cw0 - ox (cw0 - cw1): arithmetic, not a
+    // table or a select between fields, which the compiler turns into an indexed load from scratch when the colour is
+    // not a constant.
+    long long b0, d1, d2, d3;
+    int cw0, cw1;
+    __device__ __forceinline__ void init(const long long* cpos, int w, int gx, int gy)
+    {
+        cw0 = (w + 1) >> 1;
+        cw1 = w >> 1;
+        const long long c0 = cpos[0] + (long long)(2 * gy) * cw0 + 2 * gx;
+        const long long c1 = cpos[1] + (long long)(2 * gy) * cw1 + 2 * gx;
+        const long long c2 = cpos[2] + (long long)(2 * gy) * cw0 + 2 * gx;
+        const long long c3 = cpos[3] + (long long)(2 * gy) * cw1 + 2 * gx;
+        b0 = c0;
+        d1 = c1 - c0;
+        d2 = c2 - c0;
+        d3 = c3 - c2 - c1 + c0;
+    }
+    __device__ __forceinline__ long long at(int lx, int ly) const
+    {
+        const long long ox = lx & 1, oy = ly & 1;
+        const long long b = b0 + ox * d1 + oy * d2 + (ox & oy) * d3;
+        const long long w = cw0 - ox * (cw0 - cw1);
+        return b + (long long)(ly >> 1) * w + (lx >> 1);
+    }
+};
 
-    const long long gi = blockIdx.x;
+template <int C0>
+struct RefineLane
+{
+    const double* S;
+    long long sn;
+    BlockPos bp;
+    int px0, py0, w0, h0;   // the block's first texel and the plane's size
+
+    __device__ __forceinline__ bool inside(int lx, int ly) const
+    {
+        const int px = px0 + lx, py = py0 + ly;
+        return px >= 0 && px < w0 && py >= 0 && py < h0;
+    }
+
+    // Entry e of the stencil block `slot` of texel (lx, ly).
+    __device__ __forceinline__ double entry(int lx, int ly, int slot, int e) const
+    {
+        return S[(long long)(slot * C0 * C0 + e) * sn + bp.at(lx, ly)];
+    }
+
+    // a2's entry (texel (ux, uy), channel i; texel (vx, vy), channel j) for two texels of the block that are equal or
+    // 8-neighbours: the forward slots (1, 0), (-1, 1), (0, 1), (1, 1) are u's blocks, the other four v's, transposed.
+    __device__ __forceinline__ double a2(int ux, int uy, int i, int vx, int vy, int j) const
+    {
+        if (!inside(ux, uy) || !inside(vx, vy))
+            return 0.0;
+        const int dx = vx - ux, dy = vy - uy;
+        if (dx == 0 && dy == 0)
+            return entry(ux, uy, 0, i * C0 + j);
+        if (dy == 1 || (dy == 0 && dx == 1))
+            return entry(ux, uy, dy == 0 ? 1 : dx + 3, i * C0 + j);
+        return entry(vx, vy, dy == 0 ? 1 : 3 - dx, j * C0 + i);
+    }
+};
+
+__device__ __forceinline__ int sel_get(unsigned long long bits, int t)
+{
+    return (int)((bits >> (3 * t)) & 7ull);
+}
+
+__device__ __forceinline__ unsigned long long sel_set(unsigned long long bits, int t, int v)
+{
+    return (bits & ~(7ull << (3 * t))) | ((unsigned long long)v << (3 * t));
+}
+
+// The plane value of selector s of a block with endpoints r0 > r1 (eight-value mode), computed rather than looked up
+// (a table indexed by a lane's own selector would live in scratch).
+__device__ __forceinline__ float level0_value(float lo, float hi, int r0, int r1, int s)
+{
+    return bc0_value(lo, hi, bc4_entry8(r0, r1, s));
+}
+
+// The texels of the block that are texel (x, y) itself or its 8-neighbours, in increasing index order.
+#define FOR_BLOCK_NEIGHBOURS(x, y, nx, ny, BODY)                                                                       \
+    _Pragma("unroll") for (int dy_ = -1; dy_ <= 1; dy_++)                                                              \
+        _Pragma("unroll") for (int dx_ = -1; dx_ <= 1; dx_++)                                                          \
+        {                                                                                                              \
+            const int nx = (x) + dx_, ny = (y) + dy_;                                                                  \
+            if (nx >= 0 && nx < 4 && ny >= 0 && ny < 4)                                                                \
+            {                                                                                                          \
+                BODY                                                                                                   \
+            }                                                                                                          \
+        }
+
+template <int C0>
+__global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int colour_index)
+{
+    constexpr int N = 16 * C0;
+    // The block's two n-vectors live in LDS, one column per lane (entry i of lane l at [i * 64 + l]): no lane reads
+    // another's column, so no barrier is needed.
+    __shared__ double s_a1[N * 64], s_uv[N * 64];
+    double* a1 = s_a1 + threadIdx.x;
+    double* uv = s_uv + threadIdx.x;
+
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= colour.start[colour.n])
+        return;
     const int p = range_plane(colour, gi);
     const PlaneGeom& g = a.planes.p[p];
     const long long ci = gi - colour.start[p];
@@ -152,294 +259,289 @@ __global__ __launch_bounds__(RB_THREADS) void k_bc_refine(BcArgs a, Ranges colou
     const int cbx = (bxn - cx + 1) / 2;
     const int gx = cx + 2 * (int)(ci % cbx);
     const int gy = cy + 2 * (int)(ci / cbx);
-    const int w0 = g.w0, h0 = g.h0, c0 = a.c0;
-    const int n = 16 * c0;
-    const int tid = threadIdx.x;
+    const int w0 = g.w0;
     const long long base = g.off0;
-    const double* stencil = a.stencil;
-    const size_t bstride = (size_t)(STENCIL_BLOCKS * c0 * c0);
 
-    for (int i = tid; i < n * n; i += RB_THREADS)
-        a2[i] = 0.0;
-    for (int i = tid; i < n; i += RB_THREADS)
-    {
-        a1[i] = 0.0;
-        dv[i] = 0.0;
-        uv[i] = 0.0;
-    }
-    __syncthreads();
+    RefineLane<C0> R;
+    R.S = a.stencil;
+    R.sn = a.sm.sn;
+    R.px0 = gx * 4;
+    R.py0 = gy * 4;
+    R.w0 = w0;
+    R.h0 = g.h0;
+    R.bp.init(a.sm.cpos[p], w0, gx, gy);
+    // The chain index of texel (lx, ly) of the block (or of a neighbour just outside it).
+    auto tix = [=](int lx, int ly) { return base + (long long)(R.py0 + ly) * w0 + (R.px0 + lx); };
 
-    // ---- the block's quadratic, one thread per texel; every entry of a2 receives at most one addition.
-    if (tid < 16)
+    // ---- a1: each texel's gradient plus the shares of its neighbours outside the block, slot by slot, the forward
+    // neighbour's then the backward one's (slot 0 is the texel itself, inside the block).
+#pragma unroll 1
+    for (int u = 0; u < 16; u++)
     {
-        const int px = gx * 4 + (tid & 3), py = gy * 4 + (tid >> 2);
-        if (px < w0 && py < h0)
+        const int lx = u & 3, ly = u >> 2;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            uv[(u * C0 + i) * 64] = 0.0;
+        if (!R.inside(lx, ly))
         {
-            const long long ti = base + (long long)py * w0 + px;
-            const double* own = stencil + ti * bstride;
-            double acc[MAX_CH];
-            for (int i = 0; i < c0; i++)
-                acc[i] = a.grad[ti * c0 + i];
-            for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                a1[(u * C0 + i) * 64] = 0.0;
+            continue;
+        }
+        const long long ti = tix(lx, ly);
+        double acc[C0];
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            acc[i] = a.grad[ti * C0 + i];
+#pragma unroll
+        for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
+        {
+            int dx, dy;
+            stencil_offset(slot, dx, dy);
+            const int fx = lx + dx, fy = ly + dy;
+            if (!(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && R.inside(fx, fy))
             {
-                int dx, dy;
-                stencil_offset(slot, dx, dy);
-                const int fx = px + dx, fy = py + dy;
-                if (fx >= 0 && fx < w0 && fy >= 0 && fy < h0)
-                {
-                    const double* blk = own + (size_t)slot * c0 * c0;
-                    const int lx = fx - gx * 4, ly = fy - gy * 4;
-                    if (lx >= 0 && lx < 4 && ly >= 0 && ly < 4)
-                    {
-                        const int ft = ly * 4 + lx;
-                        for (int i = 0; i < c0; i++)
-                            for (int j = 0; j < c0; j++)
-                            {
-                                a2[(tid * c0 + i) * n + (ft * c0 + j)] += blk[i * c0 + j];
-                                if (slot != 0)
-                                    a2[(ft * c0 + j) * n + (tid * c0 + i)] += blk[i * c0 + j];
-                            }
-                    }
-                    else
-                    {
-                        const double* xn = a.delta + (base + (long long)fy * w0 + fx) * c0;
-                        for (int i = 0; i < c0; i++)
-                            for (int j = 0; j < c0; j++)
-                                acc[i] += blk[i * c0 + j] * xn[j];
-                    }
-                }
-                if (slot == 0)
-                    continue;
-                const int ax = px - dx, ay = py - dy;
-                const int lx = ax - gx * 4, ly = ay - gy * 4;
-                if (ax < 0 || ax >= w0 || ay < 0 || ay >= h0 || (lx >= 0 && lx < 4 && ly >= 0 && ly < 4))
-                    continue;
-                const long long nt = base + (long long)ay * w0 + ax;
-                const double* blk = stencil + nt * bstride + (size_t)slot * c0 * c0;
-                const double* xn = a.delta + nt * c0;
-                for (int i = 0; i < c0; i++)
-                    for (int j = 0; j < c0; j++)
-                        acc[i] += blk[j * c0 + i] * xn[j];
+                const double* xn = a.delta + tix(fx, fy) * C0;
+                double xv[C0];
+#pragma unroll
+                for (int j = 0; j < C0; j++)
+                    xv[j] = xn[j];
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+#pragma unroll
+                    for (int j = 0; j < C0; j++)
+                        acc[i] += R.entry(lx, ly, slot, i * C0 + j) * xv[j];
             }
-            for (int i = 0; i < c0; i++)
+            const int bx = lx - dx, by = ly - dy;
+            if (!(bx >= 0 && bx < 4 && by >= 0 && by < 4) && R.inside(bx, by))
             {
-                a1[tid * c0 + i] = acc[i];
-                dv[tid * c0 + i] = (double)a.v0[ti * c0 + i] - (double)a.prev[ti * c0 + i];
+                const double* xn = a.delta + tix(bx, by) * C0;
+                double xv[C0];
+#pragma unroll
+                for (int j = 0; j < C0; j++)
+                    xv[j] = xn[j];
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+#pragma unroll
+                    for (int j = 0; j < C0; j++)
+                        acc[i] += R.entry(bx, by, slot, j * C0 + i) * xv[j];
             }
         }
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            a1[(u * C0 + i) * 64] = acc[i];
     }
-    __syncthreads();
+
+    // ---- uv = a2 dv, scattered a column at a time in increasing column order, so every row receives its terms in
+    // the order the reference's row sums add them.
+#pragma unroll 1
+    for (int v = 0; v < 16; v++)
+    {
+        const int vx = v & 3, vy = v >> 2;
+        if (!R.inside(vx, vy))
+            continue;
+        const long long tv = tix(vx, vy);
+#pragma unroll
+        for (int j = 0; j < C0; j++)
+        {
+            const double dv = (double)a.v0[tv * C0 + j] - (double)a.prev[tv * C0 + j];
+            FOR_BLOCK_NEIGHBOURS(vx, vy, ux, uy, {
+                _Pragma("unroll")
+                for (int i = 0; i < C0; i++)
+                    uv[((uy * 4 + ux) * C0 + i) * 64] += R.a2(ux, uy, i, vx, vy, j) * dv;
+            })
+        }
+    }
 
     const long long gb = g.offb + (long long)gy * bxn + gx;
-    if (tid < n)
+    int ep0[C0], ep1[C0];
+    unsigned long long sel[C0];
+#pragma unroll
+    for (int c = 0; c < C0; c++)
     {
-        double s = 0.0;
-        for (int j = 0; j < n; j++)
-            s += a2[tid * n + j] * dv[j];
-        uv[tid] = s;
+        ep0[c] = a.ep[(gb * C0 + c) * 2];
+        ep1[c] = a.ep[(gb * C0 + c) * 2 + 1];
+        const uint8_t* s = a.sel + (gb * C0 + c) * 16;
+        unsigned long long bits = 0;
+#pragma unroll
+        for (int t = 0; t < 16; t++)
+            bits |= (unsigned long long)(s[t] & 7) << (3 * t);
+        sel[c] = bits;
     }
-    if (tid < 2 * c0)
-        s_ep[tid] = a.ep[(gb * c0 + tid / 2) * 2 + (tid & 1)];
-    if (tid < n)
-        s_sel[tid] = a.sel[gb * c0 * 16 + tid];
-    __syncthreads();
-
-    double taken = 0.0;   // thread 0's running total
-    const float* lo0 = a.lo0;
-    const float* hi0 = a.hi0;
+    double taken = 0.0;
 
     // ---- (i) the endpoints of each channel
-    for (int c = 0; c < c0; c++)
+#pragma unroll
+    for (int c = 0; c < C0; c++)
     {
-        const double step = ((double)hi0[c] - (double)lo0[c]) / 255.0;
-        if (tid < 16)
+        const float lo = a.lo0[c], hi = a.hi0[c];
+        const double step = ((double)hi - (double)lo) / 255.0;
+        double bp = 0.0, bq = 0.0, pap = 0.0, paq = 0.0, qaq = 0.0;
+#pragma unroll 1
+        for (int u = 0; u < 16; u++)
         {
-            const int idx = tid * c0 + c;
-            const double aw = bc4_weight8(s_sel[c * 16 + tid]);
+            const int ux = u & 3, uy = u >> 2;
+            const int idx = u * C0 + c;
+            const double aw = bc4_weight8(sel_get(sel[c], u));
             const double pp = step * aw;
             const double qq = step * (1.0 - aw);
-            const double b = a1[idx] + uv[idx];
+            const double b = a1[idx * 64] + uv[idx * 64];
             double ap = 0.0, aq = 0.0;
-            for (int j = 0; j < 16; j++)
-            {
-                const double e = a2[idx * n + (j * c0 + c)];
-                const double aj = bc4_weight8(s_sel[c * 16 + j]);
+            FOR_BLOCK_NEIGHBOURS(ux, uy, vx, vy, {
+                const double e = R.a2(ux, uy, c, vx, vy, c);
+                const double aj = bc4_weight8(sel_get(sel[c], vy * 4 + vx));
                 ap += e * step * aj;
                 aq += e * step * (1.0 - aj);
-            }
-            t5[0][tid] = b * pp;
-            t5[1][tid] = b * qq;
-            t5[2][tid] = pp * ap;
-            t5[3][tid] = pp * aq;
-            t5[4][tid] = qq * aq;
+            })
+            bp += b * pp;
+            bq += b * qq;
+            pap += pp * ap;
+            paq += pp * aq;
+            qaq += qq * aq;
         }
-        __syncthreads();
-        if (tid == 0)
+        const int r0 = ep0[c], r1 = ep1[c];
+        const double det = pap * qaq - paq * paq;
+        double d0 = 0.0, d1 = 0.0;
+        if (det > 1e-12 * pap * qaq)
         {
-            double bp = 0.0, bq = 0.0, pap = 0.0, paq = 0.0, qaq = 0.0;
-            for (int t = 0; t < 16; t++)
+            d0 = (paq * bq - qaq * bp) / det;
+            d1 = (paq * bp - pap * bq) / det;
+        }
+        const double v0r = (double)r0 + d0, v1r = (double)r1 + d1;
+        const int c0r = (int)floor(v0r + 0.5), c1r = (int)floor(v1r + 0.5);
+        double best = 0.0;
+        int best0 = r0, best1 = r1;
+#pragma unroll
+        for (int e0 = -1; e0 <= 1; e0++)
+#pragma unroll
+            for (int e1 = -1; e1 <= 1; e1++)
             {
-                bp += t5[0][t];
-                bq += t5[1][t];
-                pap += t5[2][t];
-                paq += t5[3][t];
-                qaq += t5[4][t];
-            }
-            const int r0 = s_ep[2 * c], r1 = s_ep[2 * c + 1];
-            const double det = pap * qaq - paq * paq;
-            double d0 = 0.0, d1 = 0.0;
-            if (det > 1e-12 * pap * qaq)
-            {
-                d0 = (paq * bq - qaq * bp) / det;
-                d1 = (paq * bp - pap * bq) / det;
-            }
-            const double v0r = (double)r0 + d0, v1r = (double)r1 + d1;
-            const int c0r = (int)floor(v0r + 0.5), c1r = (int)floor(v1r + 0.5);
-            double best = 0.0;
-            int best0 = r0, best1 = r1;
-            for (int e0 = -1; e0 <= 1; e0++)
-                for (int e1 = -1; e1 <= 1; e1++)
+                const int x0 = c0r + e0 < 0 ? 0 : (c0r + e0 > 255 ? 255 : c0r + e0);
+                const int x1 = c1r + e1 < 0 ? 0 : (c1r + e1 > 255 ? 255 : c1r + e1);
+                if (x0 <= x1)
+                    continue;
+                const double f0 = (double)(x0 - r0), f1 = (double)(x1 - r1);
+                const double de = 2.0 * (bp * f0 + bq * f1) + pap * f0 * f0 + 2.0 * paq * f0 * f1 + qaq * f1 * f1;
+                if (de < best)
                 {
-                    const int a0 = c0r + e0 < 0 ? 0 : (c0r + e0 > 255 ? 255 : c0r + e0);
-                    const int a1i = c1r + e1 < 0 ? 0 : (c1r + e1 > 255 ? 255 : c1r + e1);
-                    if (a0 <= a1i)
-                        continue;
-                    const double f0 = (double)(a0 - r0), f1 = (double)(a1i - r1);
-                    const double de =
-                        2.0 * (bp * f0 + bq * f1) + pap * f0 * f0 + 2.0 * paq * f0 * f1 + qaq * f1 * f1;
-                    if (de < best)
-                    {
-                        best = de;
-                        best0 = a0;
-                        best1 = a1i;
-                    }
+                    best = de;
+                    best0 = x0;
+                    best1 = x1;
                 }
-            s_new[0] = best0;
-            s_new[1] = best1;
-            s_new[2] = (best0 == s_ep[2 * c] && best1 == s_ep[2 * c + 1]) ? 0 : 1;
-        }
-        __syncthreads();
-        if (!s_new[2])
+            }
+        if (best0 == r0 && best1 == r1)
             continue;
-        const int new0 = s_new[0], new1 = s_new[1];
-        // The candidate weighed again in the arithmetic the plane is written in.
-        if (tid < 16)
+        // The candidate weighed again in the arithmetic the plane is written in: texel v moves by
+        // next(v) - now(v), the two palettes' values at its selector.
+        auto dlt = [=](int v) {
+            const int s = sel_get(sel[c], v);
+            return (double)level0_value(lo, hi, best0, best1, s) - (double)level0_value(lo, hi, r0, r1, s);
+        };
+        double exact = 0.0;
+#pragma unroll 1
+        for (int u = 0; u < 16; u++)
         {
-            const float now = bc0_value(lo0[c], hi0[c], bc4_entry8(s_ep[2 * c], s_ep[2 * c + 1], s_sel[c * 16 + tid]));
-            const float next = bc0_value(lo0[c], hi0[c], bc4_entry8(new0, new1, s_sel[c * 16 + tid]));
-            dlt[tid] = (double)next - (double)now;
-        }
-        __syncthreads();
-        if (tid < 16)
-        {
-            const int idx = tid * c0 + c;
+            const int ux = u & 3, uy = u >> 2;
+            const int idx = u * C0 + c;
             double quad = 0.0;
-            for (int j = 0; j < 16; j++)
-                quad += a2[idx * n + (j * c0 + c)] * dlt[j];
-            t5[0][tid] = dlt[tid] * (2.0 * (a1[idx] + uv[idx]) + quad);
+            FOR_BLOCK_NEIGHBOURS(ux, uy, vx, vy, { quad += R.a2(ux, uy, c, vx, vy, c) * dlt(vy * 4 + vx); })
+            exact += dlt(u) * (2.0 * (a1[idx * 64] + uv[idx * 64]) + quad);
         }
-        __syncthreads();
-        if (tid == 0)
+        if (!(exact < 0.0))
+            continue;
+        taken += exact;
+#pragma unroll 1
+        for (int u = 0; u < 16; u++)
         {
-            double exact = 0.0;
-            for (int t = 0; t < 16; t++)
-                exact += t5[0][t];
-            s_new[2] = exact < 0.0 ? 1 : 0;
-            if (exact < 0.0)
-                taken += exact;
-        }
-        __syncthreads();
-        if (s_new[2])
-        {
-            if (tid < n)
+            const int ux = u & 3, uy = u >> 2;
+#pragma unroll
+            for (int i = 0; i < C0; i++)
             {
                 double add = 0.0;
-                for (int j = 0; j < 16; j++)
-                    add += a2[tid * n + (j * c0 + c)] * dlt[j];
-                uv[tid] += add;
+                FOR_BLOCK_NEIGHBOURS(ux, uy, vx, vy, { add += R.a2(ux, uy, i, vx, vy, c) * dlt(vy * 4 + vx); })
+                uv[(u * C0 + i) * 64] += add;
             }
-            __syncthreads();
-            if (tid == 0)
-            {
-                s_ep[2 * c] = new0;
-                s_ep[2 * c + 1] = new1;
-            }
-            if (tid < 16)
-                dv[tid * c0 + c] += dlt[tid];
         }
-        __syncthreads();
+        ep0[c] = best0;
+        ep1[c] = best1;
     }
 
     // ---- (ii) the selectors, two Gauss-Seidel passes of exact 8-way argmins
+    float pal[C0][8];
+#pragma unroll
+    for (int c = 0; c < C0; c++)
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            pal[c][k] = bc0_value(a.lo0[c], a.hi0[c], bc4_entry8(ep0[c], ep1[c], k));
+#pragma unroll 1
     for (int pass = 0; pass < 2; pass++)
-        for (int c = 0; c < c0; c++)
+#pragma unroll
+        for (int c = 0; c < C0; c++)
+#pragma unroll 1
             for (int t = 0; t < 16; t++)
             {
-                if (tid == 0)
+                const int tx = t & 3, ty = t >> 2;
+                const int i = t * C0 + c;
+                const double b = a1[i * 64] + uv[i * 64], aa = R.a2(tx, ty, c, tx, ty, c);
+                const int s = sel_get(sel[c], t);
+                const double now = (double)level0_value(a.lo0[c], a.hi0[c], ep0[c], ep1[c], s);
+                double best = 0.0, move = 0.0;
+                int which = s;
+#pragma unroll
+                for (int kk = 0; kk < 8; kk++)
                 {
-                    const int i = t * c0 + c;
-                    const double b = a1[i] + uv[i], aa = a2[i * n + i];
-                    const int r0 = s_ep[2 * c], r1 = s_ep[2 * c + 1];
-                    const double now = (double)bc0_value(lo0[c], hi0[c], bc4_entry8(r0, r1, s_sel[c * 16 + t]));
-                    double best = 0.0, move = 0.0;
-                    int which = s_sel[c * 16 + t];
-                    for (int kk = 0; kk < 8; kk++)
+                    const double d = (double)pal[c][kk] - now;
+                    const double de = 2.0 * b * d + aa * d * d;
+                    if (de < best)
                     {
-                        const double d = (double)bc0_value(lo0[c], hi0[c], bc4_entry8(r0, r1, kk)) - now;
-                        const double de = 2.0 * b * d + aa * d * d;
-                        if (de < best)
-                        {
-                            best = de;
-                            move = d;
-                            which = kk;
-                        }
-                    }
-                    s_i = best < 0.0 ? i : -1;
-                    s_move = move;
-                    s_which = which;
-                    if (best < 0.0)
-                        taken += best;
-                }
-                __syncthreads();
-                const int i = s_i;
-                if (i >= 0)
-                {
-                    if (tid < n)
-                        uv[tid] += a2[tid * n + i] * s_move;
-                    __syncthreads();
-                    if (tid == 0)
-                    {
-                        s_sel[c * 16 + t] = (unsigned char)s_which;
-                        dv[i] += s_move;
+                        best = de;
+                        move = d;
+                        which = kk;
                     }
                 }
-                __syncthreads();
+                if (best < 0.0)
+                {
+                    taken += best;
+                    FOR_BLOCK_NEIGHBOURS(tx, ty, ux, uy, {
+                        _Pragma("unroll")
+                        for (int ii = 0; ii < C0; ii++)
+                            uv[((uy * 4 + ux) * C0 + ii) * 64] += R.a2(ux, uy, ii, tx, ty, c) * move;
+                    })
+                    sel[c] = sel_set(sel[c], t, which);
+                }
             }
 
     // ---- what the block leaves behind
-    if (tid < 2 * c0)
-        a.ep[(gb * c0 + tid / 2) * 2 + (tid & 1)] = (uint8_t)s_ep[tid];
-    if (tid < n)
-        a.sel[gb * c0 * 16 + tid] = s_sel[tid];
-    if (tid < 16)
+#pragma unroll
+    for (int c = 0; c < C0; c++)
     {
-        const int px = gx * 4 + (tid & 3), py = gy * 4 + (tid >> 2);
-        if (px < w0 && py < h0)
+        a.ep[(gb * C0 + c) * 2] = (uint8_t)ep0[c];
+        a.ep[(gb * C0 + c) * 2 + 1] = (uint8_t)ep1[c];
+        uint8_t* s = a.sel + (gb * C0 + c) * 16;
+#pragma unroll
+        for (int t = 0; t < 16; t++)
+            s[t] = (uint8_t)sel_get(sel[c], t);
+    }
+#pragma unroll 1
+    for (int u = 0; u < 16; u++)
+    {
+        const int lx = u & 3, ly = u >> 2;
+        if (!R.inside(lx, ly))
+            continue;
+        const long long ti = tix(lx, ly);
+#pragma unroll
+        for (int c = 0; c < C0; c++)
         {
-            const long long ti = base + (long long)py * w0 + px;
-            for (int c = 0; c < c0; c++)
-            {
-                const float byte = bc4_entry8(s_ep[2 * c], s_ep[2 * c + 1], s_sel[c * 16 + tid]);
-                const float value = bc0_value(lo0[c], hi0[c], byte);
-                const long long idx = ti * c0 + c;
-                a.v0[idx] = value;
-                a.k0[idx] = bc0_index(byte);
-                a.delta[idx] = (double)value - (double)a.prev[idx];
-            }
+            const float byte = bc4_entry8(ep0[c], ep1[c], sel_get(sel[c], u));
+            const float value = bc0_value(a.lo0[c], a.hi0[c], byte);
+            const long long idx = ti * C0 + c;
+            a.v0[idx] = value;
+            a.k0[idx] = bc0_index(byte);
+            a.delta[idx] = (double)value - (double)a.prev[idx];
         }
     }
-    if (tid == 0)
-        a.taken[gb] = taken;
+    a.taken[gb] = taken;
 }
 
 // The .dds blocks of every file of every plane (bc4_pack, shared/bc_pack.h:51-60), and each block's squared error in
@@ -505,6 +607,7 @@ BcArgs bc_args(Ctx& c)
     a.delta = c.lat0.x.p;
     a.prev = c.lat0.prev.p;
     a.stencil = c.lat0.stencil.p;
+    a.sm = stencil_map(c, true);
     a.grad = c.lat0.grad.p;
     a.lo0 = c.grid.p;
     a.hi0 = c.grid.p + 4;
@@ -559,7 +662,14 @@ void bc_refine_pass(Ctx& c, double* taken, double* improved)
         const double c0 = c.c0;
         KernelTimer kt(c.stream, "k_bc_refine", (double)n * 16.0 * (40.0 * c0 * c0 + 8.0 * c0 + 8.0 * c0 + 9.0 * c0),
                        0.0);
-        hipLaunchKernelGGL(k_bc_refine, dim3((unsigned)n), dim3(RB_THREADS), 0, c.stream, a, colour[col], col);
+        const dim3 grid(blocks_for(n, 64));
+        switch (c.c0)
+        {
+        case 1: hipLaunchKernelGGL(k_bc_refine<1>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
+        case 2: hipLaunchKernelGGL(k_bc_refine<2>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
+        case 3: hipLaunchKernelGGL(k_bc_refine<3>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
+        default: hipLaunchKernelGGL(k_bc_refine<4>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
+        }
         HIP_OK(hipGetLastError());
     }
     std::vector<double> t((size_t)c.nb_total);
