@@ -219,8 +219,11 @@ __device__ __forceinline__ void monomials(const float* s, double* mono)
             mono[n++] = (double)s[i] * (double)s[j];
 }
 
-// One site's terms for the level-1 (LEVEL0 false) or the level-0 (LEVEL0 true) assembly.
-template <bool LEVEL0>
+// One site's terms for the level-1 (LEVEL0 false) or the level-0 (LEVEL0 true) assembly, instantiated per channel
+// counts so every per-site array is indexed by constants and lives in registers; the outputs run to the cap under a
+// runtime guard. The sums per output (the residual, then each channel's acc) keep the reference's operand order
+// (src/cpu/cpu_solve_level1.cpp:262-331).
+template <bool LEVEL0, int C0, int C1>
 __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 {
     const long long gs = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -234,88 +237,122 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
     const int j = (int)(si % per_pixel);
     const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
     const int px = (int)(pixel % w0), py = (int)(pixel / w0);
-    const int c0 = a.c0, c1 = a.c1, nout = a.nout, nin = a.nin;
+    const int nout = a.nout, nin = a.nin;
     const float* src = a.src + g.off0 * nout;
-    const float* v0 = a.v0 + g.off0 * c0;
-    const float* v1 = a.v1 + g.off1 * c1;
+    const float* v0 = a.v0 + g.off0 * C0;
+    const float* v1 = a.v1 + g.off1 * C1;
     float dx, dy;
     site_delta(a.k, j, px, py, dx, dy);
     const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
-    float sbar[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
-    sample_bilinear(v1, w1, c1, tp, cbar);
+    float sbar[C0], cbar[C1], target[MAX_NOUT];
+    sample_bilinear_r<C1>(v1, w1, C1, tp, cbar);
     if (j == 0)
     {
         const size_t pi = (size_t)py * w0 + px;
-        for (int i = 0; i < c0; i++)
-            sbar[i] = v0[pi * c0 + i];
-        for (int r = 0; r < nout; r++)
-            target[r] = src[pi * nout + r];
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            sbar[i] = v0[pi * C0 + i];
+#pragma unroll
+        for (int r = 0; r < MAX_NOUT; r++)
+            if (r < nout)
+                target[r] = src[pi * nout + r];
     }
     else
     {
         const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-        sample_bilinear(v0, w0, c0, t0, sbar);
-        sample_bilinear(src, w0, nout, t0, target);
+        sample_bilinear_r<C0>(v0, w0, C0, t0, sbar);
+        sample_bilinear_r<MAX_NOUT>(src, w0, nout, t0, target);
+    }
+    constexpr int C = LEVEL0 ? C0 : C1;   // the solved latent's channels
+    double acc[C];
+#pragma unroll
+    for (int aa = 0; aa < C; aa++)
+        acc[aa] = 0.0;
+    // Per output r: the row's matrix entries for the solved latent (G = A + sum_i sbar_i M_i for level 1,
+    // Q = B + sum_q cbar_q N_q for level 0), the fp32 residual, then each channel's cw_r G_ra res_r added in r order.
+#pragma unroll
+    for (int r = 0; r < MAX_NOUT; r++)
+    {
+        if (r >= nout)
+            continue;
+        const float* row = a.weights + (size_t)r * nin;
+        float out = a.bias[r];
+        float m[C];
+        if (!LEVEL0)
+        {
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                out += row[C1 + i] * sbar[i];
+#pragma unroll
+            for (int q = 0; q < C1; q++)
+            {
+                float gg = row[q];
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                    gg += sbar[i] * row[C1 + C0 + i * C1 + q];
+                m[q] = gg;
+                out += gg * cbar[q];
+            }
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < C1; q++)
+                out += row[q] * cbar[q];
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+            {
+                float qq = row[C1 + i];
+#pragma unroll
+                for (int q = 0; q < C1; q++)
+                    qq += cbar[q] * row[C1 + C0 + i * C1 + q];
+                m[i] = qq;
+                out += qq * sbar[i];
+            }
+        }
+        const float res = out - target[r];
+#pragma unroll
+        for (int aa = 0; aa < C; aa++)
+            acc[aa] += (double)a.cw[r] * (double)m[aa] * (double)res;
     }
     if (!LEVEL0)
     {
-        // G = A + sum_i sbar_i M_i and the residual r = b + B sbar + G cbar - target, in fp32.
-        float gmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
-        for (int r = 0; r < nout; r++)
-        {
-            const float* row = a.weights + (size_t)r * nin;
-            float out = a.bias[r];
-            for (int i = 0; i < c0; i++)
-                out += row[c1 + i] * sbar[i];
-            for (int q = 0; q < c1; q++)
-            {
-                float gg = row[q];
-                for (int i = 0; i < c0; i++)
-                    gg += sbar[i] * row[c1 + c0 + i * c1 + q];
-                gmat[r * c1 + q] = gg;
-                out += gg * cbar[q];
-            }
-            res[r] = out - target[r];
-        }
-        for (int i = 0; i < c0; i++)
-            a.site_f[gs * c0 + i] = sbar[i];
-        for (int aa = 0; aa < c1; aa++)
-        {
-            double acc = 0.0;
-            for (int r = 0; r < nout; r++)
-                acc += (double)a.cw[r] * (double)gmat[r * c1 + aa] * (double)res[r];
-            a.site_d[gs * c1 + aa] = acc;
-        }
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            a.site_f[gs * C0 + i] = sbar[i];
     }
     else
     {
-        // Q = B + sum_q cbar_q N_q and the residual r = b + A cbar + Q sbar - target, in fp32.
-        float qmat[MAX_NOUT * MAX_CH], res[MAX_NOUT];
-        for (int r = 0; r < nout; r++)
-        {
-            const float* row = a.weights + (size_t)r * nin;
-            float out = a.bias[r];
-            for (int q = 0; q < c1; q++)
-                out += row[q] * cbar[q];
-            for (int i = 0; i < c0; i++)
-            {
-                float qq = row[c1 + i];
-                for (int q = 0; q < c1; q++)
-                    qq += cbar[q] * row[c1 + c0 + i * c1 + q];
-                qmat[r * c0 + i] = qq;
-                out += qq * sbar[i];
-            }
-            res[r] = out - target[r];
-        }
-        for (int q = 0; q < c1; q++)
-            a.site_f[gs * c1 + q] = cbar[q];
-        for (int aa = 0; aa < c0; aa++)
-        {
-            double acc = 0.0;
-            for (int r = 0; r < nout; r++)
-                acc += (double)a.cw[r] * (double)qmat[r * c0 + aa] * (double)res[r];
-            a.site_d[gs * c0 + aa] = acc;
-        }
+#pragma unroll
+        for (int q = 0; q < C1; q++)
+            a.site_f[gs * C1 + q] = cbar[q];
+    }
+#pragma unroll
+    for (int aa = 0; aa < C; aa++)
+        a.site_d[gs * C + aa] = acc[aa];
+}
+
+template <bool LEVEL0, int C0>
+void launch_site_terms_c1(int c1, dim3 grid, hipStream_t s, const AssembleArgs& a)
+{
+    switch (c1)
+    {
+    case 1: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 1>), grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 2>), grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 3>), grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 4>), grid, dim3(256), 0, s, a); break;
+    }
+}
+
+template <bool LEVEL0>
+void launch_site_terms(int c0, int c1, dim3 grid, hipStream_t s, const AssembleArgs& a)
+{
+    switch (c0)
+    {
+    case 1: launch_site_terms_c1<LEVEL0, 1>(c1, grid, s, a); break;
+    case 2: launch_site_terms_c1<LEVEL0, 2>(c1, grid, s, a); break;
+    case 3: launch_site_terms_c1<LEVEL0, 3>(c1, grid, s, a); break;
+    default: launch_site_terms_c1<LEVEL0, 4>(c1, grid, s, a); break;
     }
 }
 
@@ -615,8 +652,8 @@ __global__ void k_diag_fold(ChunkArgs a, double ridge)
     s[SC_LAMBDA] = unknowns > 0 ? ridge * sum / (double)unknowns : 0.0;
 }
 
-// Two inner products over the chunk's values, each folded serially (exact) or by tree (fast): sum a_i b_i and
-// sum c_i d_i.
+// The exact form's two inner products over the chunk's values, sum a_i b_i and sum c_i d_i, each folded serially in
+// index order: the products staged through LDS, lanes 0 and 1 adding them.
 __global__ __launch_bounds__(64) void k_dot_chunks(ChunkArgs a, const double* x0, const double* y0, const double* x1,
                                                    const double* y1)
 {
@@ -626,51 +663,62 @@ __global__ __launch_bounds__(64) void k_dot_chunks(ChunkArgs a, const double* x0
     if (a.L.scalars[(size_t)p * SC_COUNT + SC_ACTIVE] == 0.0)
         return;
     const int lane = threadIdx.x;
-    double acc = 0.0;   // the exact form: lane 0 folds the first product, lane 1 the second
-    double s0 = 0.0, s1 = 0.0;
+    double acc = 0.0;   // lane 0 folds the first product, lane 1 the second
     constexpr int ROWS = 8;   // 512 values staged per batch: one memory latency per batch
     __shared__ double buf[2][64 * ROWS];
     for (long long base = vb; base < ve; base += 64 * ROWS)
     {
         const int n = (int)(ve - base < 64 * ROWS ? ve - base : 64 * ROWS);
-        double t0[ROWS], t1[ROWS];
 #pragma unroll
         for (int u = 0; u < ROWS; u++)
         {
             const long long i = base + u * 64 + lane;
-            t0[u] = i < ve ? x0[i] * y0[i] : 0.0;
-            t1[u] = (x1 && i < ve) ? x1[i] * y1[i] : 0.0;
-        }
-        if (!a.exact)
-        {
-#pragma unroll
-            for (int u = 0; u < ROWS; u++)
-            {
-                s0 += tree_lanes(t0[u]);
-                s1 += tree_lanes(t1[u]);
-            }
-            continue;
-        }
-#pragma unroll
-        for (int u = 0; u < ROWS; u++)
-        {
-            buf[0][u * 64 + lane] = t0[u];
-            buf[1][u * 64 + lane] = t1[u];
+            buf[0][u * 64 + lane] = i < ve ? x0[i] * y0[i] : 0.0;
+            buf[1][u * 64 + lane] = (x1 && i < ve) ? x1[i] * y1[i] : 0.0;
         }
         __syncthreads();
         if (lane < 2)
             acc = fold_serial16(buf[lane], n, acc);
         __syncthreads();
     }
-    if (a.exact)
+    if (lane < 2)
+        a.partial[blockIdx.x * 2 + lane] = acc;
+}
+
+// The fast form's two inner products: one 256-lane workgroup per chunk, each lane's own strided values, then trees.
+constexpr int DOT_FAST_THREADS = 256;
+
+__global__ __launch_bounds__(DOT_FAST_THREADS) void k_dot_fast(ChunkArgs a, const double* x0, const double* y0,
+                                                               const double* x1, const double* y1)
+{
+    __shared__ double part[DOT_FAST_THREADS / 64][2];
+    int p;
+    long long vb, ve;
+    chunk_bounds(a, blockIdx.x, p, vb, ve);
+    if (a.L.scalars[(size_t)p * SC_COUNT + SC_ACTIVE] == 0.0)
+        return;
+    double s0 = 0.0, s1 = 0.0;
+    for (long long i = vb + threadIdx.x; i < ve; i += DOT_FAST_THREADS)
     {
-        s0 = __shfl(acc, 0);
-        s1 = __shfl(acc, 1);
+        s0 += x0[i] * y0[i];
+        if (x1)
+            s1 += x1[i] * y1[i];
     }
-    if (lane == 0)
+    s0 = tree_lanes(s0);
+    s1 = tree_lanes(s1);
+    const int wave = threadIdx.x / 64;
+    if (threadIdx.x % 64 == 0)
     {
-        a.partial[blockIdx.x * 2] = s0;
-        a.partial[blockIdx.x * 2 + 1] = s1;
+        part[wave][0] = s0;
+        part[wave][1] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2)
+    {
+        double t = 0.0;
+        for (int w = 0; w < DOT_FAST_THREADS / 64; w++)
+            t += part[w][threadIdx.x];
+        a.partial[blockIdx.x * 2 + threadIdx.x] = t;
     }
 }
 
@@ -1364,9 +1412,9 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
                        0.0);
         if (level0)
-            hipLaunchKernelGGL(k_site_terms<true>, dim3(blocks_for(acc, 256)), dim3(256), 0, c.stream, a);
+            launch_site_terms<true>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
         else
-            hipLaunchKernelGGL(k_site_terms<false>, dim3(blocks_for(acc, 256)), dim3(256), 0, c.stream, a);
+            launch_site_terms<false>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
     }
     HIP_OK(hipGetLastError());
     {
@@ -1408,7 +1456,11 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     // (the exact form) or by a wavefront tree (the fast form).
     auto dot = [&](const double* x0, const double* y0, const double* x1, const double* y1, double bytes) {
         KernelTimer kt(c.stream, level0 ? "k_dot0" : "k_dot1", bytes, 0.0);
-        hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, x0, y0, x1, y1);
+        if (ch.exact)
+            hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, x0, y0, x1, y1);
+        else
+            hipLaunchKernelGGL(k_dot_fast, dim3((unsigned)nch), dim3(DOT_FAST_THREADS), 0, c.stream, ch, x0, y0, x1,
+                               y1);
     };
     hipLaunchKernelGGL(k_cg_start, dim3(tb), dim3(256), 0, c.stream, L);
     dot(L.r, L.r, L.r, L.z, 16.0 * (double)n * L.c);
