@@ -187,6 +187,13 @@ inline StencilMap stencil_map(const Ctx& c, bool level0)
     return m;
 }
 
+// The fast-form objective's tile of pixels (one workgroup each) and the number of tiles of a plane.
+constexpr int OBJ_TW = 32, OBJ_TH = 8;
+inline long long objective_tiles(const PlaneGeom& g)
+{
+    return (long long)((g.w0 + OBJ_TW - 1) / OBJ_TW) * ((g.h0 + OBJ_TH - 1) / OBJ_TH);
+}
+
 // ---- launchers (kernel files)
 // objective.hip
 void objective_pass(Ctx& c, int k, double* sums_out /* nplanes * 3 */);

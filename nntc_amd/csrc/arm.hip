@@ -622,7 +622,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.scalars.alloc((size_t)SC_COUNT * md->nplanes);
         long long chunks_obj = 0;
         for (int p = 0; p < md->nplanes; p++)
-            chunks_obj += (c.planes.p[p].n0 * 5 + CHUNK - 1) / CHUNK + 1;
+            chunks_obj += std::max((c.planes.p[p].n0 * 5 + CHUNK - 1) / CHUNK + 1, objective_tiles(c.planes.p[p]));
         c.partial.alloc((size_t)std::max<long long>(chunks_obj * 3, (o0 / CHUNK + md->nplanes + 1) * 8));
         const int mm = c.nin + 1;
         const int entries = mm * (mm + 1) / 2 + nout * mm;

@@ -1,20 +1,21 @@
 // objective.hip: the objective E, the decode of a stored level to bytes, and the centre residual, on gfx950.
 //
 // E (src/objective.cu:123-279, src/cpu/cpu_objective.cpp:112-291) is the cw-weighted squared error of the decoded
-// material over every site of every plane - the centre and K fractional sites per pixel - in double, plus the
-// unweighted centre and fractional sums. One launch covers every plane: the site space of the chain is cut into
-// chunks of CHUNK consecutive sites of one plane, the reference CPU backend's own reduction unit.
+// material over every site of every plane - the centre and K fractional sites per pixel - plus the unweighted centre
+// and fractional sums. One launch covers every plane.
 //
-// Exact form: each chunk is one wavefront. Its 64 lanes evaluate 64 sites at a time in parallel (sampling, phi, the
-// 3T x nin decode in double) and stage every site's per-output terms in LDS; then lanes 0, 1, 2 fold the weighted,
-// centre and fractional terms serially in site-then-output order - the same additions in the same order as the
-// reference's chunk loop, so each chunk's three sums are its bits. The host folds the chunk sums in chunk order
+// Exact form: the site space of each plane is cut into chunks of CHUNK consecutive sites, the reference CPU backend's
+// own reduction unit, one wavefront each. Its 64 lanes evaluate 64 sites at a time in parallel (the double sampling
+// rule, phi, the 3T x nin decode in double) and stage every site's per-output terms in LDS; then lanes 0, 1, 2 fold
+// the weighted, centre and fractional terms serially in site-then-output order - the same additions in the same order
+// as the reference's chunk loop, so each chunk's three sums are its bits. The host folds the chunk sums in chunk order
 // (src/cpu/pool.h:314-324).
-// Fast form: one 256-lane workgroup per chunk, each lane summing its own sites, then trees; last bits differ.
+// Fast form: LDS-staged pixel tiles, fp32 sampling and decode, double sums per tile (k_objective_tiles below).
 //
 // Roofline: per pass the algorithmic bytes are the three planes read once, 4N(nout + c0) + 4 N1 c1; the flops are
 // S 2 (4 c1 + 4 (c0 + nout) K / (K + 1) + c0 c1 + nout nin) for S = N (1 + K) sites, about 41 flop/byte at nout 12:
-// compute-side of the fp64 ridge, so this kernel's fraction is reported against fp64 peak.
+// compute-side of the ridge, so this kernel's fraction is reported against the vector peak (78.6 TFLOP/s, fp64 or
+// fp32 without packed operations).
 
 #include "arm.h"
 
@@ -71,7 +72,7 @@ __device__ __forceinline__ void objective_site(const float* v0, const float* v1,
 struct ObjArgs
 {
     Planes planes;
-    Ranges chunks;            // chunk index space: plane p owns chunks [start[p], start[p+1])
+    Ranges chunks;            // reduction units (chunks or tiles): plane p owns [start[p], start[p+1])
     const float* src;
     const float* v0;
     const float* v1;
@@ -149,56 +150,209 @@ __global__ __launch_bounds__(OBJ_WAVE) void k_objective(ObjArgs a)
         a.partial[chunk * 3 + lane] = acc;
 }
 
-// The fast form: one 256-lane workgroup per chunk, each lane summing its own sites (16 of them, strided so a wavefront
-// reads consecutive sites), then the wavefront trees and the four wavefronts' partials added in order. Sixteen
-// wavefronts per SIMD over a launch instead of three, and no serial fold.
-constexpr int OBJ_FAST_THREADS = 256;
+// ---------------------------------------------------------------------------------------------------------------------
+// The fast form: LDS-staged tiles
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// One 256-lane workgroup per OBJ_TW x OBJ_TH tile of one plane's pixels, one pixel (its 1 + K sites) per lane. The
+// workgroup first copies, with coalesced row reads, the tile and its one-pixel halo of the source and level 0, the
+// level-1 texels the tile's sites can tap, and the decoder, into LDS; every tap is then an LDS read. Each input byte
+// leaves HBM once per tile (plus the halo) instead of once per tap, and the decode runs in fp32 with fused
+// multiply-adds. Sampling follows the fp32 hardware rule the assembly uses (tap_pixel), so this form's E is the
+// objective of the same model the solves see; the per-tile sums are in double.
+constexpr int OBJ_L0_PIX = (OBJ_TW + 2) * (OBJ_TH + 2);
+constexpr int OBJ_L1_W = OBJ_TW + 6, OBJ_L1_H = OBJ_TH + 6;   // bounds the level-1 box for w1 <= w0, h1 <= h0
 
-__global__ __launch_bounds__(OBJ_FAST_THREADS) void k_objective_fast(ObjArgs a)
+// The level-1 texels the taps of sites at pixel columns [p0, p0 + n) can reach, margins for rounding included.
+__device__ __forceinline__ void level1_box(int p0, int n, int w0, int w1, int& lo, int& hi)
 {
-    __shared__ double part[OBJ_FAST_THREADS / 64][3];
-    const long long chunk = blockIdx.x;
-    const int p = range_plane(a.chunks, chunk);
+    const float xmin = (((float)p0 + 0.125f) * (float)w1) / (float)w0 - 0.5f;
+    const float xmax = (((float)(p0 + n) - 0.125f) * (float)w1) / (float)w0 - 0.5f;
+    lo = max(0, (int)floorf(xmin) - 1);
+    hi = min(w1 - 1, (int)floorf(xmax) + 2);
+}
+
+template <int C0, int C1>
+__global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
+{
+    constexpr int NIN = C1 + C0 + C0 * C1;
+    constexpr int NINP = (NIN + 3) & ~3;   // decoder rows padded to whole float4s
+    __shared__ float s_l0[OBJ_L0_PIX * (MAX_NOUT + MAX_CH)];   // per staged pixel: nout source values, then C0
+    __shared__ float s_l1[OBJ_L1_W * OBJ_L1_H * MAX_CH];
+    __shared__ __align__(16) float s_w[MAX_NOUT * ((MAX_NIN + 3) & ~3)];
+    __shared__ float s_b[MAX_NOUT], s_cw[MAX_NOUT];
+    __shared__ double part[4][3];
+
+    const int tile = blockIdx.x;
+    const int p = range_plane(a.chunks, tile);
     const PlaneGeom& g = a.planes.p[p];
-    const long long sites = g.n0 * (long long)(1 + a.k);
-    const long long begin = (chunk - a.chunks.start[p]) * (long long)CHUNK;
-    const long long end = begin + CHUNK < sites ? begin + CHUNK : sites;
-    const float* src = a.src + g.off0 * a.nout;
-    const float* v0 = a.v0 + g.off0 * a.c0;
-    const float* v1 = a.v1 + g.off1 * a.c1;
-    const int nout = a.nout;
-    double s[3] = { 0.0, 0.0, 0.0 };
-    for (long long si = begin + threadIdx.x; si < end; si += OBJ_FAST_THREADS)
+    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1, nout = a.nout;
+    const int tiles_x = (w0 + OBJ_TW - 1) / OBJ_TW;
+    const int t = tile - (int)a.chunks.start[p];
+    const int tx0 = (t % tiles_x) * OBJ_TW, ty0 = (t / tiles_x) * OBJ_TH;
+    const int tid = threadIdx.x;
+
+    // ---- staging
+    const int bx0 = max(0, tx0 - 1), bx1 = min(w0 - 1, tx0 + OBJ_TW), by0 = max(0, ty0 - 1),
+              by1 = min(h0 - 1, ty0 + OBJ_TH);
+    const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
+    const int P = nout + C0;   // floats per staged pixel
+    const float* src = a.src + g.off0 * nout;
+    const float* v0 = a.v0 + g.off0 * C0;
+    for (int i = tid; i < bw * bh * nout; i += 256)
     {
-        double e[MAX_NOUT];
-        int j;
-        objective_site(v0, v1, src, g.w0, g.h0, g.w1, g.h1, a.c0, a.c1, a.nin, nout, a.weights, a.bias, a.k, si, e, j);
-        double we = 0.0, ee = 0.0;
-#pragma unroll
-        for (int r = 0; r < MAX_NOUT; r++)
-            if (r < nout)
-            {
-                we += (double)a.cw[r] * e[r] * e[r];
-                ee += e[r] * e[r];
-            }
-        s[0] += we;
-        s[j == 0 ? 1 : 2] += ee;
+        const int row = i / (bw * nout), rem = i - row * bw * nout;
+        const int pix = rem / nout, r = rem - pix * nout;
+        s_l0[(row * bw + pix) * P + r] = src[((size_t)(by0 + row) * w0 + bx0) * nout + rem];
     }
+    for (int i = tid; i < bw * bh * C0; i += 256)
+    {
+        const int row = i / (bw * C0), rem = i - row * bw * C0;
+        const int pix = rem / C0, q = rem - pix * C0;
+        s_l0[(row * bw + pix) * P + nout + q] = v0[((size_t)(by0 + row) * w0 + bx0) * C0 + rem];
+    }
+    int lx0, lx1, ly0, ly1;
+    level1_box(tx0, OBJ_TW, w0, w1, lx0, lx1);
+    level1_box(ty0, OBJ_TH, h0, h1, ly0, ly1);
+    const int lw = lx1 - lx0 + 1, lh = ly1 - ly0 + 1;
+    const float* v1 = a.v1 + g.off1 * C1;
+    for (int i = tid; i < lw * lh * C1; i += 256)
+    {
+        const int row = i / (lw * C1), rem = i - row * lw * C1;
+        s_l1[row * lw * C1 + rem] = v1[((size_t)(ly0 + row) * w1 + lx0) * C1 + rem];
+    }
+    for (int i = tid; i < nout * NINP; i += 256)
+    {
+        const int r = i / NINP, q = i - r * NINP;
+        s_w[i] = q < NIN ? a.weights[r * NIN + q] : 0.0f;
+    }
+    if (tid < nout)
+    {
+        s_b[tid] = a.bias[tid];
+        s_cw[tid] = a.cw[tid];
+    }
+    __syncthreads();
+
+    // ---- one pixel per lane
+    const int px = tx0 + tid % OBJ_TW, py = ty0 + tid / OBJ_TW;
+    float acc_w = 0.0f, acc_c = 0.0f, acc_f = 0.0f;
+    if (px < w0 && py < h0)
+        for (int j = 0; j <= a.k; j++)
+        {
+            float dx, dy;
+            site_delta(a.k, j, px, py, dx, dy);
+            float phi[NIN], target[MAX_NOUT];
+            // level 1, from its staged box (a tap outside the box cannot occur; the clamp keeps LDS in bounds)
+            const Tap t1 = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+            {
+                const int xa = min(max(t1.x0 - lx0, 0), lw - 1), xb = min(max(t1.x1 - lx0, 0), lw - 1);
+                const int ya = min(max(t1.y0 - ly0, 0), lh - 1), yb = min(max(t1.y1 - ly0, 0), lh - 1);
+                const float w00 = (1.0f - t1.fx) * (1.0f - t1.fy), w10 = t1.fx * (1.0f - t1.fy);
+                const float w01 = (1.0f - t1.fx) * t1.fy, w11 = t1.fx * t1.fy;
+#pragma unroll
+                for (int q = 0; q < C1; q++)
+                    phi[q] = w00 * s_l1[(ya * lw + xa) * C1 + q] + w10 * s_l1[(ya * lw + xb) * C1 + q] +
+                             w01 * s_l1[(yb * lw + xa) * C1 + q] + w11 * s_l1[(yb * lw + xb) * C1 + q];
+            }
+            // level 0 and the target: the texel itself at the centre, four taps at a fractional site
+            if (j == 0)
+            {
+                const float* pp = s_l0 + ((py - by0) * bw + (px - bx0)) * P;
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                    phi[C1 + i] = pp[nout + i];
+#pragma unroll
+                for (int r = 0; r < MAX_NOUT; r++)
+                    if (r < nout)
+                        target[r] = pp[r];
+            }
+            else
+            {
+                const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+                const float* pa = s_l0 + ((t0.y0 - by0) * bw + (t0.x0 - bx0)) * P;
+                const float* pb = s_l0 + ((t0.y0 - by0) * bw + (t0.x1 - bx0)) * P;
+                const float* pe = s_l0 + ((t0.y1 - by0) * bw + (t0.x0 - bx0)) * P;
+                const float* pf = s_l0 + ((t0.y1 - by0) * bw + (t0.x1 - bx0)) * P;
+                const float w00 = (1.0f - t0.fx) * (1.0f - t0.fy), w10 = t0.fx * (1.0f - t0.fy);
+                const float w01 = (1.0f - t0.fx) * t0.fy, w11 = t0.fx * t0.fy;
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                    phi[C1 + i] = w00 * pa[nout + i] + w10 * pb[nout + i] + w01 * pe[nout + i] + w11 * pf[nout + i];
+#pragma unroll
+                for (int r = 0; r < MAX_NOUT; r++)
+                    if (r < nout)
+                        target[r] = w00 * pa[r] + w10 * pb[r] + w01 * pe[r] + w11 * pf[r];
+            }
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+#pragma unroll
+                for (int q = 0; q < C1; q++)
+                    phi[C1 + C0 + i * C1 + q] = phi[C1 + i] * phi[q];
+            // The decoder is re-read from LDS for every site (a broadcast read): kept in registers across the sites it
+            // would take nout x nin of them.
+            asm volatile("" ::: "memory");
+            float we = 0.0f, ee = 0.0f;
+#pragma unroll
+            for (int r = 0; r < MAX_NOUT; r++)
+                if (r < nout)
+                {
+                    float out = s_b[r];
+                    const float4* row = reinterpret_cast<const float4*>(s_w + r * NINP);
+#pragma unroll
+                    for (int i4 = 0; i4 < NINP / 4; i4++)
+                    {
+                        const float4 wv = row[i4];
+                        const float ww[4] = { wv.x, wv.y, wv.z, wv.w };
+#pragma unroll
+                        for (int u = 0; u < 4; u++)
+                            if (i4 * 4 + u < NIN)
+                                out = fmaf(ww[u], phi[i4 * 4 + u], out);
+                    }
+                    const float e = out - target[r];
+                    we = fmaf(s_cw[r] * e, e, we);
+                    ee = fmaf(e, e, ee);
+                }
+            acc_w += we;
+            if (j == 0)
+                acc_c += ee;
+            else
+                acc_f += ee;
+        }
+
+    // ---- the tile's three sums: lane values in double, wavefront trees, the four wavefronts in order
+    double s[3] = { (double)acc_w, (double)acc_c, (double)acc_f };
 #pragma unroll
     for (int q = 0; q < 3; q++)
         for (int off = 32; off > 0; off >>= 1)
             s[q] += __shfl_xor(s[q], off);
-    const int wave = threadIdx.x / 64;
-    if (threadIdx.x % 64 == 0)
+    if (tid % 64 == 0)
         for (int q = 0; q < 3; q++)
-            part[wave][q] = s[q];
+            part[tid / 64][q] = s[q];
     __syncthreads();
-    if (threadIdx.x < 3)
+    if (tid < 3)
+        a.partial[(size_t)tile * 3 + tid] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
+}
+
+template <int C0>
+void launch_objective_tiles_c1(int c1, unsigned grid, hipStream_t s, const ObjArgs& a)
+{
+    switch (c1)
     {
-        double t = 0.0;
-        for (int w = 0; w < OBJ_FAST_THREADS / 64; w++)
-            t += part[w][threadIdx.x];
-        a.partial[chunk * 3 + threadIdx.x] = t;
+    case 1: hipLaunchKernelGGL((k_objective_tiles<C0, 1>), dim3(grid), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_objective_tiles<C0, 2>), dim3(grid), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_objective_tiles<C0, 3>), dim3(grid), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_objective_tiles<C0, 4>), dim3(grid), dim3(256), 0, s, a); break;
+    }
+}
+
+void launch_objective_tiles(int c0, int c1, unsigned grid, hipStream_t s, const ObjArgs& a)
+{
+    switch (c0)
+    {
+    case 1: launch_objective_tiles_c1<1>(c1, grid, s, a); break;
+    case 2: launch_objective_tiles_c1<2>(c1, grid, s, a); break;
+    case 3: launch_objective_tiles_c1<3>(c1, grid, s, a); break;
+    default: launch_objective_tiles_c1<4>(c1, grid, s, a); break;
     }
 }
 
@@ -250,18 +404,28 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a)
 
 void objective_pass(Ctx& c, int k, double* sums_out)
 {
+    const bool exact = c.form == NNTC_FORM_EXACT;
     ObjArgs a;
     a.planes = c.planes;
     a.chunks.n = c.planes.n;
+    // The reduction units: chunks of CHUNK sites (exact form) or OBJ_TW x OBJ_TH pixel tiles (fast form), per plane.
     long long total = 0;
     for (int p = 0; p < c.planes.n; p++)
     {
+        const PlaneGeom& g = c.g(p);
         a.chunks.start[p] = total;
-        total += (c.g(p).n0 * (1 + k) + CHUNK - 1) / CHUNK;
+        if (exact)
+            total += (g.n0 * (1 + k) + CHUNK - 1) / CHUNK;
+        else
+        {
+            if (g.w1 > g.w0 || g.h1 > g.h0)
+                throw Refused { "objective: the tiled form needs level 1 no larger than level 0" };
+            total += objective_tiles(g);
+        }
     }
     a.chunks.start[c.planes.n] = total;
     if ((size_t)total * 3 > c.partial.n)
-        throw Refused { "objective: chunk partials exceed the workspace" };
+        throw Refused { "objective: reduction partials exceed the workspace" };
     a.src = c.src.p;
     a.v0 = c.v0.p;
     a.v1 = c.v1.p;
@@ -280,11 +444,11 @@ void objective_pass(Ctx& c, int k, double* sums_out)
         const double bytes = 4.0 * n0 * (c.nout + c.c0) + 4.0 * n1 * c.c1;
         const double flops = s * 2.0 *
                              (4.0 * c.c1 + 4.0 * (c.c0 + c.nout) * k / (k + 1.0) + c.c0 * c.c1 + (double)c.nout * c.nin);
-        KernelTimer kt(c.stream, "k_objective", bytes, flops);
-        if (c.form == NNTC_FORM_EXACT)
+        KernelTimer kt(c.stream, exact ? "k_objective" : "k_objective_tiles", bytes, flops);
+        if (exact)
             hipLaunchKernelGGL(k_objective, dim3((unsigned)total), dim3(OBJ_WAVE), 0, c.stream, a);
         else
-            hipLaunchKernelGGL(k_objective_fast, dim3((unsigned)total), dim3(OBJ_FAST_THREADS), 0, c.stream, a);
+            launch_objective_tiles(c.c0, c.c1, (unsigned)total, c.stream, a);
     }
     HIP_OK(hipGetLastError());
     std::vector<double> part((size_t)total * 3);
