@@ -224,6 +224,12 @@ int nntc_hip_level0_download_values(nntc_hip_ctx* ctx, int plane, float* v0, siz
 /* Replaces be::decode_plane (src/backend.h:132): out = saturate(W phi + b) rounded to bytes, w0*h0*nout bytes. */
 int nntc_hip_decode_plane(nntc_hip_ctx* ctx, int plane, uint8_t* rgb8, size_t n);
 
+/* Replaces the host loop image_psnr_texture (src/image.h:387-407) that the reference runs over decode_plane's bytes:
+ * the base plane is decoded on the device and each texture's squared byte error against the source over the top-left
+ * width x height is summed there. sse has textures (nout / 3) entries; the sums are integers, the reference's double
+ * sums' exact values. */
+int nntc_hip_texture_sse(nntc_hip_ctx* ctx, int width, int height, unsigned long long* sse);
+
 /* Replaces be::objective_eval (src/backend.h:133). per_site has nplanes entries. */
 int nntc_hip_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* r);
 

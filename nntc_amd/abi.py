@@ -158,6 +158,7 @@ SIGNATURES = {
     "level0_upload": (C.c_int, [C.c_void_p, C.c_int, _f32p, C.c_size_t]),
     "level0_download_values": (C.c_int, [C.c_void_p, C.c_int, _f32p, C.c_size_t, _P(C.c_size_t)]),
     "decode_plane": (C.c_int, [C.c_void_p, C.c_int, _u8p, C.c_size_t]),
+    "texture_sse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_ulonglong)]),
     "objective_eval": (C.c_int, [C.c_void_p, C.c_int, _f64p, _P(Objective)]),
     "objective_total_ms": (C.c_double, []),
     "objective_passes": (C.c_longlong, []),
@@ -323,6 +324,11 @@ class Session:
         out = np.zeros(pl.w0 * pl.h0 * self.nout, dtype=np.uint8)
         self.call("decode_plane", p, _ptr(out, C.c_uint8), out.size)
         return out
+
+    def texture_sse(self, width, height):
+        out = (C.c_ulonglong * (self.nout // 3))()
+        self.call("texture_sse", width, height, out)
+        return list(out)
 
     def init_level0(self, luma=(0.299, 0.587, 0.114), luma_channel0=False):
         l = np.array(luma, dtype=np.float32)

@@ -153,6 +153,7 @@ struct Ctx
     DevBuf<uint8_t> rgb8;                  // one plane's decode
     DevBuf<float> site_f;                  // per site of the chain: the other latent's sample (assembly pass 1)
     DevBuf<double> site_d;                 // per site: the residual pulled back through the decoder
+    DevBuf<unsigned long long> tex_sse;    // per texture: the base decode's squared byte error
 
     // Pinned host staging for the small per-call readbacks.
     double* host_scalars = nullptr;
@@ -199,6 +200,7 @@ inline long long objective_tiles(const PlaneGeom& g)
 void objective_pass(Ctx& c, int k, double* sums_out /* nplanes * 3 */);
 void decode_pass(Ctx& c, int plane, uint8_t* out);
 void residual_pass(Ctx& c, int plane, float* resid);
+void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out);
 // decoder.hip
 void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc);
 // latent.hip

@@ -54,6 +54,7 @@ template struct DevBuf<float>;
 template struct DevBuf<double>;
 template struct DevBuf<uint8_t>;
 template struct DevBuf<unsigned int>;
+template struct DevBuf<unsigned long long>;
 
 Profiler& profiler()
 {
@@ -641,6 +642,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.work_f.alloc(256 * 2 * MAX_CH);
         c.work_u.alloc(MAX_CH * 2 * 256 + 64);
         c.rgb8.alloc((size_t)c.planes.p[0].n0 * nout);
+        c.tex_sse.alloc(NNTC_MAX_NOUT / 3);
         // The assembly's site buffer: every site of the chain at the largest K the encoder takes (4).
         c.site_f.alloc((size_t)o0 * 5 * std::max(c0, c1));
         c.site_d.alloc((size_t)o0 * 5 * std::max(c0, c1));
@@ -1180,6 +1182,16 @@ int nntc_hip_decode_plane(nntc_hip_ctx* h, int plane, uint8_t* rgb8, size_t n)
         check_plane(c, plane);
         decode_pass(c, plane, c.rgb8.p);
         download(c, (const uint8_t*)c.rgb8.p, (size_t)c.g(plane).n0 * c.nout, rgb8, n, nullptr);
+    });
+}
+
+int nntc_hip_texture_sse(nntc_hip_ctx* h, int width, int height, unsigned long long* sse)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        if (width < 0 || height < 0 || width > c.g(0).w0 || height > c.g(0).h0)
+            throw Refused { "texture_sse: the extent exceeds the base plane" };
+        texture_sse_pass(c, width, height, sse);
     });
 }
 

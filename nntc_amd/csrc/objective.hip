@@ -400,7 +400,50 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a)
     }
 }
 
+// Each texture's squared byte error of the decoded base plane against the source over the top-left width x height
+// (image_psnr_texture, src/image.h:387-407): the source value saturated and rounded half away from zero to a byte, the
+// difference squared, summed as integers - exact in any order, so the sums are the reference's double sums' values.
+// One atomic per wavefront and texture.
+__global__ __launch_bounds__(256) void k_texture_sse(const uint8_t* rgb8, const float* src, int w0, int nout,
+                                                     int width, int height, unsigned long long* sse)
+{
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool live = idx < (long long)width * height;
+    const int x = live ? (int)(idx % width) : 0, y = live ? (int)(idx / width) : 0;
+    const size_t base = ((size_t)y * w0 + x) * nout;
+    for (int t = 0; t < nout / 3; t++)
+    {
+        unsigned long long s = 0;
+        if (live)
+            for (int ch = 0; ch < 3; ch++)
+            {
+                const float f = src[base + 3 * t + ch];
+                const int a = (int)lroundf(fminf(1.0f, fmaxf(0.0f, f)) * 255.0f);
+                const int d = a - (int)rgb8[base + 3 * t + ch];
+                s += (unsigned long long)(d * d);
+            }
+        for (int off = 32; off > 0; off >>= 1)
+            s += __shfl_xor(s, off);
+        if (threadIdx.x % 64 == 0 && s)
+            atomicAdd(sse + t, s);
+    }
+}
+
 }   // namespace
+
+void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out)
+{
+    const PlaneGeom& g = c.g(0);
+    decode_pass(c, 0, c.rgb8.p);
+    const int textures = c.nout / 3;
+    HIP_OK(hipMemsetAsync(c.tex_sse.p, 0, sizeof(unsigned long long) * textures, c.stream));
+    hipLaunchKernelGGL(k_texture_sse, dim3(blocks_for((long long)width * height, 256)), dim3(256), 0, c.stream,
+                       c.rgb8.p, c.src.p, g.w0, c.nout, width, height, c.tex_sse.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(sse_out, c.tex_sse.p, sizeof(unsigned long long) * textures, hipMemcpyDeviceToHost,
+                          c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
 
 void objective_pass(Ctx& c, int k, double* sums_out)
 {

@@ -478,12 +478,19 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     for (int pass = 0; pass < 2; pass++)
 #pragma unroll
         for (int c = 0; c < C0; c++)
-#pragma unroll 1
+        {
+            // The sixteen diagonal entries the channel's steps read, issued together: one memory latency per
+            // channel rather than one per step.
+            double diag[16];
+#pragma unroll
+            for (int t = 0; t < 16; t++)
+                diag[t] = R.a2(t & 3, t >> 2, c, t & 3, t >> 2, c);
+#pragma unroll
             for (int t = 0; t < 16; t++)
             {
                 const int tx = t & 3, ty = t >> 2;
                 const int i = t * C0 + c;
-                const double b = a1[i * 64] + uv[i * 64], aa = R.a2(tx, ty, c, tx, ty, c);
+                const double b = a1[i * 64] + uv[i * 64], aa = diag[t];
                 const int s = sel_get(sel[c], t);
                 const double now = (double)level0_value(a.lo0[c], a.hi0[c], ep0[c], ep1[c], s);
                 double best = 0.0, move = 0.0;
@@ -511,6 +518,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                     sel[c] = sel_set(sel[c], t, which);
                 }
             }
+        }
 
     // ---- what the block leaves behind
 #pragma unroll

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -86,6 +87,33 @@ void Encoder::check(int status, const char* what)
 {
     if (status != 0)
         throw Failure { std::string(what) + ": " + seam.last_error() };
+    if (!call_times)
+        return;
+    const auto now = std::chrono::steady_clock::now();
+    const double ms = std::chrono::duration<double, std::milli>(now - last_check).count();
+    last_check = now;
+    for (auto& e : call_ms)
+        if (e.first == what)
+        {
+            e.second.first += ms;
+            e.second.second++;
+            return;
+        }
+    call_ms.push_back({ what, { ms, 1 } });
+}
+
+void Encoder::print_call_times()
+{
+    if (!call_times)
+        return;
+    std::vector<std::pair<std::string, std::pair<double, long long>>> v = call_ms;
+    std::sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.second.first > b.second.first; });
+    double total = 0.0;
+    for (auto& e : v)
+        total += e.second.first;
+    std::fprintf(stderr, "call times (wall ms, the call and the host work before it): total %.1f\n", total);
+    for (auto& e : v)
+        std::fprintf(stderr, "  %-32s %6lld calls %9.2f ms\n", e.first.c_str(), e.second.second, e.second.first);
 }
 
 }   // namespace nntc_host
@@ -123,6 +151,9 @@ void run_encode(Encoder& E)
     std::vector<nntc_level1_report> lr((size_t)np), lr0((size_t)np);
 
     const auto t_start = std::chrono::steady_clock::now();
+    E.call_times = std::getenv("NNTC_CALL_TIMES") != nullptr;
+    E.call_ms.clear();
+    E.last_check = t_start;
     const long long passes0 = S.objective_passes();
     const double obj_ms0 = S.objective_total_ms();
     long long tally0[4];
@@ -448,7 +479,9 @@ void run_encode(Encoder& E)
 
     nntc_objective shipped;
     obj(shipped);
-    E.decode_base();
+    // The shipped decode's per-texture error, summed on the device; the bytes themselves are fetched only when asked.
+    cks(S.texture_sse(d, E.source_w, E.source_h, E.texture_se), "texture_sse");
+    E.recon.clear();
     rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
     rep.e_shipped = shipped.e;
     rep.centre_psnr = mse_to_db(shipped.centre_mse);
@@ -468,6 +501,7 @@ void run_encode(Encoder& E)
         rep.ridge_tally[i] = tally[i] - tally0[i];
     for (int tx = 0; tx < E.desc.textures; tx++)
         rep.psnr[tx] = E.texture_psnr(tx);
+    E.print_call_times();
     if (!o.quiet)
     {
         std::printf("\nreport\n  backend      %s\n", S.name.c_str());
@@ -652,26 +686,14 @@ void Encoder::decode_base()
     check(seam.decode_plane(ctx, 0, recon.data(), recon.size()), "decode_plane");
 }
 
-// image_psnr_texture (src/image.h:387-407) over the original extent, both sides rounded to 8 bits.
+// image_psnr_texture (src/image.h:387-407) over the original extent, both sides rounded to 8 bits, from the arm's
+// integer error sum (the reference's double sum of the same integer squares has the same value).
 double Encoder::texture_psnr(int t) const
 {
-    const int nout = desc.nout, w0 = desc.planes[0].w0;
-    double se = 0.0;
-    size_t n = 0;
-    for (int y = 0; y < source_h; y++)
-        for (int x = 0; x < source_w; x++)
-            for (int c = 0; c < 3; c++)
-            {
-                const float f = chain[0][((size_t)y * w0 + x) * nout + 3 * t + c];
-                const int a = (int)std::lround(std::min(1.0f, std::max(0.0f, f)) * 255.0f);
-                const int b = recon[((size_t)y * w0 + x) * nout + 3 * t + c];
-                const double dd = (double)(a - b);
-                se += dd * dd;
-                n++;
-            }
+    const size_t n = (size_t)source_w * source_h * 3;
     if (n == 0)
         return 0.0;
-    const double mse = se / (double)n;
+    const double mse = (double)texture_se[t] / (double)n;
     return mse > 0.0 ? 10.0 * std::log10(255.0 * 255.0 / mse) : 100.0;
 }
 
@@ -836,7 +858,19 @@ int nntc_encoder_level0_blocks(const nntc_encoder* h, int plane, int file, uint8
 
 int nntc_encoder_recon(const nntc_encoder* h, uint8_t* out, size_t n, size_t* count)
 {
-    const Encoder& e = *reinterpret_cast<const Encoder*>(h);
+    Encoder& e = *const_cast<Encoder*>(reinterpret_cast<const Encoder*>(h));
+    if (e.ran && e.recon.empty())
+    {
+        try
+        {
+            e.decode_base();
+        }
+        catch (const Failure& f)
+        {
+            g_error = f.what;
+            return 1;
+        }
+    }
     if (count)
         *count = e.recon.size();
     std::memcpy(out, e.recon.data(), std::min(n, e.recon.size()));

@@ -4,6 +4,7 @@
 #include "nntc_encode.h"
 #include "nntc_hip.h"
 
+#include <chrono>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -50,6 +51,7 @@ struct Failure
     X(level0_upload)                                                                                                   \
     X(level0_download_values)                                                                                          \
     X(decode_plane)                                                                                                    \
+    X(texture_sse)                                                                                                     \
     X(objective_eval)                                                                                                  \
     X(objective_total_ms)                                                                                              \
     X(objective_passes)                                                                                                \
@@ -78,13 +80,21 @@ struct Encoder
     nntc_hip_ctx* ctx = nullptr;
     std::vector<std::vector<float>> chain;   // the source at every stored level, interleaved 3T channels
     std::vector<double> per_site;            // the mip weights' per-site weight of every plane
-    std::vector<uint8_t> recon;              // the base plane decoded to bytes
+    std::vector<uint8_t> recon;              // the base plane decoded to bytes, fetched when first asked for
+    unsigned long long texture_se[NNTC_MAX_NOUT / 3] = {};   // the shipped decode's squared byte error per texture
     std::vector<std::string> inputs;
     int source_w = 0, source_h = 0;
     nntc_encode_report report {};
     bool ran = false;
 
+    // Wall time per seam call name, when NNTC_CALL_TIMES is set in the environment: each check() charges the time
+    // since the previous one (the call itself and the host work that prepared it) to the call's name.
+    bool call_times = false;
+    std::vector<std::pair<std::string, std::pair<double, long long>>> call_ms;
+    std::chrono::steady_clock::time_point last_check {};
+
     void check(int status, const char* what);
+    void print_call_times();
     void setup(int textures, int width, int height, const uint8_t* const* rgb);
     void box_reorder();
     void verify_level1_on_grid();

@@ -26,6 +26,7 @@ struct RefCtx
 {
     Model m;
     be::Device* d = nullptr;
+    Image base;   // the source's base level, for the texture error sums
 };
 
 thread_local std::string g_error;
@@ -206,6 +207,7 @@ nntc_hip_ctx* nntc_ref_device_create(const nntc_model_desc* md, const float* con
         chain[i].nc = m.nout;
         chain[i].v.assign(source_chain[i], source_chain[i] + (size_t)chain[i].w * chain[i].h * m.nout);
     }
+    r->base = chain[0];
     r->d = be::device_create(m, chain);
     return reinterpret_cast<nntc_hip_ctx*>(r);
 }
@@ -487,6 +489,29 @@ int nntc_ref_decode_plane(nntc_hip_ctx* ctx, int plane, uint8_t* rgb8, size_t n)
     std::vector<uint8_t> v;
     be::decode_plane(r->d, m, plane, v);
     return download(v, rgb8, n, nullptr);
+}
+
+// The reference's image_psnr_texture (src/image.h:387-407) squared-error sum over decode_plane's bytes, per texture.
+int nntc_ref_texture_sse(nntc_hip_ctx* ctx, int width, int height, unsigned long long* sse)
+{
+    REF_CTX;
+    std::vector<uint8_t> rec;
+    be::decode_plane(r->d, m, 0, rec);
+    for (int t = 0; t < m.textures; t++)
+    {
+        unsigned long long s = 0;
+        for (int y = 0; y < height; y++)
+            for (int x = 0; x < width; x++)
+                for (int c = 0; c < 3; c++)
+                {
+                    const float f = r->base.at(x, y)[3 * t + c];
+                    const int a = (int)std::lround(std::min(1.0f, std::max(0.0f, f)) * 255.0f);
+                    const int d = a - (int)rec[((size_t)y * r->base.w + x) * m.nout + 3 * t + c];
+                    s += (unsigned long long)(d * d);
+                }
+        sse[t] = s;
+    }
+    return 0;
 }
 
 int nntc_ref_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* o)

@@ -78,3 +78,7 @@ def test_lockstep_exact(name, hip_arm, ref_arm):
             np.testing.assert_array_equal(h.bc_blocks(p, f), r.download_blocks(p, f, h.bc_blocks(p, f).size * 2),
                                           err_msg=f"blocks plane {p} file {f}")
         np.testing.assert_array_equal(h.decode_plane(p), r.decode_plane(p), err_msg=f"decode plane {p}")
+    # The per-texture error sums the PSNR report is made of, over the whole base and over a crop.
+    base = h.plane(0)
+    for wh in ((base.w0, base.h0), (max(1, base.w0 - 3), max(1, base.h0 // 2))):
+        assert h.texture_sse(*wh) == r.texture_sse(*wh), f"texture sse over {wh}"
