@@ -627,7 +627,10 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.partial.alloc((size_t)std::max<long long>(chunks_obj * 3, (o0 / CHUNK + md->nplanes + 1) * 8));
         const int mm = c.nin + 1;
         const int entries = mm * (mm + 1) / 2 + nout * mm;
-        c.ls_partial.alloc((size_t)md->nplanes * entries * SITE_BLOCKS);
+        long long tiles = 0;   // the fast form's Gram keeps one partial per pixel tile
+        for (int p = 0; p < md->nplanes; p++)
+            tiles += objective_tiles(c.planes.p[p]);
+        c.ls_partial.alloc((size_t)entries * (size_t)std::max<long long>(md->nplanes * SITE_BLOCKS, tiles));
         c.ls_sum.alloc((size_t)entries + MAX_PLANES);
         c.obj.alloc((size_t)md->nplanes * 3);
         c.resid.alloc((size_t)o0 * nout);

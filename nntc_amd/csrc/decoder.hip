@@ -10,10 +10,11 @@
 // by its site weight. That is the reference CPU backend's order exactly, so A and rhs are its bits.
 //
 // Roofline: the useful flops are 2 S (mm (mm + 1) / 2 + nout mm) for S sites (4.19 GF at 4 x 1024^2); the fp64 dot
-// products run on the vector ALUs, so the bound is fp64 peak. The MFMA split-precision form replaces this tile
-// product when the exact form is not asked for.
+// products run on the vector ALUs, so the bound is fp64 peak. The fast form replaces this with the matrix-core Gram
+// below (k_gram_tiles).
 
 #include "arm.h"
+#include "tile_stage.h"
 
 namespace nntc
 {
@@ -144,10 +145,280 @@ __global__ __launch_bounds__(64) void k_ls_reduce(const double* partial, const d
     out[e] = total;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// The fast form: the Gram on the matrix cores, BF16 hi/lo split products accumulated in fp32
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// With u = [phi; 1; t] (D = mm + nout values per site), every normal-equation entry is an entry of the Gram
+// G = sum u u^T: A = G[0:mm, 0:mm], rhs = G[mm + c, i]. One 256-lane workgroup per pixel tile (tile_stage.h) builds the
+// u of its 1 + K sites per pixel, splits each value into bf16 hi + lo (x - hi is exact in fp32, so hi + lo carries 16
+// of x's 24 mantissa bits), stores them feature-major in LDS, and each wavefront multiplies its 64 sites' slice on the
+// matrix cores: G_tile += hi hi^T + hi lo^T + lo hi^T, v_mfma_f32_32x32x16_bf16 per 32 x 32 block of G and 16
+// sites, accumulated in fp32 over the wavefront's 320 sites. The four wavefronts' blocks are added in double, in order,
+// and the tile's entries written; a second kernel folds the tiles of each plane in order, times the plane's weight.
+// Roofline: 2 S D^2 MFMA flops per pass (three split products), bounded by the feature pass that feeds them.
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short short8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+struct GramArgs
+{
+    Planes planes;
+    Ranges tiles;          // plane p owns tiles [start[p], start[p+1])
+    const float* src;
+    const float* v0;
+    const float* v1;
+    int nout, k, mm, tri, entries;
+    double* partial;       // tile * entries + e
+};
+
+// x as bf16 rounded to nearest even (finite inputs), and the bf16 of what it leaves.
+__device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
+{
+    const unsigned int b = __float_as_uint(x);
+    hi = (unsigned short)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+    const float rest = x - __uint_as_float((unsigned int)hi << 16);
+    const unsigned int r = __float_as_uint(rest);
+    lo = (unsigned short)((r + 0x7FFFu + ((r >> 16) & 1u)) >> 16);
+}
+
+template <int C0, int C1, int DB>
+__global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
+{
+    constexpr int NIN = C1 + C0 + C0 * C1, MM = NIN + 1;
+    constexpr int DP = 32 * DB;                 // features padded to whole 32-blocks
+    constexpr int NT = DB * (DB + 1) / 2;       // the blocks of G's upper block triangle
+    __shared__ float s_l0[TILE_L0_FLOATS];
+    __shared__ float s_l1[TILE_L1_FLOATS];
+    __shared__ __align__(16) unsigned short s_hi[DP * 256], s_lo[DP * 256];   // [feature][site of this round]
+    __shared__ double s_g[DP * DP];
+
+    const int tile = blockIdx.x;
+    const int p = range_plane(a.tiles, tile);
+    const int nout = a.nout, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const TileGeom tg = stage_tile<C0, C1>(a.planes.p[p], tile - (int)a.tiles.start[p], nout, a.src, a.v0, a.v1,
+                                           s_l0, s_l1);
+    for (int i = tid; i < DP * DP; i += 256)
+        s_g[i] = 0.0;
+    __syncthreads();
+
+    const int px = tg.tx0 + tid % TILE_W, py = tg.ty0 + tid / TILE_W;
+    const bool live = px < tg.w0 && py < tg.h0;
+    floatx16 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            acc[t][r] = 0.0f;
+
+    for (int j = 0; j <= a.k; j++)
+    {
+        // ---- this round's site of every lane, as a column of bf16 hi and lo values (zeros past the plane)
+        {
+            float phi[NIN], target[MAX_NOUT];
+            if (live)
+                tile_site<C0, C1>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
+#pragma unroll
+            for (int f = 0; f < DP; f++)
+            {
+                float x = 0.0f;
+                if (live)
+                {
+                    if (f < NIN)
+                        x = phi[f];
+                    else if (f == NIN)
+                        x = 1.0f;
+                    else if (f - MM < nout && f - MM < MAX_NOUT)
+                        x = target[f - MM];
+                }
+                unsigned short hi, lo;
+                split_bf16(x, hi, lo);
+                s_hi[f * 256 + tid] = hi;
+                s_lo[f * 256 + tid] = lo;
+            }
+        }
+        __syncthreads();
+        // ---- the wavefront's 64 sites, 16 per product; lane (feature l % 32, sites 8 (l / 32) .. + 7)
+#pragma unroll
+        for (int ks = 0; ks < 4; ks++)
+        {
+            const int site0 = 64 * wave + 16 * ks + 8 * (lane / 32);
+            bf16x8 hv[DB], lv[DB];
+#pragma unroll
+            for (int b = 0; b < DB; b++)
+            {
+                const int f = 32 * b + lane % 32;
+                hv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_hi + f * 256 + site0));
+                lv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_lo + f * 256 + site0));
+            }
+            int t = 0;
+#pragma unroll
+            for (int bi = 0; bi < DB; bi++)
+#pragma unroll
+                for (int bj = bi; bj < DB; bj++, t++)
+                {
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], hv[bj], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], lv[bj], acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lv[bi], hv[bj], acc[t], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+
+    // ---- the four wavefronts' blocks added into G in double, wavefront by wavefront; lane l, value r holds
+    // row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of its 32 x 32 block
+    for (int w = 0; w < 4; w++)
+    {
+        if (wave == w)
+        {
+            int t = 0;
+#pragma unroll
+            for (int bi = 0; bi < DB; bi++)
+#pragma unroll
+                for (int bj = bi; bj < DB; bj++, t++)
+#pragma unroll
+                    for (int r = 0; r < 16; r++)
+                    {
+                        const int m = 32 * bi + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                        const int n = 32 * bj + lane % 32;
+                        s_g[m * DP + n] += (double)acc[t][r];
+                    }
+        }
+        __syncthreads();
+    }
+    // ---- the tile's entries: A's upper triangle row by row, then rhs (target c, feature i); G's lower half is the
+    // mirror of the blocks above
+    for (int e = tid; e < a.entries; e += 256)
+    {
+        int i, jj;
+        if (e < a.tri)
+        {
+            int rest = e;
+            i = 0;
+            while (rest >= MM - i)
+                rest -= MM - i++;
+            jj = i + rest;
+        }
+        else
+        {
+            const int e2 = e - a.tri;
+            i = e2 % MM;           // feature
+            jj = MM + e2 / MM;     // target, always in a block at or right of the feature's
+        }
+        const int bi = i / 32, bj = jj / 32;
+        a.partial[(size_t)tile * a.entries + e] = bi <= bj ? s_g[i * DP + jj] : s_g[jj * DP + i];
+    }
+}
+
+// Per entry: each plane's tiles in index order, times the plane's weight, the planes in order.
+__global__ __launch_bounds__(64) void k_gram_reduce(const double* partial, const double* omega, Ranges tiles,
+                                                    int entries, double* out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= entries)
+        return;
+    double total = 0.0;
+    for (int p = 0; p < tiles.n; p++)
+    {
+        double sum = 0.0;
+        for (long long t = tiles.start[p]; t < tiles.start[p + 1]; t++)
+            sum += partial[(size_t)t * entries + e];
+        total += omega[p] * sum;
+    }
+    out[e] = total;
+}
+
+template <int C0, int C1>
+void launch_gram_tiles_db(int db, unsigned grid, hipStream_t s, const GramArgs& a)
+{
+    if (db == 1)
+        hipLaunchKernelGGL((k_gram_tiles<C0, C1, 1>), dim3(grid), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_gram_tiles<C0, C1, 2>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <int C0>
+void launch_gram_tiles_c1(int c1, int db, unsigned grid, hipStream_t s, const GramArgs& a)
+{
+    switch (c1)
+    {
+    case 1: launch_gram_tiles_db<C0, 1>(db, grid, s, a); break;
+    case 2: launch_gram_tiles_db<C0, 2>(db, grid, s, a); break;
+    case 3: launch_gram_tiles_db<C0, 3>(db, grid, s, a); break;
+    default: launch_gram_tiles_db<C0, 4>(db, grid, s, a); break;
+    }
+}
+
+void launch_gram_tiles(int c0, int c1, int db, unsigned grid, hipStream_t s, const GramArgs& a)
+{
+    switch (c0)
+    {
+    case 1: launch_gram_tiles_c1<1>(c1, db, grid, s, a); break;
+    case 2: launch_gram_tiles_c1<2>(c1, db, grid, s, a); break;
+    case 3: launch_gram_tiles_c1<3>(c1, db, grid, s, a); break;
+    default: launch_gram_tiles_c1<4>(c1, db, grid, s, a); break;
+    }
+}
+
+// The fast form's accumulation: tiled MFMA Gram, then the per-plane weighted fold.
+void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
+{
+    GramArgs a;
+    a.planes = c.planes;
+    a.tiles.n = c.planes.n;
+    long long total = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        const PlaneGeom& g = c.g(p);
+        if (g.w1 > g.w0 || g.h1 > g.h0)
+            throw Refused { "decoder: the tiled Gram needs level 1 no larger than level 0" };
+        a.tiles.start[p] = total;
+        total += objective_tiles(g);
+    }
+    a.tiles.start[c.planes.n] = total;
+    a.src = c.src.p;
+    a.v0 = c.v0.p;
+    a.v1 = c.v1.p;
+    a.nout = c.nout;
+    a.k = k;
+    a.mm = c.nin + 1;
+    a.tri = a.mm * (a.mm + 1) / 2;
+    a.entries = a.tri + c.nout * a.mm;
+    a.partial = c.ls_partial.p;
+    if ((size_t)total * a.entries > c.ls_partial.n)
+        throw Refused { "decoder: tile partials exceed the workspace" };
+    const int dims = a.mm + c.nout;
+    const int db = dims <= 32 ? 1 : 2;
+    if (dims > 64)
+        throw Refused { "decoder: more than 64 Gram features" };
+    {
+        const double n0 = (double)c.n0_total, n1 = (double)c.n1_total, s = n0 * (1 + k);
+        const double bytes = 4.0 * n0 * (c.nout + c.c0) + 4.0 * n1 * c.c1;
+        const double flops = 2.0 * s * ((double)a.mm * (a.mm + 1) / 2.0 + (double)c.nout * a.mm);
+        KernelTimer kt(c.stream, "k_gram_tiles", bytes, flops);
+        launch_gram_tiles(c.c0, c.c1, db, (unsigned)total, c.stream, a);
+    }
+    HIP_OK(hipGetLastError());
+    double* d_omega = c.ls_sum.p + a.entries;
+    HIP_OK(hipMemcpyAsync(d_omega, omega, sizeof(double) * c.planes.n, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_gram_reduce, dim3(blocks_for(a.entries, 64)), dim3(64), 0, c.stream,
+                       (const double*)a.partial, (const double*)d_omega, a.tiles, a.entries, c.ls_sum.p);
+    HIP_OK(hipGetLastError());
+    acc.resize((size_t)a.entries);
+    HIP_OK(hipMemcpyAsync(acc.data(), c.ls_sum.p, sizeof(double) * a.entries, hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
 }   // namespace
 
 void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
 {
+    if (c.form != NNTC_FORM_EXACT)
+    {
+        gram_accumulate(c, k, omega, acc);
+        return;
+    }
     LsArgs a;
     a.planes = c.planes;
     a.src = c.src.p;

@@ -18,6 +18,7 @@
 // fp32 without packed operations).
 
 #include "arm.h"
+#include "tile_stage.h"
 
 namespace nntc
 {
@@ -151,76 +152,25 @@ __global__ __launch_bounds__(OBJ_WAVE) void k_objective(ObjArgs a)
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// The fast form: LDS-staged tiles
+// The fast form: LDS-staged tiles (tile_stage.h), the decode in fp32 with fused multiply-adds, the tile's sums in double
 // ---------------------------------------------------------------------------------------------------------------------
-//
-// One 256-lane workgroup per OBJ_TW x OBJ_TH tile of one plane's pixels, one pixel (its 1 + K sites) per lane. The
-// workgroup first copies, with coalesced row reads, the tile and its one-pixel halo of the source and level 0, the
-// level-1 texels the tile's sites can tap, and the decoder, into LDS; every tap is then an LDS read. Each input byte
-// leaves HBM once per tile (plus the halo) instead of once per tap, and the decode runs in fp32 with fused
-// multiply-adds. Sampling follows the fp32 hardware rule the assembly uses (tap_pixel), so this form's E is the
-// objective of the same model the solves see; the per-tile sums are in double.
-constexpr int OBJ_L0_PIX = (OBJ_TW + 2) * (OBJ_TH + 2);
-constexpr int OBJ_L1_W = OBJ_TW + 6, OBJ_L1_H = OBJ_TH + 6;   // bounds the level-1 box for w1 <= w0, h1 <= h0
-
-// The level-1 texels the taps of sites at pixel columns [p0, p0 + n) can reach, margins for rounding included.
-__device__ __forceinline__ void level1_box(int p0, int n, int w0, int w1, int& lo, int& hi)
-{
-    const float xmin = (((float)p0 + 0.125f) * (float)w1) / (float)w0 - 0.5f;
-    const float xmax = (((float)(p0 + n) - 0.125f) * (float)w1) / (float)w0 - 0.5f;
-    lo = max(0, (int)floorf(xmin) - 1);
-    hi = min(w1 - 1, (int)floorf(xmax) + 2);
-}
 
 template <int C0, int C1>
 __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
 {
     constexpr int NIN = C1 + C0 + C0 * C1;
     constexpr int NINP = (NIN + 3) & ~3;   // decoder rows padded to whole float4s
-    __shared__ float s_l0[OBJ_L0_PIX * (MAX_NOUT + MAX_CH)];   // per staged pixel: nout source values, then C0
-    __shared__ float s_l1[OBJ_L1_W * OBJ_L1_H * MAX_CH];
+    __shared__ float s_l0[TILE_L0_FLOATS];
+    __shared__ float s_l1[TILE_L1_FLOATS];
     __shared__ __align__(16) float s_w[MAX_NOUT * ((MAX_NIN + 3) & ~3)];
     __shared__ float s_b[MAX_NOUT], s_cw[MAX_NOUT];
     __shared__ double part[4][3];
 
     const int tile = blockIdx.x;
     const int p = range_plane(a.chunks, tile);
-    const PlaneGeom& g = a.planes.p[p];
-    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1, nout = a.nout;
-    const int tiles_x = (w0 + OBJ_TW - 1) / OBJ_TW;
-    const int t = tile - (int)a.chunks.start[p];
-    const int tx0 = (t % tiles_x) * OBJ_TW, ty0 = (t / tiles_x) * OBJ_TH;
-    const int tid = threadIdx.x;
-
-    // ---- staging
-    const int bx0 = max(0, tx0 - 1), bx1 = min(w0 - 1, tx0 + OBJ_TW), by0 = max(0, ty0 - 1),
-              by1 = min(h0 - 1, ty0 + OBJ_TH);
-    const int bw = bx1 - bx0 + 1, bh = by1 - by0 + 1;
-    const int P = nout + C0;   // floats per staged pixel
-    const float* src = a.src + g.off0 * nout;
-    const float* v0 = a.v0 + g.off0 * C0;
-    for (int i = tid; i < bw * bh * nout; i += 256)
-    {
-        const int row = i / (bw * nout), rem = i - row * bw * nout;
-        const int pix = rem / nout, r = rem - pix * nout;
-        s_l0[(row * bw + pix) * P + r] = src[((size_t)(by0 + row) * w0 + bx0) * nout + rem];
-    }
-    for (int i = tid; i < bw * bh * C0; i += 256)
-    {
-        const int row = i / (bw * C0), rem = i - row * bw * C0;
-        const int pix = rem / C0, q = rem - pix * C0;
-        s_l0[(row * bw + pix) * P + nout + q] = v0[((size_t)(by0 + row) * w0 + bx0) * C0 + rem];
-    }
-    int lx0, lx1, ly0, ly1;
-    level1_box(tx0, OBJ_TW, w0, w1, lx0, lx1);
-    level1_box(ty0, OBJ_TH, h0, h1, ly0, ly1);
-    const int lw = lx1 - lx0 + 1, lh = ly1 - ly0 + 1;
-    const float* v1 = a.v1 + g.off1 * C1;
-    for (int i = tid; i < lw * lh * C1; i += 256)
-    {
-        const int row = i / (lw * C1), rem = i - row * lw * C1;
-        s_l1[row * lw * C1 + rem] = v1[((size_t)(ly0 + row) * w1 + lx0) * C1 + rem];
-    }
+    const int nout = a.nout, tid = threadIdx.x;
+    const TileGeom tg = stage_tile<C0, C1>(a.planes.p[p], tile - (int)a.chunks.start[p], nout, a.src, a.v0, a.v1,
+                                           s_l0, s_l1);
     for (int i = tid; i < nout * NINP; i += 256)
     {
         const int r = i / NINP, q = i - r * NINP;
@@ -234,60 +184,13 @@ __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
     __syncthreads();
 
     // ---- one pixel per lane
-    const int px = tx0 + tid % OBJ_TW, py = ty0 + tid / OBJ_TW;
+    const int px = tg.tx0 + tid % TILE_W, py = tg.ty0 + tid / TILE_W;
     float acc_w = 0.0f, acc_c = 0.0f, acc_f = 0.0f;
-    if (px < w0 && py < h0)
+    if (px < tg.w0 && py < tg.h0)
         for (int j = 0; j <= a.k; j++)
         {
-            float dx, dy;
-            site_delta(a.k, j, px, py, dx, dy);
             float phi[NIN], target[MAX_NOUT];
-            // level 1, from its staged box (a tap outside the box cannot occur; the clamp keeps LDS in bounds)
-            const Tap t1 = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
-            {
-                const int xa = min(max(t1.x0 - lx0, 0), lw - 1), xb = min(max(t1.x1 - lx0, 0), lw - 1);
-                const int ya = min(max(t1.y0 - ly0, 0), lh - 1), yb = min(max(t1.y1 - ly0, 0), lh - 1);
-                const float w00 = (1.0f - t1.fx) * (1.0f - t1.fy), w10 = t1.fx * (1.0f - t1.fy);
-                const float w01 = (1.0f - t1.fx) * t1.fy, w11 = t1.fx * t1.fy;
-#pragma unroll
-                for (int q = 0; q < C1; q++)
-                    phi[q] = w00 * s_l1[(ya * lw + xa) * C1 + q] + w10 * s_l1[(ya * lw + xb) * C1 + q] +
-                             w01 * s_l1[(yb * lw + xa) * C1 + q] + w11 * s_l1[(yb * lw + xb) * C1 + q];
-            }
-            // level 0 and the target: the texel itself at the centre, four taps at a fractional site
-            if (j == 0)
-            {
-                const float* pp = s_l0 + ((py - by0) * bw + (px - bx0)) * P;
-#pragma unroll
-                for (int i = 0; i < C0; i++)
-                    phi[C1 + i] = pp[nout + i];
-#pragma unroll
-                for (int r = 0; r < MAX_NOUT; r++)
-                    if (r < nout)
-                        target[r] = pp[r];
-            }
-            else
-            {
-                const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-                const float* pa = s_l0 + ((t0.y0 - by0) * bw + (t0.x0 - bx0)) * P;
-                const float* pb = s_l0 + ((t0.y0 - by0) * bw + (t0.x1 - bx0)) * P;
-                const float* pe = s_l0 + ((t0.y1 - by0) * bw + (t0.x0 - bx0)) * P;
-                const float* pf = s_l0 + ((t0.y1 - by0) * bw + (t0.x1 - bx0)) * P;
-                const float w00 = (1.0f - t0.fx) * (1.0f - t0.fy), w10 = t0.fx * (1.0f - t0.fy);
-                const float w01 = (1.0f - t0.fx) * t0.fy, w11 = t0.fx * t0.fy;
-#pragma unroll
-                for (int i = 0; i < C0; i++)
-                    phi[C1 + i] = w00 * pa[nout + i] + w10 * pb[nout + i] + w01 * pe[nout + i] + w11 * pf[nout + i];
-#pragma unroll
-                for (int r = 0; r < MAX_NOUT; r++)
-                    if (r < nout)
-                        target[r] = w00 * pa[r] + w10 * pb[r] + w01 * pe[r] + w11 * pf[r];
-            }
-#pragma unroll
-            for (int i = 0; i < C0; i++)
-#pragma unroll
-                for (int q = 0; q < C1; q++)
-                    phi[C1 + C0 + i * C1 + q] = phi[C1 + i] * phi[q];
+            tile_site<C0, C1>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
             // The decoder is re-read from LDS for every site (a broadcast read): kept in registers across the sites it
             // would take nout x nin of them.
             asm volatile("" ::: "memory");

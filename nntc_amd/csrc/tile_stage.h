@@ -1,0 +1,146 @@
+// tile_stage.h: the fast form's LDS-staged pixel tiles, shared by the tiled objective and the tiled decoder Gram.
+//
+// A 256-lane workgroup owns a TILE_W x TILE_H tile of one plane's pixels, one pixel (its 1 + K sites) per lane. It
+// first copies, with coalesced row reads, the tile and its one-pixel halo of the source and level 0, and the level-1
+// texels the tile's sites can tap, into LDS; every tap of every site is then an LDS read, and each input byte leaves
+// HBM once per tile (plus the halo) instead of once per tap. Sampling follows the fp32 hardware bilinear rule of the
+// assembly (tap_pixel, src/sample.cuh:43-72), so the tiled kernels see the model the latent solves see.
+
+#pragma once
+
+#include "arm.h"
+
+namespace nntc
+{
+
+constexpr int TILE_W = OBJ_TW, TILE_H = OBJ_TH;
+constexpr int TILE_L0_PIX = (TILE_W + 2) * (TILE_H + 2);
+constexpr int TILE_L1_W = TILE_W + 6, TILE_L1_H = TILE_H + 6;   // bounds the level-1 box when w1 <= w0, h1 <= h0
+constexpr int TILE_L0_FLOATS = TILE_L0_PIX * (MAX_NOUT + MAX_CH);
+constexpr int TILE_L1_FLOATS = TILE_L1_W * TILE_L1_H * MAX_CH;
+
+// The level-1 texels the taps of sites at pixel columns [p0, p0 + n) can reach (sub-texel offsets stay within
+// +-3/8), with a margin of one texel each side for rounding.
+__device__ __forceinline__ void level1_box(int p0, int n, int w0, int w1, int& lo, int& hi)
+{
+    const float xmin = (((float)p0 + 0.125f) * (float)w1) / (float)w0 - 0.5f;
+    const float xmax = (((float)(p0 + n) - 0.125f) * (float)w1) / (float)w0 - 0.5f;
+    lo = max(0, (int)floorf(xmin) - 1);
+    hi = min(w1 - 1, (int)floorf(xmax) + 2);
+}
+
+struct TileGeom
+{
+    int w0, h0, w1, h1, nout;
+    int tx0, ty0;            // the tile's first pixel
+    int bx0, by0, bw, bh;    // the staged level-0 box (tile plus halo, clipped to the plane)
+    int lx0, ly0, lw, lh;    // the staged level-1 box
+    int P;                   // floats per staged level-0 pixel: nout source values, then C0
+};
+
+// Stage tile t (row-major over the plane's tiles) of plane g. The caller synchronises before reading.
+template <int C0, int C1>
+__device__ __forceinline__ TileGeom stage_tile(const PlaneGeom& g, int t, int nout, const float* src_chain,
+                                               const float* v0_chain, const float* v1_chain, float* s_l0,
+                                               float* s_l1)
+{
+    TileGeom tg;
+    tg.w0 = g.w0;
+    tg.h0 = g.h0;
+    tg.w1 = g.w1;
+    tg.h1 = g.h1;
+    tg.nout = nout;
+    const int tiles_x = (g.w0 + TILE_W - 1) / TILE_W;
+    tg.tx0 = (t % tiles_x) * TILE_W;
+    tg.ty0 = (t / tiles_x) * TILE_H;
+    tg.bx0 = max(0, tg.tx0 - 1);
+    tg.by0 = max(0, tg.ty0 - 1);
+    tg.bw = min(g.w0 - 1, tg.tx0 + TILE_W) - tg.bx0 + 1;
+    tg.bh = min(g.h0 - 1, tg.ty0 + TILE_H) - tg.by0 + 1;
+    tg.P = nout + C0;
+    const int tid = threadIdx.x, nthreads = blockDim.x;
+    const float* src = src_chain + g.off0 * nout;
+    const float* v0 = v0_chain + g.off0 * C0;
+    for (int i = tid; i < tg.bw * tg.bh * nout; i += nthreads)
+    {
+        const int row = i / (tg.bw * nout), rem = i - row * tg.bw * nout;
+        const int pix = rem / nout, r = rem - pix * nout;
+        s_l0[(row * tg.bw + pix) * tg.P + r] = src[((size_t)(tg.by0 + row) * g.w0 + tg.bx0) * nout + rem];
+    }
+    for (int i = tid; i < tg.bw * tg.bh * C0; i += nthreads)
+    {
+        const int row = i / (tg.bw * C0), rem = i - row * tg.bw * C0;
+        const int pix = rem / C0, q = rem - pix * C0;
+        s_l0[(row * tg.bw + pix) * tg.P + nout + q] = v0[((size_t)(tg.by0 + row) * g.w0 + tg.bx0) * C0 + rem];
+    }
+    int lx1, ly1;
+    level1_box(tg.tx0, TILE_W, g.w0, g.w1, tg.lx0, lx1);
+    level1_box(tg.ty0, TILE_H, g.h0, g.h1, tg.ly0, ly1);
+    tg.lw = lx1 - tg.lx0 + 1;
+    tg.lh = ly1 - tg.ly0 + 1;
+    const float* v1 = v1_chain + g.off1 * C1;
+    for (int i = tid; i < tg.lw * tg.lh * C1; i += nthreads)
+    {
+        const int row = i / (tg.lw * C1), rem = i - row * tg.lw * C1;
+        s_l1[row * tg.lw * C1 + rem] = v1[((size_t)(tg.ly0 + row) * g.w1 + tg.lx0) * C1 + rem];
+    }
+    return tg;
+}
+
+// Site j of pixel (px, py) of the staged tile: phi = [c (C1)][s (C0)][s_i c_q] and the target (nout values), fp32.
+template <int C0, int C1>
+__device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0, const float* s_l1, int k, int j,
+                                          int px, int py, float* phi, float* target)
+{
+    float dx, dy;
+    site_delta(k, j, px, py, dx, dy);
+    // Level 1 from its staged box (a tap outside the box cannot occur; the clamp keeps LDS reads in bounds).
+    const Tap t1 = tap_pixel(px, py, dx, dy, tg.w0, tg.h0, tg.w1, tg.h1);
+    {
+        const int lw = tg.lw, lh = tg.lh;
+        const int xa = min(max(t1.x0 - tg.lx0, 0), lw - 1), xb = min(max(t1.x1 - tg.lx0, 0), lw - 1);
+        const int ya = min(max(t1.y0 - tg.ly0, 0), lh - 1), yb = min(max(t1.y1 - tg.ly0, 0), lh - 1);
+        const float w00 = (1.0f - t1.fx) * (1.0f - t1.fy), w10 = t1.fx * (1.0f - t1.fy);
+        const float w01 = (1.0f - t1.fx) * t1.fy, w11 = t1.fx * t1.fy;
+#pragma unroll
+        for (int q = 0; q < C1; q++)
+            phi[q] = w00 * s_l1[(ya * lw + xa) * C1 + q] + w10 * s_l1[(ya * lw + xb) * C1 + q] +
+                     w01 * s_l1[(yb * lw + xa) * C1 + q] + w11 * s_l1[(yb * lw + xb) * C1 + q];
+    }
+    const int nout = tg.nout, P = tg.P;
+    if (j == 0)
+    {
+        const float* pp = s_l0 + ((py - tg.by0) * tg.bw + (px - tg.bx0)) * P;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            phi[C1 + i] = pp[nout + i];
+#pragma unroll
+        for (int r = 0; r < MAX_NOUT; r++)
+            if (r < nout)
+                target[r] = pp[r];
+    }
+    else
+    {
+        const Tap t0 = tap_pixel(px, py, dx, dy, tg.w0, tg.h0, tg.w0, tg.h0);
+        const float* pa = s_l0 + ((t0.y0 - tg.by0) * tg.bw + (t0.x0 - tg.bx0)) * P;
+        const float* pb = s_l0 + ((t0.y0 - tg.by0) * tg.bw + (t0.x1 - tg.bx0)) * P;
+        const float* pe = s_l0 + ((t0.y1 - tg.by0) * tg.bw + (t0.x0 - tg.bx0)) * P;
+        const float* pf = s_l0 + ((t0.y1 - tg.by0) * tg.bw + (t0.x1 - tg.bx0)) * P;
+        const float w00 = (1.0f - t0.fx) * (1.0f - t0.fy), w10 = t0.fx * (1.0f - t0.fy);
+        const float w01 = (1.0f - t0.fx) * t0.fy, w11 = t0.fx * t0.fy;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            phi[C1 + i] = w00 * pa[nout + i] + w10 * pb[nout + i] + w01 * pe[nout + i] + w11 * pf[nout + i];
+#pragma unroll
+        for (int r = 0; r < MAX_NOUT; r++)
+            if (r < nout)
+                target[r] = w00 * pa[r] + w10 * pb[r] + w01 * pe[r] + w11 * pf[r];
+    }
+#pragma unroll
+    for (int i = 0; i < C0; i++)
+#pragma unroll
+        for (int q = 0; q < C1; q++)
+            phi[C1 + C0 + i * C1 + q] = phi[C1 + i] * phi[q];
+}
+
+}   // namespace nntc
