@@ -125,24 +125,27 @@ __global__ __launch_bounds__(LS_TILE) void k_ls_accumulate(LsArgs a)
     }
 }
 
-// Per entry: the planes in index order, inside each plane the SITE_BLOCKS partials in index order, each plane's sum
-// times its own site weight.
+// Per entry (one wavefront each): the planes in index order, inside each plane the SITE_BLOCKS partials in index
+// order, each plane's sum times its own site weight. The wavefront stages a plane's partials in LDS with coalesced
+// reads and lane 0 adds them in order, sixteen read ahead of their additions.
 __global__ __launch_bounds__(64) void k_ls_reduce(const double* partial, const double* omega, int planes, int entries,
                                                   double* out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= entries)
-        return;
+    __shared__ double row[SITE_BLOCKS];
+    const int e = blockIdx.x, lane = threadIdx.x;
     double total = 0.0;
     for (int p = 0; p < planes; p++)
     {
-        const double* row = partial + ((size_t)p * entries + e) * SITE_BLOCKS;
-        double sum = 0.0;
-        for (int b = 0; b < SITE_BLOCKS; b++)
-            sum += row[b];
-        total += omega[p] * sum;
+        const double* src = partial + ((size_t)p * entries + e) * SITE_BLOCKS;
+        for (int b = lane; b < SITE_BLOCKS; b += 64)
+            row[b] = src[b];
+        __syncthreads();
+        if (lane == 0)
+            total += omega[p] * fold_serial16(row, SITE_BLOCKS, 0.0);
+        __syncthreads();
     }
-    out[e] = total;
+    if (lane == 0)
+        out[e] = total;
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -311,22 +314,29 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
     }
 }
 
-// Per entry: each plane's tiles in index order, times the plane's weight, the planes in order.
-__global__ __launch_bounds__(64) void k_gram_reduce(const double* partial, const double* omega, Ranges tiles,
-                                                    int entries, double* out)
+// One 256-lane workgroup per entry: each plane's tiles summed by the lanes (strided) and a tree, times the plane's
+// weight, the planes added in order.
+__global__ __launch_bounds__(256) void k_gram_reduce(const double* partial, const double* omega, Ranges tiles,
+                                                     int entries, double* out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= entries)
-        return;
+    __shared__ double part[4];
+    const int e = blockIdx.x, tid = threadIdx.x;
     double total = 0.0;
     for (int p = 0; p < tiles.n; p++)
     {
         double sum = 0.0;
-        for (long long t = tiles.start[p]; t < tiles.start[p + 1]; t++)
+        for (long long t = tiles.start[p] + tid; t < tiles.start[p + 1]; t += 256)
             sum += partial[(size_t)t * entries + e];
-        total += omega[p] * sum;
+        for (int off = 32; off > 0; off >>= 1)
+            sum += __shfl_xor(sum, off);
+        if (tid % 64 == 0)
+            part[tid / 64] = sum;
+        __syncthreads();
+        total += omega[p] * (part[0] + part[1] + part[2] + part[3]);
+        __syncthreads();
     }
-    out[e] = total;
+    if (tid == 0)
+        out[e] = total;
 }
 
 template <int C0, int C1>
@@ -402,8 +412,8 @@ void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& ac
     HIP_OK(hipGetLastError());
     double* d_omega = c.ls_sum.p + a.entries;
     HIP_OK(hipMemcpyAsync(d_omega, omega, sizeof(double) * c.planes.n, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_gram_reduce, dim3(blocks_for(a.entries, 64)), dim3(64), 0, c.stream,
-                       (const double*)a.partial, (const double*)d_omega, a.tiles, a.entries, c.ls_sum.p);
+    hipLaunchKernelGGL(k_gram_reduce, dim3(a.entries), dim3(256), 0, c.stream, (const double*)a.partial,
+                       (const double*)d_omega, a.tiles, a.entries, c.ls_sum.p);
     HIP_OK(hipGetLastError());
     acc.resize((size_t)a.entries);
     HIP_OK(hipMemcpyAsync(acc.data(), c.ls_sum.p, sizeof(double) * a.entries, hipMemcpyDeviceToHost, c.stream));
@@ -446,7 +456,7 @@ void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
     // omega travels through the scalar workspace's tail: nplanes doubles uploaded with the launch.
     double* d_omega = c.ls_sum.p + a.entries;
     HIP_OK(hipMemcpyAsync(d_omega, omega, sizeof(double) * c.planes.n, hipMemcpyHostToDevice, c.stream));
-    hipLaunchKernelGGL(k_ls_reduce, dim3(blocks_for(a.entries, 64)), dim3(64), 0, c.stream, (const double*)a.partial,
+    hipLaunchKernelGGL(k_ls_reduce, dim3(a.entries), dim3(64), 0, c.stream, (const double*)a.partial,
                        (const double*)d_omega, c.planes.n, a.entries, c.ls_sum.p);
     HIP_OK(hipGetLastError());
     acc.resize((size_t)a.entries);

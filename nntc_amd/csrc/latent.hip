@@ -100,26 +100,6 @@ __device__ __forceinline__ double fold_lanes(double acc, double v, int count)
     return acc;
 }
 
-// acc plus t[0] + t[1] + ... + t[n-1], added in index order; sixteen values are read ahead of their sixteen dependent
-// additions so the chain waits on the adds alone.
-__device__ __forceinline__ double fold_serial16(const double* t, int n, double acc)
-{
-    int i = 0;
-    for (; i + 16 <= n; i += 16)
-    {
-        double v[16];
-#pragma unroll
-        for (int u = 0; u < 16; u++)
-            v[u] = t[i + u];
-#pragma unroll
-        for (int u = 0; u < 16; u++)
-            acc += v[u];
-    }
-    for (; i < n; i++)
-        acc += t[i];
-    return acc;
-}
-
 // The fast form's wavefront tree: every lane ends holding the sum of all 64.
 __device__ __forceinline__ double tree_lanes(double v)
 {

@@ -382,6 +382,26 @@ struct Ranges
     long long start[MAX_PLANES + 1];
 };
 
+// acc plus t[0] + t[1] + ... + t[n-1], added in index order (the exact form's serial folds); sixteen values are read
+// ahead of their sixteen dependent additions so the chain waits on the adds alone.
+NNTC_HD double fold_serial16(const double* t, int n, double acc)
+{
+    int i = 0;
+    for (; i + 16 <= n; i += 16)
+    {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            v[u] = t[i + u];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            acc += v[u];
+    }
+    for (; i < n; i++)
+        acc += t[i];
+    return acc;
+}
+
 // The plane whose range holds global index g (ranges are contiguous and increasing).
 NNTC_HD int range_plane(const Ranges& r, long long g)
 {
