@@ -179,33 +179,26 @@ struct RefineLane
 {
     const double* S;
     long long sn;
-    BlockPos bp;
-    int px0, py0, w0, h0;   // the block's first texel and the plane's size
+    const long long* pos;   // the lane's LDS column of its sixteen texels' stencil positions: pos[u * 64]
+    unsigned int inside;    // bit u: texel u of the block is in the plane
 
-    __device__ __forceinline__ bool inside(int lx, int ly) const
-    {
-        const int px = px0 + lx, py = py0 + ly;
-        return px >= 0 && px < w0 && py >= 0 && py < h0;
-    }
+    __device__ __forceinline__ bool in(int u) const { return (inside >> u) & 1u; }
+    __device__ __forceinline__ long long at(int u) const { return pos[u * 64]; }
 
-    // Entry e of the stencil block `slot` of texel (lx, ly).
-    __device__ __forceinline__ double entry(int lx, int ly, int slot, int e) const
+    // a2's entry (texel u, channel i; texel u + (dx, dy), channel j) for a neighbour inside the block; dx, dy, i and j
+    // are constants after unrolling, so the entry's offset is uniform and the address one add. The forward slots
+    // (1, 0), (-1, 1), (0, 1), (1, 1) are u's blocks, the other four the neighbour's, transposed; zero unless both
+    // texels are in the plane.
+    __device__ __forceinline__ double a2(int u, int i, int dx, int dy, int j) const
     {
-        return S[(long long)(slot * C0 * C0 + e) * sn + bp.at(lx, ly)];
-    }
-
-    // a2's entry (texel (ux, uy), channel i; texel (vx, vy), channel j) for two texels of the block that are equal or
-    // 8-neighbours: the forward slots (1, 0), (-1, 1), (0, 1), (1, 1) are u's blocks, the other four v's, transposed.
-    __device__ __forceinline__ double a2(int ux, int uy, int i, int vx, int vy, int j) const
-    {
-        if (!inside(ux, uy) || !inside(vx, vy))
+        const int v = u + 4 * dy + dx;
+        if (!(in(u) && in(v)))
             return 0.0;
-        const int dx = vx - ux, dy = vy - uy;
         if (dx == 0 && dy == 0)
-            return entry(ux, uy, 0, i * C0 + j);
+            return S[(long long)(i * C0 + j) * sn + at(u)];
         if (dy == 1 || (dy == 0 && dx == 1))
-            return entry(ux, uy, dy == 0 ? 1 : dx + 3, i * C0 + j);
-        return entry(vx, vy, dy == 0 ? 1 : 3 - dx, j * C0 + i);
+            return S[(long long)((dy == 0 ? 1 : dx + 3) * C0 * C0 + i * C0 + j) * sn + at(u)];
+        return S[(long long)((dy == 0 ? 1 : 3 - dx) * C0 * C0 + j * C0 + i) * sn + at(v)];
     }
 };
 
@@ -226,13 +219,13 @@ __device__ __forceinline__ float level0_value(float lo, float hi, int r0, int r1
     return bc0_value(lo, hi, bc4_entry8(r0, r1, s));
 }
 
-// The texels of the block that are texel (x, y) itself or its 8-neighbours, in increasing index order.
-#define FOR_BLOCK_NEIGHBOURS(x, y, nx, ny, BODY)                                                                       \
-    _Pragma("unroll") for (int dy_ = -1; dy_ <= 1; dy_++)                                                              \
-        _Pragma("unroll") for (int dx_ = -1; dx_ <= 1; dx_++)                                                          \
+// The block's texels (x + dx, y + dy) that are texel (x, y) itself or its 8-neighbours, in increasing index order
+// (dy, then dx); dx and dy are constants in BODY.
+#define FOR_NEIGHBOURS(x, y, dx, dy, BODY)                                                                             \
+    _Pragma("unroll") for (int dy = -1; dy <= 1; dy++)                                                                 \
+        _Pragma("unroll") for (int dx = -1; dx <= 1; dx++)                                                             \
         {                                                                                                              \
-            const int nx = (x) + dx_, ny = (y) + dy_;                                                                  \
-            if (nx >= 0 && nx < 4 && ny >= 0 && ny < 4)                                                                \
+            if ((x) + dx >= 0 && (x) + dx < 4 && (y) + dy >= 0 && (y) + dy < 4)                                        \
             {                                                                                                          \
                 BODY                                                                                                   \
             }                                                                                                          \
@@ -242,9 +235,10 @@ template <int C0>
 __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int colour_index)
 {
     constexpr int N = 16 * C0;
-    // The block's two n-vectors live in LDS, one column per lane (entry i of lane l at [i * 64 + l]): no lane reads
-    // another's column, so no barrier is needed.
+    // The block's two n-vectors and its texels' positions live in LDS, one column per lane (entry i of lane l at
+    // [i * 64 + l]): no lane reads another's column, so no barrier is needed.
     __shared__ double s_a1[N * 64], s_uv[N * 64];
+    __shared__ long long s_pos[16 * 64];
     double* a1 = s_a1 + threadIdx.x;
     double* uv = s_uv + threadIdx.x;
 
@@ -259,19 +253,32 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     const int cbx = (bxn - cx + 1) / 2;
     const int gx = cx + 2 * (int)(ci % cbx);
     const int gy = cy + 2 * (int)(ci / cbx);
-    const int w0 = g.w0;
+    const int w0 = g.w0, h0 = g.h0;
     const long long base = g.off0;
+    const int px0 = gx * 4, py0 = gy * 4;
+    const double* S = a.stencil;
+    const long long sn = a.sm.sn;
+
+    BlockPos bp;
+    bp.init(a.sm.cpos[p], w0, gx, gy);
+    auto inside_xy = [=](int lx, int ly) {
+        return px0 + lx >= 0 && px0 + lx < w0 && py0 + ly >= 0 && py0 + ly < h0;
+    };
+    // The chain index of texel (lx, ly) of the block (or of a neighbour just outside it).
+    auto tix = [=](int lx, int ly) { return base + (long long)(py0 + ly) * w0 + (px0 + lx); };
 
     RefineLane<C0> R;
-    R.S = a.stencil;
-    R.sn = a.sm.sn;
-    R.px0 = gx * 4;
-    R.py0 = gy * 4;
-    R.w0 = w0;
-    R.h0 = g.h0;
-    R.bp.init(a.sm.cpos[p], w0, gx, gy);
-    // The chain index of texel (lx, ly) of the block (or of a neighbour just outside it).
-    auto tix = [=](int lx, int ly) { return base + (long long)(R.py0 + ly) * w0 + (R.px0 + lx); };
+    R.S = S;
+    R.sn = sn;
+    R.pos = s_pos + threadIdx.x;
+    R.inside = 0;
+#pragma unroll
+    for (int u = 0; u < 16; u++)
+    {
+        s_pos[u * 64 + threadIdx.x] = bp.at(u & 3, u >> 2);
+        if (inside_xy(u & 3, u >> 2))
+            R.inside |= 1u << u;
+    }
 
     // ---- a1: each texel's gradient plus the shares of its neighbours outside the block, slot by slot, the forward
     // neighbour's then the backward one's (slot 0 is the texel itself, inside the block).
@@ -282,7 +289,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
 #pragma unroll
         for (int i = 0; i < C0; i++)
             uv[(u * C0 + i) * 64] = 0.0;
-        if (!R.inside(lx, ly))
+        if (!R.in(u))
         {
 #pragma unroll
             for (int i = 0; i < C0; i++)
@@ -290,6 +297,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             continue;
         }
         const long long ti = tix(lx, ly);
+        const long long pu = R.at(u);
         double acc[C0];
 #pragma unroll
         for (int i = 0; i < C0; i++)
@@ -300,7 +308,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             int dx, dy;
             stencil_offset(slot, dx, dy);
             const int fx = lx + dx, fy = ly + dy;
-            if (!(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && R.inside(fx, fy))
+            if (!(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && inside_xy(fx, fy))
             {
                 const double* xn = a.delta + tix(fx, fy) * C0;
                 double xv[C0];
@@ -311,12 +319,13 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += R.entry(lx, ly, slot, i * C0 + j) * xv[j];
+                        acc[i] += S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + pu] * xv[j];
             }
             const int bx = lx - dx, by = ly - dy;
-            if (!(bx >= 0 && bx < 4 && by >= 0 && by < 4) && R.inside(bx, by))
+            if (!(bx >= 0 && bx < 4 && by >= 0 && by < 4) && inside_xy(bx, by))
             {
                 const double* xn = a.delta + tix(bx, by) * C0;
+                const long long pb = bp.at(bx, by);
                 double xv[C0];
 #pragma unroll
                 for (int j = 0; j < C0; j++)
@@ -325,7 +334,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += R.entry(bx, by, slot, j * C0 + i) * xv[j];
+                        acc[i] += S[(long long)(slot * C0 * C0 + j * C0 + i) * sn + pb] * xv[j];
             }
         }
 #pragma unroll
@@ -338,18 +347,18 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
 #pragma unroll 1
     for (int v = 0; v < 16; v++)
     {
-        const int vx = v & 3, vy = v >> 2;
-        if (!R.inside(vx, vy))
+        if (!R.in(v))
             continue;
+        const int vx = v & 3, vy = v >> 2;
         const long long tv = tix(vx, vy);
 #pragma unroll
         for (int j = 0; j < C0; j++)
         {
             const double dv = (double)a.v0[tv * C0 + j] - (double)a.prev[tv * C0 + j];
-            FOR_BLOCK_NEIGHBOURS(vx, vy, ux, uy, {
-                _Pragma("unroll")
-                for (int i = 0; i < C0; i++)
-                    uv[((uy * 4 + ux) * C0 + i) * 64] += R.a2(ux, uy, i, vx, vy, j) * dv;
+            FOR_NEIGHBOURS(vx, vy, dx, dy, {
+                const int u = v + 4 * dy + dx;
+                _Pragma("unroll") for (int i = 0; i < C0; i++)
+                    uv[(u * C0 + i) * 64] += R.a2(u, i, -dx, -dy, j) * dv;
             })
         }
     }
@@ -377,24 +386,25 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     {
         const float lo = a.lo0[c], hi = a.hi0[c];
         const double step = ((double)hi - (double)lo) / 255.0;
-        double bp = 0.0, bq = 0.0, pap = 0.0, paq = 0.0, qaq = 0.0;
+        const unsigned long long sc = sel[c];
+        double bp_ = 0.0, bq = 0.0, pap = 0.0, paq = 0.0, qaq = 0.0;
 #pragma unroll 1
         for (int u = 0; u < 16; u++)
         {
             const int ux = u & 3, uy = u >> 2;
             const int idx = u * C0 + c;
-            const double aw = bc4_weight8(sel_get(sel[c], u));
+            const double aw = bc4_weight8(sel_get(sc, u));
             const double pp = step * aw;
             const double qq = step * (1.0 - aw);
             const double b = a1[idx * 64] + uv[idx * 64];
             double ap = 0.0, aq = 0.0;
-            FOR_BLOCK_NEIGHBOURS(ux, uy, vx, vy, {
-                const double e = R.a2(ux, uy, c, vx, vy, c);
-                const double aj = bc4_weight8(sel_get(sel[c], vy * 4 + vx));
+            FOR_NEIGHBOURS(ux, uy, dx, dy, {
+                const double e = R.a2(u, c, dx, dy, c);
+                const double aj = bc4_weight8(sel_get(sc, u + 4 * dy + dx));
                 ap += e * step * aj;
                 aq += e * step * (1.0 - aj);
             })
-            bp += b * pp;
+            bp_ += b * pp;
             bq += b * qq;
             pap += pp * ap;
             paq += pp * aq;
@@ -405,8 +415,8 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         double d0 = 0.0, d1 = 0.0;
         if (det > 1e-12 * pap * qaq)
         {
-            d0 = (paq * bq - qaq * bp) / det;
-            d1 = (paq * bp - pap * bq) / det;
+            d0 = (paq * bq - qaq * bp_) / det;
+            d1 = (paq * bp_ - pap * bq) / det;
         }
         const double v0r = (double)r0 + d0, v1r = (double)r1 + d1;
         const int c0r = (int)floor(v0r + 0.5), c1r = (int)floor(v1r + 0.5);
@@ -422,7 +432,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 if (x0 <= x1)
                     continue;
                 const double f0 = (double)(x0 - r0), f1 = (double)(x1 - r1);
-                const double de = 2.0 * (bp * f0 + bq * f1) + pap * f0 * f0 + 2.0 * paq * f0 * f1 + qaq * f1 * f1;
+                const double de = 2.0 * (bp_ * f0 + bq * f1) + pap * f0 * f0 + 2.0 * paq * f0 * f1 + qaq * f1 * f1;
                 if (de < best)
                 {
                     best = de;
@@ -435,7 +445,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         // The candidate weighed again in the arithmetic the plane is written in: texel v moves by
         // next(v) - now(v), the two palettes' values at its selector.
         auto dlt = [=](int v) {
-            const int s = sel_get(sel[c], v);
+            const int s = sel_get(sc, v);
             return (double)level0_value(lo, hi, best0, best1, s) - (double)level0_value(lo, hi, r0, r1, s);
         };
         double exact = 0.0;
@@ -445,7 +455,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             const int ux = u & 3, uy = u >> 2;
             const int idx = u * C0 + c;
             double quad = 0.0;
-            FOR_BLOCK_NEIGHBOURS(ux, uy, vx, vy, { quad += R.a2(ux, uy, c, vx, vy, c) * dlt(vy * 4 + vx); })
+            FOR_NEIGHBOURS(ux, uy, dx, dy, { quad += R.a2(u, c, dx, dy, c) * dlt(u + 4 * dy + dx); })
             exact += dlt(u) * (2.0 * (a1[idx * 64] + uv[idx * 64]) + quad);
         }
         if (!(exact < 0.0))
@@ -459,7 +469,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             for (int i = 0; i < C0; i++)
             {
                 double add = 0.0;
-                FOR_BLOCK_NEIGHBOURS(ux, uy, vx, vy, { add += R.a2(ux, uy, i, vx, vy, c) * dlt(vy * 4 + vx); })
+                FOR_NEIGHBOURS(ux, uy, dx, dy, { add += R.a2(u, i, dx, dy, c) * dlt(u + 4 * dy + dx); })
                 uv[(u * C0 + i) * 64] += add;
             }
         }
@@ -484,7 +494,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             double diag[16];
 #pragma unroll
             for (int t = 0; t < 16; t++)
-                diag[t] = R.a2(t & 3, t >> 2, c, t & 3, t >> 2, c);
+                diag[t] = R.a2(t, c, 0, 0, c);
 #pragma unroll
             for (int t = 0; t < 16; t++)
             {
@@ -510,10 +520,10 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 if (best < 0.0)
                 {
                     taken += best;
-                    FOR_BLOCK_NEIGHBOURS(tx, ty, ux, uy, {
-                        _Pragma("unroll")
-                        for (int ii = 0; ii < C0; ii++)
-                            uv[((uy * 4 + ux) * C0 + ii) * 64] += R.a2(ux, uy, ii, tx, ty, c) * move;
+                    FOR_NEIGHBOURS(tx, ty, dx, dy, {
+                        const int u = t + 4 * dy + dx;
+                        _Pragma("unroll") for (int ii = 0; ii < C0; ii++)
+                            uv[(u * C0 + ii) * 64] += R.a2(u, ii, -dx, -dy, c) * move;
                     })
                     sel[c] = sel_set(sel[c], t, which);
                 }
@@ -534,10 +544,9 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
 #pragma unroll 1
     for (int u = 0; u < 16; u++)
     {
-        const int lx = u & 3, ly = u >> 2;
-        if (!R.inside(lx, ly))
+        if (!R.in(u))
             continue;
-        const long long ti = tix(lx, ly);
+        const long long ti = tix(u & 3, u >> 2);
 #pragma unroll
         for (int c = 0; c < C0; c++)
         {
