@@ -244,7 +244,8 @@ long long nntc_hip_objective_passes(void);
  * (src/export.cpp:294-331) and the packing error is measured against it (src/export.cpp:283-292). Model::k0 is the
  * level-0 index plane as the encode last downloaded it, so here it is an argument: `k0_mirror[plane]` points at the
  * caller's copy of each plane's indices, or k0_mirror is NULL, which means "the device's level-0 indices as they stand"
- * for bc_pack_prepare and "the mirror the previous bc_* call used" for the other two. */
+ * for a seeding bc_pack_prepare (the reference downloads Model::k0 right before each) and "the mirror the previous
+ * bc_* call used" otherwise. */
 int nntc_hip_bc_pack_prepare(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, int seed,
                              const uint8_t* const* k0_mirror, double* pack_psnr, size_t* pack_texels, double* ms);
 int nntc_hip_bc_refine(nntc_hip_ctx* ctx, int passes, const uint8_t* const* k0_mirror, nntc_bc_refine_report* rep,
@@ -261,6 +262,13 @@ int nntc_hip_bc_blocks_download(nntc_hip_ctx* ctx, int plane, int file, uint8_t*
 int nntc_hip_bc_state_save(nntc_hip_ctx* ctx, int plane, uint8_t* ep, size_t n_ep, uint8_t* sel, size_t n_sel);
 int nntc_hip_bc_state_restore(nntc_hip_ctx* ctx, int plane, const uint8_t* ep, size_t n_ep, const uint8_t* sel,
                               size_t n_sel);
+
+/* Replace the outer repack loop's host round trip (src/main.cpp:2561-2588: every plane's level-0 values and indices,
+ * level-1 values and indices and BC endpoints and selectors downloaded before a pass and uploaded again when it is
+ * rejected, with Model::k0 kept beside them) by a copy on the device: state_snapshot keeps all of it, the index mirror
+ * included, and state_restore puts it back. One snapshot is held; restoring without one is an error. */
+int nntc_hip_state_snapshot(nntc_hip_ctx* ctx);
+int nntc_hip_state_restore(nntc_hip_ctx* ctx);
 
 /* Per-kernel timing (not in the reference's seam; the benchmark's instrument). While enabled, every launch of the
  * arm's named kernels is bracketed by HIP events on the arm's stream and its algorithmic bytes and flops are added

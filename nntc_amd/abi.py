@@ -169,6 +169,8 @@ SIGNATURES = {
     "bc_blocks_download": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _u8p, C.c_size_t, _P(C.c_size_t)]),
     "bc_state_save": (C.c_int, [C.c_void_p, C.c_int, _u8p, C.c_size_t, _u8p, C.c_size_t]),
     "bc_state_restore": (C.c_int, [C.c_void_p, C.c_int, _u8p, C.c_size_t, _u8p, C.c_size_t]),
+    "state_snapshot": (C.c_int, [C.c_void_p]),
+    "state_restore": (C.c_int, [C.c_void_p]),
 }
 
 

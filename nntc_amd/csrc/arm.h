@@ -146,6 +146,12 @@ struct Ctx
     DevBuf<uint8_t> bc_ep, bc_sel;         // blocks * c0 * 2, blocks * c0 * 16
     DevBuf<uint8_t> bc_blocks;             // blocks * files * 16 (file f of block b at (b * files + f) * 16)
     DevBuf<uint8_t> k0_mirror;             // the host's Model::k0 as the bc calls read it
+    struct Snapshot                         // the outer repack loop's saved state (state_snapshot / state_restore)
+    {
+        DevBuf<float> v0, v1;
+        DevBuf<uint8_t> k0, k1, ep, sel, mirror;
+        bool held = false;
+    } snap;
     DevBuf<double> bc_taken;               // per block: the accepted decrease
     DevBuf<double> bc_se;                  // per plane: the packing error
     DevBuf<float> work_f;                  // staging for grid fits (values of one channel)

@@ -1248,7 +1248,7 @@ int nntc_hip_bc_pack_prepare(nntc_hip_ctx* h, const nntc_model_state* st, int k,
         Ctx& c = h->c;
         double ms = assemble_for_refine(c, st, k);
         const double t0 = now_ms();
-        set_k0_mirror(c, k0_mirror, true);
+        set_k0_mirror(c, k0_mirror, seed != 0);
         if (seed)
             bc_seed(c);
         bc_decode(c);
@@ -1378,6 +1378,54 @@ int nntc_hip_bc_state_restore(nntc_hip_ctx* h, int plane, const uint8_t* ep, siz
             throw Refused { "BC state restore was handed more blocks than the plane holds" };
         HIP_OK(hipMemcpyAsync(c.bc_ep.p + g.offb * c.c0 * 2, ep, n_ep, hipMemcpyHostToDevice, c.stream));
         HIP_OK(hipMemcpyAsync(c.bc_sel.p + g.offb * c.c0 * 16, sel, n_sel, hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int nntc_hip_state_snapshot(nntc_hip_ctx* h)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        Ctx::Snapshot& s = c.snap;
+        if (!s.v0.p)
+        {
+            s.v0.alloc(c.v0.n);
+            s.v1.alloc(c.v1.n);
+            s.k0.alloc(c.k0.n);
+            s.k1.alloc(c.k1.n);
+            s.ep.alloc(c.bc_ep.n);
+            s.sel.alloc(c.bc_sel.n);
+            s.mirror.alloc(c.k0_mirror.n);
+            c.bytes += s.v0.bytes() + s.v1.bytes() + s.k0.bytes() + s.k1.bytes() + s.ep.bytes() + s.sel.bytes() +
+                       s.mirror.bytes();
+        }
+        const auto d2d = hipMemcpyDeviceToDevice;
+        HIP_OK(hipMemcpyAsync(s.v0.p, c.v0.p, c.v0.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(s.v1.p, c.v1.p, c.v1.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(s.k0.p, c.k0.p, c.k0.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(s.k1.p, c.k1.p, c.k1.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(s.ep.p, c.bc_ep.p, c.bc_ep.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(s.sel.p, c.bc_sel.p, c.bc_sel.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(s.mirror.p, c.k0_mirror.p, c.k0_mirror.bytes(), d2d, c.stream));
+        s.held = true;
+    });
+}
+
+int nntc_hip_state_restore(nntc_hip_ctx* h)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        Ctx::Snapshot& s = c.snap;
+        if (!s.held)
+            throw Refused { "state_restore: no snapshot is held" };
+        const auto d2d = hipMemcpyDeviceToDevice;
+        HIP_OK(hipMemcpyAsync(c.v0.p, s.v0.p, c.v0.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(c.v1.p, s.v1.p, c.v1.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(c.k0.p, s.k0.p, c.k0.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(c.k1.p, s.k1.p, c.k1.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(c.bc_ep.p, s.ep.p, c.bc_ep.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(c.bc_sel.p, s.sel.p, c.bc_sel.bytes(), d2d, c.stream));
+        HIP_OK(hipMemcpyAsync(c.k0_mirror.p, s.mirror.p, c.k0_mirror.bytes(), d2d, c.stream));
         HIP_OK(hipStreamSynchronize(c.stream));
     });
 }

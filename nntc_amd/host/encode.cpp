@@ -123,18 +123,6 @@ using namespace nntc_host;
 namespace
 {
 
-// The stored planes' values and indices, the decoder, the grids and the BC state: what a rejected outer pass is put
-// back to (src/main.cpp:2561-2588).
-struct Saved
-{
-    std::vector<std::vector<float>> v0, v1;
-    std::vector<std::vector<uint8_t>> k0dev, k1dev, ep, sel;
-    std::vector<std::vector<uint8_t>> k0;
-    nntc_model_state state;
-    double pack_psnr = 0.0, pack_psnr_seed = 0.0;
-    size_t pack_texels = 0;
-};
-
 void run_encode(Encoder& E)
 {
     Seam& S = E.seam;
@@ -314,22 +302,9 @@ void run_encode(Encoder& E)
         cks(S.quantise_level0(d, &st, o.q1_range), "quantise_level0");
     cks(S.solve_decoder(d, &st, k, ps, &t), "solve_decoder");
     ms_a += t;
-    std::vector<std::vector<uint8_t>> k0((size_t)np);
-    auto download_k0 = [&](std::vector<std::vector<uint8_t>>& out) {
-        for (int p = 0; p < np; p++)
-        {
-            const nntc_plane_size& pl = E.desc.planes[p];
-            out[(size_t)p].resize((size_t)pl.w0 * pl.h0 * E.desc.c0);
-            cks(S.level0_download(d, p, out[(size_t)p].data(), out[(size_t)p].size(), nullptr), "level0_download");
-        }
-    };
-    download_k0(k0);
-    std::vector<const uint8_t*> k0p((size_t)np);
-    auto mirror = [&]() {
-        for (int p = 0; p < np; p++)
-            k0p[(size_t)p] = k0[(size_t)p].data();
-        return k0p.data();
-    };
+    // Model::k0, the index mirror the bc calls read, stays on the device: a seeding pack takes the device's indices as
+    // they stand (where the reference downloads them), the other calls the mirror already held.
+    auto mirror = []() -> const uint8_t* const* { return nullptr; };
 
     // ---- the post-fit BC pack, its refinement and the refit
     double pack_psnr = 0.0, pack_psnr_seed = 0.0, ms_pack = 0.0, ms_refine = 0.0;
@@ -377,38 +352,12 @@ void run_encode(Encoder& E)
     for (int pass = 1; pass <= o.bc_outer; pass++)
     {
         rep.outer_tried++;
-        Saved keep;
-        keep.v0.resize((size_t)np);
-        keep.v1.resize((size_t)np);
-        keep.k0dev.resize((size_t)np);
-        keep.k1dev.resize((size_t)np);
-        keep.ep.resize((size_t)np);
-        keep.sel.resize((size_t)np);
-        for (int p = 0; p < np; p++)
-        {
-            const nntc_plane_size& pl = E.desc.planes[p];
-            const size_t n0 = (size_t)pl.w0 * pl.h0 * E.desc.c0, n1 = (size_t)pl.w1 * pl.h1 * E.desc.c1;
-            const size_t nb = (size_t)((pl.w0 + 3) / 4) * ((pl.h0 + 3) / 4);
-            keep.v0[(size_t)p].resize(n0);
-            keep.k0dev[(size_t)p].resize(n0);
-            keep.v1[(size_t)p].resize(n1);
-            keep.k1dev[(size_t)p].resize(n1);
-            keep.ep[(size_t)p].resize(nb * E.desc.c0 * 2);
-            keep.sel[(size_t)p].resize(nb * E.desc.c0 * 16);
-            cks(S.level0_download_values(d, p, keep.v0[(size_t)p].data(), n0, nullptr), "level0_download_values");
-            cks(S.level0_download(d, p, keep.k0dev[(size_t)p].data(), n0, nullptr), "level0_download");
-            cks(S.level1_download(d, p, keep.v1[(size_t)p].data(), n1, nullptr), "level1_download");
-            cks(S.level1_download_indices(d, p, keep.k1dev[(size_t)p].data(), n1, nullptr),
-                "level1_download_indices");
-            cks(S.bc_state_save(d, p, keep.ep[(size_t)p].data(), keep.ep[(size_t)p].size(),
-                                keep.sel[(size_t)p].data(), keep.sel[(size_t)p].size()),
-                "bc_state_save");
-        }
-        keep.k0 = k0;
-        keep.state = st;
-        keep.pack_psnr = pack_psnr;
-        keep.pack_psnr_seed = pack_psnr_seed;
-        keep.pack_texels = pack_texels;
+        // The state a rejected pass is put back to, kept on the device (the reference's host round trip,
+        // src/main.cpp:2561-2588).
+        cks(S.state_snapshot(d), "state_snapshot");
+        const nntc_model_state keep_state = st;
+        const double keep_pack_psnr = pack_psnr, keep_pack_psnr_seed = pack_psnr_seed;
+        const size_t keep_pack_texels = pack_texels;
 
         refit_level1();
         cks(S.solve_decoder(d, &st, k, ps, &t), "solve_decoder");
@@ -421,7 +370,6 @@ void run_encode(Encoder& E)
             cks(S.snap_level0_on_grid(d, &st), "snap_level0_on_grid");
         }
         ms_c += t;
-        download_k0(k0);
         cks(S.bc_pack_prepare(d, &st, k, 1, mirror(), &pack_psnr_seed, &pack_texels, &t), "bc_pack_prepare");
         ms_pack += t;
         cks(S.bc_refine(d, o.bc_refine, mirror(), rr.data(), &pack_psnr, &pack_texels, &t), "bc_refine");
@@ -440,23 +388,11 @@ void run_encode(Encoder& E)
         if (!o.quiet)
             std::printf("level 0 outer : pass %d rejected: shipped E %.9e -> E %.9e, the previous pass restored\n",
                         pass, e_ship, probe.e);
-        for (int p = 0; p < np; p++)
-        {
-            cks(S.level0_upload(d, p, keep.v0[(size_t)p].data(), keep.v0[(size_t)p].size()), "level0_upload");
-            cks(S.level0_upload_indices(d, p, keep.k0dev[(size_t)p].data(), keep.k0dev[(size_t)p].size()),
-                "level0_upload_indices");
-            cks(S.level1_upload(d, p, keep.v1[(size_t)p].data(), keep.v1[(size_t)p].size()), "level1_upload");
-            cks(S.level1_upload_indices(d, p, keep.k1dev[(size_t)p].data(), keep.k1dev[(size_t)p].size()),
-                "level1_upload_indices");
-            cks(S.bc_state_restore(d, p, keep.ep[(size_t)p].data(), keep.ep[(size_t)p].size(),
-                                   keep.sel[(size_t)p].data(), keep.sel[(size_t)p].size()),
-                "bc_state_restore");
-        }
-        k0 = keep.k0;
-        st = keep.state;
-        pack_psnr = keep.pack_psnr;
-        pack_psnr_seed = keep.pack_psnr_seed;
-        pack_texels = keep.pack_texels;
+        cks(S.state_restore(d), "state_restore");
+        st = keep_state;
+        pack_psnr = keep_pack_psnr;
+        pack_psnr_seed = keep_pack_psnr_seed;
+        pack_texels = keep_pack_texels;
         cks(S.upload_decoder_to_device(d, &st), "upload_decoder_to_device");
         cks(S.upload_grids_to_device(d, &st), "upload_grids_to_device");
         // The blocks the file holds are a function of the restored endpoints and selectors.

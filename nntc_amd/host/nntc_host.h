@@ -60,7 +60,9 @@ struct Failure
     X(bc_blocks_from_device)                                                                                           \
     X(bc_blocks_download)                                                                                              \
     X(bc_state_save)                                                                                                   \
-    X(bc_state_restore)
+    X(bc_state_restore)                                                                                                \
+    X(state_snapshot)                                                                                                  \
+    X(state_restore)
 
 struct Seam
 {

@@ -27,6 +27,12 @@ struct RefCtx
     Model m;
     be::Device* d = nullptr;
     Image base;   // the source's base level, for the texture error sums
+    struct Saved  // state_snapshot's copy
+    {
+        std::vector<std::vector<float>> v0, v1;
+        std::vector<std::vector<uint8_t>> k0, k1, ep, sel, mirror;
+        bool held = false;
+    } saved;
 };
 
 thread_local std::string g_error;
@@ -563,8 +569,8 @@ long long nntc_ref_objective_passes(void)
     return be::objective_passes();
 }
 
-// Model::k0 as the C ABI defines it: the caller's planes when given; otherwise, for the prepare, the device's indices
-// as they stand, and for the later calls the mirror already held.
+// Model::k0 as the C ABI defines it: the caller's planes when given; otherwise, for a seeding prepare, the device's
+// indices as they stand, and for the other calls the mirror already held.
 static void set_k0_mirror(RefCtx* r, const uint8_t* const* k0, bool snapshot)
 {
     Model& m = r->m;
@@ -587,7 +593,7 @@ int nntc_ref_bc_pack_prepare(nntc_hip_ctx* ctx, const nntc_model_state* st, int 
 {
     REF_CTX;
     state_to_model(st, m);
-    set_k0_mirror(r, k0_mirror, true);
+    set_k0_mirror(r, k0_mirror, seed != 0);
     double psnr = 0.0;
     size_t texels = 0;
     const double t = be::bc_pack_prepare(r->d, m, k, seed != 0, psnr, texels);
@@ -671,6 +677,51 @@ int nntc_ref_bc_state_restore(nntc_hip_ctx* ctx, int plane, const uint8_t* ep, s
     e[(size_t)plane].assign(ep, ep + n_ep);
     s[(size_t)plane].assign(sel, sel + n_sel);
     be::bc_state_restore(r->d, m, e, s);
+    return 0;
+}
+
+// The reference's own round trip (src/main.cpp:2561-2588): every plane downloaded into the context, Model::k0 beside
+// them, and uploaded back on restore.
+int nntc_ref_state_snapshot(nntc_hip_ctx* ctx)
+{
+    REF_CTX;
+    RefCtx::Saved& s = r->saved;
+    const size_t np = m.planes.size();
+    s.v0.assign(np, {});
+    s.v1.assign(np, {});
+    s.k0.assign(np, {});
+    s.k1.assign(np, {});
+    for (size_t p = 0; p < np; p++)
+    {
+        be::level0_download_values(r->d, m, (int)p, s.v0[p]);
+        be::level0_download(r->d, m, (int)p, s.k0[p]);
+        be::level1_download(r->d, m, (int)p, s.v1[p]);
+        be::level1_download_indices(r->d, m, (int)p, s.k1[p]);
+    }
+    be::bc_state_save(r->d, m, s.ep, s.sel);
+    s.mirror = m.k0;
+    s.held = true;
+    return 0;
+}
+
+int nntc_ref_state_restore(nntc_hip_ctx* ctx)
+{
+    REF_CTX;
+    RefCtx::Saved& s = r->saved;
+    if (!s.held)
+    {
+        g_error = "state_restore: no snapshot is held";
+        return 1;
+    }
+    for (size_t p = 0; p < m.planes.size(); p++)
+    {
+        be::level0_upload(r->d, m, (int)p, s.v0[p]);
+        be::level0_upload_indices(r->d, m, (int)p, s.k0[p]);
+        be::level1_upload(r->d, m, (int)p, s.v1[p]);
+        be::level1_upload_indices(r->d, m, (int)p, s.k1[p]);
+    }
+    be::bc_state_restore(r->d, m, s.ep, s.sel);
+    m.k0 = s.mirror;
     return 0;
 }
 
