@@ -362,17 +362,28 @@ __device__ __forceinline__ int merge_taps(const int* cx, const int* cy, const fl
 // Texel ti's five blocks and gradient. LEVEL0 false: a level-1 texel, every site whose level-1 taps reach it, C = c1,
 // CS = c0. LEVEL0 true: a level-0 texel (block (c')), the 3x3-pixel neighbourhood with the centre site read nearest as
 // one tap of weight 1, C = c0, CS = c1.
-template <bool LEVEL0, int C, int CS>
+//
+// G lanes share a texel. G = 1 (the exact form): the one lane visits the footprint's sites in the reference's order
+// (pixel row, pixel, site), so every sum receives the reference's operands in the reference's order. G > 1 (the fast
+// form): lane r of the group visits the sites r, r + G, ... of that order and the group's partial sums meet in a
+// fixed shuffle tree - deterministic, the last bits not the reference's. A level-1 texel's footprint holds about four
+// hundred sites; spread over sixteen lanes the gather has sixteen times the wavefronts to hide its loads behind.
+template <bool LEVEL0, int C, int CS, int G>
 __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
 {
     constexpr int NM = 1 + CS + CS * (CS + 1) / 2;
     const LatentView& L = a.L;
-    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= L.texels.start[L.texels.n])
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gi = gt / G;
+    const int sub = (int)(gt % G);
+    const long long total = L.texels.start[L.texels.n];
+    // With G > 1 every lane of a group must reach the shuffles, so a lane past the end idles instead of leaving.
+    if (G == 1 && gi >= total)
         return;
-    const int p = range_plane(L.texels, gi);
+    const bool live = gi < total;
+    const int p = range_plane(L.texels, live ? gi : total - 1);
     const PlaneGeom& g = L.planes.p[p];
-    const long long ti = gi - L.texels.start[p];
+    const long long ti = (live ? gi : total - 1) - L.texels.start[p];
     const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
     const int tw_ = LEVEL0 ? w0 : w1;
     const int tx = (int)(ti % tw_), ty = (int)(ti / tw_);
@@ -401,67 +412,104 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
         footprint_range(tx, w0, w1, px_lo, px_hi);
         footprint_range(ty, h0, h1, py_lo, py_hi);
     }
-
-    for (int py = py_lo; py <= py_hi; py++)
-        for (int px = px_lo; px <= px_hi; px++)
-            for (int j = 0; j <= a.k; j++)
+    if (!live)
+        py_hi = py_lo - 1;   // no sites: the lane only joins the shuffles
+    // The sites in the reference's order (pixel row, pixel, site): lane `sub` of the group takes every G-th.
+    const int nxp = px_hi - px_lo + 1;
+    const int visits = nxp * (py_hi - py_lo + 1) * per_pixel;
+    int px = px_lo, py = py_lo, j = 0;
+    for (int v = sub; v < visits; v += G)
+    {
+        if (G > 1)
+        {
+            const int pix = v / per_pixel;
+            j = v - pix * per_pixel;
+            px = px_lo + pix % nxp;
+            py = py_lo + pix / nxp;
+        }
+        else if (v > 0 && ++j > a.k)   // one lane: step to the next site without a division
+        {
+            j = 0;
+            if (++px > px_hi)
             {
-                float dx, dy;
-                site_delta(a.k, j, px, py, dx, dy);
-                int nx[4], ny[4];
-                float tw[4];
-                int taps;
-                if (LEVEL0 && j == 0)
-                {
-                    nx[0] = px;
-                    ny[0] = py;
-                    tw[0] = 1.0f;
-                    taps = 1;
-                }
-                else
-                {
-                    const Tap t = LEVEL0 ? tap_pixel(px, py, dx, dy, w0, h0, w0, h0)
-                                         : tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
-                    const int cx[4] = { t.x0, t.x1, t.x0, t.x1 };
-                    const int cy[4] = { t.y0, t.y0, t.y1, t.y1 };
-                    const float cwt[4] = { (1.0f - t.fx) * (1.0f - t.fy), t.fx * (1.0f - t.fy),
-                                           (1.0f - t.fx) * t.fy, t.fx * t.fy };
-                    taps = merge_taps(cx, cy, cwt, nx, ny, tw);
-                }
-                int self = -1;
-                for (int e = 0; e < taps; e++)
-                    if (nx[e] == tx && ny[e] == ty)
-                        self = e;
-                if (self < 0)
-                    continue;
-                const float w_self = tw[self];
-                const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
-                float sv[CS];
-#pragma unroll
-                for (int i = 0; i < CS; i++)
-                    sv[i] = a.site_f[s * CS + i];
-                double mono[NM];
-                monomials<CS>(sv, mono);
-#pragma unroll
-                for (int aa = 0; aa < C; aa++)
-                    gacc[aa] += (double)w_self * a.site_d[s * C + aa];
-                for (int e = 0; e < taps; e++)
-                {
-                    const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
-                    if (slot < 0)
-                        continue;
-                    const double scale = (double)w_self * (double)tw[e];
-#pragma unroll
-                    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
-                        if (sl == slot)
-#pragma unroll
-                            for (int i = 0; i < NM; i++)
-                                sums[sl][i] += scale * mono[i];
-                }
+                px = px_lo;
+                ++py;
             }
+        }
+        float dx, dy;
+        site_delta(a.k, j, px, py, dx, dy);
+        int nx[4], ny[4];
+        float tw[4];
+        int taps;
+        if (LEVEL0 && j == 0)
+        {
+            nx[0] = px;
+            ny[0] = py;
+            tw[0] = 1.0f;
+            taps = 1;
+        }
+        else
+        {
+            const Tap t = LEVEL0 ? tap_pixel(px, py, dx, dy, w0, h0, w0, h0)
+                                 : tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
+            const int cx[4] = { t.x0, t.x1, t.x0, t.x1 };
+            const int cy[4] = { t.y0, t.y0, t.y1, t.y1 };
+            const float cwt[4] = { (1.0f - t.fx) * (1.0f - t.fy), t.fx * (1.0f - t.fy), (1.0f - t.fx) * t.fy,
+                                   t.fx * t.fy };
+            taps = merge_taps(cx, cy, cwt, nx, ny, tw);
+        }
+        int self = -1;
+        for (int e = 0; e < taps; e++)
+            if (nx[e] == tx && ny[e] == ty)
+                self = e;
+        if (self < 0)
+            continue;
+        const float w_self = tw[self];
+        const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
+        float sv[CS];
+#pragma unroll
+        for (int i = 0; i < CS; i++)
+            sv[i] = a.site_f[s * CS + i];
+        double mono[NM];
+        monomials<CS>(sv, mono);
+#pragma unroll
+        for (int aa = 0; aa < C; aa++)
+            gacc[aa] += (double)w_self * a.site_d[s * C + aa];
+        for (int e = 0; e < taps; e++)
+        {
+            const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
+            if (slot < 0)
+                continue;
+            const double scale = (double)w_self * (double)tw[e];
+#pragma unroll
+            for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+                if (sl == slot)
+#pragma unroll
+                    for (int i = 0; i < NM; i++)
+                        sums[sl][i] += scale * mono[i];
+        }
+    }
+
+    if (G > 1)
+    {
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1)
+        {
+#pragma unroll
+            for (int s = 0; s < STENCIL_BLOCKS; s++)
+#pragma unroll
+                for (int i = 0; i < NM; i++)
+                    sums[s][i] += __shfl_xor(sums[s][i], off);
+#pragma unroll
+            for (int i = 0; i < C; i++)
+                gacc[i] += __shfl_xor(gacc[i], off);
+        }
+    }
+    if (!live || sub != 0)
+        return;
 
     // The five owned blocks expanded from the monomial sums; a zero sum is skipped.
-    const long long gt = L.texels.start[p] + ti;
+    const long long gtx = L.texels.start[p] + ti;
     const long long sn = L.sm.sn;
     double* out_h = L.stencil + stencil_pos(L.sm, p, tw_, tx, ty);
 #pragma unroll
@@ -488,30 +536,31 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
     }
 #pragma unroll
     for (int i = 0; i < C; i++)
-        L.grad[gt * C + i] = gacc[i];
+        L.grad[gtx * C + i] = gacc[i];
 }
 
-template <bool LEVEL0, int C>
+template <bool LEVEL0, int C, int G>
 void launch_gather_cs(int cs, dim3 grid, hipStream_t s, const AssembleArgs& a)
 {
     switch (cs)
     {
-    case 1: hipLaunchKernelGGL((k_gather<LEVEL0, C, 1>), grid, dim3(128), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_gather<LEVEL0, C, 2>), grid, dim3(128), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_gather<LEVEL0, C, 3>), grid, dim3(128), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_gather<LEVEL0, C, 4>), grid, dim3(128), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_gather<LEVEL0, C, 1, G>), grid, dim3(128), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_gather<LEVEL0, C, 2, G>), grid, dim3(128), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_gather<LEVEL0, C, 3, G>), grid, dim3(128), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_gather<LEVEL0, C, 4, G>), grid, dim3(128), 0, s, a); break;
     }
 }
 
-template <bool LEVEL0>
-void launch_gather(int c, int cs, dim3 grid, hipStream_t s, const AssembleArgs& a)
+template <bool LEVEL0, int G>
+void launch_gather(int c, int cs, long long texels, hipStream_t s, const AssembleArgs& a)
 {
+    const dim3 grid(blocks_for(texels * G, 128));
     switch (c)
     {
-    case 1: launch_gather_cs<LEVEL0, 1>(cs, grid, s, a); break;
-    case 2: launch_gather_cs<LEVEL0, 2>(cs, grid, s, a); break;
-    case 3: launch_gather_cs<LEVEL0, 3>(cs, grid, s, a); break;
-    default: launch_gather_cs<LEVEL0, 4>(cs, grid, s, a); break;
+    case 1: launch_gather_cs<LEVEL0, 1, G>(cs, grid, s, a); break;
+    case 2: launch_gather_cs<LEVEL0, 2, G>(cs, grid, s, a); break;
+    case 3: launch_gather_cs<LEVEL0, 3, G>(cs, grid, s, a); break;
+    default: launch_gather_cs<LEVEL0, 4, G>(cs, grid, s, a); break;
     }
 }
 
@@ -1403,10 +1452,19 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         // site once.
         const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
         KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
-        if (level0)
-            launch_gather<true>(L.c, cs, dim3(blocks_for(n, 128)), c.stream, a);
+        // The exact form gathers a texel per lane in the reference's order; the fast form spreads a level-1 texel's
+        // four-hundred-site footprint over sixteen lanes (level 0's footprints are small enough for one).
+        if (c.form == NNTC_FORM_EXACT)
+        {
+            if (level0)
+                launch_gather<true, 1>(L.c, cs, n, c.stream, a);
+            else
+                launch_gather<false, 1>(L.c, cs, n, c.stream, a);
+        }
+        else if (level0)
+            launch_gather<true, 1>(L.c, cs, n, c.stream, a);
         else
-            launch_gather<false>(L.c, cs, dim3(blocks_for(n, 128)), c.stream, a);
+            launch_gather<false, 16>(L.c, cs, n, c.stream, a);
     }
     HIP_OK(hipGetLastError());
     ChunkArgs ch = make_chunks(c, L);
