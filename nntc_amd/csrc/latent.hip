@@ -203,7 +203,7 @@ __device__ __forceinline__ void monomials(const float* s, double* mono)
 // counts so every per-site array is indexed by constants and lives in registers; the outputs run to the cap under a
 // runtime guard. The sums per output (the residual, then each channel's acc) keep the reference's operand order
 // (src/cpu/cpu_solve_level1.cpp:262-331).
-template <bool LEVEL0, int C0, int C1>
+template <bool LEVEL0, int C0, int C1, bool DIRECT>
 __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 {
     const long long gs = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -244,10 +244,14 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
         sample_bilinear_r<MAX_NOUT>(src, w0, nout, t0, target);
     }
     constexpr int C = LEVEL0 ? C0 : C1;   // the solved latent's channels
-    double acc[C];
+    constexpr int NH = C * (C + 1) / 2;
+    double acc[C], hq[NH];
 #pragma unroll
     for (int aa = 0; aa < C; aa++)
         acc[aa] = 0.0;
+#pragma unroll
+    for (int e = 0; e < NH; e++)
+        hq[e] = 0.0;
     // Per output r: the row's matrix entries for the solved latent (G = A + sum_i sbar_i M_i for level 1,
     // Q = B + sum_q cbar_q N_q for level 0), the fp32 residual, then each channel's cw_r G_ra res_r added in r order.
 #pragma unroll
@@ -294,8 +298,24 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 #pragma unroll
         for (int aa = 0; aa < C; aa++)
             acc[aa] += (double)a.cw[r] * (double)m[aa] * (double)res;
+        if (DIRECT)
+        {
+            int e = 0;
+#pragma unroll
+            for (int i = 0; i < C; i++)
+#pragma unroll
+                for (int jj = i; jj < C; jj++, e++)
+                    hq[e] += (double)a.cw[r] * (double)m[i] * (double)m[jj];
+        }
     }
-    if (!LEVEL0)
+    if (DIRECT)
+    {
+        // The site's own block Q^T diag(cw) Q, upper triangle, in place of the sample it is a function of.
+#pragma unroll
+        for (int e = 0; e < NH; e++)
+            a.site_f[gs * NH + e] = (float)hq[e];
+    }
+    else if (!LEVEL0)
     {
 #pragma unroll
         for (int i = 0; i < C0; i++)
@@ -312,27 +332,27 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
         a.site_d[gs * C + aa] = acc[aa];
 }
 
-template <bool LEVEL0, int C0>
+template <bool LEVEL0, int C0, bool DIRECT>
 void launch_site_terms_c1(int c1, dim3 grid, hipStream_t s, const AssembleArgs& a)
 {
     switch (c1)
     {
-    case 1: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 1>), grid, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 2>), grid, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 3>), grid, dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 4>), grid, dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 1, DIRECT>), grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 2, DIRECT>), grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 3, DIRECT>), grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 4, DIRECT>), grid, dim3(256), 0, s, a); break;
     }
 }
 
-template <bool LEVEL0>
+template <bool LEVEL0, bool DIRECT>
 void launch_site_terms(int c0, int c1, dim3 grid, hipStream_t s, const AssembleArgs& a)
 {
     switch (c0)
     {
-    case 1: launch_site_terms_c1<LEVEL0, 1>(c1, grid, s, a); break;
-    case 2: launch_site_terms_c1<LEVEL0, 2>(c1, grid, s, a); break;
-    case 3: launch_site_terms_c1<LEVEL0, 3>(c1, grid, s, a); break;
-    default: launch_site_terms_c1<LEVEL0, 4>(c1, grid, s, a); break;
+    case 1: launch_site_terms_c1<LEVEL0, 1, DIRECT>(c1, grid, s, a); break;
+    case 2: launch_site_terms_c1<LEVEL0, 2, DIRECT>(c1, grid, s, a); break;
+    case 3: launch_site_terms_c1<LEVEL0, 3, DIRECT>(c1, grid, s, a); break;
+    default: launch_site_terms_c1<LEVEL0, 4, DIRECT>(c1, grid, s, a); break;
     }
 }
 
@@ -564,6 +584,115 @@ void launch_gather(int c, int cs, long long texels, hipStream_t s, const Assembl
     }
 }
 
+// The fast form's level-0 gather: each site stores its own block Q^T diag(cw) Q (upper triangle, fp32) in place of
+// the level-1 sample, so a texel adds scale x that block into each owned slot - C (C + 1) / 2 values per tap instead of
+// the 1 + c1 + c1 (c1 + 1) / 2 monomials and their expansion, and a handful of registers instead of five slots of
+// monomial sums. The blocks are symmetric, so each slot's block is written from its triangle.
+template <int C>
+__global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
+{
+    constexpr int NH = C * (C + 1) / 2;
+    const LatentView& L = a.L;
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gi >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gi);
+    const PlaneGeom& g = L.planes.p[p];
+    const long long ti = gi - L.texels.start[p];
+    const int w0 = g.w0, h0 = g.h0;
+    const int tx = (int)(ti % w0), ty = (int)(ti / w0);
+    const int per_pixel = 1 + a.k;
+    const long long site0 = a.sites.start[p];
+
+    double hs[STENCIL_BLOCKS][NH], gacc[C];
+#pragma unroll
+    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+#pragma unroll
+        for (int e = 0; e < NH; e++)
+            hs[sl][e] = 0.0;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        gacc[i] = 0.0;
+
+    int px_lo, px_hi, py_lo, py_hi;
+    footprint_range(tx, w0, w0, px_lo, px_hi);
+    footprint_range(ty, h0, h0, py_lo, py_hi);
+    for (int py = py_lo; py <= py_hi; py++)
+        for (int px = px_lo; px <= px_hi; px++)
+            for (int j = 0; j <= a.k; j++)
+            {
+                int nx[4], ny[4];
+                float tw[4];
+                int taps;
+                if (j == 0)
+                {
+                    nx[0] = px;
+                    ny[0] = py;
+                    tw[0] = 1.0f;
+                    taps = 1;
+                }
+                else
+                {
+                    float dx, dy;
+                    site_delta(a.k, j, px, py, dx, dy);
+                    const Tap t = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
+                    const int cx[4] = { t.x0, t.x1, t.x0, t.x1 };
+                    const int cy[4] = { t.y0, t.y0, t.y1, t.y1 };
+                    const float cwt[4] = { (1.0f - t.fx) * (1.0f - t.fy), t.fx * (1.0f - t.fy), (1.0f - t.fx) * t.fy,
+                                           t.fx * t.fy };
+                    taps = merge_taps(cx, cy, cwt, nx, ny, tw);
+                }
+                int self = -1;
+                for (int e = 0; e < taps; e++)
+                    if (nx[e] == tx && ny[e] == ty)
+                        self = e;
+                if (self < 0)
+                    continue;
+                const float w_self = tw[self];
+                const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
+                float hv[NH];
+#pragma unroll
+                for (int e = 0; e < NH; e++)
+                    hv[e] = a.site_f[s * NH + e];
+#pragma unroll
+                for (int aa = 0; aa < C; aa++)
+                    gacc[aa] += (double)w_self * a.site_d[s * C + aa];
+                for (int e = 0; e < taps; e++)
+                {
+                    const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
+                    if (slot < 0)
+                        continue;
+                    const double scale = (double)w_self * (double)tw[e];
+#pragma unroll
+                    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+                        if (sl == slot)
+#pragma unroll
+                            for (int q = 0; q < NH; q++)
+                                hs[sl][q] += scale * (double)hv[q];
+                }
+            }
+
+    const long long sn = L.sm.sn;
+    double* out_h = L.stencil + stencil_pos(L.sm, p, w0, tx, ty);
+#pragma unroll
+    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+    {
+        int e = 0;
+#pragma unroll
+        for (int i = 0; i < C; i++)
+#pragma unroll
+            for (int jj = i; jj < C; jj++, e++)
+            {
+                out_h[(sl * C * C + i * C + jj) * sn] = hs[sl][e];
+                out_h[(sl * C * C + jj * C + i) * sn] = hs[sl][e];
+            }
+    }
+    const long long gtx = L.texels.start[p] + ti;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        L.grad[gtx * C + i] = gacc[i];
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // Fixed-order reductions
 // ---------------------------------------------------------------------------------------------------------------------
@@ -586,6 +715,54 @@ __device__ __forceinline__ void chunk_bounds(const ChunkArgs& a, long long chunk
     const long long te = tb + CHUNK < texels ? tb + CHUNK : texels;
     vb = (a.L.texels.start[p] + tb) * a.L.c;
     ve = (a.L.texels.start[p] + te) * a.L.c;
+}
+
+// The fast form's diagonal statistics per chunk: one 256-lane workgroup, each lane's strided values, then trees.
+__global__ __launch_bounds__(256) void k_diag_fast(ChunkArgs a)
+{
+    __shared__ double part[4][3];
+    int p;
+    long long vb, ve;
+    chunk_bounds(a, blockIdx.x, p, vb, ve);
+    const int c = a.L.c, w = a.L.pw[p];
+    const long long t0 = a.L.texels.start[p];
+    double sum = 0.0, lo = 1e300, hi = -1e300;
+    for (long long i = vb + threadIdx.x; i < ve; i += 256)
+    {
+        const long long t = i / c - t0;
+        const int q = (int)(i % c);
+        const double x = a.L.stencil[(q * c + q) * a.L.sm.sn + stencil_pos(a.L.sm, p, w, (int)(t % w), (int)(t / w))];
+        sum += x;
+        lo = x < lo ? x : lo;
+        hi = x > hi ? x : hi;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+    {
+        sum += __shfl_xor(sum, off);
+        const double ol = __shfl_xor(lo, off), oh = __shfl_xor(hi, off);
+        lo = ol < lo ? ol : lo;
+        hi = oh > hi ? oh : hi;
+    }
+    if (threadIdx.x % 64 == 0)
+    {
+        part[threadIdx.x / 64][0] = sum;
+        part[threadIdx.x / 64][1] = lo;
+        part[threadIdx.x / 64][2] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double s = 0.0, l = 1e300, h = -1e300;
+        for (int wv = 0; wv < 4; wv++)
+        {
+            s += part[wv][0];
+            l = part[wv][1] < l ? part[wv][1] : l;
+            h = part[wv][2] > h ? part[wv][2] : h;
+        }
+        a.partial[blockIdx.x * 3] = s;
+        a.partial[blockIdx.x * 3 + 1] = l;
+        a.partial[blockIdx.x * 3 + 2] = h;
+    }
 }
 
 // The diagonal of H over the chunk's texels: its sum (serial, texel then channel), smallest and largest.
@@ -626,7 +803,7 @@ __global__ __launch_bounds__(64) void k_diag_chunks(ChunkArgs a)
             const int count = row >= end ? 0 : (int)(end - row < 64 ? end - row : 64);
             if (count == 0)
                 continue;
-            // Minimum and maximum are exact in any order; the sum is folded serially in the exact form.
+            // Minimum and maximum are exact in any order; the sum is folded serially below.
             double l = lane < count ? x[u] : 1e300, h = lane < count ? x[u] : -1e300;
             for (int off = 32; off > 0; off >>= 1)
             {
@@ -636,20 +813,15 @@ __global__ __launch_bounds__(64) void k_diag_chunks(ChunkArgs a)
             }
             lo = l < lo ? l : lo;
             hi = h > hi ? h : hi;
-            if (!a.exact)
-                sum += tree_lanes(lane < count ? x[u] : 0.0);
         }
-        if (a.exact)
-        {
-            // Staged through LDS so lane 0 folds the batch in index order (texel then channel).
+        // Staged through LDS so lane 0 folds the batch in index order (texel then channel).
 #pragma unroll
-            for (int u = 0; u < ROWS; u++)
-                buf[u * 64 + lane] = x[u];
-            __syncthreads();
-            if (lane == 0)
-                sum = fold_serial16(buf, n, sum);
-            __syncthreads();
-        }
+        for (int u = 0; u < ROWS; u++)
+            buf[u * 64 + lane] = x[u];
+        __syncthreads();
+        if (lane == 0)
+            sum = fold_serial16(buf, n, sum);
+        __syncthreads();
     }
     if (lane == 0)
     {
@@ -1432,7 +1604,10 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     }
     a.sites.start[c.planes.n] = acc;
     const int cs = level0 ? c.c1 : c.c0;   // the other latent's channels: the sample each site stores
-    if (k > 4 || (size_t)acc * (size_t)std::max(cs, L.c) > c.site_f.n)
+    // The fast form's level 0 stores each site's own block (its triangle) instead of the sample.
+    const bool direct = level0 && c.form != NNTC_FORM_EXACT;
+    const int per_site_f = direct ? L.c * (L.c + 1) / 2 : cs;
+    if (k > 4 || (size_t)acc * (size_t)per_site_f > c.site_f.n || (size_t)acc * (size_t)L.c > c.site_d.n)
         throw Refused { "assembly: the site buffer holds K <= 4" };
     const long long n = L.texels.start[L.texels.n];
     {
@@ -1440,10 +1615,12 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         const double in = 4.0 * c.n0_total * (c.nout + c.c0) + 4.0 * c.n1_total * c.c1;
         KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
                        0.0);
-        if (level0)
-            launch_site_terms<true>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
+        if (level0 && direct)
+            launch_site_terms<true, true>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
+        else if (level0)
+            launch_site_terms<true, false>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
         else
-            launch_site_terms<false>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
+            launch_site_terms<false, false>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
     }
     HIP_OK(hipGetLastError());
     {
@@ -1454,15 +1631,24 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
         // The exact form gathers a texel per lane in the reference's order; the fast form spreads a level-1 texel's
         // four-hundred-site footprint over sixteen lanes (level 0's footprints are small enough for one).
-        if (c.form == NNTC_FORM_EXACT)
+        if (direct)
+        {
+            const dim3 grid(blocks_for(n, 128));
+            switch (L.c)
+            {
+            case 1: hipLaunchKernelGGL(k_gather_direct<1>, grid, dim3(128), 0, c.stream, a); break;
+            case 2: hipLaunchKernelGGL(k_gather_direct<2>, grid, dim3(128), 0, c.stream, a); break;
+            case 3: hipLaunchKernelGGL(k_gather_direct<3>, grid, dim3(128), 0, c.stream, a); break;
+            default: hipLaunchKernelGGL(k_gather_direct<4>, grid, dim3(128), 0, c.stream, a); break;
+            }
+        }
+        else if (c.form == NNTC_FORM_EXACT)
         {
             if (level0)
                 launch_gather<true, 1>(L.c, cs, n, c.stream, a);
             else
                 launch_gather<false, 1>(L.c, cs, n, c.stream, a);
         }
-        else if (level0)
-            launch_gather<true, 1>(L.c, cs, n, c.stream, a);
         else
             launch_gather<false, 16>(L.c, cs, n, c.stream, a);
     }
@@ -1471,7 +1657,10 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     const long long nch = ch.chunks.start[ch.chunks.n];
     {
         KernelTimer kt(c.stream, "k_diag", 8.0 * (double)n * L.c, 0.0);
-        hipLaunchKernelGGL(k_diag_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+        if (ch.exact)
+            hipLaunchKernelGGL(k_diag_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
+        else
+            hipLaunchKernelGGL(k_diag_fast, dim3((unsigned)nch), dim3(256), 0, c.stream, ch);
     }
     hipLaunchKernelGGL(k_diag_fold, dim3(1), dim3(64), 0, c.stream, ch, ridge);
     HIP_OK(hipGetLastError());
