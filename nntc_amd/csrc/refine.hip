@@ -128,21 +128,22 @@ __global__ __launch_bounds__(256) void k_bc_decode(BcArgs a, long long total)
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
-// The refinement, one LANE per 4x4 block (src/cpu/cpu_refine_bc.cpp:138-384)
+// The refinement, one sixteen-lane DPP row per 4x4 block (src/cpu/cpu_refine_bc.cpp:138-384)
 // ---------------------------------------------------------------------------------------------------------------------
 //
 // The reference refines a block with a chain of small serial decisions (a least-squares endpoint step, nine candidates,
-// then 2 x 16 x c0 selector argmins, each one seeing the moves before it). Spread over a workgroup, every decision is
-// one thread's work between two barriers; here a lane carries one block through the whole chain and a wavefront
-// advances 64 blocks' chains at once, with no barrier. The lane keeps the block's two n-vectors (a1: the linear term
-// from the texels around the block; uv: the quadratic applied to the block's own corrections, kept current as the block
-// moves) in its own column of LDS, and reads the block's quadratic a2 straight from block (c')'s stencil when
-// a step needs an entry: entry (texel u, channel i; texel v, channel j) of a2 is one entry of a stencil block of u or
-// of v (the transpose of v's when v owns the pair), zero unless u and v are neighbours in the plane.
+// then 2 x 16 x c0 selector argmins, each one seeing the moves before it). Here a row of sixteen lanes carries one
+// block and a wavefront four: lane u holds texel u's rows of the block's two n-vectors (a1: the linear term from the
+// texels around the block; uv: the quadratic applied to the block's own corrections, kept current as the block moves)
+// in registers, and reads its rows of the block's quadratic a2 straight from block (c')'s stencil: entry (texel u,
+// channel i; texel v, channel j) of a2 is one entry of a stencil block of u or of v (the transpose of v's when v owns
+// the pair), zero unless u and v are neighbours in the plane. Every decision is taken by all sixteen lanes from the same
+// row-broadcast operands, so the row agrees on it without a barrier or a vote.
 //
-// Every sum folds its terms in the reference's order. The reference's sums run over all sixteen texels of the block;
-// the terms skipped here are products with a zero entry of a2, which add a zero to a sum that starts at +0.0 - a sum
-// such a zero cannot change.
+// Every sum folds its terms in the reference's order: a sum over the block's texels is gathered lane by lane with row
+// broadcasts and added in increasing texel order, as the reference CPU backend folds it. The reference's row sums run
+// over all sixteen texels of the block; the terms skipped here are products with a zero entry of a2, which add a zero
+// to a sum that starts at +0.0 - a sum such a zero cannot change.
 
 // Texel (lx, ly) of block (gx, gy), lx and ly in -1..4, in the stencil's runs: base[col] + (ly >> 1) cw[col] + (lx >> 1).
 struct BlockPos
@@ -174,33 +175,77 @@ struct BlockPos
     }
 };
 
+// Texel u of a block, as its lane sees the block.
 template <int C0>
-struct RefineLane
+struct BlockLane
 {
     const double* S;
     long long sn;
-    const long long* pos;   // the lane's LDS column of its sixteen texels' stencil positions: pos[u * 64]
-    unsigned int inside;    // bit u: texel u of the block is in the plane
+    BlockPos bp;
+    int u, ux, uy;
+    long long pu;           // texel u's stencil position
+    unsigned int inside;    // bit v: texel v of the block is in the plane
 
-    __device__ __forceinline__ bool in(int u) const { return (inside >> u) & 1u; }
-    __device__ __forceinline__ long long at(int u) const { return pos[u * 64]; }
+    __device__ __forceinline__ bool in(int v) const { return (inside >> v) & 1u; }
+
+    // The lane's geometry made opaque to the optimiser, so that addresses and offsets derived from it are computed where
+    // they are used rather than hoisted out of a loop and held in registers across it.
+    __device__ __forceinline__ void opaque()
+    {
+        asm volatile("" : "+v"(pu), "+v"(bp.b0), "+v"(bp.d1), "+v"(bp.d2), "+v"(bp.d3));
+        asm volatile("" : "+v"(ux), "+v"(uy), "+v"(u), "+v"(inside), "+v"(bp.cw0), "+v"(bp.cw1));
+    }
 
     // a2's entry (texel u, channel i; texel u + (dx, dy), channel j) for a neighbour inside the block; dx, dy, i and j
-    // are constants after unrolling, so the entry's offset is uniform and the address one add. The forward slots
-    // (1, 0), (-1, 1), (0, 1), (1, 1) are u's blocks, the other four the neighbour's, transposed; zero unless both
-    // texels are in the plane.
-    __device__ __forceinline__ double a2(int u, int i, int dx, int dy, int j) const
+    // are constants after unrolling, so the entry's offset is uniform. The forward slots (1, 0), (-1, 1), (0, 1),
+    // (1, 1) are u's blocks, the other four the neighbour's, transposed; zero unless both texels are in the plane.
+    __device__ __forceinline__ double a2(int i, int dx, int dy, int j) const
     {
-        const int v = u + 4 * dy + dx;
-        if (!(in(u) && in(v)))
+        if (!(in(u) && in(u + 4 * dy + dx)))
             return 0.0;
         if (dx == 0 && dy == 0)
-            return S[(long long)(i * C0 + j) * sn + at(u)];
+            return S[(long long)(i * C0 + j) * sn + pu];
         if (dy == 1 || (dy == 0 && dx == 1))
-            return S[(long long)((dy == 0 ? 1 : dx + 3) * C0 * C0 + i * C0 + j) * sn + at(u)];
-        return S[(long long)((dy == 0 ? 1 : 3 - dx) * C0 * C0 + j * C0 + i) * sn + at(v)];
+            return S[(long long)((dy == 0 ? 1 : dx + 3) * C0 * C0 + i * C0 + j) * sn + pu];
+        return S[(long long)((dy == 0 ? 1 : 3 - dx) * C0 * C0 + j * C0 + i) * sn + bp.at(ux + dx, uy + dy)];
     }
 };
+
+// Lane K of each sixteen-lane row, to the whole row (DPP row_newbcast).
+template <int K>
+__device__ __forceinline__ double row_bcast(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), 0x150 + K, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), 0x150 + K, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Lane i + OFF of lane i's row (DPP row_shl / row_shr); a lane whose source falls outside the row reads zero.
+template <int OFF>
+__device__ __forceinline__ double row_from(double x)
+{
+    if constexpr (OFF == 0)
+        return x;
+    else
+    {
+        constexpr int ctrl = OFF > 0 ? 0x100 + OFF : 0x110 - OFF;
+        const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), ctrl, 0xF, 0xF, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), ctrl, 0xF, 0xF, false);
+        return __hiloint2double(hi, lo);
+    }
+}
+
+// ((0.0 + t_0) + t_1) + ... + t_15 over the row's lanes in lane order: a sum over the block's texels in increasing
+// texel order, the same on every lane of the row.
+template <int K = 0>
+__device__ __forceinline__ double row_fold(double t, double acc = 0.0)
+{
+    acc += row_bcast<K>(t);
+    if constexpr (K < 15)
+        return row_fold<K + 1>(t, acc);
+    else
+        return acc;
+}
 
 __device__ __forceinline__ int sel_get(unsigned long long bits, int t)
 {
@@ -231,20 +276,69 @@ __device__ __forceinline__ float level0_value(float lo, float hi, int r0, int r1
             }                                                                                                          \
         }
 
+// The nine texels (x + DX, y + DY) around texel (x, y), itself included, in increasing index order (DY, then DX), with
+// DX and DY literal constants (for the DPP shifts).
+#define NEIGHBOURS9(M) M(-1, -1) M(0, -1) M(1, -1) M(-1, 0) M(0, 0) M(1, 0) M(-1, 1) M(0, 1) M(1, 1)
+
+// Selector steps T..15 of one channel of one Gauss-Seidel pass: texel T's exact 8-way argmin, taken from texel T's
+// linear term and diagonal broadcast over the row, and on a strict decrease the move added into the rows of T's
+// neighbours. Lane u's entry (u, ii; T, c) sits in its LDS column at e[k * 64 * C0 + ii] for T = u + offset k: an
+// offset that differs from lane to lane, which in registers would take a select per slot.
+template <int C0, int T>
+__device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, const double (&a1)[C0], double (&uv)[C0],
+                                               const double* e, double diag, const float (&pal)[8], float lo, float hi,
+                                               int r0, int r1, unsigned long long& sc, double& taken)
+{
+    constexpr int tx = T & 3, ty = T >> 2;
+    const double b = row_bcast<T>(a1[c] + uv[c]);
+    const double aa = row_bcast<T>(diag);
+    const int s = sel_get(sc, T);
+    const double now = (double)level0_value(lo, hi, r0, r1, s);
+    double best = 0.0, move = 0.0;
+    int which = s;
+#pragma unroll
+    for (int kk = 0; kk < 8; kk++)
+    {
+        const double d = (double)pal[kk] - now;
+        const double de = 2.0 * b * d + aa * d * d;
+        if (de < best)
+        {
+            best = de;
+            move = d;
+            which = kk;
+        }
+    }
+    if (best < 0.0)
+    {
+        taken += best;
+        const int dx = tx - L.ux, dy = ty - L.uy;
+        if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
+        {
+            const double* ek = e + ((dy + 1) * 3 + dx + 1) * 64 * C0;
+#pragma unroll
+            for (int ii = 0; ii < C0; ii++)
+                uv[ii] += ek[ii] * move;
+        }
+        // Opaque, so the shift is not folded into the argmin's selects as eight shifted constants (which VOP3 cannot
+        // encode, so each would hold a register).
+        asm volatile("" : "+v"(which));
+        sc = sel_set(sc, T, which);
+    }
+    if constexpr (T < 15)
+        selector_steps<C0, T + 1>(L, c, a1, uv, e, diag, pal, lo, hi, r0, r1, sc, taken);
+}
+
 template <int C0>
 __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int colour_index)
 {
-    constexpr int N = 16 * C0;
-    // The block's two n-vectors and its texels' positions live in LDS, one column per lane (entry i of lane l at
-    // [i * 64 + l]): no lane reads another's column, so no barrier is needed.
-    __shared__ double s_a1[N * 64], s_uv[N * 64];
-    __shared__ long long s_pos[16 * 64];
-    double* a1 = s_a1 + threadIdx.x;
-    double* uv = s_uv + threadIdx.x;
-
-    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // Four blocks per wavefront, one per sixteen-lane row; a row past the colour's last block leaves as a whole (the
+    // rows of a wavefront never read each other's lanes).
+    const long long gi = (long long)blockIdx.x * 4 + (threadIdx.x >> 4);
     if (gi >= colour.start[colour.n])
         return;
+    // The selector steps' entries, one column per lane ([slot][lane][channel]); a lane reads only its own column.
+    __shared__ double s_e[9 * 64 * C0];
+
     const int p = range_plane(colour, gi);
     const PlaneGeom& g = a.planes.p[p];
     const long long ci = gi - colour.start[p];
@@ -259,55 +353,46 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     const double* S = a.stencil;
     const long long sn = a.sm.sn;
 
-    BlockPos bp;
-    bp.init(a.sm.cpos[p], w0, gx, gy);
+    BlockLane<C0> L;   // not const: opaque() launders its fields
+    L.S = S;
+    L.sn = sn;
+    L.bp.init(a.sm.cpos[p], w0, gx, gy);
+    const int u = threadIdx.x & 15, ux = u & 3, uy = u >> 2;
+    L.u = u;
+    L.ux = ux;
+    L.uy = uy;
+    L.pu = L.bp.at(ux, uy);
+    {
+        const int nx = min(4, w0 - px0), ny = min(4, h0 - py0);
+        const unsigned int row = (1u << nx) - 1u;
+        L.inside = (row | row << 4 | row << 8 | row << 12) & ((1u << (4 * ny)) - 1u);
+    }
+    const bool in_u = L.in(u);
     auto inside_xy = [=](int lx, int ly) {
         return px0 + lx >= 0 && px0 + lx < w0 && py0 + ly >= 0 && py0 + ly < h0;
     };
     // The chain index of texel (lx, ly) of the block (or of a neighbour just outside it).
     auto tix = [=](int lx, int ly) { return base + (long long)(py0 + ly) * w0 + (px0 + lx); };
+    const long long tu = tix(ux, uy);
 
-    RefineLane<C0> R;
-    R.S = S;
-    R.sn = sn;
-    R.pos = s_pos + threadIdx.x;
-    R.inside = 0;
-#pragma unroll
-    for (int u = 0; u < 16; u++)
-    {
-        s_pos[u * 64 + threadIdx.x] = bp.at(u & 3, u >> 2);
-        if (inside_xy(u & 3, u >> 2))
-            R.inside |= 1u << u;
-    }
-
-    // ---- a1: each texel's gradient plus the shares of its neighbours outside the block, slot by slot, the forward
+    // ---- a1: the texel's gradient plus the shares of its neighbours outside the block, slot by slot, the forward
     // neighbour's then the backward one's (slot 0 is the texel itself, inside the block).
-#pragma unroll 1
-    for (int u = 0; u < 16; u++)
+    double a1[C0], uv[C0];
+#pragma unroll
+    for (int i = 0; i < C0; i++)
+        a1[i] = 0.0;
+    if (in_u)
     {
-        const int lx = u & 3, ly = u >> 2;
-#pragma unroll
-        for (int i = 0; i < C0; i++)
-            uv[(u * C0 + i) * 64] = 0.0;
-        if (!R.in(u))
-        {
-#pragma unroll
-            for (int i = 0; i < C0; i++)
-                a1[(u * C0 + i) * 64] = 0.0;
-            continue;
-        }
-        const long long ti = tix(lx, ly);
-        const long long pu = R.at(u);
         double acc[C0];
 #pragma unroll
         for (int i = 0; i < C0; i++)
-            acc[i] = a.grad[ti * C0 + i];
+            acc[i] = a.grad[tu * C0 + i];
 #pragma unroll
         for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
         {
             int dx, dy;
             stencil_offset(slot, dx, dy);
-            const int fx = lx + dx, fy = ly + dy;
+            const int fx = ux + dx, fy = uy + dy;
             if (!(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && inside_xy(fx, fy))
             {
                 const double* xn = a.delta + tix(fx, fy) * C0;
@@ -319,13 +404,13 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + pu] * xv[j];
+                        acc[i] += S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + L.pu] * xv[j];
             }
-            const int bx = lx - dx, by = ly - dy;
+            const int bx = ux - dx, by = uy - dy;
             if (!(bx >= 0 && bx < 4 && by >= 0 && by < 4) && inside_xy(bx, by))
             {
                 const double* xn = a.delta + tix(bx, by) * C0;
-                const long long pb = bp.at(bx, by);
+                const long long pb = L.bp.at(bx, by);
                 double xv[C0];
 #pragma unroll
                 for (int j = 0; j < C0; j++)
@@ -339,29 +424,28 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         }
 #pragma unroll
         for (int i = 0; i < C0; i++)
-            a1[(u * C0 + i) * 64] = acc[i];
+            a1[i] = acc[i];
     }
 
-    // ---- uv = a2 dv, scattered a column at a time in increasing column order, so every row receives its terms in
-    // the order the reference's row sums add them.
-#pragma unroll 1
-    for (int v = 0; v < 16; v++)
-    {
-        if (!R.in(v))
-            continue;
-        const int vx = v & 3, vy = v >> 2;
-        const long long tv = tix(vx, vy);
+    // ---- uv = a2 dv: the lane's rows, their terms added in increasing column order (neighbour, then channel), the
+    // order of the reference's row sums; each neighbour's correction is read from its lane.
+    double dv[C0];
 #pragma unroll
-        for (int j = 0; j < C0; j++)
-        {
-            const double dv = (double)a.v0[tv * C0 + j] - (double)a.prev[tv * C0 + j];
-            FOR_NEIGHBOURS(vx, vy, dx, dy, {
-                const int u = v + 4 * dy + dx;
-                _Pragma("unroll") for (int i = 0; i < C0; i++)
-                    uv[(u * C0 + i) * 64] += R.a2(u, i, -dx, -dy, j) * dv;
-            })
-        }
+    for (int j = 0; j < C0; j++)
+    {
+        dv[j] = in_u ? (double)a.v0[tu * C0 + j] - (double)a.prev[tu * C0 + j] : 0.0;
+        uv[j] = 0.0;
     }
+#define NNTC_UV_TERM(DX, DY)                                                                                           \
+    {                                                                                                                  \
+        double dn[C0];                                                                                                 \
+        _Pragma("unroll") for (int j = 0; j < C0; j++) dn[j] = row_from<4 * (DY) + (DX)>(dv[j]);                       \
+        if (ux + (DX) >= 0 && ux + (DX) < 4 && uy + (DY) >= 0 && uy + (DY) < 4 && L.in(u + 4 * (DY) + (DX)))           \
+            _Pragma("unroll") for (int j = 0; j < C0; j++) _Pragma("unroll") for (int i = 0; i < C0; i++)              \
+                uv[i] += L.a2(i, (DX), (DY), j) * dn[j];                                                               \
+    }
+    NEIGHBOURS9(NNTC_UV_TERM)
+#undef NNTC_UV_TERM
 
     const long long gb = g.offb + (long long)gy * bxn + gx;
     int ep0[C0], ep1[C0];
@@ -380,35 +464,31 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     }
     double taken = 0.0;
 
-    // ---- (i) the endpoints of each channel
+    // ---- (i) the endpoints of each channel: each lane's terms of the five sums, folded over the row in texel order
 #pragma unroll
     for (int c = 0; c < C0; c++)
     {
         const float lo = a.lo0[c], hi = a.hi0[c];
         const double step = ((double)hi - (double)lo) / 255.0;
         const unsigned long long sc = sel[c];
-        double bp_ = 0.0, bq = 0.0, pap = 0.0, paq = 0.0, qaq = 0.0;
-#pragma unroll 1
-        for (int u = 0; u < 16; u++)
+        double bp_, bq, pap, paq, qaq;
         {
-            const int ux = u & 3, uy = u >> 2;
-            const int idx = u * C0 + c;
             const double aw = bc4_weight8(sel_get(sc, u));
             const double pp = step * aw;
             const double qq = step * (1.0 - aw);
-            const double b = a1[idx * 64] + uv[idx * 64];
+            const double b = a1[c] + uv[c];
             double ap = 0.0, aq = 0.0;
             FOR_NEIGHBOURS(ux, uy, dx, dy, {
-                const double e = R.a2(u, c, dx, dy, c);
+                const double e = L.a2(c, dx, dy, c);
                 const double aj = bc4_weight8(sel_get(sc, u + 4 * dy + dx));
                 ap += e * step * aj;
                 aq += e * step * (1.0 - aj);
             })
-            bp_ += b * pp;
-            bq += b * qq;
-            pap += pp * ap;
-            paq += pp * aq;
-            qaq += qq * aq;
+            bp_ = row_fold(b * pp);
+            bq = row_fold(b * qq);
+            pap = row_fold(pp * ap);
+            paq = row_fold(pp * aq);
+            qaq = row_fold(qq * aq);
         }
         const int r0 = ep0[c], r1 = ep1[c];
         const double det = pap * qaq - paq * paq;
@@ -448,30 +528,18 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             const int s = sel_get(sc, v);
             return (double)level0_value(lo, hi, best0, best1, s) - (double)level0_value(lo, hi, r0, r1, s);
         };
-        double exact = 0.0;
-#pragma unroll 1
-        for (int u = 0; u < 16; u++)
-        {
-            const int ux = u & 3, uy = u >> 2;
-            const int idx = u * C0 + c;
-            double quad = 0.0;
-            FOR_NEIGHBOURS(ux, uy, dx, dy, { quad += R.a2(u, c, dx, dy, c) * dlt(u + 4 * dy + dx); })
-            exact += dlt(u) * (2.0 * (a1[idx * 64] + uv[idx * 64]) + quad);
-        }
+        double quad = 0.0;
+        FOR_NEIGHBOURS(ux, uy, dx, dy, { quad += L.a2(c, dx, dy, c) * dlt(u + 4 * dy + dx); })
+        const double exact = row_fold(dlt(u) * (2.0 * (a1[c] + uv[c]) + quad));
         if (!(exact < 0.0))
             continue;
         taken += exact;
-#pragma unroll 1
-        for (int u = 0; u < 16; u++)
-        {
-            const int ux = u & 3, uy = u >> 2;
 #pragma unroll
-            for (int i = 0; i < C0; i++)
-            {
-                double add = 0.0;
-                FOR_NEIGHBOURS(ux, uy, dx, dy, { add += R.a2(u, i, dx, dy, c) * dlt(u + 4 * dy + dx); })
-                uv[(u * C0 + i) * 64] += add;
-            }
+        for (int i = 0; i < C0; i++)
+        {
+            double add = 0.0;
+            FOR_NEIGHBOURS(ux, uy, dx, dy, { add += L.a2(i, dx, dy, c) * dlt(u + 4 * dy + dx); })
+            uv[i] += add;
         }
         ep0[c] = best0;
         ep1[c] = best1;
@@ -489,76 +557,48 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
 #pragma unroll
         for (int c = 0; c < C0; c++)
         {
-            // The sixteen diagonal entries the channel's steps read, issued together: one memory latency per
-            // channel rather than one per step.
-            double diag[16];
-#pragma unroll
-            for (int t = 0; t < 16; t++)
-                diag[t] = R.a2(t, c, 0, 0, c);
-#pragma unroll
-            for (int t = 0; t < 16; t++)
-            {
-                const int tx = t & 3, ty = t >> 2;
-                const int i = t * C0 + c;
-                const double b = a1[i * 64] + uv[i * 64], aa = diag[t];
-                const int s = sel_get(sel[c], t);
-                const double now = (double)level0_value(a.lo0[c], a.hi0[c], ep0[c], ep1[c], s);
-                double best = 0.0, move = 0.0;
-                int which = s;
-#pragma unroll
-                for (int kk = 0; kk < 8; kk++)
-                {
-                    const double d = (double)pal[c][kk] - now;
-                    const double de = 2.0 * b * d + aa * d * d;
-                    if (de < best)
-                    {
-                        best = de;
-                        move = d;
-                        which = kk;
-                    }
-                }
-                if (best < 0.0)
-                {
-                    taken += best;
-                    FOR_NEIGHBOURS(tx, ty, dx, dy, {
-                        const int u = t + 4 * dy + dx;
-                        _Pragma("unroll") for (int ii = 0; ii < C0; ii++)
-                            uv[(u * C0 + ii) * 64] += R.a2(u, ii, -dx, -dy, c) * move;
-                    })
-                    sel[c] = sel_set(sel[c], t, which);
-                }
-            }
+            // Addresses are formed in the channel that uses them: hoisted out of the pass loop, every channel's and
+            // every step's would be live at once.
+            L.opaque();
+            int lane_off = (int)threadIdx.x * C0;
+            asm volatile("" : "+v"(lane_off));
+            double* const ec = s_e + lane_off;
+            // The lane's entries (u, ii; u + (dx, dy), c) that the channel's steps read, issued together: one memory
+            // latency per channel rather than one per step.
+            FOR_NEIGHBOURS(L.ux, L.uy, dx, dy, {
+                _Pragma("unroll") for (int ii = 0; ii < C0; ii++) ec[((dy + 1) * 3 + dx + 1) * 64 * C0 + ii] =
+                    L.a2(ii, dx, dy, c);
+            })
+            const double diag = L.a2(c, 0, 0, c);
+            selector_steps<C0, 0>(L, c, a1, uv, ec, diag, pal[c], a.lo0[c], a.hi0[c], ep0[c], ep1[c], sel[c], taken);
         }
 
-    // ---- what the block leaves behind
+    // ---- what the block leaves behind: lane u its texel's selectors and plane values, lane 0 the block's endpoints
 #pragma unroll
     for (int c = 0; c < C0; c++)
     {
-        a.ep[(gb * C0 + c) * 2] = (uint8_t)ep0[c];
-        a.ep[(gb * C0 + c) * 2 + 1] = (uint8_t)ep1[c];
-        uint8_t* s = a.sel + (gb * C0 + c) * 16;
-#pragma unroll
-        for (int t = 0; t < 16; t++)
-            s[t] = (uint8_t)sel_get(sel[c], t);
+        if (u == 0)
+        {
+            a.ep[(gb * C0 + c) * 2] = (uint8_t)ep0[c];
+            a.ep[(gb * C0 + c) * 2 + 1] = (uint8_t)ep1[c];
+        }
+        a.sel[(gb * C0 + c) * 16 + u] = (uint8_t)sel_get(sel[c], u);
     }
-#pragma unroll 1
-    for (int u = 0; u < 16; u++)
+    if (in_u)
     {
-        if (!R.in(u))
-            continue;
-        const long long ti = tix(u & 3, u >> 2);
 #pragma unroll
         for (int c = 0; c < C0; c++)
         {
             const float byte = bc4_entry8(ep0[c], ep1[c], sel_get(sel[c], u));
             const float value = bc0_value(a.lo0[c], a.hi0[c], byte);
-            const long long idx = ti * C0 + c;
+            const long long idx = tu * C0 + c;
             a.v0[idx] = value;
             a.k0[idx] = bc0_index(byte);
             a.delta[idx] = (double)value - (double)a.prev[idx];
         }
     }
-    a.taken[gb] = taken;
+    if (u == 0)
+        a.taken[gb] = taken;
 }
 
 // The .dds blocks of every file of every plane (bc4_pack, shared/bc_pack.h:51-60), and each block's squared error in
@@ -679,7 +719,7 @@ void bc_refine_pass(Ctx& c, double* taken, double* improved)
         const double c0 = c.c0;
         KernelTimer kt(c.stream, "k_bc_refine", (double)n * 16.0 * (40.0 * c0 * c0 + 8.0 * c0 + 8.0 * c0 + 9.0 * c0),
                        0.0);
-        const dim3 grid(blocks_for(n, 64));
+        const dim3 grid(blocks_for(n, 4));   // four blocks per 64-lane workgroup
         switch (c.c0)
         {
         case 1: hipLaunchKernelGGL(k_bc_refine<1>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
