@@ -142,7 +142,7 @@ struct Ctx
     DevBuf<double> obj;                    // nplanes * 3
     DevBuf<float> resid;                   // base texels * nout
     DevBuf<double> cov_partial;            // SITE_BLOCKS * entries
-    DevBuf<unsigned int> counters;         // per plane
+    DevBuf<unsigned int> counters;         // per plane, MOVED_SLOTS counters MOVED_STRIDE apart
     DevBuf<uint8_t> bc_ep, bc_sel;         // blocks * c0 * 2, blocks * c0 * 16
     DevBuf<uint8_t> bc_blocks;             // blocks * files * 16 (file f of block b at (b * files + f) * 16)
     DevBuf<uint8_t> k0_mirror;             // the host's Model::k0 as the bc calls read it
@@ -195,6 +195,10 @@ inline StencilMap stencil_map(const Ctx& c, bool level0)
 }
 
 // The fast-form objective's tile of pixels (one workgroup each) and the number of tiles of a plane.
+// The quantised sweeps' moved-index counters: MOVED_SLOTS per plane, MOVED_STRIDE unsigned ints (one 128-byte line)
+// apart, summed on the host after the sweeps.
+constexpr int MOVED_SLOTS = 64, MOVED_STRIDE = 32;
+
 constexpr int OBJ_TW = 32, OBJ_TH = 8;
 inline long long objective_tiles(const PlaneGeom& g)
 {

@@ -448,10 +448,14 @@ double sweep_block(Ctx& c, const nntc_model_state* st, int k, double ridge, int 
     const float ms_sw = c.ms_since(c.ev0);
     const int ch = level0 ? c.c0 : c.c1;
     read_scalars(c, ch, rep);
-    std::vector<unsigned int> moved((size_t)c.planes.n);
-    HIP_OK(hipMemcpyAsync(moved.data(), c.counters.p, moved.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
+    std::vector<unsigned int> slots((size_t)c.planes.n * MOVED_SLOTS * MOVED_STRIDE);
+    HIP_OK(hipMemcpyAsync(slots.data(), c.counters.p, slots.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
                           c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
+    std::vector<unsigned long long> moved((size_t)c.planes.n, 0);
+    for (int p = 0; p < c.planes.n; p++)
+        for (int s = 0; s < MOVED_SLOTS; s++)
+            moved[(size_t)p] += slots[((size_t)p * MOVED_SLOTS + s) * MOVED_STRIDE];
     for (int p = 0; p < c.planes.n; p++)
     {
         rep[p].quantised = 1;
@@ -635,7 +639,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.obj.alloc((size_t)md->nplanes * 3);
         c.resid.alloc((size_t)o0 * nout);
         c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout));
-        c.counters.alloc((size_t)md->nplanes);
+        c.counters.alloc((size_t)md->nplanes * MOVED_SLOTS * MOVED_STRIDE);
         c.bc_ep.alloc((size_t)ob * c0 * 2);
         c.bc_sel.alloc((size_t)ob * c0 * 16);
         c.bc_blocks.alloc((size_t)ob * 32);

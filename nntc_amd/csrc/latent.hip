@@ -1364,17 +1364,100 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
     const bool live = gi < colour.start[colour.n];
     const int p = range_plane(colour, live ? gi : colour.start[colour.n] - 1);
     const unsigned int moved = live ? sweep_texel<C>(L, colour, colour_index, gi, p) : 0u;
-    // One atomic per wavefront and plane: the wavefront's lanes of its first lane's plane are summed by shuffles; a
-    // lane of another plane (a wavefront across a plane boundary) adds its own.
-    const int p0 = __shfl(p, 0);
-    unsigned int same = p == p0 ? moved : 0u;
+    // One atomic per workgroup: the lanes of the plane of the workgroup's first texel are summed by shuffles and LDS;
+    // a lane of another plane (a workgroup across a plane boundary) adds its own. The workgroups spread their adds
+    // over MOVED_SLOTS counters per plane on separate cache lines, so they do not queue on one address.
+    __shared__ unsigned int s_wave[4];
+    const long long g0 = (long long)blockIdx.x * blockDim.x;
+    const int pw = range_plane(colour, g0);
+    unsigned int same = p == pw ? moved : 0u;
     for (int off = 32; off > 0; off >>= 1)
         same += __shfl_xor(same, off);
-    if (threadIdx.x % 64 == 0 && same)
-        atomicAdd(changed + p0, same);
-    if (p != p0 && moved)
-        atomicAdd(changed + p, moved);
+    if (threadIdx.x % 64 == 0)
+        s_wave[threadIdx.x / 64] = same;
+    if (p != pw && moved)
+        atomicAdd(changed + ((size_t)p * MOVED_SLOTS + blockIdx.x % MOVED_SLOTS) * MOVED_STRIDE, moved);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const unsigned int sum = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        if (sum)
+            atomicAdd(changed + ((size_t)pw * MOVED_SLOTS + blockIdx.x % MOVED_SLOTS) * MOVED_STRIDE, sum);
+    }
 }
+
+// One neighbour group of a texel's sweep: the neighbour's correction, the stencil block that couples it to the texel
+// (entry (i, j) at s[i * C + j], transposed for a backward neighbour, whose block it is) and a 0/1 weight. Every load is
+// unconditional: a neighbour outside the plane reads the texel's own entries (a valid address, finite values) and
+// weighs its terms by an opaque 0.0 (1.0 inside). x * 1.0 is x, and x * 0.0 is a zero that leaves the sum unchanged
+// (the sum starts at +0.0 and so is never -0.0); the weight is opaque so that the compiler cannot turn it back into a
+// branch around the loads, which would serialise them.
+template <int C>
+struct SweepNeighbour
+{
+    double x[C], s[C * C], keep;
+
+    // Group g: slot 1 + g / 2, forward for even g, backward for odd g.
+    __device__ __forceinline__ void load(const LatentView& L, const double* S0, long long sn, const double* delta0,
+                                         const long long (&cbase)[4], int g, int tx, int ty, int w, int h,
+                                         long long base, long long t, long long pos)
+    {
+        // The addresses hang on an opaque zero made here, so the loads cannot be issued before this point.
+        long long z = 0;
+        asm volatile("" : "+v"(z));
+        const double* S = S0 + z;
+        const double* delta = delta0 + z;
+        int dx, dy;
+        stencil_offset(1 + g / 2, dx, dy);
+        const int slot = 1 + g / 2;
+        if (g % 2 == 0)
+        {
+            const int fx = tx + dx, fy = ty + dy;
+            const bool ok = fx >= 0 && fx < w && fy >= 0 && fy < h;
+            keep = ok ? 1.0 : 0.0;
+            asm volatile("" : "+v"(keep));
+            const double* xn = delta + (ok ? base + (long long)fy * w + fx : t) * C;
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                x[j] = xn[j];
+#pragma unroll
+            for (int e = 0; e < C * C; e++)
+                s[e] = S[(slot * C * C + e) * sn + pos];
+        }
+        else
+        {
+            const int bx = tx - dx, by = ty - dy;
+            const bool ok = bx >= 0 && bx < w && by >= 0 && by < h;
+            keep = ok ? 1.0 : 0.0;
+            asm volatile("" : "+v"(keep));
+            const int bxs = ok ? bx : tx, bys = ok ? by : ty;
+            const int ncx = bxs & 1, ncy = bys & 1;
+            const long long nbase = ncy ? (ncx ? cbase[3] : cbase[2]) : (ncx ? cbase[1] : cbase[0]);
+            const long long npos = nbase + (long long)(bys >> 1) * ((w - ncx + 1) >> 1) + (bxs >> 1);
+            const double* xn = delta + (base + (long long)bys * w + bxs) * C;
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                x[j] = xn[j];
+#pragma unroll
+            for (int i = 0; i < C; i++)
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    s[i * C + j] = S[(slot * C * C + j * C + i) * sn + npos];
+        }
+    }
+
+    __device__ __forceinline__ void add_to(double (&nb)[C]) const
+    {
+#pragma unroll
+        for (int i = 0; i < C; i++)
+#pragma unroll
+            for (int j = 0; j < C; j++)
+            {
+                const double term = s[i * C + j] * x[j];
+                nb[i] += term * keep;
+            }
+    }
+};
 
 // One texel of the sweep's colour: returns 1 per channel whose index changed.
 template <int C>
@@ -1395,72 +1478,82 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
     const long long pos = L.sm.cpos[p][colour_index] + ci;
     double* delta = L.x;
 
-    double nb[C];
+    // The plane's four colour bases in the stencil's runs, read once: a neighbour's position is then arithmetic and
+    // selects, not a dependent load (opaque, so the selects are not folded back into an indexed load).
+    long long cbase[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+    {
+        cbase[k] = L.sm.cpos[p][k];
+        asm volatile("" : "+v"(cbase[k]));
+    }
+
+    // The eight neighbour groups (slots 1..4, forward then backward) pipelined: group g's loads are issued D groups
+    // before its terms are added, so about D groups are in flight per lane - enough, over the wavefronts a SIMD holds,
+    // to cover the memory latency, without holding every group's values at once. The order is pinned by data: the
+    // loads wait on an opaque value made when the group starts, and the sums are made opaque after each group's
+    // terms, and volatile asm statements keep their order. The terms reach each sum slot by slot, forward then
+    // backward, channel by channel, as in a loop over the neighbours.
+    constexpr int D = C <= 2 ? 4 : 2;
+    SweepNeighbour<C> q[8];
+    double nb[C], dl[C], own[C * C], gr[C];
+    float pvs[C];
+    uint8_t kold[C];
 #pragma unroll
     for (int i = 0; i < C; i++)
         nb[i] = 0.0;
 #pragma unroll
-    for (int slot = 1; slot < STENCIL_BLOCKS; slot++)
+    for (int g = 0; g < 8 + D; g++)
     {
-        int dx, dy;
-        stencil_offset(slot, dx, dy);
-        const int fx = tx + dx, fy = ty + dy;
-        if (fx >= 0 && fx < w && fy >= 0 && fy < h)
+        if (g < 8)
+            q[g].load(L, S, sn, delta, cbase, g, tx, ty, w, h, base, t, pos);
+        if (g == 8)
         {
-            const double* xn = delta + (base + (long long)fy * w + fx) * C;
-            double xv[C];
+            // The texel's own values, behind the last groups.
 #pragma unroll
             for (int j = 0; j < C; j++)
-                xv[j] = xn[j];
+            {
+                kold[j] = L.k[t * C + j];
+                dl[j] = delta[t * C + j];
+                gr[j] = L.grad[t * C + j];
+                pvs[j] = L.prev[t * C + j];
+            }
 #pragma unroll
-            for (int i = 0; i < C; i++)
-#pragma unroll
-                for (int j = 0; j < C; j++)
-                    nb[i] += S[(slot * C * C + i * C + j) * sn + pos] * xv[j];
+            for (int e = 0; e < C * C; e++)
+                own[e] = S[e * sn + pos];
         }
-        const int bx = tx - dx, by = ty - dy;
-        if (bx >= 0 && bx < w && by >= 0 && by < h)
+        if (g >= D)
         {
-            const long long npos = stencil_pos(L.sm, p, w, bx, by);
-            const double* xn = delta + (base + (long long)by * w + bx) * C;
-            double xv[C];
-#pragma unroll
-            for (int j = 0; j < C; j++)
-                xv[j] = xn[j];
+            q[g - D].add_to(nb);
+            // The sums are made opaque here, so the group's terms are added before the next group's loads issue.
 #pragma unroll
             for (int i = 0; i < C; i++)
-#pragma unroll
-                for (int j = 0; j < C; j++)
-                    nb[i] += S[(slot * C * C + j * C + i) * sn + npos] * xv[j];
+                asm volatile("" : "+v"(nb[i]));
         }
     }
 
     // The texel's own channels in order, each seeing the ones before it already moved (the deltas kept in registers
     // are the values the reference reads back from its plane).
-    double dl[C];
-#pragma unroll
-    for (int j = 0; j < C; j++)
-        dl[j] = delta[t * C + j];
     unsigned int moved = 0;
 #pragma unroll
     for (int q = 0; q < C; q++)
     {
         const long long idx = t * C + q;
         const float l = L.lo[q], span = L.hi[q] - L.lo[q];
-        const double aq = S[(q * C + q) * sn + pos] + lambda;
+        const double aq = own[q * C + q] + lambda;
         if (!(aq > 0.0) || !(span > 0.0f))
             continue;
         double rest = nb[q];
 #pragma unroll
         for (int j = 0; j < C; j++)
             if (j != q)
-                rest += S[(q * C + j) * sn + pos] * dl[j];
-        const double bq = L.grad[idx] + rest;
-        const float pv = L.prev[idx];
+                rest += own[q * C + j] * dl[j];
+        const double bq = gr[q] + rest;
+        const float pv = pvs[q];
         const double want = (double)pv - bq / aq;
         const int kk = level1_index(l, L.hi[q], L.bits, (float)want);
         const float value = level1_value(l, L.hi[q], L.bits, kk);
-        if ((int)L.k[idx] != kk)
+        if ((int)kold[q] != kk)
         {
             L.k[idx] = (uint8_t)kk;
             moved++;
@@ -1772,7 +1865,7 @@ void copy_prev_zero_delta(Ctx& c, bool level0)
 void quant_sweeps(Ctx& c, bool level0, int sweeps)
 {
     LatentView L = make_view(c, level0);
-    HIP_OK(hipMemsetAsync(c.counters.p, 0, sizeof(unsigned int) * c.planes.n, c.stream));
+    HIP_OK(hipMemsetAsync(c.counters.p, 0, c.counters.bytes(), c.stream));
     Ranges colour[4];
     for (int col = 0; col < 4; col++)
     {
