@@ -185,31 +185,74 @@ struct BlockLane
     int u, ux, uy;
     long long pu;           // texel u's stencil position
     unsigned int inside;    // bit v: texel v of the block is in the plane
+    unsigned int nbmask;    // bit k = (dy + 1) * 3 + dx + 1: texel u + (dx, dy) is in the block, and it and u in the plane
 
     __device__ __forceinline__ bool in(int v) const { return (inside >> v) & 1u; }
+
+    __device__ __forceinline__ void init_neighbours()
+    {
+        nbmask = 0;
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++)
+            {
+                const bool ok = ux + dx >= 0 && ux + dx < 4 && uy + dy >= 0 && uy + dy < 4 && in(u) &&
+                                in((u + 4 * dy + dx) & 15);
+                nbmask |= (ok ? 1u : 0u) << ((dy + 1) * 3 + dx + 1);
+            }
+    }
 
     // The lane's geometry made opaque to the optimiser, so that addresses and offsets derived from it are computed where
     // they are used rather than hoisted out of a loop and held in registers across it.
     __device__ __forceinline__ void opaque()
     {
         asm volatile("" : "+v"(pu), "+v"(bp.b0), "+v"(bp.d1), "+v"(bp.d2), "+v"(bp.d3));
-        asm volatile("" : "+v"(ux), "+v"(uy), "+v"(u), "+v"(inside), "+v"(bp.cw0), "+v"(bp.cw1));
+        asm volatile("" : "+v"(ux), "+v"(uy), "+v"(u), "+v"(nbmask), "+v"(bp.cw0), "+v"(bp.cw1));
     }
 
-    // a2's entry (texel u, channel i; texel u + (dx, dy), channel j) for a neighbour inside the block; dx, dy, i and j
-    // are constants after unrolling, so the entry's offset is uniform. The forward slots (1, 0), (-1, 1), (0, 1),
-    // (1, 1) are u's blocks, the other four the neighbour's, transposed; zero unless both texels are in the plane.
+    // 1.0 for a neighbour k whose entries a2 holds, 0.0 otherwise; opaque, so that an entry weighted by it is loaded
+    // whatever its value rather than behind a branch.
+    __device__ __forceinline__ double keep(int k) const
+    {
+        double w = ((nbmask >> k) & 1u) ? 1.0 : 0.0;
+        asm volatile("" : "+v"(w));
+        return w;
+    }
+
+    // a2's entry (texel u, channel i; texel u + (dx, dy), channel j), weighted by keep: dx, dy, i and j are constants
+    // after unrolling, so the entry's offset is uniform. The forward slots (1, 0), (-1, 1), (0, 1), (1, 1) are u's
+    // blocks, the other four the neighbour's, transposed. A neighbour a2 does not hold reads the lane's own run instead
+    // (a valid address, a finite value) and is weighted by 0.0: the entry is then a zero, as a2's is.
     __device__ __forceinline__ double a2(int i, int dx, int dy, int j) const
     {
-        if (!(in(u) && in(u + 4 * dy + dx)))
-            return 0.0;
+        const int k = (dy + 1) * 3 + dx + 1;
+        const double* e;
         if (dx == 0 && dy == 0)
-            return S[(long long)(i * C0 + j) * sn + pu];
-        if (dy == 1 || (dy == 0 && dx == 1))
-            return S[(long long)((dy == 0 ? 1 : dx + 3) * C0 * C0 + i * C0 + j) * sn + pu];
-        return S[(long long)((dy == 0 ? 1 : 3 - dx) * C0 * C0 + j * C0 + i) * sn + bp.at(ux + dx, uy + dy)];
+            e = S + (long long)(i * C0 + j) * sn + pu;
+        else if (dy == 1 || (dy == 0 && dx == 1))
+            e = S + (long long)((dy == 0 ? 1 : dx + 3) * C0 * C0 + i * C0 + j) * sn + pu;
+        else
+        {
+            const long long pv = ((nbmask >> k) & 1u) ? bp.at(ux + dx, uy + dy) : pu;
+            e = S + (long long)((dy == 0 ? 1 : 3 - dx) * C0 * C0 + j * C0 + i) * sn + pv;
+        }
+        return *e * keep(k);
+    }
+
+    // The lane's rows of a2's column channel j: E[k][i] = a2(i, neighbour k, j).
+    __device__ __forceinline__ void column(int j, double (&E)[9][C0]) const
+    {
+#pragma unroll
+        for (int dy = -1; dy <= 1; dy++)
+#pragma unroll
+            for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                    E[(dy + 1) * 3 + dx + 1][i] = a2(i, dx, dy, j);
     }
 };
+
 
 // Lane K of each sixteen-lane row, to the whole row (DPP row_newbcast).
 template <int K>
@@ -246,6 +289,23 @@ __device__ __forceinline__ double row_fold(double t, double acc = 0.0)
     else
         return acc;
 }
+
+// v(u + (DX, DY)) for each of the nine neighbours, read from their lanes (a neighbour outside the block reads another
+// lane's or a zero: a2 weighs it by zero).
+#define NNTC_FROM_NEIGHBOUR(DX, DY) out[((DY) + 1) * 3 + (DX) + 1] = row_from<4 * (DY) + (DX)>(v);
+__device__ __forceinline__ void from_neighbours(double v, double (&out)[9])
+{
+    NNTC_FROM_NEIGHBOUR(-1, -1)
+    NNTC_FROM_NEIGHBOUR(0, -1)
+    NNTC_FROM_NEIGHBOUR(1, -1)
+    NNTC_FROM_NEIGHBOUR(-1, 0)
+    NNTC_FROM_NEIGHBOUR(0, 0)
+    NNTC_FROM_NEIGHBOUR(1, 0)
+    NNTC_FROM_NEIGHBOUR(-1, 1)
+    NNTC_FROM_NEIGHBOUR(0, 1)
+    NNTC_FROM_NEIGHBOUR(1, 1)
+}
+#undef NNTC_FROM_NEIGHBOUR
 
 __device__ __forceinline__ int sel_get(unsigned long long bits, int t)
 {
@@ -293,7 +353,13 @@ __device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, co
     const double b = row_bcast<T>(a1[c] + uv[c]);
     const double aa = row_bcast<T>(diag);
     const int s = sel_get(sc, T);
-    const double now = (double)level0_value(lo, hi, r0, r1, s);
+    // The texel's value now, its selector's palette entry (picked by selects: an array indexed by a lane's value would
+    // live in scratch).
+    float pnow = pal[0];
+#pragma unroll
+    for (int kk = 1; kk < 8; kk++)
+        pnow = s == kk ? pal[kk] : pnow;
+    const double now = (double)pnow;
     double best = 0.0, move = 0.0;
     int which = s;
 #pragma unroll
@@ -326,6 +392,57 @@ __device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, co
     }
     if constexpr (T < 15)
         selector_steps<C0, T + 1>(L, c, a1, uv, e, diag, pal, lo, hi, r0, r1, sc, taken);
+}
+
+// The lane's entries a2(i, neighbour K, j) of the initial uv = a2 dv, e[j][i]. The geometry is made opaque first, so
+// the loads cannot be issued before this point.
+template <int C0, int K>
+__device__ __forceinline__ void uv_entries(BlockLane<C0>& L, double (&e)[C0][C0])
+{
+    constexpr int dx = K % 3 - 1, dy = K / 3 - 1;
+    L.opaque();
+#pragma unroll
+    for (int j = 0; j < C0; j++)
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            e[j][i] = L.a2(i, dx, dy, j);
+}
+
+// Neighbour K's terms of the lane's rows of uv, channel by channel, with the neighbour's correction read from its lane;
+// uv is made opaque after them, so they are added before the loads that follow are issued.
+template <int C0, int K>
+__device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (&dv)[C0], double (&uv)[C0])
+{
+    constexpr int dx = K % 3 - 1, dy = K / 3 - 1;
+    double dn[C0];
+#pragma unroll
+    for (int j = 0; j < C0; j++)
+        dn[j] = row_from<4 * dy + dx>(dv[j]);
+#pragma unroll
+    for (int j = 0; j < C0; j++)
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            uv[i] += e[j][i] * dn[j];
+#pragma unroll
+    for (int i = 0; i < C0; i++)
+        asm volatile("" : "+v"(uv[i]));
+}
+
+// uv = a2 dv over the nine neighbours in increasing order (the order of the reference's row sums), neighbour K + 1's
+// entries loaded while neighbour K's terms are added: two neighbours' entries in flight, not all nine.
+template <int C0, int K>
+__device__ __forceinline__ void uv_pipeline(BlockLane<C0>& L, const double (&dv)[C0], double (&uv)[C0],
+                                            const double (&cur)[C0][C0])
+{
+    if constexpr (K < 8)
+    {
+        double nxt[C0][C0];
+        uv_entries<C0, K + 1>(L, nxt);
+        uv_add<C0, K>(cur, dv, uv);
+        uv_pipeline<C0, K + 1>(L, dv, uv, nxt);
+    }
+    else
+        uv_add<C0, K>(cur, dv, uv);
 }
 
 template <int C0>
@@ -361,13 +478,16 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     L.u = u;
     L.ux = ux;
     L.uy = uy;
-    L.pu = L.bp.at(ux, uy);
     {
         const int nx = min(4, w0 - px0), ny = min(4, h0 - py0);
         const unsigned int row = (1u << nx) - 1u;
         L.inside = (row | row << 4 | row << 8 | row << 12) & ((1u << (4 * ny)) - 1u);
     }
     const bool in_u = L.in(u);
+    // A texel outside the plane takes the block's first texel's position: its entries are read (weighted by zero), so
+    // the address must be one the stencil holds.
+    L.pu = in_u ? L.bp.at(ux, uy) : L.bp.at(0, 0);
+    L.init_neighbours();
     auto inside_xy = [=](int lx, int ly) {
         return px0 + lx >= 0 && px0 + lx < w0 && py0 + ly >= 0 && py0 + ly < h0;
     };
@@ -392,10 +512,16 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         {
             int dx, dy;
             stencil_offset(slot, dx, dy);
+            // A neighbour that adds nothing (inside the block, or outside the plane) reads the texel's own values (valid
+            // addresses, finite values) with an opaque weight 0.0: its terms are zeros. A zero leaves the sum unchanged
+            // but for turning a -0.0 into +0.0, and a1 enters every use as a1 + uv, with uv a sum from +0.0 that never
+            // holds -0.0: either zero gives the same a1 + uv.
             const int fx = ux + dx, fy = uy + dy;
-            if (!(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && inside_xy(fx, fy))
+            const bool fok = !(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && inside_xy(fx, fy);
+            double fkeep = fok ? 1.0 : 0.0;
+            asm volatile("" : "+v"(fkeep));
             {
-                const double* xn = a.delta + tix(fx, fy) * C0;
+                const double* xn = a.delta + (fok ? tix(fx, fy) : tu) * C0;
                 double xv[C0];
 #pragma unroll
                 for (int j = 0; j < C0; j++)
@@ -404,13 +530,15 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + L.pu] * xv[j];
+                        acc[i] += (S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + L.pu] * xv[j]) * fkeep;
             }
             const int bx = ux - dx, by = uy - dy;
-            if (!(bx >= 0 && bx < 4 && by >= 0 && by < 4) && inside_xy(bx, by))
+            const bool bok = !(bx >= 0 && bx < 4 && by >= 0 && by < 4) && inside_xy(bx, by);
+            double bkeep = bok ? 1.0 : 0.0;
+            asm volatile("" : "+v"(bkeep));
             {
-                const double* xn = a.delta + tix(bx, by) * C0;
-                const long long pb = L.bp.at(bx, by);
+                const double* xn = a.delta + (bok ? tix(bx, by) : tu) * C0;
+                const long long pb = bok ? L.bp.at(bx, by) : L.pu;
                 double xv[C0];
 #pragma unroll
                 for (int j = 0; j < C0; j++)
@@ -419,7 +547,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += S[(long long)(slot * C0 * C0 + j * C0 + i) * sn + pb] * xv[j];
+                        acc[i] += (S[(long long)(slot * C0 * C0 + j * C0 + i) * sn + pb] * xv[j]) * bkeep;
             }
         }
 #pragma unroll
@@ -436,16 +564,11 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         dv[j] = in_u ? (double)a.v0[tu * C0 + j] - (double)a.prev[tu * C0 + j] : 0.0;
         uv[j] = 0.0;
     }
-#define NNTC_UV_TERM(DX, DY)                                                                                           \
-    {                                                                                                                  \
-        double dn[C0];                                                                                                 \
-        _Pragma("unroll") for (int j = 0; j < C0; j++) dn[j] = row_from<4 * (DY) + (DX)>(dv[j]);                       \
-        if (ux + (DX) >= 0 && ux + (DX) < 4 && uy + (DY) >= 0 && uy + (DY) < 4 && L.in(u + 4 * (DY) + (DX)))           \
-            _Pragma("unroll") for (int j = 0; j < C0; j++) _Pragma("unroll") for (int i = 0; i < C0; i++)              \
-                uv[i] += L.a2(i, (DX), (DY), j) * dn[j];                                                               \
+    {
+        double e0[C0][C0];
+        uv_entries<C0, 0>(L, e0);
+        uv_pipeline<C0, 0>(L, dv, uv, e0);
     }
-    NEIGHBOURS9(NNTC_UV_TERM)
-#undef NNTC_UV_TERM
 
     const long long gb = g.offb + (long long)gy * bxn + gx;
     int ep0[C0], ep1[C0];
@@ -471,19 +594,27 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         const float lo = a.lo0[c], hi = a.hi0[c];
         const double step = ((double)hi - (double)lo) / 255.0;
         const unsigned long long sc = sel[c];
+        // The lane's rows of a2's column c (zero where a2 holds nothing): the endpoint step, its check and its update
+        // read these alone, so a term for a neighbour outside the block or the plane is a zero, which leaves the sum
+        // (from +0.0, never -0.0) unchanged, as leaving it out would.
+        double E[9][C0];
+        L.column(c, E);
         double bp_, bq, pap, paq, qaq;
         {
+            // The texel's weight on the high endpoint, and its neighbours' from their lanes.
             const double aw = bc4_weight8(sel_get(sc, u));
+            double aj[9];
+            from_neighbours(aw, aj);
             const double pp = step * aw;
             const double qq = step * (1.0 - aw);
             const double b = a1[c] + uv[c];
             double ap = 0.0, aq = 0.0;
-            FOR_NEIGHBOURS(ux, uy, dx, dy, {
-                const double e = L.a2(c, dx, dy, c);
-                const double aj = bc4_weight8(sel_get(sc, u + 4 * dy + dx));
-                ap += e * step * aj;
-                aq += e * step * (1.0 - aj);
-            })
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+            {
+                ap += E[k][c] * step * aj[k];
+                aq += E[k][c] * step * (1.0 - aj[k]);
+            }
             bp_ = row_fold(b * pp);
             bq = row_fold(b * qq);
             pap = row_fold(pp * ap);
@@ -523,14 +654,17 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         if (best0 == r0 && best1 == r1)
             continue;
         // The candidate weighed again in the arithmetic the plane is written in: texel v moves by
-        // next(v) - now(v), the two palettes' values at its selector.
-        auto dlt = [=](int v) {
-            const int s = sel_get(sc, v);
-            return (double)level0_value(lo, hi, best0, best1, s) - (double)level0_value(lo, hi, r0, r1, s);
-        };
+        // next(v) - now(v), the two palettes' values at its selector; the lane works out its own texel's move and reads
+        // its neighbours' from their lanes.
+        const int su = sel_get(sc, u);
+        const double dlt = (double)level0_value(lo, hi, best0, best1, su) - (double)level0_value(lo, hi, r0, r1, su);
+        double dn[9];
+        from_neighbours(dlt, dn);
         double quad = 0.0;
-        FOR_NEIGHBOURS(ux, uy, dx, dy, { quad += L.a2(c, dx, dy, c) * dlt(u + 4 * dy + dx); })
-        const double exact = row_fold(dlt(u) * (2.0 * (a1[c] + uv[c]) + quad));
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            quad += E[k][c] * dn[k];
+        const double exact = row_fold(dlt * (2.0 * (a1[c] + uv[c]) + quad));
         if (!(exact < 0.0))
             continue;
         taken += exact;
@@ -538,7 +672,9 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         for (int i = 0; i < C0; i++)
         {
             double add = 0.0;
-            FOR_NEIGHBOURS(ux, uy, dx, dy, { add += L.a2(i, dx, dy, c) * dlt(u + 4 * dy + dx); })
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                add += E[k][i] * dn[k];
             uv[i] += add;
         }
         ep0[c] = best0;
@@ -565,11 +701,14 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             double* const ec = s_e + lane_off;
             // The lane's entries (u, ii; u + (dx, dy), c) that the channel's steps read, issued together: one memory
             // latency per channel rather than one per step.
-            FOR_NEIGHBOURS(L.ux, L.uy, dx, dy, {
-                _Pragma("unroll") for (int ii = 0; ii < C0; ii++) ec[((dy + 1) * 3 + dx + 1) * 64 * C0 + ii] =
-                    L.a2(ii, dx, dy, c);
-            })
-            const double diag = L.a2(c, 0, 0, c);
+            double E[9][C0];
+            L.column(c, E);
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+#pragma unroll
+                for (int ii = 0; ii < C0; ii++)
+                    ec[k * 64 * C0 + ii] = E[k][ii];
+            const double diag = E[4][c];
             selector_steps<C0, 0>(L, c, a1, uv, ec, diag, pal[c], a.lo0[c], a.hi0[c], ep0[c], ep1[c], sel[c], taken);
         }
 
