@@ -273,8 +273,10 @@ int nntc_hip_state_restore(nntc_hip_ctx* ctx);
 /* Per-kernel timing (not in the reference's seam; the benchmark's instrument). While enabled, every launch of the
  * arm's named kernels is bracketed by HIP events on the arm's stream and its algorithmic bytes and flops are added
  * up. nntc_hip_kernel_stats reads one kernel's totals (index-based enumeration: name_out receives the name); it
- * returns 0, or nonzero past the last kernel. */
+ * returns 0, or nonzero past the last kernel. nntc_hip_profile_only limits the timing to the kernel of that name
+ * (NULL: every kernel), so that a timed run carries one kernel's events only. */
 void nntc_hip_profile_enable(int on);
+void nntc_hip_profile_only(const char* kernel);
 void nntc_hip_profile_reset(void);
 int nntc_hip_kernel_stats(int index, char* name_out, size_t name_len, long long* launches, double* ms, double* bytes,
                           double* flops);

@@ -99,6 +99,7 @@ struct KernelStat
 struct Profiler
 {
     bool on = false;
+    std::string only;                   // when set, the one kernel timed
     std::vector<std::pair<std::string, KernelStat>> stats;
     std::vector<hipEvent_t> pool;
     KernelStat& get(const char* name);
@@ -194,11 +195,11 @@ inline StencilMap stencil_map(const Ctx& c, bool level0)
     return m;
 }
 
-// The fast-form objective's tile of pixels (one workgroup each) and the number of tiles of a plane.
 // The quantised sweeps' moved-index counters: MOVED_SLOTS per plane, MOVED_STRIDE unsigned ints (one 128-byte line)
 // apart, summed on the host after the sweeps.
 constexpr int MOVED_SLOTS = 64, MOVED_STRIDE = 32;
 
+// The fast-form objective's tile of pixels (one workgroup each) and the number of tiles of a plane.
 constexpr int OBJ_TW = 32, OBJ_TH = 8;
 inline long long objective_tiles(const PlaneGeom& g)
 {

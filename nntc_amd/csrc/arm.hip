@@ -117,7 +117,7 @@ Profiler::~Profiler()
 KernelTimer::KernelTimer(hipStream_t s, const char* name, double bytes, double flops) : stream(s)
 {
     Profiler& p = profiler();
-    if (!p.on)
+    if (!p.on || (!p.only.empty() && p.only != name))
         return;
     st = &p.get(name);
     st->launches++;
@@ -1342,6 +1342,11 @@ int nntc_hip_bc_state_save(nntc_hip_ctx* h, int plane, uint8_t* ep, size_t n_ep,
 void nntc_hip_profile_enable(int on)
 {
     profiler().on = on != 0;
+}
+
+void nntc_hip_profile_only(const char* kernel)
+{
+    profiler().only = kernel ? kernel : "";
 }
 
 void nntc_hip_profile_reset(void)
