@@ -379,31 +379,22 @@ __device__ __forceinline__ int merge_taps(const int* cx, const int* cy, const fl
     return taps;
 }
 
-// Texel ti's five blocks and gradient. LEVEL0 false: a level-1 texel, every site whose level-1 taps reach it, C = c1,
-// CS = c0. LEVEL0 true: a level-0 texel (block (c')), the 3x3-pixel neighbourhood with the centre site read nearest as
-// one tap of weight 1, C = c0, CS = c1.
-//
-// G lanes share a texel. G = 1 (the exact form): the one lane visits the footprint's sites in the reference's order
-// (pixel row, pixel, site), so every sum receives the reference's operands in the reference's order. G > 1 (the fast
-// form): lane r of the group visits the sites r, r + G, ... of that order and the group's partial sums meet in a
-// fixed shuffle tree - deterministic, the last bits not the reference's. A level-1 texel's footprint holds about four
-// hundred sites; spread over sixteen lanes the gather has sixteen times the wavefronts to hide its loads behind.
-template <bool LEVEL0, int C, int CS, int G>
+// Texel ti's five blocks and gradient in the exact form. LEVEL0 false: a level-1 texel, every site whose level-1 taps
+// reach it, C = c1, CS = c0. LEVEL0 true: a level-0 texel (block (c')), the 3x3-pixel neighbourhood with the centre site
+// read nearest as one tap of weight 1, C = c0, CS = c1. The lane visits the footprint's sites in the reference's order
+// (pixel row, pixel, site), so every sum receives the reference's operands in the reference's order.
+template <bool LEVEL0, int C, int CS>
 __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
 {
     constexpr int NM = 1 + CS + CS * (CS + 1) / 2;
     const LatentView& L = a.L;
-    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long gi = gt / G;
-    const int sub = (int)(gt % G);
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = L.texels.start[L.texels.n];
-    // With G > 1 every lane of a group must reach the shuffles, so a lane past the end idles instead of leaving.
-    if (G == 1 && gi >= total)
+    if (gi >= total)
         return;
-    const bool live = gi < total;
-    const int p = range_plane(L.texels, live ? gi : total - 1);
+    const int p = range_plane(L.texels, gi);
     const PlaneGeom& g = L.planes.p[p];
-    const long long ti = (live ? gi : total - 1) - L.texels.start[p];
+    const long long ti = gi - L.texels.start[p];
     const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
     const int tw_ = LEVEL0 ? w0 : w1;
     const int tx = (int)(ti % tw_), ty = (int)(ti / tw_);
@@ -432,22 +423,13 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
         footprint_range(tx, w0, w1, px_lo, px_hi);
         footprint_range(ty, h0, h1, py_lo, py_hi);
     }
-    if (!live)
-        py_hi = py_lo - 1;   // no sites: the lane only joins the shuffles
-    // The sites in the reference's order (pixel row, pixel, site): lane `sub` of the group takes every G-th.
+    // The sites in the reference's order (pixel row, pixel, site), stepped without a division.
     const int nxp = px_hi - px_lo + 1;
     const int visits = nxp * (py_hi - py_lo + 1) * per_pixel;
     int px = px_lo, py = py_lo, j = 0;
-    for (int v = sub; v < visits; v += G)
+    for (int v = 0; v < visits; v++)
     {
-        if (G > 1)
-        {
-            const int pix = v / per_pixel;
-            j = v - pix * per_pixel;
-            px = px_lo + pix % nxp;
-            py = py_lo + pix / nxp;
-        }
-        else if (v > 0 && ++j > a.k)   // one lane: step to the next site without a division
+        if (v > 0 && ++j > a.k)
         {
             j = 0;
             if (++px > px_hi)
@@ -510,24 +492,6 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
         }
     }
 
-    if (G > 1)
-    {
-#pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1)
-        {
-#pragma unroll
-            for (int s = 0; s < STENCIL_BLOCKS; s++)
-#pragma unroll
-                for (int i = 0; i < NM; i++)
-                    sums[s][i] += __shfl_xor(sums[s][i], off);
-#pragma unroll
-            for (int i = 0; i < C; i++)
-                gacc[i] += __shfl_xor(gacc[i], off);
-        }
-    }
-    if (!live || sub != 0)
-        return;
-
     // The five owned blocks expanded from the monomial sums; a zero sum is skipped.
     const long long gtx = L.texels.start[p] + ti;
     const long long sn = L.sm.sn;
@@ -559,28 +523,236 @@ __global__ __launch_bounds__(128) void k_gather(AssembleArgs a)
         L.grad[gtx * C + i] = gacc[i];
 }
 
-template <bool LEVEL0, int C, int G>
+template <bool LEVEL0, int C>
 void launch_gather_cs(int cs, dim3 grid, hipStream_t s, const AssembleArgs& a)
 {
     switch (cs)
     {
-    case 1: hipLaunchKernelGGL((k_gather<LEVEL0, C, 1, G>), grid, dim3(128), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_gather<LEVEL0, C, 2, G>), grid, dim3(128), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_gather<LEVEL0, C, 3, G>), grid, dim3(128), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_gather<LEVEL0, C, 4, G>), grid, dim3(128), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_gather<LEVEL0, C, 1>), grid, dim3(128), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_gather<LEVEL0, C, 2>), grid, dim3(128), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_gather<LEVEL0, C, 3>), grid, dim3(128), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_gather<LEVEL0, C, 4>), grid, dim3(128), 0, s, a); break;
     }
 }
 
-template <bool LEVEL0, int G>
+template <bool LEVEL0>
 void launch_gather(int c, int cs, long long texels, hipStream_t s, const AssembleArgs& a)
 {
-    const dim3 grid(blocks_for(texels * G, 128));
+    const dim3 grid(blocks_for(texels, 128));
     switch (c)
     {
-    case 1: launch_gather_cs<LEVEL0, 1, G>(cs, grid, s, a); break;
-    case 2: launch_gather_cs<LEVEL0, 2, G>(cs, grid, s, a); break;
-    case 3: launch_gather_cs<LEVEL0, 3, G>(cs, grid, s, a); break;
-    default: launch_gather_cs<LEVEL0, 4, G>(cs, grid, s, a); break;
+    case 1: launch_gather_cs<LEVEL0, 1>(cs, grid, s, a); break;
+    case 2: launch_gather_cs<LEVEL0, 2>(cs, grid, s, a); break;
+    case 3: launch_gather_cs<LEVEL0, 3>(cs, grid, s, a); break;
+    default: launch_gather_cs<LEVEL0, 4>(cs, grid, s, a); break;
+    }
+}
+
+// The fast form's view of a site's taps: a tap texel's weight separates into a column weight times a row weight (both
+// taps of a clamped pair add up at the one texel), so the weights of a texel's five owned slots are products of five
+// column and row weights - no merge of the four taps and no test per tap and slot.
+__device__ __forceinline__ float tap_weight(int x, int x0, int x1, float f)
+{
+    return (x0 == x ? 1.0f - f : 0.0f) + (x1 == x ? f : 0.0f);
+}
+
+// w_self (the site's weight at texel (tx, ty)) times its weight at each owned slot's texel; false when the texel holds no
+// weight of the site.
+__device__ __forceinline__ bool slot_scales(const Tap& t, int tx, int ty, double (&scale)[STENCIL_BLOCKS], float& w_self)
+{
+    const float wxm = tap_weight(tx - 1, t.x0, t.x1, t.fx), wx0 = tap_weight(tx, t.x0, t.x1, t.fx),
+                wxp = tap_weight(tx + 1, t.x0, t.x1, t.fx);
+    const float wy0 = tap_weight(ty, t.y0, t.y1, t.fy), wyp = tap_weight(ty + 1, t.y0, t.y1, t.fy);
+    w_self = wx0 * wy0;
+    if (w_self == 0.0f)
+        return false;
+    const double ws = (double)w_self;
+    scale[0] = ws * (double)(wx0 * wy0);
+    scale[1] = ws * (double)(wxp * wy0);
+    scale[2] = ws * (double)(wxm * wyp);
+    scale[3] = ws * (double)(wx0 * wyp);
+    scale[4] = ws * (double)(wxp * wyp);
+    return true;
+}
+
+// The taps of site j of pixel (px, py) into a w x h plane; level 0's centre site reads its own texel with weight 1.
+__device__ __forceinline__ Tap site_taps(int k, int j, int px, int py, int w0, int h0, int w, int h, bool nearest_centre)
+{
+    if (nearest_centre && j == 0)
+    {
+        Tap t;
+        t.x0 = t.x1 = px;
+        t.y0 = t.y1 = py;
+        t.fx = t.fy = 0.0f;
+        return t;
+    }
+    float dx, dy;
+    site_delta(k, j, px, py, dx, dy);
+    return tap_pixel(px, py, dx, dy, w0, h0, w, h);
+}
+
+// One DPP move of a double (both halves) under the data-parallel-primitive control CTRL, within the row.
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double x)
+{
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// The sum over an aligned group of G lanes (G a power of two up to 16), on every lane of the group: pairs of lanes
+// (quad_perm [1,0,3,2], then [2,3,0,1]), then halves of eight (row_half_mirror) and of sixteen (row_mirror). Each level
+// adds two partial sums that every lane of the pair holds in the other order, and addition commutes, so every lane of
+// the group ends with the same bits.
+template <int G>
+__device__ __forceinline__ double group_sum(double x)
+{
+    if constexpr (G >= 2)
+        x += dpp_double<0xB1>(x);
+    if constexpr (G >= 4)
+        x += dpp_double<0x4E>(x);
+    if constexpr (G >= 8)
+        x += dpp_double<0x141>(x);
+    if constexpr (G >= 16)
+        x += dpp_double<0x140>(x);
+    return x;
+}
+
+// The fast form's level-1 gather: G lanes per texel take the footprint's sites in turn (site j fastest, then pixel,
+// then pixel row), each adding its five slots' terms branch-free with fused multiply-adds; the group's sums meet in a
+// fixed butterfly (deterministic).
+template <int C, int CS, int G>
+__global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
+{
+    constexpr int NM = 1 + CS + CS * (CS + 1) / 2;
+    const LatentView& L = a.L;
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gi = gt / G;
+    const int sub = (int)(gt % G);
+    const long long total = L.texels.start[L.texels.n];
+    // Every lane of a group must reach the shuffles, so a lane past the end idles instead of leaving.
+    const bool live = gi < total;
+    const int p = range_plane(L.texels, live ? gi : total - 1);
+    const PlaneGeom& g = L.planes.p[p];
+    const long long ti = (live ? gi : total - 1) - L.texels.start[p];
+    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
+    const int tx = (int)(ti % w1), ty = (int)(ti / w1);
+    const int per_pixel = 1 + a.k;
+    const long long site0 = a.sites.start[p];
+
+    double sums[STENCIL_BLOCKS][NM], gacc[C];
+#pragma unroll
+    for (int s = 0; s < STENCIL_BLOCKS; s++)
+#pragma unroll
+        for (int i = 0; i < NM; i++)
+            sums[s][i] = 0.0;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        gacc[i] = 0.0;
+
+    int px_lo, px_hi, py_lo, py_hi;
+    footprint_range(tx, w0, w1, px_lo, px_hi);
+    footprint_range(ty, h0, h1, py_lo, py_hi);
+    const int nxp = px_hi - px_lo + 1;
+    const int visits = live ? nxp * (py_hi - py_lo + 1) * per_pixel : 0;
+    // Lane sub's first site and its stride of G sites as (pixels, sites) steps: no division in the loop.
+    int j = sub % per_pixel, pix = sub / per_pixel;
+    int px = px_lo + pix % nxp, py = py_lo + pix / nxp;
+    const int dj = G % per_pixel, dpix = G / per_pixel;
+    const int dpx = dpix % nxp, dpy = dpix / nxp;
+    for (int v = sub; v < visits; v += G)
+    {
+        const Tap t = site_taps(a.k, j, px, py, w0, h0, w1, h1, false);
+        double scale[STENCIL_BLOCKS];
+        float w_self;
+        if (slot_scales(t, tx, ty, scale, w_self))
+        {
+            const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
+            float sv[CS];
+#pragma unroll
+            for (int i = 0; i < CS; i++)
+                sv[i] = a.site_f[s * CS + i];
+            double mono[NM];
+            monomials<CS>(sv, mono);
+#pragma unroll
+            for (int aa = 0; aa < C; aa++)
+                gacc[aa] = fma((double)w_self, a.site_d[s * C + aa], gacc[aa]);
+#pragma unroll
+            for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+#pragma unroll
+                for (int i = 0; i < NM; i++)
+                    sums[sl][i] = fma(scale[sl], mono[i], sums[sl][i]);
+        }
+        // The next site: j by dj, the pixel by dpix (carrying into the row), with one carry from j.
+        j += dj;
+        int cpx = dpx;
+        if (j >= per_pixel)
+        {
+            j -= per_pixel;
+            cpx++;
+        }
+        px += cpx;
+        py += dpy;
+        if (px > px_hi)
+        {
+            px -= nxp;
+            py++;
+        }
+    }
+
+#pragma unroll
+    for (int s = 0; s < STENCIL_BLOCKS; s++)
+#pragma unroll
+        for (int i = 0; i < NM; i++)
+            sums[s][i] = group_sum<G>(sums[s][i]);
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        gacc[i] = group_sum<G>(gacc[i]);
+    if (!live)
+        return;
+
+    // The five owned blocks expanded from the monomial sums, which the butterfly left on every lane of the group: lane
+    // sub writes entries sub, sub + G, ... of each block.
+    const long long gtx = L.texels.start[p] + ti;
+    const long long sn = L.sm.sn;
+    double* out_h = L.stencil + stencil_pos(L.sm, p, w1, tx, ty);
+    for (int e = sub; e < C * C; e += G)
+    {
+        double mat[NM];
+#pragma unroll
+        for (int i = 0; i < NM; i++)
+            mat[i] = a.mono_p[(size_t)i * C * C + e];
+#pragma unroll
+        for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
+        {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; i++)
+                v = fma(sums[slot][i], mat[i], v);
+            out_h[(slot * C * C + e) * sn] = v;
+        }
+    }
+    if (sub < C)
+    {
+        // gacc[sub], picked by selects (a register array indexed by a lane's value would live in scratch).
+        double gv = gacc[0];
+#pragma unroll
+        for (int i = 1; i < C; i++)
+            gv = sub == i ? gacc[i] : gv;
+        L.grad[gtx * C + sub] = gv;
+    }
+}
+
+template <int C, int G>
+void launch_gather_fast1(int cs, long long texels, hipStream_t s, const AssembleArgs& a)
+{
+    const dim3 grid(blocks_for(texels * G, 128));
+    switch (cs)
+    {
+    case 1: hipLaunchKernelGGL((k_gather_fast1<C, 1, G>), grid, dim3(128), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_gather_fast1<C, 2, G>), grid, dim3(128), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_gather_fast1<C, 3, G>), grid, dim3(128), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_gather_fast1<C, 4, G>), grid, dim3(128), 0, s, a); break;
     }
 }
 
@@ -621,34 +793,11 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
         for (int px = px_lo; px <= px_hi; px++)
             for (int j = 0; j <= a.k; j++)
             {
-                int nx[4], ny[4];
-                float tw[4];
-                int taps;
-                if (j == 0)
-                {
-                    nx[0] = px;
-                    ny[0] = py;
-                    tw[0] = 1.0f;
-                    taps = 1;
-                }
-                else
-                {
-                    float dx, dy;
-                    site_delta(a.k, j, px, py, dx, dy);
-                    const Tap t = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-                    const int cx[4] = { t.x0, t.x1, t.x0, t.x1 };
-                    const int cy[4] = { t.y0, t.y0, t.y1, t.y1 };
-                    const float cwt[4] = { (1.0f - t.fx) * (1.0f - t.fy), t.fx * (1.0f - t.fy), (1.0f - t.fx) * t.fy,
-                                           t.fx * t.fy };
-                    taps = merge_taps(cx, cy, cwt, nx, ny, tw);
-                }
-                int self = -1;
-                for (int e = 0; e < taps; e++)
-                    if (nx[e] == tx && ny[e] == ty)
-                        self = e;
-                if (self < 0)
+                const Tap t = site_taps(a.k, j, px, py, w0, h0, w0, h0, true);
+                double scale[STENCIL_BLOCKS];
+                float w_self;
+                if (!slot_scales(t, tx, ty, scale, w_self))
                     continue;
-                const float w_self = tw[self];
                 const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
                 float hv[NH];
 #pragma unroll
@@ -656,20 +805,12 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
                     hv[e] = a.site_f[s * NH + e];
 #pragma unroll
                 for (int aa = 0; aa < C; aa++)
-                    gacc[aa] += (double)w_self * a.site_d[s * C + aa];
-                for (int e = 0; e < taps; e++)
-                {
-                    const int slot = stencil_slot(nx[e] - tx, ny[e] - ty);
-                    if (slot < 0)
-                        continue;
-                    const double scale = (double)w_self * (double)tw[e];
+                    gacc[aa] = fma((double)w_self, a.site_d[s * C + aa], gacc[aa]);
 #pragma unroll
-                    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
-                        if (sl == slot)
+                for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
 #pragma unroll
-                            for (int q = 0; q < NH; q++)
-                                hs[sl][q] += scale * (double)hv[q];
-                }
+                    for (int q = 0; q < NH; q++)
+                        hs[sl][q] = fma(scale[sl], (double)hv[q], hs[sl][q]);
             }
 
     const long long sn = L.sm.sn;
@@ -1723,7 +1864,7 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
         KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
         // The exact form gathers a texel per lane in the reference's order; the fast form spreads a level-1 texel's
-        // four-hundred-site footprint over sixteen lanes (level 0's footprints are small enough for one).
+        // footprint (125 sites at K = 4) over eight lanes and gathers level 0's per-site blocks a texel per lane.
         if (direct)
         {
             const dim3 grid(blocks_for(n, 128));
@@ -1738,12 +1879,20 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         else if (c.form == NNTC_FORM_EXACT)
         {
             if (level0)
-                launch_gather<true, 1>(L.c, cs, n, c.stream, a);
+                launch_gather<true>(L.c, cs, n, c.stream, a);
             else
-                launch_gather<false, 1>(L.c, cs, n, c.stream, a);
+                launch_gather<false>(L.c, cs, n, c.stream, a);
         }
         else
-            launch_gather<false, 16>(L.c, cs, n, c.stream, a);
+        {
+            switch (L.c)
+            {
+            case 1: launch_gather_fast1<1, 8>(cs, n, c.stream, a); break;
+            case 2: launch_gather_fast1<2, 8>(cs, n, c.stream, a); break;
+            case 3: launch_gather_fast1<3, 8>(cs, n, c.stream, a); break;
+            default: launch_gather_fast1<4, 8>(cs, n, c.stream, a); break;
+            }
+        }
     }
     HIP_OK(hipGetLastError());
     ChunkArgs ch = make_chunks(c, L);
