@@ -618,6 +618,45 @@ __device__ __forceinline__ double group_sum(double x)
     return x;
 }
 
+// Lane sub's walk over a footprint's sites in the order (pixel row, pixel, site j), taking every G-th site of the
+// order: the stride of G sites is split once into (rows, pixels, sites) steps, so the walk has no division.
+struct SiteWalk
+{
+    int j, px, py, px_lo, px_hi, nxp, per_pixel, dj, dpx, dpy;
+    __device__ __forceinline__ void init(int sub, int g, int lo_x, int hi_x, int lo_y, int pp)
+    {
+        px_lo = lo_x;
+        px_hi = hi_x;
+        nxp = hi_x - lo_x + 1;
+        per_pixel = pp;
+        j = sub % pp;
+        const int pix = sub / pp;
+        px = lo_x + pix % nxp;
+        py = lo_y + pix / nxp;
+        dj = g % pp;
+        const int dpix = g / pp;
+        dpx = dpix % nxp;
+        dpy = dpix / nxp;
+    }
+    __device__ __forceinline__ void next()
+    {
+        j += dj;
+        int cpx = dpx;
+        if (j >= per_pixel)
+        {
+            j -= per_pixel;
+            cpx++;
+        }
+        px += cpx;
+        py += dpy;
+        if (px > px_hi)
+        {
+            px -= nxp;
+            py++;
+        }
+    }
+};
+
 // The fast form's level-1 gather: G lanes per texel take the footprint's sites in turn (site j fastest, then pixel,
 // then pixel row), each adding its five slots' terms branch-free with fused multiply-adds; the group's sums meet in a
 // fixed butterfly (deterministic).
@@ -653,15 +692,12 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
     int px_lo, px_hi, py_lo, py_hi;
     footprint_range(tx, w0, w1, px_lo, px_hi);
     footprint_range(ty, h0, h1, py_lo, py_hi);
-    const int nxp = px_hi - px_lo + 1;
-    const int visits = live ? nxp * (py_hi - py_lo + 1) * per_pixel : 0;
-    // Lane sub's first site and its stride of G sites as (pixels, sites) steps: no division in the loop.
-    int j = sub % per_pixel, pix = sub / per_pixel;
-    int px = px_lo + pix % nxp, py = py_lo + pix / nxp;
-    const int dj = G % per_pixel, dpix = G / per_pixel;
-    const int dpx = dpix % nxp, dpy = dpix / nxp;
-    for (int v = sub; v < visits; v += G)
+    const int visits = live ? (px_hi - px_lo + 1) * (py_hi - py_lo + 1) * per_pixel : 0;
+    SiteWalk wk;
+    wk.init(sub, G, px_lo, px_hi, py_lo, per_pixel);
+    for (int v = sub; v < visits; v += G, wk.next())
     {
+        const int j = wk.j, px = wk.px, py = wk.py;
         const Tap t = site_taps(a.k, j, px, py, w0, h0, w1, h1, false);
         double scale[STENCIL_BLOCKS];
         float w_self;
@@ -682,21 +718,6 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
 #pragma unroll
                 for (int i = 0; i < NM; i++)
                     sums[sl][i] = fma(scale[sl], mono[i], sums[sl][i]);
-        }
-        // The next site: j by dj, the pixel by dpix (carrying into the row), with one carry from j.
-        j += dj;
-        int cpx = dpx;
-        if (j >= per_pixel)
-        {
-            j -= per_pixel;
-            cpx++;
-        }
-        px += cpx;
-        py += dpy;
-        if (px > px_hi)
-        {
-            px -= nxp;
-            py++;
         }
     }
 
@@ -758,19 +779,22 @@ void launch_gather_fast1(int cs, long long texels, hipStream_t s, const Assemble
 
 // The fast form's level-0 gather: each site stores its own block Q^T diag(cw) Q (upper triangle, fp32) in place of
 // the level-1 sample, so a texel adds scale x that block into each owned slot - C (C + 1) / 2 values per tap instead of
-// the 1 + c1 + c1 (c1 + 1) / 2 monomials and their expansion, and a handful of registers instead of five slots of
-// monomial sums. The blocks are symmetric, so each slot's block is written from its triangle.
-template <int C>
+// the 1 + c1 + c1 (c1 + 1) / 2 monomials and their expansion. G lanes share a texel's 3x3-pixel footprint (45 sites at
+// K = 4) and meet in the butterfly; the blocks are symmetric, so each slot's block is written from its triangle.
+template <int C, int G>
 __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
 {
     constexpr int NH = C * (C + 1) / 2;
     const LatentView& L = a.L;
-    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= L.texels.start[L.texels.n])
-        return;
-    const int p = range_plane(L.texels, gi);
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gi = gt / G;
+    const int sub = (int)(gt % G);
+    const long long total = L.texels.start[L.texels.n];
+    // Every lane of a group must reach the butterfly, so a lane past the end idles instead of leaving.
+    const bool live = gi < total;
+    const int p = range_plane(L.texels, live ? gi : total - 1);
     const PlaneGeom& g = L.planes.p[p];
-    const long long ti = gi - L.texels.start[p];
+    const long long ti = (live ? gi : total - 1) - L.texels.start[p];
     const int w0 = g.w0, h0 = g.h0;
     const int tx = (int)(ti % w0), ty = (int)(ti / w0);
     const int per_pixel = 1 + a.k;
@@ -789,49 +813,70 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
     int px_lo, px_hi, py_lo, py_hi;
     footprint_range(tx, w0, w0, px_lo, px_hi);
     footprint_range(ty, h0, h0, py_lo, py_hi);
-    for (int py = py_lo; py <= py_hi; py++)
-        for (int px = px_lo; px <= px_hi; px++)
-            for (int j = 0; j <= a.k; j++)
-            {
-                const Tap t = site_taps(a.k, j, px, py, w0, h0, w0, h0, true);
-                double scale[STENCIL_BLOCKS];
-                float w_self;
-                if (!slot_scales(t, tx, ty, scale, w_self))
-                    continue;
-                const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
-                float hv[NH];
+    const int visits = live ? (px_hi - px_lo + 1) * (py_hi - py_lo + 1) * per_pixel : 0;
+    SiteWalk wk;
+    wk.init(sub, G, px_lo, px_hi, py_lo, per_pixel);
+    for (int v = sub; v < visits; v += G, wk.next())
+    {
+        const int j = wk.j, px = wk.px, py = wk.py;
+        const Tap t = site_taps(a.k, j, px, py, w0, h0, w0, h0, true);
+        double scale[STENCIL_BLOCKS];
+        float w_self;
+        if (!slot_scales(t, tx, ty, scale, w_self))
+            continue;
+        const long long s = site0 + ((long long)py * w0 + px) * per_pixel + j;
+        float hv[NH];
 #pragma unroll
-                for (int e = 0; e < NH; e++)
-                    hv[e] = a.site_f[s * NH + e];
+        for (int e = 0; e < NH; e++)
+            hv[e] = a.site_f[s * NH + e];
 #pragma unroll
-                for (int aa = 0; aa < C; aa++)
-                    gacc[aa] = fma((double)w_self, a.site_d[s * C + aa], gacc[aa]);
+        for (int aa = 0; aa < C; aa++)
+            gacc[aa] = fma((double)w_self, a.site_d[s * C + aa], gacc[aa]);
 #pragma unroll
-                for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+        for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
 #pragma unroll
-                    for (int q = 0; q < NH; q++)
-                        hs[sl][q] = fma(scale[sl], (double)hv[q], hs[sl][q]);
-            }
+            for (int q = 0; q < NH; q++)
+                hs[sl][q] = fma(scale[sl], (double)hv[q], hs[sl][q]);
+    }
 
-    const long long sn = L.sm.sn;
-    double* out_h = L.stencil + stencil_pos(L.sm, p, w0, tx, ty);
 #pragma unroll
     for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
-    {
-        int e = 0;
 #pragma unroll
-        for (int i = 0; i < C; i++)
-#pragma unroll
-            for (int jj = i; jj < C; jj++, e++)
-            {
-                out_h[(sl * C * C + i * C + jj) * sn] = hs[sl][e];
-                out_h[(sl * C * C + jj * C + i) * sn] = hs[sl][e];
-            }
-    }
-    const long long gtx = L.texels.start[p] + ti;
+        for (int q = 0; q < NH; q++)
+            hs[sl][q] = group_sum<G>(hs[sl][q]);
 #pragma unroll
     for (int i = 0; i < C; i++)
-        L.grad[gtx * C + i] = gacc[i];
+        gacc[i] = group_sum<G>(gacc[i]);
+    if (!live)
+        return;
+
+    // Lane sub writes entries sub, sub + G, ... of each slot's C x C block, entry (i, jj) from the triangle's
+    // (min, max) element, picked by selects (a register array indexed by a lane's value would live in scratch).
+    const long long sn = L.sm.sn;
+    double* out_h = L.stencil + stencil_pos(L.sm, p, w0, tx, ty);
+    for (int e = sub; e < C * C; e += G)
+    {
+        const int i = e / C, jj = e % C;
+        const int lo = i < jj ? i : jj, hi = i < jj ? jj : i;
+        const int tri = lo * C - lo * (lo - 1) / 2 + (hi - lo);
+#pragma unroll
+        for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+        {
+            double v = hs[sl][0];
+#pragma unroll
+            for (int q = 1; q < NH; q++)
+                v = tri == q ? hs[sl][q] : v;
+            out_h[(sl * C * C + e) * sn] = v;
+        }
+    }
+    if (sub < C)
+    {
+        double gv = gacc[0];
+#pragma unroll
+        for (int i = 1; i < C; i++)
+            gv = sub == i ? gacc[i] : gv;
+        L.grad[(L.texels.start[p] + ti) * C + sub] = gv;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -1864,16 +1909,17 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
         KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
         // The exact form gathers a texel per lane in the reference's order; the fast form spreads a level-1 texel's
-        // footprint (125 sites at K = 4) over eight lanes and gathers level 0's per-site blocks a texel per lane.
+        // footprint (125 sites at K = 4) over eight lanes and level 0's per-site blocks (45 sites) over two.
         if (direct)
         {
-            const dim3 grid(blocks_for(n, 128));
+            constexpr int G0 = 2;
+            const dim3 grid(blocks_for(n * G0, 128));
             switch (L.c)
             {
-            case 1: hipLaunchKernelGGL(k_gather_direct<1>, grid, dim3(128), 0, c.stream, a); break;
-            case 2: hipLaunchKernelGGL(k_gather_direct<2>, grid, dim3(128), 0, c.stream, a); break;
-            case 3: hipLaunchKernelGGL(k_gather_direct<3>, grid, dim3(128), 0, c.stream, a); break;
-            default: hipLaunchKernelGGL(k_gather_direct<4>, grid, dim3(128), 0, c.stream, a); break;
+            case 1: hipLaunchKernelGGL((k_gather_direct<1, G0>), grid, dim3(128), 0, c.stream, a); break;
+            case 2: hipLaunchKernelGGL((k_gather_direct<2, G0>), grid, dim3(128), 0, c.stream, a); break;
+            case 3: hipLaunchKernelGGL((k_gather_direct<3, G0>), grid, dim3(128), 0, c.stream, a); break;
+            default: hipLaunchKernelGGL((k_gather_direct<4, G0>), grid, dim3(128), 0, c.stream, a); break;
             }
         }
         else if (c.form == NNTC_FORM_EXACT)
