@@ -164,6 +164,14 @@ struct Ctx
 
     // Pinned host staging for the small per-call readbacks.
     double* host_scalars = nullptr;
+    // Pinned host staging for readbacks and small uploads, grown on demand: a copy between device memory and pageable
+    // host memory is staged by the runtime through buffers of its own, in pieces, each a copy kernel and a wait.
+    void* host_stage = nullptr;
+    size_t host_stage_bytes = 0;
+    bool stage_in_flight = false;   // an upload may still be reading the staging area
+    // The staging area, at least `bytes` long. The host may write it (host_writes) only once no upload still reads
+    // it; a copy into it on the stream is ordered after any such upload by the stream itself.
+    void* stage(size_t bytes, bool host_writes);
 
     size_t bytes = 0;
 
@@ -175,6 +183,31 @@ struct Ctx
     const PlaneGeom& g(int p) const { return planes.p[p]; }
     float ms_since(hipEvent_t a);
 };
+
+// n values read back from the device through the pinned staging area; the pointer holds until the next staging use.
+template <class T>
+const T* read_back(Ctx& c, const T* dev, size_t n)
+{
+    T* h = static_cast<T*>(c.stage(n * sizeof(T), false));
+    if (n)
+        HIP_OK(hipMemcpyAsync(h, dev, n * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+    HIP_OK(hipStreamSynchronize(c.stream));
+    c.stage_in_flight = false;
+    return h;
+}
+
+// n values uploaded through the pinned staging area, asynchronously (the next staging use waits for it).
+template <class T>
+void upload_staged(Ctx& c, T* dev, const T* host, size_t n)
+{
+    if (!n)
+        return;
+    T* h = static_cast<T*>(c.stage(n * sizeof(T), true));
+    for (size_t i = 0; i < n; i++)
+        h[i] = host[i];
+    HIP_OK(hipMemcpyAsync(dev, h, n * sizeof(T), hipMemcpyHostToDevice, c.stream));
+    c.stage_in_flight = true;
+}
 
 // The stencil layout of one latent's chain (see StencilMap).
 inline StencilMap stencil_map(const Ctx& c, bool level0)

@@ -140,6 +140,26 @@ KernelTimer::~KernelTimer()
     }
 }
 
+void* Ctx::stage(size_t bytes, bool host_writes)
+{
+    if (stage_in_flight && (host_writes || bytes > host_stage_bytes))
+    {
+        HIP_OK(hipStreamSynchronize(stream));
+        stage_in_flight = false;
+    }
+    if (bytes > host_stage_bytes)
+    {
+        if (host_stage)
+            HIP_OK(hipHostFree(host_stage));
+        host_stage = nullptr;
+        host_stage_bytes = 0;
+        const size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
+        HIP_OK(hipHostMalloc(&host_stage, want, hipHostMallocDefault));
+        host_stage_bytes = want;
+    }
+    return host_stage;
+}
+
 float Ctx::ms_since(hipEvent_t a)
 {
     HIP_OK(hipEventRecord(ev1, stream));
@@ -448,10 +468,8 @@ double sweep_block(Ctx& c, const nntc_model_state* st, int k, double ridge, int 
     const float ms_sw = c.ms_since(c.ev0);
     const int ch = level0 ? c.c0 : c.c1;
     read_scalars(c, ch, rep);
-    std::vector<unsigned int> slots((size_t)c.planes.n * MOVED_SLOTS * MOVED_STRIDE);
-    HIP_OK(hipMemcpyAsync(slots.data(), c.counters.p, slots.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
-                          c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const unsigned int* slots = read_back(c, (const unsigned int*)c.counters.p,
+                                          (size_t)c.planes.n * MOVED_SLOTS * MOVED_STRIDE);
     std::vector<unsigned long long> moved((size_t)c.planes.n, 0);
     for (int p = 0; p < c.planes.n; p++)
         for (int s = 0; s < MOVED_SLOTS; s++)
@@ -714,6 +732,8 @@ void nntc_hip_device_destroy(nntc_hip_ctx* h)
         (void)hipStreamSynchronize(c.stream);
     if (c.host_scalars)
         (void)hipHostFree(c.host_scalars);
+    if (c.host_stage)
+        (void)hipHostFree(c.host_stage);
     if (c.ev0)
         (void)hipEventDestroy(c.ev0);
     if (c.ev1)

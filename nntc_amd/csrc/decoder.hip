@@ -411,13 +411,12 @@ void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& ac
     }
     HIP_OK(hipGetLastError());
     double* d_omega = c.ls_sum.p + a.entries;
-    HIP_OK(hipMemcpyAsync(d_omega, omega, sizeof(double) * c.planes.n, hipMemcpyHostToDevice, c.stream));
+    upload_staged(c, d_omega, omega, (size_t)c.planes.n);
     hipLaunchKernelGGL(k_gram_reduce, dim3(a.entries), dim3(256), 0, c.stream, (const double*)a.partial,
                        (const double*)d_omega, a.tiles, a.entries, c.ls_sum.p);
     HIP_OK(hipGetLastError());
-    acc.resize((size_t)a.entries);
-    HIP_OK(hipMemcpyAsync(acc.data(), c.ls_sum.p, sizeof(double) * a.entries, hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const double* h = read_back(c, (const double*)c.ls_sum.p, (size_t)a.entries);
+    acc.assign(h, h + a.entries);
 }
 
 }   // namespace
@@ -455,13 +454,12 @@ void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
     HIP_OK(hipGetLastError());
     // omega travels through the scalar workspace's tail: nplanes doubles uploaded with the launch.
     double* d_omega = c.ls_sum.p + a.entries;
-    HIP_OK(hipMemcpyAsync(d_omega, omega, sizeof(double) * c.planes.n, hipMemcpyHostToDevice, c.stream));
+    upload_staged(c, d_omega, omega, (size_t)c.planes.n);
     hipLaunchKernelGGL(k_ls_reduce, dim3(a.entries), dim3(64), 0, c.stream, (const double*)a.partial,
                        (const double*)d_omega, c.planes.n, a.entries, c.ls_sum.p);
     HIP_OK(hipGetLastError());
-    acc.resize((size_t)a.entries);
-    HIP_OK(hipMemcpyAsync(acc.data(), c.ls_sum.p, sizeof(double) * a.entries, hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const double* h = read_back(c, (const double*)c.ls_sum.p, (size_t)a.entries);
+    acc.assign(h, h + a.entries);
 }
 
 }   // namespace nntc

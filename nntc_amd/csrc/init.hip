@@ -312,9 +312,7 @@ void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi)
         float* out = c.work_f.p;
         hipLaunchKernelGGL(k_minmax, dim3(blocks), dim3(256), 0, c.stream, v, n, ch, out);
         HIP_OK(hipGetLastError());
-        std::vector<float> h((size_t)blocks * 2 * MAX_CH);
-        HIP_OK(hipMemcpyAsync(h.data(), out, h.size() * sizeof(float), hipMemcpyDeviceToHost, c.stream));
-        HIP_OK(hipStreamSynchronize(c.stream));
+        const float* h = read_back(c, (const float*)out, (size_t)blocks * 2 * MAX_CH);
         for (int b = 0; b < blocks; b++)
             for (int q = 0; q < ch; q++)
             {
@@ -338,18 +336,15 @@ void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi)
     unsigned int mask = 0;
     unsigned int* d_hist = c.work_u.p;
     unsigned int* d_prefix = c.work_u.p + MAX_CH * 2 * 256;
-    std::vector<unsigned int> hist((size_t)ch * 2 * 256);
     for (int shift = 24; shift >= 0; shift -= 8)
     {
         HIP_OK(hipMemsetAsync(d_hist, 0, sizeof(unsigned int) * MAX_CH * 2 * 256, c.stream));
-        HIP_OK(hipMemcpyAsync(d_prefix, prefix, sizeof(prefix), hipMemcpyHostToDevice, c.stream));
+        upload_staged(c, d_prefix, prefix, MAX_CH * 2);
         const unsigned blocks = (unsigned)std::min<long long>(1024, (n + 255) / 256);
         hipLaunchKernelGGL(k_radix_hist, dim3(blocks), dim3(256), 0, c.stream, v, n, ch, shift,
                            (const unsigned int*)d_prefix, mask, d_hist);
         HIP_OK(hipGetLastError());
-        HIP_OK(hipMemcpyAsync(hist.data(), d_hist, hist.size() * sizeof(unsigned int), hipMemcpyDeviceToHost,
-                              c.stream));
-        HIP_OK(hipStreamSynchronize(c.stream));
+        const unsigned int* hist = read_back(c, (const unsigned int*)d_hist, (size_t)ch * 2 * 256);
         for (int i = 0; i < ch * 2; i++)
         {
             unsigned long long cum = 0;
@@ -386,10 +381,7 @@ void cov_partials(Ctx& c, const float* data, long long texels, std::vector<doubl
     hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, entries), dim3(256), 0, c.stream, data, texels, c.nout,
                        c.cov_partial.p);
     HIP_OK(hipGetLastError());
-    std::vector<double> part((size_t)SITE_BLOCKS * entries);
-    HIP_OK(hipMemcpyAsync(part.data(), c.cov_partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost,
-                          c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const double* part = read_back(c, (const double*)c.cov_partial.p, (size_t)SITE_BLOCKS * entries);
     for (int e = 0; e < entries; e++)
         for (int b = 0; b < SITE_BLOCKS; b++)
             raw[(size_t)e] += weight * part[(size_t)b * entries + e];
@@ -407,12 +399,10 @@ double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, 
     double* out = c.cov_partial.p;
     hipLaunchKernelGGL(k_proj_peak, dim3(blocks), dim3(256), 0, c.stream, resid, texels, c.nout, dir, out);
     HIP_OK(hipGetLastError());
-    std::vector<double> h(blocks);
-    HIP_OK(hipMemcpyAsync(h.data(), out, blocks * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const double* h = read_back(c, (const double*)out, (size_t)blocks);
     double peak = 0.0;
-    for (double v : h)
-        peak = std::max(peak, v);
+    for (unsigned b = 0; b < blocks; b++)
+        peak = std::max(peak, h[b]);
     return peak;
 }
 

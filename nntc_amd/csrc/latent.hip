@@ -753,14 +753,14 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
             out_h[(slot * C * C + e) * sn] = v;
         }
     }
-    if (sub < C)
+    for (int ch = sub; ch < C; ch += G)
     {
-        // gacc[sub], picked by selects (a register array indexed by a lane's value would live in scratch).
+        // gacc[ch], picked by selects (a register array indexed by a lane's value would live in scratch).
         double gv = gacc[0];
 #pragma unroll
         for (int i = 1; i < C; i++)
-            gv = sub == i ? gacc[i] : gv;
-        L.grad[gtx * C + sub] = gv;
+            gv = ch == i ? gacc[i] : gv;
+        L.grad[gtx * C + ch] = gv;
     }
 }
 
@@ -869,13 +869,13 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
             out_h[(sl * C * C + e) * sn] = v;
         }
     }
-    if (sub < C)
+    for (int ch = sub; ch < C; ch += G)
     {
         double gv = gacc[0];
 #pragma unroll
         for (int i = 1; i < C; i++)
-            gv = sub == i ? gacc[i] : gv;
-        L.grad[(L.texels.start[p] + ti) * C + sub] = gv;
+            gv = ch == i ? gacc[i] : gv;
+        L.grad[(L.texels.start[p] + ti) * C + ch] = gv;
     }
 }
 
@@ -1954,6 +1954,18 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     HIP_OK(hipGetLastError());
 }
 
+namespace
+{
+
+// Every plane's conjugate gradients marked active.
+__global__ __launch_bounds__(64) void k_mark_active(double* scalars, int np)
+{
+    if ((int)threadIdx.x < np)
+        scalars[(size_t)threadIdx.x * SC_COUNT + SC_ACTIVE] = 1.0;
+}
+
+}   // namespace
+
 void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
 {
     LatentView L = make_view(c, level0);
@@ -1963,10 +1975,8 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     const int np = c.planes.n;
     const unsigned tb = blocks_for(n, 256);
     // Every plane starts active.
-    std::vector<double> one((size_t)np, 1.0);
-    for (int p = 0; p < np; p++)
-        HIP_OK(hipMemcpyAsync(c.scalars.p + (size_t)p * SC_COUNT + SC_ACTIVE, &one[(size_t)p], sizeof(double),
-                              hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(k_mark_active, dim3(1), dim3(64), 0, c.stream, c.scalars.p, np);
+    HIP_OK(hipGetLastError());
     // Two inner products per chunk, into the chunk partials: one wavefront per chunk, folded serially in index order
     // (the exact form) or by a wavefront tree (the fast form).
     auto dot = [&](const double* x0, const double* y0, const double* x1, const double* y1, double bytes) {

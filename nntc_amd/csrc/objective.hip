@@ -397,9 +397,7 @@ void objective_pass(Ctx& c, int k, double* sums_out)
             launch_objective_tiles(c.c0, c.c1, (unsigned)total, c.stream, a);
     }
     HIP_OK(hipGetLastError());
-    std::vector<double> part((size_t)total * 3);
-    HIP_OK(hipMemcpyAsync(part.data(), c.partial.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const double* part = read_back(c, (const double*)c.partial.p, (size_t)total * 3);
     // The chunk sums folded in chunk order, from zero (src/cpu/pool.h:314-324).
     for (int p = 0; p < c.planes.n; p++)
     {

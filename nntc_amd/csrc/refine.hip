@@ -868,9 +868,7 @@ void bc_refine_pass(Ctx& c, double* taken, double* improved)
         }
         HIP_OK(hipGetLastError());
     }
-    std::vector<double> t((size_t)c.nb_total);
-    HIP_OK(hipMemcpyAsync(t.data(), c.bc_taken.p, t.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const double* t = read_back(c, (const double*)c.bc_taken.p, (size_t)c.nb_total);
     const size_t chunk = 64;
     for (int p = 0; p < c.planes.n; p++)
     {
@@ -915,11 +913,9 @@ void bc_pack_blocks(Ctx& c, double& se, size_t& n)
     BcArgs a = bc_args(c);
     hipLaunchKernelGGL(k_bc_pack, dim3(blocks_for(c.nb_total, 256)), dim3(256), 0, c.stream, a, c.nb_total);
     HIP_OK(hipGetLastError());
-    std::vector<double> h((size_t)c.nb_total);
-    HIP_OK(hipMemcpyAsync(h.data(), c.bc_se.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
-    for (double v : h)
-        se += v;
+    const double* h = read_back(c, (const double*)c.bc_se.p, (size_t)c.nb_total);
+    for (long long b = 0; b < c.nb_total; b++)
+        se += h[b];
     for (int p = 0; p < c.planes.n; p++)
         n += (size_t)c.g(p).n0 * (size_t)c.c0;
 }
