@@ -308,9 +308,13 @@ void upload_decoder(Ctx& c, const nntc_model_state* st)
     {
         c.h_w.assign(st->w, st->w + nw);
         c.h_b.assign(st->b, st->b + c.nout);
-        HIP_OK(hipMemcpyAsync(c.weights.p, st->w, nw * sizeof(float), hipMemcpyHostToDevice, c.stream));
-        HIP_OK(hipMemcpyAsync(c.bias.p, st->b, c.nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
-        HIP_OK(hipStreamSynchronize(c.stream));
+        // Both through the pinned staging area, asynchronously: the launches that read them follow on the stream.
+        float* h = static_cast<float*>(c.stage((nw + c.nout) * sizeof(float), true));
+        std::memcpy(h, st->w, nw * sizeof(float));
+        std::memcpy(h + nw, st->b, c.nout * sizeof(float));
+        HIP_OK(hipMemcpyAsync(c.weights.p, h, nw * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(c.bias.p, h + nw, c.nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        c.stage_in_flight = true;
     }
 }
 
@@ -341,8 +345,7 @@ void upload_grids(Ctx& c, const nntc_model_state* st)
     std::memcpy(c.h_hi0, g + 4, sizeof(c.h_hi0));
     std::memcpy(c.h_lo1, g + 8, sizeof(c.h_lo1));
     std::memcpy(c.h_hi1, g + 12, sizeof(c.h_hi1));
-    HIP_OK(hipMemcpyAsync(c.grid.p, g, sizeof(g), hipMemcpyHostToDevice, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    upload_staged(c, c.grid.p, g, sizeof(g) / sizeof(g[0]));
 }
 
 // A plane-sized download (or, with n smaller, its head).
@@ -658,6 +661,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.resid.alloc((size_t)o0 * nout);
         c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout));
         c.counters.alloc((size_t)md->nplanes * MOVED_SLOTS * MOVED_STRIDE);
+        c.cg_partial.alloc(2 * ((size_t)std::max(o0, o1) / 256 + MAX_PLANES + 1));
         c.bc_ep.alloc((size_t)ob * c0 * 2);
         c.bc_sel.alloc((size_t)ob * c0 * 16);
         c.bc_blocks.alloc((size_t)ob * 32);

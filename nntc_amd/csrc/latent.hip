@@ -1751,6 +1751,210 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
     return moved;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// The fast form's conjugate gradients: the inner products inside the vector launches
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// The mat-vec and the update each produce the iteration's inner products as workgroup partials over chunks of
+// FAST_CG_CHUNK texels of one plane, and a one-workgroup-per-plane fold derives the scalars the next launch reads:
+// alpha after the mat-vec; |r|^2, r.z, beta, the residual and, at a probe iteration, the stopping bar after the update.
+// The two inner-product launches are gone, and with them two passes over the vectors. (The folds stay launches of
+// their own: a plane's last workgroup folding in place would need an agent-scope release and acquire per workgroup,
+// which on a chip of eight L2 caches writes back and invalidates them.)
+
+constexpr int FAST_CG_CHUNK = 256;
+
+struct FastCg
+{
+    LatentView L;
+    Ranges chunks;            // per plane: its chunks of FAST_CG_CHUNK texels
+    double* partial;          // two per chunk
+    int probe;                // the fold after the update applies the stopping bar
+};
+
+// The workgroup's chunk: its plane and its texels [t0, t1) (chain indices).
+__device__ __forceinline__ int fast_chunk(const FastCg& a, long long& t0, long long& t1)
+{
+    const int p = range_plane(a.chunks, blockIdx.x);
+    const long long texels = a.L.texels.start[p + 1] - a.L.texels.start[p];
+    const long long b = ((long long)blockIdx.x - a.chunks.start[p]) * FAST_CG_CHUNK;
+    t0 = a.L.texels.start[p] + b;
+    t1 = a.L.texels.start[p] + (b + FAST_CG_CHUNK < texels ? b + FAST_CG_CHUNK : texels);
+    return p;
+}
+
+// The sum over the 256-lane workgroup: each wave's tree, then the four waves in order (every lane gets the total).
+__device__ __forceinline__ double block_sum256(double v, double* s4)
+{
+    v = tree_lanes(v);
+    __syncthreads();
+    if (threadIdx.x % 64 == 0)
+        s4[threadIdx.x / 64] = v;
+    __syncthreads();
+    return ((s4[0] + s4[1]) + s4[2]) + s4[3];
+}
+
+// The plane's partials (component k) folded by one workgroup: lane l adds chunks l, l + 256, ... in order, then the
+// workgroup's sum - a fixed order, the same every run.
+__device__ __forceinline__ double fold_partials(const FastCg& a, int p, int k, double* s4)
+{
+    double v = 0.0;
+    for (long long ch = a.chunks.start[p] + threadIdx.x; ch < a.chunks.start[p + 1]; ch += 256)
+        v += a.partial[(size_t)ch * 2 + k];
+    return block_sum256(v, s4);
+}
+
+// Ap for the chunk's texels (the mat-vec of k_matvec, its neighbour loads pipelined as in the quantised sweep), and the
+// chunk's p.Ap.
+template <int C>
+__global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
+{
+    __shared__ double s4[4];
+    long long t0, t1;
+    const int p = fast_chunk(a, t0, t1);
+    const LatentView& L = a.L;
+    double* sc = L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;   // the whole workgroup: one plane
+    const long long gt = t0 + threadIdx.x;
+    double pap = 0.0;
+    if (gt < t1)
+    {
+        const double lambda = sc[SC_LAMBDA];
+        const long long t = gt - L.texels.start[p];
+        const int w = L.pw[p], h = L.ph[p];
+        const int tx = (int)(t % w), ty = (int)(t / w);
+        const long long base = L.texels.start[p];
+        const long long sn = L.sm.sn;
+        const double* S = L.stencil;
+        const long long pos = stencil_pos(L.sm, p, w, tx, ty);
+        long long cbase[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+        {
+            cbase[k] = L.sm.cpos[p][k];
+            asm volatile("" : "+v"(cbase[k]));
+        }
+        double xt[C], acc[C];
+#pragma unroll
+        for (int j = 0; j < C; j++)
+            xt[j] = L.p[gt * C + j];
+#pragma unroll
+        for (int i = 0; i < C; i++)
+        {
+            double sacc = lambda * xt[i];
+#pragma unroll
+            for (int j = 0; j < C; j++)
+                sacc += S[(i * C + j) * sn + pos] * xt[j];
+            acc[i] = sacc;
+        }
+        constexpr int D = C <= 2 ? 4 : 2;
+        SweepNeighbour<C> q[8];
+#pragma unroll
+        for (int g = 0; g < 8 + D; g++)
+        {
+            if (g < 8)
+                q[g].load(L, S, sn, L.p, cbase, g, tx, ty, w, h, base, gt, pos);
+            if (g >= D)
+            {
+                q[g - D].add_to(acc);
+#pragma unroll
+                for (int i = 0; i < C; i++)
+                    asm volatile("" : "+v"(acc[i]));
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < C; i++)
+        {
+            L.ap[gt * C + i] = acc[i];
+            pap += xt[i] * acc[i];
+        }
+    }
+    pap = block_sum256(pap, s4);
+    if (threadIdx.x == 0)
+        a.partial[(size_t)blockIdx.x * 2] = pap;
+}
+
+// Per plane (one workgroup each): pAp, then alpha = rz / pAp (zero on non-positive curvature).
+__global__ __launch_bounds__(256) void k_fold_alpha_fast(FastCg a)
+{
+    __shared__ double s4[4];
+    const int p = blockIdx.x;
+    double* sc = a.L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;
+    const double pap = fold_partials(a, p, 0, s4);
+    if (threadIdx.x == 0)
+    {
+        sc[SC_PAP] = pap;
+        sc[SC_ALPHA] = pap > 0.0 ? sc[SC_RZ] / pap : 0.0;
+    }
+}
+
+// x += alpha p, r -= alpha Ap, z = Minv r for the chunk's texels, and the chunk's |r|^2 and r.z.
+__global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
+{
+    __shared__ double s4[4];
+    long long t0, t1;
+    const int p = fast_chunk(a, t0, t1);
+    const LatentView& L = a.L;
+    double* sc = L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;
+    const long long gt = t0 + threadIdx.x;
+    double rr = 0.0, rz = 0.0;
+    if (gt < t1)
+    {
+        const double alpha = sc[SC_ALPHA];
+        const int c = L.c;
+        for (int i = 0; i < c; i++)
+        {
+            const long long e = gt * c + i;
+            L.x[e] += alpha * L.p[e];
+            L.r[e] += -alpha * L.ap[e];
+        }
+        apply_precond(L.precond, L.r, c, L.z, gt);
+        for (int i = 0; i < c; i++)
+        {
+            const double ri = L.r[gt * c + i];
+            rr += ri * ri;
+            rz += ri * L.z[gt * c + i];
+        }
+    }
+    rr = block_sum256(rr, s4);
+    rz = block_sum256(rz, s4);
+    if (threadIdx.x == 0)
+    {
+        a.partial[(size_t)blockIdx.x * 2] = rr;
+        a.partial[(size_t)blockIdx.x * 2 + 1] = rz;
+    }
+}
+
+// Per plane (one workgroup each): |r|^2, r.z, beta, the relative residual; at a probe iteration a plane under the bar
+// (or NaN) stops.
+__global__ __launch_bounds__(256) void k_fold_beta_fast(FastCg a)
+{
+    __shared__ double s4[4];
+    const int p = blockIdx.x;
+    double* sc = a.L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;
+    const double rr_t = fold_partials(a, p, 0, s4);
+    const double rz_t = fold_partials(a, p, 1, s4);
+    if (threadIdx.x == 0)
+    {
+        sc[SC_RR] = rr_t;
+        sc[SC_RZ_NEW] = rz_t;
+        const double rz_old = sc[SC_RZ];
+        sc[SC_BETA] = rz_old != 0.0 ? rz_t / rz_old : 0.0;
+        sc[SC_RZ] = rz_t;
+        const double rhs2 = sc[SC_RHS2];
+        sc[SC_RESID] = rhs2 > 0.0 ? sqrt((rr_t > 0.0 ? rr_t : 0.0) / rhs2) : 0.0;
+        if (a.probe && !(sc[SC_RESID] >= CG_TOLERANCE))
+            sc[SC_ACTIVE] = 0.0;
+    }
+}
+
 }   // namespace
 
 // ---------------------------------------------------------------------------------------------------------------------
@@ -1998,34 +2202,81 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     const double cc = L.c, nv = (double)n * cc;
     const double b_matvec = (double)n * 40.0 * cc * cc + 16.0 * nv;
     const double b_update = (double)n * 4.0 * cc * cc + 56.0 * nv;
+    // The fast form: the inner products inside the mat-vec and the update, folded per plane by a workgroup each.
+    FastCg fc;
+    fc.L = L;
+    fc.chunks.n = np;
+    {
+        long long acc = 0;
+        for (int p = 0; p < np; p++)
+        {
+            fc.chunks.start[p] = acc;
+            acc += (L.texels.start[p + 1] - L.texels.start[p] + FAST_CG_CHUNK - 1) / FAST_CG_CHUNK;
+        }
+        fc.chunks.start[np] = acc;
+        if ((size_t)acc * 2 > c.cg_partial.n)
+            throw Refused { "latent: CG partials exceed the workspace" };
+    }
+    fc.partial = c.cg_partial.p;
+    const unsigned fgrid = (unsigned)fc.chunks.start[np];
     for (int it = 0; it < CG_MAX_ITERATIONS; it++)
     {
+        if (!ch.exact)
         {
-            KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", b_matvec, (double)n * (2.0 * 9.0 * cc * cc));
-            switch (L.c)
+            fc.probe = (it + 1) % CG_PROBE == 0;
             {
-            case 1: hipLaunchKernelGGL(k_matvec<1>, dim3(tb), dim3(256), 0, c.stream, L); break;
-            case 2: hipLaunchKernelGGL(k_matvec<2>, dim3(tb), dim3(256), 0, c.stream, L); break;
-            case 3: hipLaunchKernelGGL(k_matvec<3>, dim3(tb), dim3(256), 0, c.stream, L); break;
-            default: hipLaunchKernelGGL(k_matvec<4>, dim3(tb), dim3(256), 0, c.stream, L); break;
+                KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", b_matvec + 16.0 * nv,
+                               (double)n * (2.0 * 9.0 * cc * cc));
+                switch (L.c)
+                {
+                case 1: hipLaunchKernelGGL(k_matvec_fast<1>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+                case 2: hipLaunchKernelGGL(k_matvec_fast<2>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+                case 3: hipLaunchKernelGGL(k_matvec_fast<3>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+                default: hipLaunchKernelGGL(k_matvec_fast<4>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+                }
             }
+            hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
+            {
+                KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
+                hipLaunchKernelGGL(k_cg_update_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
+            }
+            hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
+            {
+                KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);
+                hipLaunchKernelGGL(k_xpby, dim3(tb), dim3(256), 0, c.stream, L);
+            }
+            HIP_OK(hipGetLastError());
         }
-        dot(L.p, L.ap, nullptr, nullptr, 16.0 * nv);
-        hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(64), 0, c.stream, ch);
+        else
         {
-            KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
-            hipLaunchKernelGGL(k_cg_update, dim3(tb), dim3(256), 0, c.stream, L);
+            {
+                KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", b_matvec, (double)n * (2.0 * 9.0 * cc * cc));
+                switch (L.c)
+                {
+                case 1: hipLaunchKernelGGL(k_matvec<1>, dim3(tb), dim3(256), 0, c.stream, L); break;
+                case 2: hipLaunchKernelGGL(k_matvec<2>, dim3(tb), dim3(256), 0, c.stream, L); break;
+                case 3: hipLaunchKernelGGL(k_matvec<3>, dim3(tb), dim3(256), 0, c.stream, L); break;
+                default: hipLaunchKernelGGL(k_matvec<4>, dim3(tb), dim3(256), 0, c.stream, L); break;
+                }
+            }
+            dot(L.p, L.ap, nullptr, nullptr, 16.0 * nv);
+            hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(64), 0, c.stream, ch);
+            {
+                KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
+                hipLaunchKernelGGL(k_cg_update, dim3(tb), dim3(256), 0, c.stream, L);
+            }
+            dot(L.r, L.r, L.r, L.z, 16.0 * nv);
+            hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(64), 0, c.stream, ch, 0);
+            {
+                KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);
+                hipLaunchKernelGGL(k_xpby, dim3(tb), dim3(256), 0, c.stream, L);
+            }
+            HIP_OK(hipGetLastError());
         }
-        dot(L.r, L.r, L.r, L.z, 16.0 * nv);
-        hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(64), 0, c.stream, ch, 0);
-        {
-            KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);
-            hipLaunchKernelGGL(k_xpby, dim3(tb), dim3(256), 0, c.stream, L);
-        }
-        HIP_OK(hipGetLastError());
         if ((it + 1) % CG_PROBE != 0)
             continue;
-        hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, c.stream, L, np);
+        if (ch.exact)
+            hipLaunchKernelGGL(k_probe, dim3(1), dim3(64), 0, c.stream, L, np);
         HIP_OK(hipGetLastError());
         HIP_OK(hipMemcpyAsync(c.host_scalars, c.scalars.p, sizeof(double) * SC_COUNT * np, hipMemcpyDeviceToHost,
                               c.stream));
