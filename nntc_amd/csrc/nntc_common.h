@@ -402,6 +402,17 @@ NNTC_HD double fold_serial16(const double* t, int n, double acc)
     return acc;
 }
 
+// The workgroup's place in an index range split into eight contiguous parts, one per XCD. The dispatcher deals
+// workgroups to the chip's eight XCDs round-robin (workgroup b to XCD b % 8), and each XCD has its own L2: with the
+// identity mapping, neighbouring workgroups - which read the same cache lines of their neighbours' data - land on
+// eight different L2s, and each fetches the lines from HBM. XCD x takes the x-th part (its workgroups b = x + 8 i, in
+// order of i). A bijection of [0, n) for every n.
+__device__ __forceinline__ unsigned int xcd_block(unsigned int b, unsigned int n)
+{
+    const unsigned int q = n / 8, r = n % 8, x = b % 8, i = b / 8;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // The plane whose range holds global index g (ranges are contiguous and increasing).
 NNTC_HD int range_plane(const Ranges& r, long long g)
 {

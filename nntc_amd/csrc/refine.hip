@@ -240,17 +240,6 @@ struct BlockLane
         return *e * keep(k);
     }
 
-    // The lane's rows of a2's column channel j: E[k][i] = a2(i, neighbour k, j).
-    __device__ __forceinline__ void column(int j, double (&E)[9][C0]) const
-    {
-#pragma unroll
-        for (int dy = -1; dy <= 1; dy++)
-#pragma unroll
-            for (int dx = -1; dx <= 1; dx++)
-#pragma unroll
-                for (int i = 0; i < C0; i++)
-                    E[(dy + 1) * 3 + dx + 1][i] = a2(i, dx, dy, j);
-    }
 };
 
 
@@ -342,7 +331,7 @@ __device__ __forceinline__ float level0_value(float lo, float hi, int r0, int r1
 
 // Selector steps T..15 of one channel of one Gauss-Seidel pass: texel T's exact 8-way argmin, taken from texel T's
 // linear term and diagonal broadcast over the row, and on a strict decrease the move added into the rows of T's
-// neighbours. Lane u's entry (u, ii; T, c) sits in its LDS column at e[k * 64 * C0 + ii] for T = u + offset k: an
+// neighbours. Lane u's entry (u, ii; T, c) sits in its LDS column at e[k * C0 * 64 * C0 + ii] for T = u + offset k: an
 // offset that differs from lane to lane, which in registers would take a select per slot.
 template <int C0, int T>
 __device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, const double (&a1)[C0], double (&uv)[C0],
@@ -380,7 +369,7 @@ __device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, co
         const int dx = tx - L.ux, dy = ty - L.uy;
         if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
         {
-            const double* ek = e + ((dy + 1) * 3 + dx + 1) * 64 * C0;
+            const double* ek = e + ((dy + 1) * 3 + dx + 1) * C0 * 64 * C0;
 #pragma unroll
             for (int ii = 0; ii < C0; ii++)
                 uv[ii] += ek[ii] * move;
@@ -409,10 +398,16 @@ __device__ __forceinline__ void uv_entries(BlockLane<C0>& L, double (&e)[C0][C0]
 }
 
 // Neighbour K's terms of the lane's rows of uv, channel by channel, with the neighbour's correction read from its lane;
-// uv is made opaque after them, so they are added before the loads that follow are issued.
+// uv is made opaque after them, so they are added before the loads that follow are issued. The entries go to the
+// lane's LDS column, col[((K C0 + j) 64) C0 + i], where every later step reads them.
 template <int C0, int K>
-__device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (&dv)[C0], double (&uv)[C0])
+__device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (&dv)[C0], double (&uv)[C0], double* col)
 {
+#pragma unroll
+    for (int j = 0; j < C0; j++)
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+            col[((K * C0 + j) * 64) * C0 + i] = e[j][i];
     constexpr int dx = K % 3 - 1, dy = K / 3 - 1;
     double dn[C0];
 #pragma unroll
@@ -432,17 +427,17 @@ __device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (
 // entries loaded while neighbour K's terms are added: two neighbours' entries in flight, not all nine.
 template <int C0, int K>
 __device__ __forceinline__ void uv_pipeline(BlockLane<C0>& L, const double (&dv)[C0], double (&uv)[C0],
-                                            const double (&cur)[C0][C0])
+                                            const double (&cur)[C0][C0], double* col)
 {
     if constexpr (K < 8)
     {
         double nxt[C0][C0];
         uv_entries<C0, K + 1>(L, nxt);
-        uv_add<C0, K>(cur, dv, uv);
-        uv_pipeline<C0, K + 1>(L, dv, uv, nxt);
+        uv_add<C0, K>(cur, dv, uv, col);
+        uv_pipeline<C0, K + 1>(L, dv, uv, nxt, col);
     }
     else
-        uv_add<C0, K>(cur, dv, uv);
+        uv_add<C0, K>(cur, dv, uv, col);
 }
 
 template <int C0>
@@ -450,11 +445,15 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
 {
     // Four blocks per wavefront, one per sixteen-lane row; a row past the colour's last block leaves as a whole (the
     // rows of a wavefront never read each other's lanes).
-    const long long gi = (long long)blockIdx.x * 4 + (threadIdx.x >> 4);
+    const long long gi = (long long)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 4);
     if (gi >= colour.start[colour.n])
         return;
-    // The selector steps' entries, one column per lane ([slot][lane][channel]); a lane reads only its own column.
-    __shared__ double s_e[9 * 64 * C0];
+    // The lane's entries of a2, loaded from the stencil once (by the start of uv) and read from here by every later
+    // step: one column per lane, entry (u, i; neighbour k, j) at [((k C0 + j) 64 + lane) C0 + i]. A lane reads only
+    // its own column, so no barrier is needed. Reading the stencil again in each step instead would miss the L2 (the
+    // rows a CU's wavefronts hold exceed it) and fetch the entries from HBM once per step.
+    __shared__ double s_E[9 * C0 * 64 * C0];
+    double* const col = s_E + threadIdx.x * C0;
 
     const int p = range_plane(colour, gi);
     const PlaneGeom& g = a.planes.p[p];
@@ -567,7 +566,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     {
         double e0[C0][C0];
         uv_entries<C0, 0>(L, e0);
-        uv_pipeline<C0, 0>(L, dv, uv, e0);
+        uv_pipeline<C0, 0>(L, dv, uv, e0, col);
     }
 
     const long long gb = g.offb + (long long)gy * bxn + gx;
@@ -598,7 +597,11 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         // read these alone, so a term for a neighbour outside the block or the plane is a zero, which leaves the sum
         // (from +0.0, never -0.0) unchanged, as leaving it out would.
         double E[9][C0];
-        L.column(c, E);
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                E[k][i] = col[((k * C0 + c) * 64) * C0 + i];
         double bp_, bq, pap, paq, qaq;
         {
             // The texel's weight on the high endpoint, and its neighbours' from their lanes.
@@ -696,19 +699,10 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             // Addresses are formed in the channel that uses them: hoisted out of the pass loop, every channel's and
             // every step's would be live at once.
             L.opaque();
-            int lane_off = (int)threadIdx.x * C0;
+            int lane_off = (int)threadIdx.x * C0 + c * 64 * C0;
             asm volatile("" : "+v"(lane_off));
-            double* const ec = s_e + lane_off;
-            // The lane's entries (u, ii; u + (dx, dy), c) that the channel's steps read, issued together: one memory
-            // latency per channel rather than one per step.
-            double E[9][C0];
-            L.column(c, E);
-#pragma unroll
-            for (int k = 0; k < 9; k++)
-#pragma unroll
-                for (int ii = 0; ii < C0; ii++)
-                    ec[k * 64 * C0 + ii] = E[k][ii];
-            const double diag = E[4][c];
+            const double* const ec = s_E + lane_off;   // the lane's column of a2's column channel c
+            const double diag = ec[4 * C0 * 64 * C0 + c];
             selector_steps<C0, 0>(L, c, a1, uv, ec, diag, pal[c], a.lo0[c], a.hi0[c], ep0[c], ep1[c], sel[c], taken);
         }
 
