@@ -197,7 +197,7 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
     __shared__ __align__(16) unsigned short s_hi[DP * 256], s_lo[DP * 256];   // [feature][site of this round]
     __shared__ double s_g[DP * DP];
 
-    const int tile = blockIdx.x;
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
     const int p = range_plane(a.tiles, tile);
     const int nout = a.nout, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const TileGeom tg = stage_tile<C0, C1>(a.planes.p[p], tile - (int)a.tiles.start[p], nout, a.src, a.v0, a.v1,

@@ -206,7 +206,7 @@ __device__ __forceinline__ void monomials(const float* s, double* mono)
 template <bool LEVEL0, int C0, int C1, bool DIRECT>
 __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 {
-    const long long gs = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gs = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (gs >= a.sites.start[a.sites.n])
         return;
     const int p = range_plane(a.sites, gs);
@@ -665,7 +665,7 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
 {
     constexpr int NM = 1 + CS + CS * (CS + 1) / 2;
     const LatentView& L = a.L;
-    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gt = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const long long gi = gt / G;
     const int sub = (int)(gt % G);
     const long long total = L.texels.start[L.texels.n];
@@ -786,7 +786,7 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
 {
     constexpr int NH = C * (C + 1) / 2;
     const LatentView& L = a.L;
-    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gt = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const long long gi = gt / G;
     const int sub = (int)(gt % G);
     const long long total = L.texels.start[L.texels.n];
@@ -1546,7 +1546,7 @@ template <int C>
 __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour, int colour_index,
                                                      unsigned int* changed)
 {
-    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gi = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool live = gi < colour.start[colour.n];
     const int p = range_plane(colour, live ? gi : colour.start[colour.n] - 1);
     const unsigned int moved = live ? sweep_texel<C>(L, colour, colour_index, gi, p) : 0u;
@@ -1554,7 +1554,7 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
     // a lane of another plane (a workgroup across a plane boundary) adds its own. The workgroups spread their adds
     // over MOVED_SLOTS counters per plane on separate cache lines, so they do not queue on one address.
     __shared__ unsigned int s_wave[4];
-    const long long g0 = (long long)blockIdx.x * blockDim.x;
+    const long long g0 = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
     const int pw = range_plane(colour, g0);
     unsigned int same = p == pw ? moved : 0u;
     for (int off = 32; off > 0; off >>= 1)
@@ -1775,9 +1775,9 @@ struct FastCg
 // The workgroup's chunk: its plane and its texels [t0, t1) (chain indices).
 __device__ __forceinline__ int fast_chunk(const FastCg& a, long long& t0, long long& t1)
 {
-    const int p = range_plane(a.chunks, blockIdx.x);
+    const int p = range_plane(a.chunks, xcd_block(blockIdx.x, gridDim.x));
     const long long texels = a.L.texels.start[p + 1] - a.L.texels.start[p];
-    const long long b = ((long long)blockIdx.x - a.chunks.start[p]) * FAST_CG_CHUNK;
+    const long long b = ((long long)xcd_block(blockIdx.x, gridDim.x) - a.chunks.start[p]) * FAST_CG_CHUNK;
     t0 = a.L.texels.start[p] + b;
     t1 = a.L.texels.start[p] + (b + FAST_CG_CHUNK < texels ? b + FAST_CG_CHUNK : texels);
     return p;
@@ -1872,7 +1872,7 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
     }
     pap = block_sum256(pap, s4);
     if (threadIdx.x == 0)
-        a.partial[(size_t)blockIdx.x * 2] = pap;
+        a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2] = pap;
 }
 
 // Per plane (one workgroup each): pAp, then alpha = rz / pAp (zero on non-positive curvature).
@@ -1925,8 +1925,8 @@ __global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
     rz = block_sum256(rz, s4);
     if (threadIdx.x == 0)
     {
-        a.partial[(size_t)blockIdx.x * 2] = rr;
-        a.partial[(size_t)blockIdx.x * 2 + 1] = rz;
+        a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2] = rr;
+        a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2 + 1] = rz;
     }
 }
 

@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
     __shared__ float s_b[MAX_NOUT], s_cw[MAX_NOUT];
     __shared__ double part[4][3];
 
-    const int tile = blockIdx.x;
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
     const int p = range_plane(a.chunks, tile);
     const int nout = a.nout, tid = threadIdx.x;
     const TileGeom tg = stage_tile<C0, C1>(a.planes.p[p], tile - (int)a.chunks.start[p], nout, a.src, a.v0, a.v1,
