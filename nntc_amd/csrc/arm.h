@@ -161,7 +161,7 @@ struct Ctx
     DevBuf<uint8_t> rgb8;                  // one plane's decode
     DevBuf<float> site_f;                  // per site of the chain: the other latent's sample (assembly pass 1)
     DevBuf<double> site_d;                 // per site: the residual pulled back through the decoder
-    DevBuf<unsigned long long> tex_sse;    // per texture: the base decode's squared byte error
+    DevBuf<unsigned long long> tex_sse;    // per texture, SSE_SLOTS counters: the base decode's squared byte error
 
     // Pinned host staging for the small per-call readbacks.
     double* host_scalars = nullptr;
@@ -232,6 +232,8 @@ inline StencilMap stencil_map(const Ctx& c, bool level0)
 // The quantised sweeps' moved-index counters: MOVED_SLOTS per plane, MOVED_STRIDE unsigned ints (one 128-byte line)
 // apart, summed on the host after the sweeps.
 constexpr int MOVED_SLOTS = 64, MOVED_STRIDE = 32;
+// The PSNR sums' counters: SSE_SLOTS per texture, SSE_STRIDE 64-bit counters (one 128-byte line) apart.
+constexpr int SSE_SLOTS = 64, SSE_STRIDE = 16;
 
 // The fast-form objective's tile of pixels (one workgroup each) and the number of tiles of a plane.
 constexpr int OBJ_TW = 32, OBJ_TH = 8;

@@ -671,7 +671,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.work_f.alloc(256 * 2 * MAX_CH);
         c.work_u.alloc(MAX_CH * 2 * 256 + 64);
         c.rgb8.alloc((size_t)c.planes.p[0].n0 * nout);
-        c.tex_sse.alloc(NNTC_MAX_NOUT / 3);
+        c.tex_sse.alloc((size_t)(NNTC_MAX_NOUT / 3) * SSE_SLOTS * SSE_STRIDE);
         // The assembly's site buffer: every site of the chain at the largest K the encoder takes (4).
         c.site_f.alloc((size_t)o0 * 5 * std::max({ c0, c1, c0 * (c0 + 1) / 2 }));
         c.site_d.alloc((size_t)o0 * 5 * std::max(c0, c1));

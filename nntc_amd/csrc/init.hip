@@ -171,36 +171,47 @@ __global__ __launch_bounds__(256) void k_radix_hist(const float* v, long long n,
             atomicAdd(&hist[i], h[i]);
 }
 
-// k_cov_partial: entry e of the nout + nout^2 sums over `texels` rows of `nout` floats, lane x of workgroup b taking
-// rows b 256 + x, then every SITE_BLOCKS 256 further, and the 256 lanes halved into one.
+// k_cov_partial: entries e of the nout + nout^2 sums over `texels` rows of `nout` floats, lane x of workgroup b taking
+// rows b 256 + x, then every SITE_BLOCKS 256 further, and the 256 lanes halved into one. Workgroup (b, g) takes one
+// row of the table - the nout means (g = 0) or the nout products of column g - 1 - so a lane reads its rows once for
+// nout sums instead of once per sum; each sum keeps its own order.
 __global__ __launch_bounds__(256) void k_cov_partial(const float* data, long long texels, int nout, double* partial)
 {
-    __shared__ double lane[256];
-    const int b = blockIdx.x, e = blockIdx.y, x = threadIdx.x;
+    __shared__ double lane[MAX_NOUT][256];
+    const int b = blockIdx.x, g = blockIdx.y, x = threadIdx.x;
     const int entries = nout + nout * nout;
-    const int r = e < nout ? e : (e - nout) / nout;
-    const int s = e < nout ? -1 : (e - nout) % nout;
+    const int r = g - 1;   // the products' first column; -1 for the means
     const long long stride = (long long)SITE_BLOCKS * 256;
-    double acc = 0.0;
+    double acc[MAX_NOUT];
+#pragma unroll
+    for (int s = 0; s < MAX_NOUT; s++)
+        acc[s] = 0.0;
     for (long long row = (long long)b * 256; row < texels; row += stride)
     {
         const long long t = row + x;
         if (t < texels)
         {
-            const double a = (double)data[t * nout + r];
-            acc += s < 0 ? a : a * (double)data[t * nout + s];
+            const float* d = data + t * nout;
+            const double a = r < 0 ? 0.0 : (double)d[r];
+#pragma unroll
+            for (int s = 0; s < MAX_NOUT; s++)
+                if (s < nout)
+                    acc[s] += r < 0 ? (double)d[s] : a * (double)d[s];
         }
     }
-    lane[x] = acc;
+#pragma unroll
+    for (int s = 0; s < MAX_NOUT; s++)
+        lane[s][x] = acc[s];
     __syncthreads();
     for (int half = 128; half > 0; half >>= 1)
     {
         if (x < half)
-            lane[x] += lane[x + half];
+            for (int s = 0; s < nout; s++)
+                lane[s][x] += lane[s][x + half];
         __syncthreads();
     }
-    if (x == 0)
-        partial[(size_t)b * entries + e] = lane[0];
+    if (x < nout)
+        partial[(size_t)b * entries + (r < 0 ? x : nout + r * nout + x)] = lane[x][0];
 }
 
 struct ProjDir
@@ -378,7 +389,7 @@ void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi)
 void cov_partials(Ctx& c, const float* data, long long texels, std::vector<double>& raw, double weight)
 {
     const int entries = c.nout + c.nout * c.nout;
-    hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, entries), dim3(256), 0, c.stream, data, texels, c.nout,
+    hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, c.nout + 1), dim3(256), 0, c.stream, data, texels, c.nout,
                        c.cov_partial.p);
     HIP_OK(hipGetLastError());
     const double* part = read_back(c, (const double*)c.cov_partial.p, (size_t)SITE_BLOCKS * entries);

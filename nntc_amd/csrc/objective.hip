@@ -307,28 +307,50 @@ __global__ __launch_bounds__(256) void k_decode(DecodeArgs a)
 // (image_psnr_texture, src/image.h:387-407): the source value saturated and rounded half away from zero to a byte, the
 // difference squared, summed as integers - exact in any order, so the sums are the reference's double sums' values.
 // One atomic per wavefront and texture.
+// Integer sums, so any order gives the same total: each lane strides over the pixels, the workgroup's sums meet in
+// shuffles and LDS, and a workgroup adds them into one of SSE_SLOTS counters per texture on separate cache lines (one
+// counter per texture took tens of thousands of same-address atomics).
 __global__ __launch_bounds__(256) void k_texture_sse(const uint8_t* rgb8, const float* src, int w0, int nout,
                                                      int width, int height, unsigned long long* sse)
 {
-    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const bool live = idx < (long long)width * height;
-    const int x = live ? (int)(idx % width) : 0, y = live ? (int)(idx / width) : 0;
-    const size_t base = ((size_t)y * w0 + x) * nout;
-    for (int t = 0; t < nout / 3; t++)
+    __shared__ unsigned long long s_part[4][NNTC_MAX_NOUT / 3];
+    const int textures = nout / 3;
+    const long long n = (long long)width * height;
+    unsigned long long s[NNTC_MAX_NOUT / 3];
+#pragma unroll
+    for (int t = 0; t < NNTC_MAX_NOUT / 3; t++)
+        s[t] = 0;
+    for (long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x; idx < n;
+         idx += (long long)gridDim.x * blockDim.x)
     {
-        unsigned long long s = 0;
-        if (live)
-            for (int ch = 0; ch < 3; ch++)
-            {
-                const float f = src[base + 3 * t + ch];
-                const int a = (int)lroundf(fminf(1.0f, fmaxf(0.0f, f)) * 255.0f);
-                const int d = a - (int)rgb8[base + 3 * t + ch];
-                s += (unsigned long long)(d * d);
-            }
+        const int x = (int)(idx % width), y = (int)(idx / width);
+        const size_t base = ((size_t)y * w0 + x) * nout;
+#pragma unroll
+        for (int t = 0; t < NNTC_MAX_NOUT / 3; t++)
+            if (t < textures)
+                for (int ch = 0; ch < 3; ch++)
+                {
+                    const float f = src[base + 3 * t + ch];
+                    const int a = (int)lroundf(fminf(1.0f, fmaxf(0.0f, f)) * 255.0f);
+                    const int d = a - (int)rgb8[base + 3 * t + ch];
+                    s[t] += (unsigned long long)(d * d);
+                }
+    }
+#pragma unroll
+    for (int t = 0; t < NNTC_MAX_NOUT / 3; t++)
+    {
         for (int off = 32; off > 0; off >>= 1)
-            s += __shfl_xor(s, off);
-        if (threadIdx.x % 64 == 0 && s)
-            atomicAdd(sse + t, s);
+            s[t] += __shfl_xor(s[t], off);
+        if (threadIdx.x % 64 == 0)
+            s_part[threadIdx.x / 64][t] = s[t];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < textures)
+    {
+        const int t = threadIdx.x;
+        const unsigned long long sum = s_part[0][t] + s_part[1][t] + s_part[2][t] + s_part[3][t];
+        if (sum)
+            atomicAdd(sse + ((size_t)t * SSE_SLOTS + blockIdx.x % SSE_SLOTS) * SSE_STRIDE, sum);
     }
 }
 
@@ -339,13 +361,20 @@ void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out
     const PlaneGeom& g = c.g(0);
     decode_pass(c, 0, c.rgb8.p);
     const int textures = c.nout / 3;
-    HIP_OK(hipMemsetAsync(c.tex_sse.p, 0, sizeof(unsigned long long) * textures, c.stream));
-    hipLaunchKernelGGL(k_texture_sse, dim3(blocks_for((long long)width * height, 256)), dim3(256), 0, c.stream,
-                       c.rgb8.p, c.src.p, g.w0, c.nout, width, height, c.tex_sse.p);
+    HIP_OK(hipMemsetAsync(c.tex_sse.p, 0, c.tex_sse.bytes(), c.stream));
+    const unsigned grid = (unsigned)std::min<long long>(1024, blocks_for((long long)width * height, 256));
+    hipLaunchKernelGGL(k_texture_sse, dim3(grid), dim3(256), 0, c.stream, c.rgb8.p, c.src.p, g.w0, c.nout, width,
+                       height, c.tex_sse.p);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(sse_out, c.tex_sse.p, sizeof(unsigned long long) * textures, hipMemcpyDeviceToHost,
-                          c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+    const unsigned long long* slots = read_back(c, (const unsigned long long*)c.tex_sse.p,
+                                                (size_t)textures * SSE_SLOTS * SSE_STRIDE);
+    for (int t = 0; t < textures; t++)
+    {
+        unsigned long long sum = 0;
+        for (int s = 0; s < SSE_SLOTS; s++)
+            sum += slots[((size_t)t * SSE_SLOTS + s) * SSE_STRIDE];
+        sse_out[t] = sum;
+    }
 }
 
 void objective_pass(Ctx& c, int k, double* sums_out)
