@@ -225,23 +225,18 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
     site_delta(a.k, j, px, py, dx, dy);
     const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
     float sbar[C0], cbar[C1], target[MAX_NOUT];
-    sample_bilinear_r<C1>(v1, w1, C1, tp, cbar);
+    sample_bilinear_v<C1>(v1, w1, C1, tp, cbar);
     if (j == 0)
     {
         const size_t pi = (size_t)py * w0 + px;
-#pragma unroll
-        for (int i = 0; i < C0; i++)
-            sbar[i] = v0[pi * C0 + i];
-#pragma unroll
-        for (int r = 0; r < MAX_NOUT; r++)
-            if (r < nout)
-                target[r] = src[pi * nout + r];
+        load_texel<C0>(v0 + pi * C0, C0, sbar);
+        load_texel<MAX_NOUT>(src + pi * nout, nout, target);
     }
     else
     {
         const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-        sample_bilinear_r<C0>(v0, w0, C0, t0, sbar);
-        sample_bilinear_r<MAX_NOUT>(src, w0, nout, t0, target);
+        sample_bilinear_v<C0>(v0, w0, C0, t0, sbar);
+        sample_bilinear_v<MAX_NOUT>(src, w0, nout, t0, target);
     }
     constexpr int C = LEVEL0 ? C0 : C1;   // the solved latent's channels
     constexpr int NH = C * (C + 1) / 2;

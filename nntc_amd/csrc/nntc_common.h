@@ -189,6 +189,62 @@ NNTC_HD void sample_bilinear_r(const float* plane, int w, int c, const Tap& t, f
             out[k] = w00 * a[k] + w10 * b[k] + w01 * e[k] + w11 * f[k];
 }
 
+// Channels [0, c) of one texel of a plane of c-channel rows into out[], whole float4 (or float2) loads where the rows
+// allow: a chain array starts 256-byte aligned and its planes start at texel offsets, so with c a multiple of four (of
+// two) every row is 16-byte (8-byte) aligned. Only the load width changes, not a value.
+template <int MAXC>
+NNTC_HD void load_texel(const float* p, int c, float* out)
+{
+    if ((c & 3) == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < MAXC / 4; k++)
+            if (4 * k < c)
+            {
+                const float4 v = reinterpret_cast<const float4*>(p)[k];
+                out[4 * k] = v.x;
+                out[4 * k + 1] = v.y;
+                out[4 * k + 2] = v.z;
+                out[4 * k + 3] = v.w;
+            }
+    }
+    else if ((c & 1) == 0)
+    {
+#pragma unroll
+        for (int k = 0; k < MAXC / 2; k++)
+            if (2 * k < c)
+            {
+                const float2 v = reinterpret_cast<const float2*>(p)[k];
+                out[2 * k] = v.x;
+                out[2 * k + 1] = v.y;
+            }
+    }
+    else
+    {
+#pragma unroll
+        for (int k = 0; k < MAXC; k++)
+            if (k < c)
+                out[k] = p[k];
+    }
+}
+
+// sample_bilinear_r with each tap's channels read by load_texel; the blend is the same expression, channel by channel.
+template <int MAXC>
+NNTC_HD void sample_bilinear_v(const float* plane, int w, int c, const Tap& t, float* out)
+{
+    float a[MAXC], b[MAXC], e[MAXC], f[MAXC];
+    load_texel<MAXC>(plane + ((size_t)t.y0 * w + t.x0) * c, c, a);
+    load_texel<MAXC>(plane + ((size_t)t.y0 * w + t.x1) * c, c, b);
+    load_texel<MAXC>(plane + ((size_t)t.y1 * w + t.x0) * c, c, e);
+    load_texel<MAXC>(plane + ((size_t)t.y1 * w + t.x1) * c, c, f);
+    const float w00 = (1.0f - t.fx) * (1.0f - t.fy), w10 = t.fx * (1.0f - t.fy);
+    const float w01 = (1.0f - t.fx) * t.fy, w11 = t.fx * t.fy;
+#pragma unroll
+    for (int k = 0; k < MAXC; k++)
+        if (k < c)
+            out[k] = w00 * a[k] + w10 * b[k] + w01 * e[k] + w11 * f[k];
+}
+
 template <int MAXC>
 NNTC_HD void obj_sample_r(const float* plane, int w, int h, int c, double u, double v, double* out)
 {
