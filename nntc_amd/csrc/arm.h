@@ -79,6 +79,7 @@ struct Latent
     int c = 0;              // channels
     int bits = 8;
     DevBuf<double> stencil; // texels * 5 * c * c
+    DevBuf<float> stencil32;   // the fast form's fp32 copy, written by its gathers, read by its sweeps and mat-vec
     DevBuf<double> grad;    // texels * c
     DevBuf<float> precond;  // texels * c * c
     DevBuf<double> x, r, z, p, ap;

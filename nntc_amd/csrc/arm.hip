@@ -635,6 +635,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
             L.bits = level0 ? 8 : md->bits1;
             L.nmono = stencil_monomials(level0 ? c1 : c0);
             L.stencil.alloc((size_t)t * STENCIL_BLOCKS * L.c * L.c);
+            L.stencil32.alloc((size_t)t * STENCIL_BLOCKS * L.c * L.c);
             L.grad.alloc((size_t)t * L.c);
             L.precond.alloc((size_t)t * L.c * L.c);
             L.x.alloc((size_t)t * L.c);

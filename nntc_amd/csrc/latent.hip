@@ -117,6 +117,7 @@ struct LatentView
     int bits;
     int levels;
     double* stencil;      // entry e of the texel at position s: stencil[e * sm.sn + s]
+    float* stencil32;     // the fast form's fp32 copy, same layout
     StencilMap sm;
     double* grad;
     float* precond;
@@ -731,7 +732,9 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
     // sub writes entries sub, sub + G, ... of each block.
     const long long gtx = L.texels.start[p] + ti;
     const long long sn = L.sm.sn;
-    double* out_h = L.stencil + stencil_pos(L.sm, p, w1, tx, ty);
+    const long long opos = stencil_pos(L.sm, p, w1, tx, ty);
+    double* out_h = L.stencil + opos;
+    float* out_f = L.stencil32 + opos;
     for (int e = sub; e < C * C; e += G)
     {
         double mat[NM];
@@ -746,6 +749,7 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
             for (int i = 0; i < NM; i++)
                 v = fma(sums[slot][i], mat[i], v);
             out_h[(slot * C * C + e) * sn] = v;
+            out_f[(slot * C * C + e) * sn] = (float)v;
         }
     }
     for (int ch = sub; ch < C; ch += G)
@@ -848,7 +852,9 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
     // Lane sub writes entries sub, sub + G, ... of each slot's C x C block, entry (i, jj) from the triangle's
     // (min, max) element, picked by selects (a register array indexed by a lane's value would live in scratch).
     const long long sn = L.sm.sn;
-    double* out_h = L.stencil + stencil_pos(L.sm, p, w0, tx, ty);
+    const long long opos = stencil_pos(L.sm, p, w0, tx, ty);
+    double* out_h = L.stencil + opos;
+    float* out_f = L.stencil32 + opos;
     for (int e = sub; e < C * C; e += G)
     {
         const int i = e / C, jj = e % C;
@@ -862,6 +868,7 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
             for (int q = 1; q < NH; q++)
                 v = tri == q ? hs[sl][q] : v;
             out_h[(sl * C * C + e) * sn] = v;
+            out_f[(sl * C * C + e) * sn] = (float)v;
         }
     }
     for (int ch = sub; ch < C; ch += G)
@@ -1531,20 +1538,34 @@ __global__ __launch_bounds__(256) void k_copy_prev(LatentView L)
     L.x[e] = 0.0;
 }
 
-template <int C>
+template <int C, class T>
 __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const Ranges& colour, int colour_index,
                                                     long long gi, int p);
 
+// The stencil the sweeps and the fast mat-vec read: fp64, or the fast form's fp32 copy.
+template <class T>
+__device__ __forceinline__ const T* stencil_of(const LatentView& L);
+template <>
+__device__ __forceinline__ const double* stencil_of<double>(const LatentView& L)
+{
+    return L.stencil;
+}
+template <>
+__device__ __forceinline__ const float* stencil_of<float>(const LatentView& L)
+{
+    return L.stencil32;
+}
+
 // One colour of one sweep: every texel (tx, ty) of the colour, the neighbours' share of (H delta)_t once, then its
 // channels in turn, each one's exact grid argmin using the values the earlier channels have just taken.
-template <int C>
+template <int C, class T>
 __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour, int colour_index,
                                                      unsigned int* changed)
 {
     const long long gi = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool live = gi < colour.start[colour.n];
     const int p = range_plane(colour, live ? gi : colour.start[colour.n] - 1);
-    const unsigned int moved = live ? sweep_texel<C>(L, colour, colour_index, gi, p) : 0u;
+    const unsigned int moved = live ? sweep_texel<C, T>(L, colour, colour_index, gi, p) : 0u;
     // One atomic per workgroup: the lanes of the plane of the workgroup's first texel are summed by shuffles and LDS;
     // a lane of another plane (a workgroup across a plane boundary) adds its own. The workgroups spread their adds
     // over MOVED_SLOTS counters per plane on separate cache lines, so they do not queue on one address.
@@ -1573,20 +1594,20 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
 // weighs its terms by an opaque 0.0 (1.0 inside). x * 1.0 is x, and x * 0.0 is a zero that leaves the sum unchanged
 // (the sum starts at +0.0 and so is never -0.0); the weight is opaque so that the compiler cannot turn it back into a
 // branch around the loads, which would serialise them.
-template <int C>
+template <int C, class T = double>
 struct SweepNeighbour
 {
     double x[C], s[C * C], keep;
 
     // Group g: slot 1 + g / 2, forward for even g, backward for odd g.
-    __device__ __forceinline__ void load(const LatentView& L, const double* S0, long long sn, const double* delta0,
+    __device__ __forceinline__ void load(const LatentView& L, const T* S0, long long sn, const double* delta0,
                                          const long long (&cbase)[4], int g, int tx, int ty, int w, int h,
                                          long long base, long long t, long long pos)
     {
         // The addresses hang on an opaque zero made here, so the loads cannot be issued before this point.
         long long z = 0;
         asm volatile("" : "+v"(z));
-        const double* S = S0 + z;
+        const T* S = S0 + z;
         const double* delta = delta0 + z;
         int dx, dy;
         stencil_offset(1 + g / 2, dx, dy);
@@ -1603,7 +1624,7 @@ struct SweepNeighbour
                 x[j] = xn[j];
 #pragma unroll
             for (int e = 0; e < C * C; e++)
-                s[e] = S[(slot * C * C + e) * sn + pos];
+                s[e] = (double)S[(slot * C * C + e) * sn + pos];
         }
         else
         {
@@ -1623,7 +1644,7 @@ struct SweepNeighbour
             for (int i = 0; i < C; i++)
 #pragma unroll
                 for (int j = 0; j < C; j++)
-                    s[i * C + j] = S[(slot * C * C + j * C + i) * sn + npos];
+                    s[i * C + j] = (double)S[(slot * C * C + j * C + i) * sn + npos];
         }
     }
 
@@ -1641,7 +1662,7 @@ struct SweepNeighbour
 };
 
 // One texel of the sweep's colour: returns 1 per channel whose index changed.
-template <int C>
+template <int C, class T>
 __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const Ranges& colour, int colour_index,
                                                     long long gi, int p)
 {
@@ -1655,7 +1676,7 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
     const long long t = base + (long long)ty * w + tx;
     const double lambda = L.scalars[(size_t)p * SC_COUNT + SC_LAMBDA];
     const long long sn = L.sm.sn;
-    const double* S = L.stencil;
+    const T* S = stencil_of<T>(L);
     const long long pos = L.sm.cpos[p][colour_index] + ci;
     double* delta = L.x;
 
@@ -1676,7 +1697,7 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
     // terms, and volatile asm statements keep their order. The terms reach each sum slot by slot, forward then
     // backward, channel by channel, as in a loop over the neighbours.
     constexpr int D = C <= 2 ? 4 : 2;
-    SweepNeighbour<C> q[8];
+    SweepNeighbour<C, T> q[8];
     double nb[C], dl[C], own[C * C], gr[C];
     float pvs[C];
     uint8_t kold[C];
@@ -1701,7 +1722,7 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
             }
 #pragma unroll
             for (int e = 0; e < C * C; e++)
-                own[e] = S[e * sn + pos];
+                own[e] = (double)S[e * sn + pos];
         }
         if (g >= D)
         {
@@ -1821,7 +1842,7 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
         const int tx = (int)(t % w), ty = (int)(t / w);
         const long long base = L.texels.start[p];
         const long long sn = L.sm.sn;
-        const double* S = L.stencil;
+        const float* S = L.stencil32;   // the fast form's fp32 copy: half the mat-vec's dominant bytes
         const long long pos = stencil_pos(L.sm, p, w, tx, ty);
         long long cbase[4];
 #pragma unroll
@@ -1840,11 +1861,11 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
             double sacc = lambda * xt[i];
 #pragma unroll
             for (int j = 0; j < C; j++)
-                sacc += S[(i * C + j) * sn + pos] * xt[j];
+                sacc += (double)S[(i * C + j) * sn + pos] * xt[j];
             acc[i] = sacc;
         }
         constexpr int D = C <= 2 ? 4 : 2;
-        SweepNeighbour<C> q[8];
+        SweepNeighbour<C, float> q[8];
 #pragma unroll
         for (int g = 0; g < 8 + D; g++)
         {
@@ -1976,6 +1997,7 @@ static LatentView make_view(Ctx& c, bool level0)
     L.bits = lt.bits;
     L.levels = (1 << lt.bits) - 1;
     L.stencil = lt.stencil.p;
+    L.stencil32 = lt.stencil32.p;
     L.sm = stencil_map(c, level0);
     L.grad = lt.grad.p;
     L.precond = lt.precond.p;
@@ -2313,6 +2335,19 @@ void copy_prev_zero_delta(Ctx& c, bool level0)
     HIP_OK(hipGetLastError());
 }
 
+template <class T>
+static void launch_sweep(int ch, dim3 grid, hipStream_t s, const LatentView& L, const Ranges& colour, int col,
+                         unsigned int* counters)
+{
+    switch (ch)
+    {
+    case 1: hipLaunchKernelGGL((k_quant_sweep<1, T>), grid, dim3(256), 0, s, L, colour, col, counters); break;
+    case 2: hipLaunchKernelGGL((k_quant_sweep<2, T>), grid, dim3(256), 0, s, L, colour, col, counters); break;
+    case 3: hipLaunchKernelGGL((k_quant_sweep<3, T>), grid, dim3(256), 0, s, L, colour, col, counters); break;
+    default: hipLaunchKernelGGL((k_quant_sweep<4, T>), grid, dim3(256), 0, s, L, colour, col, counters); break;
+    }
+}
+
 void quant_sweeps(Ctx& c, bool level0, int sweeps)
 {
     LatentView L = make_view(c, level0);
@@ -2342,13 +2377,11 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
             KernelTimer kt(c.stream, level0 ? "k_quant_sweep0" : "k_quant_sweep1",
                            (double)n * (40.0 * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc), 0.0);
             const dim3 grid(blocks_for(n, 256));
-            switch (L.c)
-            {
-            case 1: hipLaunchKernelGGL(k_quant_sweep<1>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
-            case 2: hipLaunchKernelGGL(k_quant_sweep<2>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
-            case 3: hipLaunchKernelGGL(k_quant_sweep<3>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
-            default: hipLaunchKernelGGL(k_quant_sweep<4>, grid, dim3(256), 0, c.stream, L, colour[col], col, c.counters.p); break;
-            }
+            // The exact form reads the fp64 stencil; the fast form its fp32 copy (half the bytes).
+            if (c.form == NNTC_FORM_EXACT)
+                launch_sweep<double>(L.c, grid, c.stream, L, colour[col], col, c.counters.p);
+            else
+                launch_sweep<float>(L.c, grid, c.stream, L, colour[col], col, c.counters.p);
         }
     HIP_OK(hipGetLastError());
 }
