@@ -214,6 +214,10 @@ int nntc_hip_level0_upload_indices(nntc_hip_ctx* ctx, int plane, const uint8_t* 
 int nntc_hip_upload_decoder_to_device(nntc_hip_ctx* ctx, const nntc_model_state* state);
 int nntc_hip_upload_grids_to_device(nntc_hip_ctx* ctx, const nntc_model_state* state);
 int nntc_hip_level1_download(nntc_hip_ctx* ctx, int plane, float* v1, size_t n, size_t* count);
+/* The check of src/main.cpp:2696-2711 (verify_level1_on_grid, src/export.cpp:70-84): of the *values level-1 values of
+ * every plane as the device holds them, *off are not the grid value, under the state's level-1 grid, of the index
+ * beside them. Replaces the download of both and the host loop. */
+int nntc_hip_level1_off_grid(nntc_hip_ctx* ctx, const nntc_model_state* state, size_t* values, size_t* off);
 int nntc_hip_level1_delta(nntc_hip_ctx* ctx, int plane, double* delta, size_t n, size_t* count);
 int nntc_hip_level1_poke(nntc_hip_ctx* ctx, int plane, size_t index, float value);
 int nntc_hip_level1_range(nntc_hip_ctx* ctx, int plane, float* lo, float* hi);

@@ -152,6 +152,7 @@ SIGNATURES = {
     "upload_grids_to_device": (C.c_int, [C.c_void_p, _P(ModelState)]),
     "level1_download": (C.c_int, [C.c_void_p, C.c_int, _f32p, C.c_size_t, _P(C.c_size_t)]),
     "level1_delta": (C.c_int, [C.c_void_p, C.c_int, _f64p, C.c_size_t, _P(C.c_size_t)]),
+    "level1_off_grid": (C.c_int, [C.c_void_p, _P(ModelState), _P(C.c_size_t), _P(C.c_size_t)]),
     "level1_poke": (C.c_int, [C.c_void_p, C.c_int, C.c_size_t, C.c_float]),
     "level1_range": (C.c_int, [C.c_void_p, C.c_int, _f32p, _f32p]),
     "level0_download": (C.c_int, [C.c_void_p, C.c_int, _u8p, C.c_size_t, _P(C.c_size_t)]),
@@ -287,6 +288,12 @@ class Session:
     def k1(self, p):
         pl = self.plane(p)
         return self.download("level1_download_indices", p, np.uint8, pl.w1 * pl.h1 * self.desc.c1)
+
+    def level1_off_grid(self):
+        """(values, off): the level-1 values, and those not the grid value of their index under the state's grid."""
+        values, off = C.c_size_t(0), C.c_size_t(0)
+        self.call("level1_off_grid", C.byref(self.state), C.byref(values), C.byref(off))
+        return values.value, off.value
 
     def delta1(self, p):
         pl = self.plane(p)

@@ -130,7 +130,7 @@ struct Ctx
     Planes planes {};
     long long n0_total = 0, n1_total = 0, nb_total = 0;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
 
     DevBuf<float> src, v0, v1;
     DevBuf<uint8_t> k0, k1;
@@ -156,7 +156,9 @@ struct Ctx
         bool held = false;
     } snap;
     DevBuf<double> bc_taken;               // per block: the accepted decrease
-    DevBuf<double> bc_se;                  // per plane: the packing error
+    DevBuf<double> bc_se;                  // per block: the packing error
+    DevBuf<double> bc_fold;                // the packing error's total, then each refinement pass's figures per plane
+    std::vector<hipEvent_t> pass_ev;       // brackets of the refinement passes
     DevBuf<float> work_f;                  // staging for grid fits (values of one channel)
     DevBuf<unsigned int> work_u;           // radix-select histograms
     DevBuf<uint8_t> rgb8;                  // one plane's decode
@@ -166,49 +168,52 @@ struct Ctx
 
     // Pinned host staging for the small per-call readbacks.
     double* host_scalars = nullptr;
-    // Pinned host staging for readbacks and small uploads, grown on demand: a copy between device memory and pageable
-    // host memory is staged by the runtime through buffers of its own, in pieces, each a copy kernel and a wait.
+    // Pinned host staging: a copy between device memory and pageable host memory is staged by the runtime through
+    // buffers of its own, in pieces, each a copy kernel and a wait. Readbacks use one area (each waits for its copy);
+    // uploads take successive pieces of a second area, so a new upload never waits for an earlier one still in flight,
+    // and the area is reused from its start once the stream has drained.
     void* host_stage = nullptr;
     size_t host_stage_bytes = 0;
-    bool stage_in_flight = false;   // an upload may still be reading the staging area
-    // The staging area, at least `bytes` long. The host may write it (host_writes) only once no upload still reads
-    // it; a copy into it on the stream is ordered after any such upload by the stream itself.
-    void* stage(size_t bytes, bool host_writes);
+    char* host_up = nullptr;
+    size_t host_up_bytes = 0, up_head = 0;
+    void* stage(size_t bytes);          // the readback area, at least `bytes` long
+    void* upload_slot(size_t bytes);    // a piece of the upload area no queued copy still reads
 
     size_t bytes = 0;
 
     // Host-side view of the decoder and grids as last uploaded.
     std::vector<float> h_w, h_b;
     float h_lo0[4] = {}, h_hi0[4] = {}, h_lo1[4] = {}, h_hi1[4] = {};
+    float h_grid[16 + 64] = {};                 // the grid buffer as last uploaded
+    bool decoder_on_device = false, grid_on_device = false;
 
     Latent& latent(bool level0) { return level0 ? lat0 : lat1; }
     const PlaneGeom& g(int p) const { return planes.p[p]; }
     float ms_since(hipEvent_t a);
 };
 
-// n values read back from the device through the pinned staging area; the pointer holds until the next staging use.
+// n values read back from the device through the pinned readback area; the pointer holds until the next readback.
 template <class T>
 const T* read_back(Ctx& c, const T* dev, size_t n)
 {
-    T* h = static_cast<T*>(c.stage(n * sizeof(T), false));
+    T* h = static_cast<T*>(c.stage(n * sizeof(T)));
     if (n)
         HIP_OK(hipMemcpyAsync(h, dev, n * sizeof(T), hipMemcpyDeviceToHost, c.stream));
     HIP_OK(hipStreamSynchronize(c.stream));
-    c.stage_in_flight = false;
+    c.up_head = 0;   // the stream has drained: no upload reads the upload area any more
     return h;
 }
 
-// n values uploaded through the pinned staging area, asynchronously (the next staging use waits for it).
+// n values uploaded through the pinned upload area, asynchronously: the launches that read them follow on the stream.
 template <class T>
 void upload_staged(Ctx& c, T* dev, const T* host, size_t n)
 {
     if (!n)
         return;
-    T* h = static_cast<T*>(c.stage(n * sizeof(T), true));
+    T* h = static_cast<T*>(c.upload_slot(n * sizeof(T)));
     for (size_t i = 0; i < n; i++)
         h[i] = host[i];
     HIP_OK(hipMemcpyAsync(dev, h, n * sizeof(T), hipMemcpyHostToDevice, c.stream));
-    c.stage_in_flight = true;
 }
 
 // The stencil layout of one latent's chain (see StencilMap).
@@ -266,11 +271,12 @@ void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi);
 void cov_partials(Ctx& c, const float* data, long long texels, std::vector<double>& raw, double weight);
 double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, const double* mu);
 void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const double* mu, double inv, int channel);
+unsigned long long level1_off_grid(Ctx& c, const float* lo, const float* hi);
 // refine.hip
 void bc_decode(Ctx& c);
 void bc_seed(Ctx& c);
-void bc_refine_pass(Ctx& c, double* taken, double* improved);
-void bc_pack_blocks(Ctx& c, double& se, size_t& n);
+void bc_refine_pass(Ctx& c, double* fold /* device, nplanes * 2 */);
+void bc_pack_blocks(Ctx& c, double* se_total /* device, one value */);
 
 // ---- shared launch helpers
 inline unsigned int blocks_for(long long n, int threads)

@@ -140,13 +140,8 @@ KernelTimer::~KernelTimer()
     }
 }
 
-void* Ctx::stage(size_t bytes, bool host_writes)
+void* Ctx::stage(size_t bytes)
 {
-    if (stage_in_flight && (host_writes || bytes > host_stage_bytes))
-    {
-        HIP_OK(hipStreamSynchronize(stream));
-        stage_in_flight = false;
-    }
     if (bytes > host_stage_bytes)
     {
         if (host_stage)
@@ -158,6 +153,34 @@ void* Ctx::stage(size_t bytes, bool host_writes)
         host_stage_bytes = want;
     }
     return host_stage;
+}
+
+void* Ctx::upload_slot(size_t bytes)
+{
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (up_head + need > host_up_bytes)
+    {
+        if (up_head)
+        {
+            HIP_OK(hipStreamSynchronize(stream));
+            up_head = 0;
+        }
+        if (need > host_up_bytes)
+        {
+            if (host_up)
+                HIP_OK(hipHostFree(host_up));
+            host_up = nullptr;
+            host_up_bytes = 0;
+            const size_t want = need < ((size_t)1 << 20) ? ((size_t)1 << 20) : need;
+            void* ptr = nullptr;
+            HIP_OK(hipHostMalloc(&ptr, want, hipHostMallocDefault));
+            host_up = static_cast<char*>(ptr);
+            host_up_bytes = want;
+        }
+    }
+    void* slot = host_up + up_head;
+    up_head += need;
+    return slot;
 }
 
 float Ctx::ms_since(hipEvent_t a)
@@ -306,15 +329,17 @@ void upload_decoder(Ctx& c, const nntc_model_state* st)
     const size_t nw = (size_t)c.nout * c.nin;
     if (st && st->has_decoder)
     {
+        // Nothing but this upload writes the device's decoder, so when the values are those of the last upload (bit for
+        // bit) the device already holds them.
+        if (c.decoder_on_device && c.h_w.size() == nw && c.h_b.size() == (size_t)c.nout &&
+            std::memcmp(c.h_w.data(), st->w, nw * sizeof(float)) == 0 &&
+            std::memcmp(c.h_b.data(), st->b, (size_t)c.nout * sizeof(float)) == 0)
+            return;
         c.h_w.assign(st->w, st->w + nw);
         c.h_b.assign(st->b, st->b + c.nout);
-        // Both through the pinned staging area, asynchronously: the launches that read them follow on the stream.
-        float* h = static_cast<float*>(c.stage((nw + c.nout) * sizeof(float), true));
-        std::memcpy(h, st->w, nw * sizeof(float));
-        std::memcpy(h + nw, st->b, c.nout * sizeof(float));
-        HIP_OK(hipMemcpyAsync(c.weights.p, h, nw * sizeof(float), hipMemcpyHostToDevice, c.stream));
-        HIP_OK(hipMemcpyAsync(c.bias.p, h + nw, c.nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
-        c.stage_in_flight = true;
+        upload_staged(c, c.weights.p, st->w, nw);
+        upload_staged(c, c.bias.p, st->b, (size_t)c.nout);
+        c.decoder_on_device = true;
     }
 }
 
@@ -345,7 +370,12 @@ void upload_grids(Ctx& c, const nntc_model_state* st)
     std::memcpy(c.h_hi0, g + 4, sizeof(c.h_hi0));
     std::memcpy(c.h_lo1, g + 8, sizeof(c.h_lo1));
     std::memcpy(c.h_hi1, g + 12, sizeof(c.h_hi1));
+    // As with the decoder, only this upload writes the device's grids.
+    if (c.grid_on_device && std::memcmp(c.h_grid, g, sizeof(g)) == 0)
+        return;
+    std::memcpy(c.h_grid, g, sizeof(g));
     upload_staged(c, c.grid.p, g, sizeof(g) / sizeof(g[0]));
+    c.grid_on_device = true;
 }
 
 // A plane-sized download (or, with n smaller, its head).
@@ -400,11 +430,16 @@ void fit_and_snap(Ctx& c, nntc_model_state* st, bool level0, int policy)
     HIP_OK(hipStreamSynchronize(c.stream));
 }
 
-void read_scalars(Ctx& c, int channels, nntc_level1_report* rep)
+// The scalar rows copied into host_scalars; they are there once the stream has next been synchronized.
+void fetch_scalars(Ctx& c)
 {
     HIP_OK(hipMemcpyAsync(c.host_scalars, c.scalars.p, sizeof(double) * SC_COUNT * c.planes.n,
                           hipMemcpyDeviceToHost, c.stream));
-    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
+// The report fields of the scalar rows fetched last.
+void parse_scalars(Ctx& c, int channels, nntc_level1_report* rep)
+{
     for (int p = 0; p < c.planes.n; p++)
     {
         const double* s = c.host_scalars + (size_t)p * SC_COUNT;
@@ -433,14 +468,19 @@ double solve_block(Ctx& c, const nntc_model_state* st, int k, double ridge, bool
     build_block_matrices(c, level0);
     HIP_OK(hipEventRecord(c.ev0, c.stream));
     assemble(c, level0, k, ridge);
-    const float ms_asm = c.ms_since(c.ev0);
-    HIP_OK(hipEventRecord(c.ev0, c.stream));
+    HIP_OK(hipEventRecord(c.ev1, c.stream));
     std::vector<int> iterations;
     cg_solve(c, level0, iterations);
     correction_and_probe(c, level0);
-    const float ms_cg = c.ms_since(c.ev0);
+    HIP_OK(hipEventRecord(c.ev2, c.stream));
+    fetch_scalars(c);
+    HIP_OK(hipStreamSynchronize(c.stream));
+    c.up_head = 0;
     const int ch = level0 ? c.c0 : c.c1;
-    read_scalars(c, ch, rep);
+    parse_scalars(c, ch, rep);
+    float ms_asm = 0.0f, ms_cg = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms_asm, c.ev0, c.ev1));
+    HIP_OK(hipEventElapsedTime(&ms_cg, c.ev1, c.ev2));
     // Every plane of the chain runs in the same launches, so each plane's report carries the block's kernel time.
     for (int p = 0; p < c.planes.n; p++)
     {
@@ -464,15 +504,18 @@ double sweep_block(Ctx& c, const nntc_model_state* st, int k, double ridge, int 
     build_block_matrices(c, level0);
     HIP_OK(hipEventRecord(c.ev0, c.stream));
     assemble(c, level0, k, ridge);
-    const float ms_asm = c.ms_since(c.ev0);
-    HIP_OK(hipEventRecord(c.ev0, c.stream));
+    HIP_OK(hipEventRecord(c.ev1, c.stream));
     copy_prev_zero_delta(c, level0);
     quant_sweeps(c, level0, sweeps);
-    const float ms_sw = c.ms_since(c.ev0);
-    const int ch = level0 ? c.c0 : c.c1;
-    read_scalars(c, ch, rep);
+    HIP_OK(hipEventRecord(c.ev2, c.stream));
+    fetch_scalars(c);
     const unsigned int* slots = read_back(c, (const unsigned int*)c.counters.p,
                                           (size_t)c.planes.n * MOVED_SLOTS * MOVED_STRIDE);
+    const int ch = level0 ? c.c0 : c.c1;
+    parse_scalars(c, ch, rep);
+    float ms_asm = 0.0f, ms_sw = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms_asm, c.ev0, c.ev1));
+    HIP_OK(hipEventElapsedTime(&ms_sw, c.ev1, c.ev2));
     std::vector<unsigned long long> moved((size_t)c.planes.n, 0);
     for (int p = 0; p < c.planes.n; p++)
         for (int s = 0; s < MOVED_SLOTS; s++)
@@ -518,15 +561,22 @@ void set_k0_mirror(Ctx& c, const uint8_t* const* k0, bool snapshot)
         HIP_OK(hipMemcpyAsync(c.k0_mirror.p, c.k0.p, (size_t)c.n0_total * c.c0, hipMemcpyDeviceToDevice, c.stream));
 }
 
-void blocks_from_device(Ctx& c, double* pack_psnr, size_t* pack_texels)
+// The packing error and texel count of the pack on the device, from se, its total squared error.
+void pack_figures(const Ctx& c, double se, double* pack_psnr, size_t* pack_texels)
 {
-    double se = 0.0;
     size_t n = 0;
-    bc_pack_blocks(c, se, n);
+    for (int p = 0; p < c.planes.n; p++)
+        n += (size_t)c.g(p).n0 * (size_t)c.c0;
     if (pack_texels)
         *pack_texels = n;
     if (pack_psnr)
         *pack_psnr = n > 0 && se > 0.0 ? 10.0 * std::log10(255.0 * 255.0 / (se / (double)n)) : 100.0;
+}
+
+void blocks_from_device(Ctx& c, double* pack_psnr, size_t* pack_texels)
+{
+    bc_pack_blocks(c, c.bc_fold.p);
+    pack_figures(c, *read_back(c, (const double*)c.bc_fold.p, 1), pack_psnr, pack_texels);
 }
 
 }   // namespace
@@ -614,6 +664,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&c.ev0));
         HIP_OK(hipEventCreate(&c.ev1));
+        HIP_OK(hipEventCreate(&c.ev2));
 
         const int nout = c.nout, c0 = c.c0, c1 = c.c1;
         c.src.alloc((size_t)o0 * nout);
@@ -669,6 +720,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.k0_mirror.alloc((size_t)o0 * c0);
         c.bc_taken.alloc((size_t)ob);
         c.bc_se.alloc((size_t)ob);
+        c.bc_fold.alloc(1 + 2 * (size_t)md->nplanes * 16);
         c.work_f.alloc(256 * 2 * MAX_CH);
         c.work_u.alloc(MAX_CH * 2 * 256 + 64);
         c.rgb8.alloc((size_t)c.planes.p[0].n0 * nout);
@@ -739,10 +791,16 @@ void nntc_hip_device_destroy(nntc_hip_ctx* h)
         (void)hipHostFree(c.host_scalars);
     if (c.host_stage)
         (void)hipHostFree(c.host_stage);
+    if (c.host_up)
+        (void)hipHostFree(c.host_up);
     if (c.ev0)
         (void)hipEventDestroy(c.ev0);
     if (c.ev1)
         (void)hipEventDestroy(c.ev1);
+    if (c.ev2)
+        (void)hipEventDestroy(c.ev2);
+    for (hipEvent_t e : c.pass_ev)
+        (void)hipEventDestroy(e);
     hipStream_t s = c.stream;
     delete h;
     if (s)
@@ -1091,6 +1149,18 @@ int nntc_hip_level1_download_indices(nntc_hip_ctx* h, int plane, uint8_t* k1, si
     });
 }
 
+int nntc_hip_level1_off_grid(nntc_hip_ctx* h, const nntc_model_state* st, size_t* values, size_t* off)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        const unsigned long long n = level1_off_grid(c, st->lo1, st->hi1);
+        if (values)
+            *values = (size_t)(c.n1_total * c.c1);
+        if (off)
+            *off = (size_t)n;
+    });
+}
+
 int nntc_hip_level1_upload(nntc_hip_ctx* h, int plane, const float* v1, size_t n)
 {
     return guarded([&] {
@@ -1302,26 +1372,45 @@ int nntc_hip_bc_refine(nntc_hip_ctx* h, int passes, const uint8_t* const* k0_mir
                 *ms_out = 0.0;
             return;
         }
-        std::vector<double> taken((size_t)c.planes.n), improved((size_t)c.planes.n);
+        // Every pass's per-plane figures land in bc_fold after its first slot (the packing error's), so the passes,
+        // the pack and its error run back to back and are read once at the end.
+        const size_t np = (size_t)c.planes.n, need = 1 + 2 * np * (size_t)passes;
+        if (c.bc_fold.n < need)
+            c.bc_fold.alloc(need);
+        while (c.pass_ev.size() < (size_t)passes + 1)
+        {
+            hipEvent_t e = nullptr;
+            HIP_OK(hipEventCreate(&e));
+            c.pass_ev.push_back(e);
+        }
+        const double t0 = now_ms();
+        HIP_OK(hipEventRecord(c.pass_ev[0], c.stream));
         for (int pass = 0; pass < passes; pass++)
         {
-            const double t0 = now_ms();
-            bc_refine_pass(c, taken.data(), improved.data());
-            const double pass_ms = now_ms() - t0;
-            ms += pass_ms;
+            bc_refine_pass(c, c.bc_fold.p + 1 + 2 * np * (size_t)pass);
+            HIP_OK(hipEventRecord(c.pass_ev[(size_t)pass + 1], c.stream));
+        }
+        bc_pack_blocks(c, c.bc_fold.p);
+        const double* f = read_back(c, (const double*)c.bc_fold.p, need);
+        ms = now_ms() - t0;
+        for (int pass = 0; pass < passes; pass++)
+        {
+            float pass_ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&pass_ms, c.pass_ev[(size_t)pass], c.pass_ev[(size_t)pass + 1]));
             nntc_bc_refine_report r {};
             r.pass = pass + 1;
             r.ms = pass_ms;
-            for (int p = 0; p < c.planes.n; p++)
+            const double* fp = f + 1 + 2 * np * (size_t)pass;
+            for (size_t p = 0; p < np; p++)
             {
-                r.delta_e += taken[(size_t)p];
-                r.improved += (long long)improved[(size_t)p];
-                r.blocks += c.g(p).nb;
+                r.delta_e += fp[2 * p];
+                r.improved += (long long)fp[2 * p + 1];
+                r.blocks += c.g((int)p).nb;
             }
             if (rep)
                 rep[pass] = r;
         }
-        blocks_from_device(c, pack_psnr, pack_texels);
+        pack_figures(c, f[0], pack_psnr, pack_texels);
         if (ms_out)
             *ms_out = ms;
     });

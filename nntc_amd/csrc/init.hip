@@ -100,6 +100,33 @@ __global__ __launch_bounds__(256) void k_quantise(const float* in, float* v, uin
     k[i] = (uint8_t)q;
 }
 
+// Level-1 values that are not the grid value of the index beside them, counted per workgroup (one atomic each): the
+// check of src/main.cpp:2696-2711 (verify_level1_on_grid, src/export.cpp:70-84) over the planes as the device holds them.
+struct GridPair
+{
+    float lo[4], hi[4];
+};
+
+__global__ __launch_bounds__(256) void k_off_grid(const float* v, const uint8_t* k, long long n, int channels, int bits,
+                                                  GridPair g, unsigned int* off)
+{
+    __shared__ unsigned int s_off;
+    if (threadIdx.x == 0)
+        s_off = 0;
+    __syncthreads();
+    unsigned int mine = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    {
+        const int c = (int)(i % channels);
+        mine += v[i] != level1_value(g.lo[c], g.hi[c], bits, k[i]) ? 1u : 0u;
+    }
+    if (mine)
+        atomicAdd(&s_off, mine);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_off)
+        atomicAdd(off, s_off);
+}
+
 // Per-channel min/max over n interleaved values: partial per workgroup.
 __global__ __launch_bounds__(256) void k_minmax(const float* v, long long n, int channels, float* out)
 {
@@ -302,6 +329,24 @@ void quantise_values(Ctx& c, bool level0, const float* in_device, int plane)
     hipLaunchKernelGGL(k_quantise, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, in_device + begin, v + begin,
                        k + begin, n, ch, bits, lo, hi);
     HIP_OK(hipGetLastError());
+}
+
+unsigned long long level1_off_grid(Ctx& c, const float* lo, const float* hi)
+{
+    GridPair g {};
+    for (int q = 0; q < c.c1 && q < 4; q++)
+    {
+        g.lo[q] = lo[q];
+        g.hi[q] = hi[q];
+    }
+    const long long n = c.n1_total * c.c1;
+    unsigned int* off = c.work_u.p + MAX_CH * 2 * 256 + 32;   // past the radix prefixes in the histograms' tail
+    HIP_OK(hipMemsetAsync(off, 0, sizeof(unsigned int), c.stream));
+    const unsigned grid = (unsigned)std::min<long long>(blocks_for(n, 256), 2048);
+    hipLaunchKernelGGL(k_off_grid, dim3(grid > 0 ? grid : 1), dim3(256), 0, c.stream, c.v1.p, c.k1.p, n, c.c1,
+                       c.md.bits1, g, off);
+    HIP_OK(hipGetLastError());
+    return *read_back(c, (const unsigned int*)off, 1);
 }
 
 // The per-channel range of a latent over every plane (src/cpu/cpu_init.cpp:338-370).

@@ -784,6 +784,98 @@ __global__ __launch_bounds__(256) void k_bc_pack(BcArgs a, long long total)
     a.se[gb] = se;
 }
 
+// One pass's figures per plane, folded in the reference's order (src/cpu/cpu_refine_bc.cpp:398-447): per colour, the
+// colour's blocks in chunks of 64, each chunk's accepted decreases (the negative ones) and their count summed in block
+// order, the chunk sums added in chunk order; then the four colours added in order. One workgroup per plane, a group of
+// 256 lanes per colour: lane t sums chunk t of each 256, the group's lane 0 adds those in order. Skipping a block that
+// took nothing and adding +0.0 for it give the same sums, since no partial sum is ever -0.0.
+__global__ __launch_bounds__(1024) void k_bc_fold(Planes planes, const double* taken, double* out)
+{
+    __shared__ double s_t[4][256], s_m[4][256], s_col[4][2];
+    const int p = blockIdx.x, col = threadIdx.x >> 8, t = threadIdx.x & 255;
+    const PlaneGeom& g = planes.p[p];
+    const int bx = (g.w0 + 3) / 4, by = (g.h0 + 3) / 4;
+    const int cx = col & 1, cy = col >> 1;
+    const int cbx = (bx - cx + 1) / 2;
+    const int count = cbx * ((by - cy + 1) / 2);
+    const int chunks = (count + 63) / 64;
+    const int chunks_max = (((bx + 1) / 2) * ((by + 1) / 2) + 63) / 64;   // colour 0 has the most blocks
+    const double* const tp = taken + g.offb;
+    double ct = 0.0, cm = 0.0;
+    for (int base = 0; base < chunks_max; base += 256)
+    {
+        const int ch = base + t;
+        if (ch < chunks)
+        {
+            double pt = 0.0, pm = 0.0;
+            const int b0 = ch * 64, b1 = min(b0 + 64, count);
+            int bxi = b0 % cbx, byi = b0 / cbx;
+            for (int b = b0; b < b1; b++)
+            {
+                const double v = tp[(long long)(cy + 2 * byi) * bx + cx + 2 * bxi];
+                const bool neg = v < 0.0;
+                pt += neg ? v : 0.0;
+                pm += neg ? 1.0 : 0.0;
+                if (++bxi == cbx)
+                {
+                    bxi = 0;
+                    byi++;
+                }
+            }
+            s_t[col][t] = pt;
+            s_m[col][t] = pm;
+        }
+        __syncthreads();
+        if (t == 0)
+        {
+            const int n = min(chunks - base, 256);
+            for (int j = 0; j < n; j++)
+            {
+                ct += s_t[col][j];
+                cm += s_m[col][j];
+            }
+        }
+        __syncthreads();
+    }
+    if (t == 0)
+    {
+        s_col[col][0] = ct;
+        s_col[col][1] = cm;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double tk = 0.0, im = 0.0;
+        for (int q = 0; q < 4; q++)
+        {
+            tk += s_col[q][0];
+            im += s_col[q][1];
+        }
+        out[2 * p] = tk;
+        out[2 * p + 1] = im;
+    }
+}
+
+// The packing error's total over every block of the chain, in this kernel's fixed order (a report figure only: the
+// reference adds the same squared errors texel by texel, so the last bits may differ).
+__global__ __launch_bounds__(1024) void k_bc_se_total(const double* se, long long n, double* out)
+{
+    __shared__ double s[1024];
+    double acc = 0.0;
+    for (long long i = threadIdx.x; i < n; i += 1024)
+        acc += se[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1)
+    {
+        if ((int)threadIdx.x < w)
+            s[threadIdx.x] += s[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        *out = s[0];
+}
+
 BcArgs bc_args(Ctx& c)
 {
     BcArgs a;
@@ -823,10 +915,9 @@ void bc_decode(Ctx& c)
     HIP_OK(hipGetLastError());
 }
 
-// One pass: the four colours over every plane's block grid, then the pass figures folded in the reference's order:
-// per plane, per colour, the colour's blocks in chunks of 64 (a chunk's decreases summed in block order, the chunks
-// added in order), the colours added in order (src/cpu/cpu_refine_bc.cpp:398-447).
-void bc_refine_pass(Ctx& c, double* taken, double* improved)
+// One pass: the four colours over every plane's block grid, then the pass figures folded per plane on the device into
+// fold[2p] (the decrease taken) and fold[2p + 1] (the blocks improved).
+void bc_refine_pass(Ctx& c, double* fold)
 {
     BcArgs a = bc_args(c);
     Ranges colour[4];
@@ -862,56 +953,18 @@ void bc_refine_pass(Ctx& c, double* taken, double* improved)
         }
         HIP_OK(hipGetLastError());
     }
-    const double* t = read_back(c, (const double*)c.bc_taken.p, (size_t)c.nb_total);
-    const size_t chunk = 64;
-    for (int p = 0; p < c.planes.n; p++)
-    {
-        const PlaneGeom& g = c.g(p);
-        const int bx = (g.w0 + 3) / 4, by = (g.h0 + 3) / 4;
-        double tk = 0.0, im = 0.0;
-        for (int col = 0; col < 4; col++)
-        {
-            const int cx = col & 1, cy = col >> 1;
-            const int cbx = (bx - cx + 1) / 2;
-            const size_t count = (size_t)cbx * (size_t)((by - cy + 1) / 2);
-            if (count == 0)
-                continue;
-            double ct = 0.0, cm = 0.0;
-            for (size_t b0 = 0; b0 < count; b0 += chunk)
-            {
-                double pt = 0.0, pm = 0.0;
-                for (size_t b = b0; b < b0 + chunk && b < count; b++)
-                {
-                    const int gx = cx + 2 * (int)(b % (size_t)cbx);
-                    const int gy = cy + 2 * (int)(b / (size_t)cbx);
-                    const double v = t[(size_t)(g.offb + (long long)gy * bx + gx)];
-                    if (v < 0.0)
-                    {
-                        pt += v;
-                        pm += 1.0;
-                    }
-                }
-                ct += pt;
-                cm += pm;
-            }
-            tk += ct;
-            im += cm;
-        }
-        taken[p] = tk;
-        improved[p] = im;
-    }
+    hipLaunchKernelGGL(k_bc_fold, dim3((unsigned)c.planes.n), dim3(1024), 0, c.stream, c.planes,
+                       (const double*)c.bc_taken.p, fold);
+    HIP_OK(hipGetLastError());
 }
 
-void bc_pack_blocks(Ctx& c, double& se, size_t& n)
+void bc_pack_blocks(Ctx& c, double* se_total)
 {
     BcArgs a = bc_args(c);
     hipLaunchKernelGGL(k_bc_pack, dim3(blocks_for(c.nb_total, 256)), dim3(256), 0, c.stream, a, c.nb_total);
     HIP_OK(hipGetLastError());
-    const double* h = read_back(c, (const double*)c.bc_se.p, (size_t)c.nb_total);
-    for (long long b = 0; b < c.nb_total; b++)
-        se += h[b];
-    for (int p = 0; p < c.planes.n; p++)
-        n += (size_t)c.g(p).n0 * (size_t)c.c0;
+    hipLaunchKernelGGL(k_bc_se_total, dim3(1), dim3(1024), 0, c.stream, (const double*)c.bc_se.p, c.nb_total, se_total);
+    HIP_OK(hipGetLastError());
 }
 
 }   // namespace nntc

@@ -594,22 +594,7 @@ void Encoder::box_reorder()
 void Encoder::verify_level1_on_grid()
 {
     size_t values = 0, off = 0;
-    for (int p = 0; p < desc.nplanes; p++)
-    {
-        const nntc_plane_size& pl = desc.planes[p];
-        const size_t n = (size_t)pl.w1 * pl.h1 * desc.c1;
-        std::vector<float> v(n);
-        std::vector<uint8_t> kk(n);
-        check(seam.level1_download(ctx, p, v.data(), n, nullptr), "level1_download");
-        check(seam.level1_download_indices(ctx, p, kk.data(), n, nullptr), "level1_download_indices");
-        for (size_t i = 0; i < n; i++)
-        {
-            const int c = (int)(i % (size_t)desc.c1);
-            values++;
-            if (v[i] != level1_value(state.lo1[c], state.hi1[c], desc.bits1, kk[i]))
-                off++;
-        }
-    }
+    check(seam.level1_off_grid(ctx, &state, &values, &off), "level1_off_grid");
     if (off)
         throw Failure { std::to_string(off) + " of " + std::to_string(values) +
                         " level-1 values are not the grid value of the index beside them" };

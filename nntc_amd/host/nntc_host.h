@@ -47,6 +47,7 @@ struct Failure
     X(upload_decoder_to_device)                                                                                        \
     X(upload_grids_to_device)                                                                                          \
     X(level1_download)                                                                                                 \
+    X(level1_off_grid)                                                                                                 \
     X(level0_download)                                                                                                 \
     X(level0_upload)                                                                                                   \
     X(level0_download_values)                                                                                          \

@@ -437,6 +437,26 @@ int nntc_ref_level1_download(nntc_hip_ctx* ctx, int plane, float* v1, size_t n, 
     return download(v, v1, n, count);
 }
 
+// The reference's own check (src/main.cpp:2696-2711): both planes downloaded, verify_level1_on_grid over them.
+int nntc_ref_level1_off_grid(nntc_hip_ctx* ctx, const nntc_model_state* st, size_t* values, size_t* off)
+{
+    REF_CTX;
+    state_to_model(st, m);
+    std::vector<std::vector<float>> v1(m.planes.size());
+    for (size_t i = 0; i < m.planes.size(); i++)
+    {
+        be::level1_download_indices(r->d, m, (int)i, m.k1[i]);
+        be::level1_download(r->d, m, (int)i, v1[i]);
+    }
+    size_t nv = 0, no = 0;
+    verify_level1_on_grid(m, v1, nv, no);
+    if (values)
+        *values = nv;
+    if (off)
+        *off = no;
+    return 0;
+}
+
 int nntc_ref_level1_delta(nntc_hip_ctx* ctx, int plane, double* delta, size_t n, size_t* count)
 {
     REF_CTX;

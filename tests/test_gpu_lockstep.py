@@ -82,3 +82,11 @@ def test_lockstep_exact(name, hip_arm, ref_arm):
     base = h.plane(0)
     for wh in ((base.w0, base.h0), (max(1, base.w0 - 3), max(1, base.h0 // 2))):
         assert h.texture_sse(*wh) == r.texture_sse(*wh), f"texture sse over {wh}"
+    # The on-grid check made before the asset is written: no level-1 value off its index's grid value, then one value
+    # moved by an ulp on both arms, counted by both.
+    values, off = h.level1_off_grid()
+    assert (values, off) == r.level1_off_grid() and off == 0
+    v = np.float32(h.v1(0)[0])
+    for s in (h, r):
+        s.call("level1_poke", 0, 0, float(np.nextafter(v, np.float32(np.inf))))
+    assert h.level1_off_grid() == r.level1_off_grid() == (values, 1)
