@@ -268,8 +268,9 @@ void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0);
 void init_level1_box(Ctx& c);
 void quantise_values(Ctx& c, bool level0, const float* in_device, int plane);
 void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi);
-void cov_partials(Ctx& c, const float* data, long long texels, std::vector<double>& raw, double weight);
-double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, const double* mu);
+void cov_accumulate(Ctx& c, const float* data, long long texels, double weight, bool first);
+void cov_read(Ctx& c, std::vector<double>& raw);
+double proj_peak_chain(Ctx& c, const float* resid, const double* e, const double* mu);
 void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const double* mu, double inv, int channel);
 unsigned long long level1_off_grid(Ctx& c, const float* lo, const float* hi);
 // refine.hip

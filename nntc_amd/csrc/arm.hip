@@ -711,7 +711,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.ls_sum.alloc((size_t)entries + MAX_PLANES);
         c.obj.alloc((size_t)md->nplanes * 3);
         c.resid.alloc((size_t)o0 * nout);
-        c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout));
+        c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout) + (nout + nout * nout));
         c.counters.alloc((size_t)md->nplanes * MOVED_SLOTS * MOVED_STRIDE);
         c.cg_partial.alloc(2 * ((size_t)std::max(o0, o1) / 256 + MAX_PLANES + 1));
         c.bc_ep.alloc((size_t)ob * c0 * 2);
@@ -875,14 +875,18 @@ int nntc_hip_init0_residual_channel(nntc_hip_ctx* h, nntc_model_state* st, int c
         std::vector<double> raw((size_t)entries, 0.0);
         double count = 0.0;
         const int nplanes = c.md.init0_chain ? c.planes.n : 1;
+        bool first = true;
         for (int p = 0; p < nplanes; p++)
         {
             const double w = c.md.init0_chain ? plane_weight[p] : 1.0;
             if (!(w > 0.0))
                 continue;
-            cov_partials(c, c.resid.p + c.g(p).off0 * nout, c.g(p).n0, raw, w);
+            cov_accumulate(c, c.resid.p + c.g(p).off0 * nout, c.g(p).n0, w, first);
+            first = false;
             count += w * (double)c.g(p).n0;
         }
+        if (!first)
+            cov_read(c, raw);
         std::vector<double> mu((size_t)nout, 0.0), cov((size_t)nout * nout, 0.0);
         for (int rr = 0; rr < nout; rr++)
             mu[(size_t)rr] = count > 0.0 ? raw[(size_t)rr] / count : 0.0;
@@ -914,9 +918,7 @@ int nntc_hip_init0_residual_channel(nntc_hip_ctx* h, nntc_model_state* st, int c
         for (int rr = 0; rr < nout && rr < MAX_NOUT; rr++)
             r.dir[rr] = e[(size_t)rr];
         // (4a) the divisor: the largest |projection| over the whole chain.
-        double peak = 0.0;
-        for (int p = 0; p < c.planes.n; p++)
-            peak = std::max(peak, proj_peak(c, c.resid.p + c.g(p).off0 * nout, c.g(p).n0, e.data(), mu.data()));
+        const double peak = proj_peak_chain(c, c.resid.p, e.data(), mu.data());
         r.peak = peak;
         r.divisor = peak;
         // (4b) the channel on every plane, continuous under --l0 bc8.

@@ -291,6 +291,21 @@ __global__ __launch_bounds__(256) void k_seed_channel(const float* resid, long l
     v0[t * c0 + channel] = (float)(p < -1.0 ? -1.0 : (p > 1.0 ? 1.0 : p));
 }
 
+// acc[e] (from zero when `first`) plus weight x partial[b entries + e] for b = 0 .. SITE_BLOCKS - 1, in that order: the
+// host fold of each plane's workgroup partials into the raw sums, kept on the device so the chain's planes need one
+// readback.
+__global__ __launch_bounds__(256) void k_cov_fold(const double* partial, int entries, double weight, int first,
+                                                  double* acc)
+{
+    for (int e = threadIdx.x; e < entries; e += blockDim.x)
+    {
+        double a = first ? 0.0 : acc[e];
+        for (int b = 0; b < SITE_BLOCKS; b++)
+            a += weight * partial[(size_t)b * entries + e];
+        acc[e] = a;
+    }
+}
+
 }   // namespace
 
 void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0)
@@ -431,19 +446,24 @@ void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi)
         }
 }
 
-void cov_partials(Ctx& c, const float* data, long long texels, std::vector<double>& raw, double weight)
+void cov_accumulate(Ctx& c, const float* data, long long texels, double weight, bool first)
 {
     const int entries = c.nout + c.nout * c.nout;
     hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, c.nout + 1), dim3(256), 0, c.stream, data, texels, c.nout,
                        c.cov_partial.p);
+    hipLaunchKernelGGL(k_cov_fold, dim3(1), dim3(256), 0, c.stream, (const double*)c.cov_partial.p, entries, weight,
+                       first ? 1 : 0, c.cov_partial.p + (size_t)SITE_BLOCKS * entries);
     HIP_OK(hipGetLastError());
-    const double* part = read_back(c, (const double*)c.cov_partial.p, (size_t)SITE_BLOCKS * entries);
-    for (int e = 0; e < entries; e++)
-        for (int b = 0; b < SITE_BLOCKS; b++)
-            raw[(size_t)e] += weight * part[(size_t)b * entries + e];
 }
 
-double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, const double* mu)
+void cov_read(Ctx& c, std::vector<double>& raw)
+{
+    const int entries = c.nout + c.nout * c.nout;
+    const double* h = read_back(c, (const double*)c.cov_partial.p + (size_t)SITE_BLOCKS * entries, (size_t)entries);
+    raw.assign(h, h + entries);
+}
+
+double proj_peak_chain(Ctx& c, const float* resid, const double* e, const double* mu)
 {
     ProjDir dir;
     for (int r = 0; r < MAX_NOUT; r++)
@@ -451,14 +471,22 @@ double proj_peak(Ctx& c, const float* resid, long long texels, const double* e, 
         dir.e[r] = r < c.nout ? e[r] : 0.0;
         dir.mu[r] = r < c.nout ? mu[r] : 0.0;
     }
-    const unsigned blocks = (unsigned)std::max<long long>(1, std::min<long long>(256, (texels + 255) / 256));
+    // Plane p's workgroup peaks at out[256 p ..]: every plane's launch first, one readback, the maximum on the host.
     double* out = c.cov_partial.p;
-    hipLaunchKernelGGL(k_proj_peak, dim3(blocks), dim3(256), 0, c.stream, resid, texels, c.nout, dir, out);
+    std::vector<unsigned> blocks((size_t)c.planes.n);
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        const long long texels = c.g(p).n0;
+        blocks[(size_t)p] = (unsigned)std::max<long long>(1, std::min<long long>(256, (texels + 255) / 256));
+        hipLaunchKernelGGL(k_proj_peak, dim3(blocks[(size_t)p]), dim3(256), 0, c.stream, resid + c.g(p).off0 * c.nout,
+                           texels, c.nout, dir, out + (size_t)p * 256);
+    }
     HIP_OK(hipGetLastError());
-    const double* h = read_back(c, (const double*)out, (size_t)blocks);
+    const double* h = read_back(c, (const double*)out, (size_t)c.planes.n * 256);
     double peak = 0.0;
-    for (unsigned b = 0; b < blocks; b++)
-        peak = std::max(peak, h[b]);
+    for (int p = 0; p < c.planes.n; p++)
+        for (unsigned b = 0; b < blocks[(size_t)p]; b++)
+            peak = std::max(peak, h[(size_t)p * 256 + b]);
     return peak;
 }
 

@@ -377,6 +377,66 @@ void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out
     }
 }
 
+namespace
+{
+
+// Each plane's three sums from the reduction units' partials (unit u's at partial[3u .. 3u + 2]). Serially (the exact
+// form): lanes 0-2 each add one sum's partials in unit order from zero (src/cpu/pool.h:314-324). Otherwise, in this
+// kernel's fixed order: lane t adds units t, t + 256, ..., then a tree over the lanes.
+__global__ __launch_bounds__(256) void k_objective_fold(const double* partial, Ranges units, int serial, double* out)
+{
+    const int p = blockIdx.x, t = threadIdx.x;
+    const long long a = units.start[p], b = units.start[p + 1];
+    if (serial)
+    {
+        if (t < 3)
+        {
+            double acc = 0.0;
+            long long u = a;
+            for (; u + 16 <= b; u += 16)
+            {
+                double v[16];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    v[j] = partial[(u + j) * 3 + t];
+#pragma unroll
+                for (int j = 0; j < 16; j++)
+                    acc += v[j];
+            }
+            for (; u < b; u++)
+                acc += partial[u * 3 + t];
+            out[p * 3 + t] = acc;
+        }
+        return;
+    }
+    __shared__ double s[3][256];
+    double w = 0.0, ce = 0.0, fr = 0.0;
+    for (long long u = a + t; u < b; u += 256)
+    {
+        w += partial[u * 3];
+        ce += partial[u * 3 + 1];
+        fr += partial[u * 3 + 2];
+    }
+    s[0][t] = w;
+    s[1][t] = ce;
+    s[2][t] = fr;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1)
+    {
+        if (t < h)
+        {
+            s[0][t] += s[0][t + h];
+            s[1][t] += s[1][t + h];
+            s[2][t] += s[2][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < 3)
+        out[p * 3 + t] = s[t][0];
+}
+
+}   // namespace
+
 void objective_pass(Ctx& c, int k, double* sums_out)
 {
     const bool exact = c.form == NNTC_FORM_EXACT;
@@ -426,21 +486,12 @@ void objective_pass(Ctx& c, int k, double* sums_out)
             launch_objective_tiles(c.c0, c.c1, (unsigned)total, c.stream, a);
     }
     HIP_OK(hipGetLastError());
-    const double* part = read_back(c, (const double*)c.partial.p, (size_t)total * 3);
-    // The chunk sums folded in chunk order, from zero (src/cpu/pool.h:314-324).
-    for (int p = 0; p < c.planes.n; p++)
-    {
-        double w = 0.0, ce = 0.0, fr = 0.0;
-        for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
-        {
-            w += part[(size_t)ch * 3];
-            ce += part[(size_t)ch * 3 + 1];
-            fr += part[(size_t)ch * 3 + 2];
-        }
-        sums_out[3 * p] = w;
-        sums_out[3 * p + 1] = ce;
-        sums_out[3 * p + 2] = fr;
-    }
+    hipLaunchKernelGGL(k_objective_fold, dim3((unsigned)c.planes.n), dim3(256), 0, c.stream, (const double*)c.partial.p,
+                       a.chunks, exact ? 1 : 0, c.obj.p);
+    HIP_OK(hipGetLastError());
+    const double* sums = read_back(c, (const double*)c.obj.p, (size_t)c.planes.n * 3);
+    for (int i = 0; i < c.planes.n * 3; i++)
+        sums_out[i] = sums[i];
 }
 
 static DecodeArgs decode_args(Ctx& c, int plane)
