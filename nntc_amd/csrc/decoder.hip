@@ -186,15 +186,23 @@ __device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned
     lo = (unsigned short)((r + 0x7FFFu + ((r >> 16) & 1u)) >> 16);
 }
 
+// Site s of feature f in the hi / lo tables. A matrix-core operand read takes one 16-byte group of eight sites from
+// each of 32 features; with rows of 512 bytes every such group would sit in the same LDS banks, so the groups of a row
+// are XOR-swizzled by the feature and the 32 reads spread over the banks.
+__device__ __forceinline__ int gram_slot(int f, int s)
+{
+    return f * 256 + ((((s >> 3) ^ (f & 31)) << 3) | (s & 7));
+}
+
 template <int C0, int C1, int DB>
 __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
 {
     constexpr int NIN = C1 + C0 + C0 * C1, MM = NIN + 1;
     constexpr int DP = 32 * DB;                 // features padded to whole 32-blocks
     constexpr int NT = DB * (DB + 1) / 2;       // the blocks of G's upper block triangle
-    __shared__ float s_l0[TILE_L0_FLOATS];
+    __shared__ float s_l0[tile_l0_floats<C0>()];
     __shared__ float s_l1[TILE_L1_FLOATS];
-    __shared__ __align__(16) unsigned short s_hi[DP * 256], s_lo[DP * 256];   // [feature][site of this round]
+    __shared__ __align__(16) unsigned short s_hi[DP * 256], s_lo[DP * 256];   // [feature][site], gram_slot order
     __shared__ double s_g[DP * DP];
 
     const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
@@ -237,8 +245,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
                 }
                 unsigned short hi, lo;
                 split_bf16(x, hi, lo);
-                s_hi[f * 256 + tid] = hi;
-                s_lo[f * 256 + tid] = lo;
+                s_hi[gram_slot(f, tid)] = hi;
+                s_lo[gram_slot(f, tid)] = lo;
             }
         }
         __syncthreads();
@@ -252,8 +260,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
             for (int b = 0; b < DB; b++)
             {
                 const int f = 32 * b + lane % 32;
-                hv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_hi + f * 256 + site0));
-                lv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_lo + f * 256 + site0));
+                hv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_hi + gram_slot(f, site0)));
+                lv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_lo + gram_slot(f, site0)));
             }
             int t = 0;
 #pragma unroll

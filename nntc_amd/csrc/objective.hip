@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
 {
     constexpr int NIN = C1 + C0 + C0 * C1;
     constexpr int NINP = (NIN + 3) & ~3;   // decoder rows padded to whole float4s
-    __shared__ float s_l0[TILE_L0_FLOATS];
+    __shared__ float s_l0[tile_l0_floats<C0>()];
     __shared__ float s_l1[TILE_L1_FLOATS];
     __shared__ __align__(16) float s_w[MAX_NOUT * ((MAX_NIN + 3) & ~3)];
     __shared__ float s_b[MAX_NOUT], s_cw[MAX_NOUT];

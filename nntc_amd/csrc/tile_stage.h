@@ -16,7 +16,17 @@ namespace nntc
 constexpr int TILE_W = OBJ_TW, TILE_H = OBJ_TH;
 constexpr int TILE_L0_PIX = (TILE_W + 2) * (TILE_H + 2);
 constexpr int TILE_L1_W = TILE_W + 6, TILE_L1_H = TILE_H + 6;   // bounds the level-1 box when w1 <= w0, h1 <= h0
-constexpr int TILE_L0_FLOATS = TILE_L0_PIX * (MAX_NOUT + MAX_CH);
+// Floats per staged level-0 pixel: the nout source values, then C0, padded to an odd count so that lanes reading
+// neighbouring pixels land in distinct LDS banks.
+__host__ __device__ constexpr int tile_l0_stride(int nout, int c0)
+{
+    return (nout + c0) | 1;
+}
+template <int C0>
+constexpr int tile_l0_floats()
+{
+    return TILE_L0_PIX * tile_l0_stride(MAX_NOUT, C0);
+}
 constexpr int TILE_L1_FLOATS = TILE_L1_W * TILE_L1_H * MAX_CH;
 
 // The level-1 texels the taps of sites at pixel columns [p0, p0 + n) can reach (sub-texel offsets stay within
@@ -35,7 +45,7 @@ struct TileGeom
     int tx0, ty0;            // the tile's first pixel
     int bx0, by0, bw, bh;    // the staged level-0 box (tile plus halo, clipped to the plane)
     int lx0, ly0, lw, lh;    // the staged level-1 box
-    int P;                   // floats per staged level-0 pixel: nout source values, then C0
+    int P;                   // floats per staged level-0 pixel: nout source values, then C0 (tile_l0_stride)
 };
 
 // Stage tile t (row-major over the plane's tiles) of plane g. The caller synchronises before reading.
@@ -57,7 +67,7 @@ __device__ __forceinline__ TileGeom stage_tile(const PlaneGeom& g, int t, int no
     tg.by0 = max(0, tg.ty0 - 1);
     tg.bw = min(g.w0 - 1, tg.tx0 + TILE_W) - tg.bx0 + 1;
     tg.bh = min(g.h0 - 1, tg.ty0 + TILE_H) - tg.by0 + 1;
-    tg.P = nout + C0;
+    tg.P = tile_l0_stride(nout, C0);
     const int tid = threadIdx.x, nthreads = blockDim.x;
     const float* src = src_chain + g.off0 * nout;
     const float* v0 = v0_chain + g.off0 * C0;
