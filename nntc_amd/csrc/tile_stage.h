@@ -39,6 +39,16 @@ __device__ __forceinline__ void level1_box(int p0, int n, int w0, int w1, int& l
     hi = min(w1 - 1, (int)floorf(xmax) + 2);
 }
 
+// Division by a workgroup-uniform divisor d of numerators below 2^16: q = floor(n m / 2^32) with m = ceil(2^32 / d)
+// is exact, since the error n (m d - 2^32) / (d 2^32) stays below 1 / d for n, d < 2^16. A multiply-high in place of the
+// compiler's general 32-bit division sequence in the staging loops' index arithmetic.
+struct Div16
+{
+    unsigned int d, m;
+    __device__ explicit Div16(unsigned int dd) : d(dd), m(dd > 1 ? (unsigned int)((0x100000000ull + dd - 1) / dd) : 0u) {}
+    __device__ unsigned int operator()(unsigned int n) const { return d > 1 ? __umulhi(n, m) : n; }
+};
+
 struct TileGeom
 {
     int w0, h0, w1, h1, nout;
@@ -71,15 +81,35 @@ __device__ __forceinline__ TileGeom stage_tile(const PlaneGeom& g, int t, int no
     const int tid = threadIdx.x, nthreads = blockDim.x;
     const float* src = src_chain + g.off0 * nout;
     const float* v0 = v0_chain + g.off0 * C0;
-    for (int i = tid; i < tg.bw * tg.bh * nout; i += nthreads)
+    // (The staged counts stay below 2^16: (TILE_W + 2) (TILE_H + 2) pixels of at most MAX_NOUT + MAX_CH floats.)
+    const Div16 by_row(tg.bw * nout), by_nout(nout), by_row0(tg.bw * C0);
+    // Eight loads in flight per lane before their LDS stores.
+    const int n_src = tg.bw * tg.bh * nout;
+    for (int i0 = tid; i0 < n_src; i0 += 8 * nthreads)
     {
-        const int row = i / (tg.bw * nout), rem = i - row * tg.bw * nout;
-        const int pix = rem / nout, r = rem - pix * nout;
-        s_l0[(row * tg.bw + pix) * tg.P + r] = src[((size_t)(tg.by0 + row) * g.w0 + tg.bx0) * nout + rem];
+        float v[8];
+        int at[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+        {
+            const int i = i0 + u * nthreads;
+            at[u] = -1;
+            if (i < n_src)
+            {
+                const int row = (int)by_row((unsigned)i), rem = i - row * tg.bw * nout;
+                const int pix = (int)by_nout((unsigned)rem), r = rem - pix * nout;
+                v[u] = src[((size_t)(tg.by0 + row) * g.w0 + tg.bx0) * nout + rem];
+                at[u] = (row * tg.bw + pix) * tg.P + r;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (at[u] >= 0)
+                s_l0[at[u]] = v[u];
     }
     for (int i = tid; i < tg.bw * tg.bh * C0; i += nthreads)
     {
-        const int row = i / (tg.bw * C0), rem = i - row * tg.bw * C0;
+        const int row = (int)by_row0((unsigned)i), rem = i - row * tg.bw * C0;
         const int pix = rem / C0, q = rem - pix * C0;
         s_l0[(row * tg.bw + pix) * tg.P + nout + q] = v0[((size_t)(tg.by0 + row) * g.w0 + tg.bx0) * C0 + rem];
     }
@@ -89,9 +119,10 @@ __device__ __forceinline__ TileGeom stage_tile(const PlaneGeom& g, int t, int no
     tg.lw = lx1 - tg.lx0 + 1;
     tg.lh = ly1 - tg.ly0 + 1;
     const float* v1 = v1_chain + g.off1 * C1;
+    const Div16 by_row1(tg.lw * C1);
     for (int i = tid; i < tg.lw * tg.lh * C1; i += nthreads)
     {
-        const int row = i / (tg.lw * C1), rem = i - row * tg.lw * C1;
+        const int row = (int)by_row1((unsigned)i), rem = i - row * tg.lw * C1;
         s_l1[row * tg.lw * C1 + rem] = v1[((size_t)(tg.ly0 + row) * g.w1 + tg.lx0) * C1 + rem];
     }
     return tg;
