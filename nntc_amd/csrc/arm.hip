@@ -650,6 +650,10 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
                 throw Refused { "empty plane" };
             g.n0 = (long long)g.w0 * g.h0;
             g.n1 = (long long)g.w1 * g.h1;
+            // The kernels index within a plane in 32 bits: every plane's values (at most MAX_NOUT per texel), and its
+            // sites (at most five per pixel), stay below 2^31.
+            if (g.n0 * std::max(MAX_NOUT, 5) >= (1ll << 31) || g.n1 * MAX_CH >= (1ll << 31))
+                throw Refused { "plane too large: a plane's values must number fewer than 2^31" };
             g.nb = (long long)((g.w0 + 3) / 4) * ((g.h0 + 3) / 4);
             g.off0 = o0;
             g.off1 = o1;

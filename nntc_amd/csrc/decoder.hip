@@ -176,14 +176,12 @@ struct GramArgs
     double* partial;       // tile * entries + e
 };
 
-// x as bf16 rounded to nearest even (finite inputs), and the bf16 of what it leaves.
+// x as bf16 rounded to nearest even (the hardware conversion; finite inputs), and the bf16 of what it leaves.
 __device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
 {
-    const unsigned int b = __float_as_uint(x);
-    hi = (unsigned short)((b + 0x7FFFu + ((b >> 16) & 1u)) >> 16);
+    hi = __builtin_bit_cast(unsigned short, (__bf16)x);
     const float rest = x - __uint_as_float((unsigned int)hi << 16);
-    const unsigned int r = __float_as_uint(rest);
-    lo = (unsigned short)((r + 0x7FFFu + ((r >> 16) & 1u)) >> 16);
+    lo = __builtin_bit_cast(unsigned short, (__bf16)rest);
 }
 
 // Site s of feature f in the hi / lo tables. A matrix-core operand read takes one 16-byte group of eight sites from
@@ -223,6 +221,15 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
         for (int r = 0; r < 16; r++)
             acc[t][r] = 0.0f;
 
+    // Rows past the last target (the padding of G to whole 32-blocks) stay zero in every round.
+    const int D = MM + nout;
+#pragma unroll
+    for (int f = 0; f < DP; f++)
+        if (f >= D)
+        {
+            s_hi[gram_slot(f, tid)] = 0;
+            s_lo[gram_slot(f, tid)] = 0;
+        }
     for (int j = 0; j <= a.k; j++)
     {
         // ---- this round's site of every lane, as a column of bf16 hi and lo values (zeros past the plane)
@@ -233,6 +240,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
 #pragma unroll
             for (int f = 0; f < DP; f++)
             {
+                if (f >= D)
+                    continue;
                 float x = 0.0f;
                 if (live)
                 {

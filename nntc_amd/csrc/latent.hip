@@ -173,6 +173,8 @@ struct AssembleArgs
     double* site_d;         // per site: acc (c doubles)
     Ranges sites;           // per plane: its sites, w0 h0 (1 + K)
     int nmono, c0, c1, nin, nout, k;
+    Div31 by_sites;                 // division by the sites per pixel, 1 + K
+    Div31 by_w0[MAX_PLANES];        // division by each plane's width
 };
 
 NNTC_HD void footprint_range(int ti, int n0, int n1, int& lo, int& hi)
@@ -204,7 +206,10 @@ __device__ __forceinline__ void monomials(const float* s, double* mono)
 // counts so every per-site array is indexed by constants and lives in registers; the outputs run to the cap under a
 // runtime guard. The sums per output (the residual, then each channel's acc) keep the reference's operand order
 // (src/cpu/cpu_solve_level1.cpp:262-331).
-template <bool LEVEL0, int C0, int C1, bool DIRECT>
+//
+// FAST (the fast form): the same terms with every multiply-add fused and the per-site sums in fp32 (a site's sums run
+// over at most MAX_NOUT outputs), stored as doubles; the samples blend with fused multiply-adds.
+template <bool LEVEL0, int C0, int C1, bool DIRECT, bool FAST>
 __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 {
     const long long gs = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -212,12 +217,14 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
         return;
     const int p = range_plane(a.sites, gs);
     const PlaneGeom& g = a.L.planes.p[p];
-    const long long si = gs - a.sites.start[p];
-    const int per_pixel = 1 + a.k;
-    const long long pixel = si / per_pixel;
-    const int j = (int)(si % per_pixel);
+    // A plane's sites number fewer than 2^31 (checked on the host).
+    const unsigned int si = (unsigned int)(gs - a.sites.start[p]);
+    const unsigned int per_pixel = (unsigned int)(1 + a.k);
+    const unsigned int pixel = a.by_sites.div(si);
+    const int j = (int)(si - pixel * per_pixel);
     const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
-    const int px = (int)(pixel % w0), py = (int)(pixel / w0);
+    const unsigned int pyu = a.by_w0[p].div(pixel);
+    const int px = (int)(pixel - pyu * (unsigned int)w0), py = (int)pyu;
     const int nout = a.nout, nin = a.nin;
     const float* src = a.src + g.off0 * nout;
     const float* v0 = a.v0 + g.off0 * C0;
@@ -226,21 +233,111 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
     site_delta(a.k, j, px, py, dx, dy);
     const Tap tp = tap_pixel(px, py, dx, dy, w0, h0, w1, h1);
     float sbar[C0], cbar[C1], target[MAX_NOUT];
-    sample_bilinear_v<C1>(v1, w1, C1, tp, cbar);
+    sample_bilinear_v<C1, FAST>(v1, w1, C1, tp, cbar);
     if (j == 0)
     {
-        const size_t pi = (size_t)py * w0 + px;
+        const unsigned int pi = (unsigned int)py * (unsigned int)w0 + (unsigned int)px;
         load_texel<C0>(v0 + pi * C0, C0, sbar);
-        load_texel<MAX_NOUT>(src + pi * nout, nout, target);
+        load_texel<MAX_NOUT>(src + pi * (unsigned int)nout, nout, target);
     }
     else
     {
         const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
-        sample_bilinear_v<C0>(v0, w0, C0, t0, sbar);
-        sample_bilinear_v<MAX_NOUT>(src, w0, nout, t0, target);
+        sample_bilinear_v<C0, FAST>(v0, w0, C0, t0, sbar);
+        sample_bilinear_v<MAX_NOUT, FAST>(src, w0, nout, t0, target);
     }
     constexpr int C = LEVEL0 ? C0 : C1;   // the solved latent's channels
     constexpr int NH = C * (C + 1) / 2;
+    if constexpr (FAST)
+    {
+        float accf[C], hqf[NH];
+#pragma unroll
+        for (int aa = 0; aa < C; aa++)
+            accf[aa] = 0.0f;
+#pragma unroll
+        for (int e = 0; e < NH; e++)
+            hqf[e] = 0.0f;
+#pragma unroll
+        for (int r = 0; r < MAX_NOUT; r++)
+        {
+            if (r >= nout)
+                continue;
+            const float* row = a.weights + (size_t)r * nin;
+            float out = a.bias[r];
+            float m[C];
+            if (!LEVEL0)
+            {
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                    out = fmaf(row[C1 + i], sbar[i], out);
+#pragma unroll
+                for (int q = 0; q < C1; q++)
+                {
+                    float gg = row[q];
+#pragma unroll
+                    for (int i = 0; i < C0; i++)
+                        gg = fmaf(sbar[i], row[C1 + C0 + i * C1 + q], gg);
+                    m[q] = gg;
+                    out = fmaf(gg, cbar[q], out);
+                }
+            }
+            else
+            {
+#pragma unroll
+                for (int q = 0; q < C1; q++)
+                    out = fmaf(row[q], cbar[q], out);
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                {
+                    float qq = row[C1 + i];
+#pragma unroll
+                    for (int q = 0; q < C1; q++)
+                        qq = fmaf(cbar[q], row[C1 + C0 + i * C1 + q], qq);
+                    m[i] = qq;
+                    out = fmaf(qq, sbar[i], out);
+                }
+            }
+            const float wr = a.cw[r];
+            const float t = wr * (out - target[r]);
+#pragma unroll
+            for (int aa = 0; aa < C; aa++)
+                accf[aa] = fmaf(m[aa], t, accf[aa]);
+            if (DIRECT)
+            {
+                int e = 0;
+#pragma unroll
+                for (int i = 0; i < C; i++)
+                {
+                    const float wm = wr * m[i];
+#pragma unroll
+                    for (int jj = i; jj < C; jj++, e++)
+                        hqf[e] = fmaf(wm, m[jj], hqf[e]);
+                }
+            }
+        }
+        if (DIRECT)
+        {
+#pragma unroll
+            for (int e = 0; e < NH; e++)
+                a.site_f[gs * NH + e] = hqf[e];
+        }
+        else if (!LEVEL0)
+        {
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                a.site_f[gs * C0 + i] = sbar[i];
+        }
+        else
+        {
+#pragma unroll
+            for (int q = 0; q < C1; q++)
+                a.site_f[gs * C1 + q] = cbar[q];
+        }
+#pragma unroll
+        for (int aa = 0; aa < C; aa++)
+            a.site_d[gs * C + aa] = (double)accf[aa];
+        return;
+    }
     double acc[C], hq[NH];
 #pragma unroll
     for (int aa = 0; aa < C; aa++)
@@ -328,27 +425,27 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
         a.site_d[gs * C + aa] = acc[aa];
 }
 
-template <bool LEVEL0, int C0, bool DIRECT>
+template <bool LEVEL0, int C0, bool DIRECT, bool FAST>
 void launch_site_terms_c1(int c1, dim3 grid, hipStream_t s, const AssembleArgs& a)
 {
     switch (c1)
     {
-    case 1: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 1, DIRECT>), grid, dim3(256), 0, s, a); break;
-    case 2: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 2, DIRECT>), grid, dim3(256), 0, s, a); break;
-    case 3: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 3, DIRECT>), grid, dim3(256), 0, s, a); break;
-    default: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 4, DIRECT>), grid, dim3(256), 0, s, a); break;
+    case 1: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 1, DIRECT, FAST>), grid, dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 2, DIRECT, FAST>), grid, dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 3, DIRECT, FAST>), grid, dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_site_terms<LEVEL0, C0, 4, DIRECT, FAST>), grid, dim3(256), 0, s, a); break;
     }
 }
 
-template <bool LEVEL0, bool DIRECT>
+template <bool LEVEL0, bool DIRECT, bool FAST>
 void launch_site_terms(int c0, int c1, dim3 grid, hipStream_t s, const AssembleArgs& a)
 {
     switch (c0)
     {
-    case 1: launch_site_terms_c1<LEVEL0, 1, DIRECT>(c1, grid, s, a); break;
-    case 2: launch_site_terms_c1<LEVEL0, 2, DIRECT>(c1, grid, s, a); break;
-    case 3: launch_site_terms_c1<LEVEL0, 3, DIRECT>(c1, grid, s, a); break;
-    default: launch_site_terms_c1<LEVEL0, 4, DIRECT>(c1, grid, s, a); break;
+    case 1: launch_site_terms_c1<LEVEL0, 1, DIRECT, FAST>(c1, grid, s, a); break;
+    case 2: launch_site_terms_c1<LEVEL0, 2, DIRECT, FAST>(c1, grid, s, a); break;
+    case 3: launch_site_terms_c1<LEVEL0, 3, DIRECT, FAST>(c1, grid, s, a); break;
+    default: launch_site_terms_c1<LEVEL0, 4, DIRECT, FAST>(c1, grid, s, a); break;
     }
 }
 
@@ -2096,11 +2193,13 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     a.site_f = c.site_f.p;
     a.site_d = c.site_d.p;
     a.sites.n = c.planes.n;
+    a.by_sites = Div31((unsigned int)(1 + k));
     long long acc = 0;
     for (int p = 0; p < c.planes.n; p++)
     {
         a.sites.start[p] = acc;
         acc += c.g(p).n0 * (1 + k);
+        a.by_w0[p] = Div31((unsigned int)c.g(p).w0);
     }
     a.sites.start[c.planes.n] = acc;
     const int cs = level0 ? c.c1 : c.c0;   // the other latent's channels: the sample each site stores
@@ -2115,12 +2214,15 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         const double in = 4.0 * c.n0_total * (c.nout + c.c0) + 4.0 * c.n1_total * c.c1;
         KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
                        0.0);
+        const dim3 grid(blocks_for(acc, 256));
         if (level0 && direct)
-            launch_site_terms<true, true>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
+            launch_site_terms<true, true, true>(c.c0, c.c1, grid, c.stream, a);
         else if (level0)
-            launch_site_terms<true, false>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
+            launch_site_terms<true, false, false>(c.c0, c.c1, grid, c.stream, a);
+        else if (c.form != NNTC_FORM_EXACT)
+            launch_site_terms<false, false, true>(c.c0, c.c1, grid, c.stream, a);
         else
-            launch_site_terms<false, false>(c.c0, c.c1, dim3(blocks_for(acc, 256)), c.stream, a);
+            launch_site_terms<false, false, false>(c.c0, c.c1, grid, c.stream, a);
     }
     HIP_OK(hipGetLastError());
     {

@@ -49,10 +49,23 @@ NNTC_HD int clamp_index(int i, int n)
 
 // The hardware's bilinear addressing with clamp-to-edge, for a site given as a pixel of the w0 x h0 plane being
 // fitted, into a w x h plane: exact in float when the two extents are equal.
+// One axis of it: p + d itself on the pixel's own grid, else ((p + 0.5 + d) n) / n0 - 0.5 in fp32. When n is a power of
+// two and n0 = 2 n, (p + 0.5 + d) n is exact and the quotient is exactly half of p + 0.5 + d, so a multiply by one half
+// gives the division's bits without its instruction sequence.
+NNTC_HD float tap_coord(int p, float d, int n0, int n)
+{
+    if (n == n0)
+        return (float)p + d;
+    const float a = (float)p + 0.5f + d;
+    if (n0 == 2 * n && (n & (n - 1)) == 0)
+        return a * 0.5f - 0.5f;
+    return (a * (float)n) / (float)n0 - 0.5f;
+}
+
 NNTC_HD Tap tap_pixel(int px, int py, float dx, float dy, int w0, int h0, int w, int h)
 {
-    const float x = w == w0 ? (float)px + dx : (((float)px + 0.5f + dx) * (float)w) / (float)w0 - 0.5f;
-    const float y = h == h0 ? (float)py + dy : (((float)py + 0.5f + dy) * (float)h) / (float)h0 - 0.5f;
+    const float x = tap_coord(px, dx, w0, w);
+    const float y = tap_coord(py, dy, h0, h);
     const int x0 = (int)floorf(x);
     const int y0 = (int)floorf(y);
     Tap t;
@@ -228,21 +241,29 @@ NNTC_HD void load_texel(const float* p, int c, float* out)
     }
 }
 
-// sample_bilinear_r with each tap's channels read by load_texel; the blend is the same expression, channel by channel.
-template <int MAXC>
+// sample_bilinear_r with each tap's channels read by load_texel; the blend is the same expression, channel by channel
+// (FUSED: with its last three products fused into the sum, for the fast form).
+template <int MAXC, bool FUSED = false>
 NNTC_HD void sample_bilinear_v(const float* plane, int w, int c, const Tap& t, float* out)
 {
     float a[MAXC], b[MAXC], e[MAXC], f[MAXC];
-    load_texel<MAXC>(plane + ((size_t)t.y0 * w + t.x0) * c, c, a);
-    load_texel<MAXC>(plane + ((size_t)t.y0 * w + t.x1) * c, c, b);
-    load_texel<MAXC>(plane + ((size_t)t.y1 * w + t.x0) * c, c, e);
-    load_texel<MAXC>(plane + ((size_t)t.y1 * w + t.x1) * c, c, f);
+    // 32-bit offsets: a plane holds fewer than 2^31 values (checked when the device model is created).
+    const unsigned int r0 = (unsigned int)(t.y0 * w), r1 = (unsigned int)(t.y1 * w), uc = (unsigned int)c;
+    load_texel<MAXC>(plane + (r0 + (unsigned int)t.x0) * uc, c, a);
+    load_texel<MAXC>(plane + (r0 + (unsigned int)t.x1) * uc, c, b);
+    load_texel<MAXC>(plane + (r1 + (unsigned int)t.x0) * uc, c, e);
+    load_texel<MAXC>(plane + (r1 + (unsigned int)t.x1) * uc, c, f);
     const float w00 = (1.0f - t.fx) * (1.0f - t.fy), w10 = t.fx * (1.0f - t.fy);
     const float w01 = (1.0f - t.fx) * t.fy, w11 = t.fx * t.fy;
 #pragma unroll
     for (int k = 0; k < MAXC; k++)
         if (k < c)
-            out[k] = w00 * a[k] + w10 * b[k] + w01 * e[k] + w11 * f[k];
+        {
+            if (FUSED)
+                out[k] = fmaf(w11, f[k], fmaf(w01, e[k], fmaf(w10, b[k], w00 * a[k])));
+            else
+                out[k] = w00 * a[k] + w10 * b[k] + w01 * e[k] + w11 * f[k];
+        }
 }
 
 template <int MAXC>
@@ -423,6 +444,25 @@ struct PlaneGeom
     long long n0, n1;       // texels of level 0 and of level 1
     long long off0, off1;   // this plane's first texel in the chain-concatenated level-0 / level-1 arrays
     long long nb, offb;     // BC blocks ((w0 + 3) / 4) ((h0 + 3) / 4) and the first one
+};
+
+// Exact division of 0 <= n < 2^31 by a divisor 1 <= d < 2^31 chosen on the host: q = (n m) >> s with s = 31 + ceil(log2
+// d) and m = ceil(2^s / d). With e = m d - 2^s < d <= 2^(s - 31), n m / 2^s exceeds n / d by n e / (d 2^s) < 1 / d,
+// so the floor is exact; n m < 2^31 2^33 fits in 64 bits. A multiply and a shift in place of a 64-bit division.
+struct Div31
+{
+    unsigned long long m = 1ull << 31;
+    int s = 31;
+    NNTC_HD Div31() {}
+    NNTC_HD explicit Div31(unsigned int d)
+    {
+        int l = 0;
+        while ((1ull << l) < (unsigned long long)d)
+            l++;
+        s = 31 + l;
+        m = ((1ull << s) + d - 1) / d;
+    }
+    NNTC_HD unsigned int div(unsigned int n) const { return (unsigned int)(((unsigned long long)n * m) >> s); }
 };
 
 struct Planes

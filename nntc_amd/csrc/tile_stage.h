@@ -129,6 +129,8 @@ __device__ __forceinline__ TileGeom stage_tile(const PlaneGeom& g, int t, int no
 }
 
 // Site j of pixel (px, py) of the staged tile: phi = [c (C1)][s (C0)][s_i c_q] and the target (nout values), fp32.
+// The blends are w00 a + w10 b + w01 e + w11 f with the last three products fused into the sum (these tiles serve the
+// fast form only), and the LDS offsets use 24-bit multiplies (every offset is far below 2^24).
 template <int C0, int C1>
 __device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0, const float* s_l1, int k, int j,
                                           int px, int py, float* phi, float* target)
@@ -143,15 +145,19 @@ __device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0,
         const int ya = min(max(t1.y0 - tg.ly0, 0), lh - 1), yb = min(max(t1.y1 - tg.ly0, 0), lh - 1);
         const float w00 = (1.0f - t1.fx) * (1.0f - t1.fy), w10 = t1.fx * (1.0f - t1.fy);
         const float w01 = (1.0f - t1.fx) * t1.fy, w11 = t1.fx * t1.fy;
+        const int ra = __mul24(ya, lw), rb = __mul24(yb, lw);
+        const float* la = s_l1 + (ra + xa) * C1;
+        const float* lb = s_l1 + (ra + xb) * C1;
+        const float* le = s_l1 + (rb + xa) * C1;
+        const float* lf = s_l1 + (rb + xb) * C1;
 #pragma unroll
         for (int q = 0; q < C1; q++)
-            phi[q] = w00 * s_l1[(ya * lw + xa) * C1 + q] + w10 * s_l1[(ya * lw + xb) * C1 + q] +
-                     w01 * s_l1[(yb * lw + xa) * C1 + q] + w11 * s_l1[(yb * lw + xb) * C1 + q];
+            phi[q] = fmaf(w11, lf[q], fmaf(w01, le[q], fmaf(w10, lb[q], w00 * la[q])));
     }
     const int nout = tg.nout, P = tg.P;
     if (j == 0)
     {
-        const float* pp = s_l0 + ((py - tg.by0) * tg.bw + (px - tg.bx0)) * P;
+        const float* pp = s_l0 + __mul24(__mul24(py - tg.by0, tg.bw) + (px - tg.bx0), P);
 #pragma unroll
         for (int i = 0; i < C0; i++)
             phi[C1 + i] = pp[nout + i];
@@ -163,19 +169,20 @@ __device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0,
     else
     {
         const Tap t0 = tap_pixel(px, py, dx, dy, tg.w0, tg.h0, tg.w0, tg.h0);
-        const float* pa = s_l0 + ((t0.y0 - tg.by0) * tg.bw + (t0.x0 - tg.bx0)) * P;
-        const float* pb = s_l0 + ((t0.y0 - tg.by0) * tg.bw + (t0.x1 - tg.bx0)) * P;
-        const float* pe = s_l0 + ((t0.y1 - tg.by0) * tg.bw + (t0.x0 - tg.bx0)) * P;
-        const float* pf = s_l0 + ((t0.y1 - tg.by0) * tg.bw + (t0.x1 - tg.bx0)) * P;
+        const int ra = __mul24(t0.y0 - tg.by0, tg.bw), rb = __mul24(t0.y1 - tg.by0, tg.bw);
+        const float* pa = s_l0 + __mul24(ra + (t0.x0 - tg.bx0), P);
+        const float* pb = s_l0 + __mul24(ra + (t0.x1 - tg.bx0), P);
+        const float* pe = s_l0 + __mul24(rb + (t0.x0 - tg.bx0), P);
+        const float* pf = s_l0 + __mul24(rb + (t0.x1 - tg.bx0), P);
         const float w00 = (1.0f - t0.fx) * (1.0f - t0.fy), w10 = t0.fx * (1.0f - t0.fy);
         const float w01 = (1.0f - t0.fx) * t0.fy, w11 = t0.fx * t0.fy;
 #pragma unroll
         for (int i = 0; i < C0; i++)
-            phi[C1 + i] = w00 * pa[nout + i] + w10 * pb[nout + i] + w01 * pe[nout + i] + w11 * pf[nout + i];
+            phi[C1 + i] = fmaf(w11, pf[nout + i], fmaf(w01, pe[nout + i], fmaf(w10, pb[nout + i], w00 * pa[nout + i])));
 #pragma unroll
         for (int r = 0; r < MAX_NOUT; r++)
             if (r < nout)
-                target[r] = w00 * pa[r] + w10 * pb[r] + w01 * pe[r] + w11 * pf[r];
+                target[r] = fmaf(w11, pf[r], fmaf(w01, pe[r], fmaf(w10, pb[r], w00 * pa[r])));
     }
 #pragma unroll
     for (int i = 0; i < C0; i++)
