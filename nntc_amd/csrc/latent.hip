@@ -171,10 +171,12 @@ struct AssembleArgs
     const double* mono_p;
     float* site_f;          // per site: the other latent's sample (cs floats)
     double* site_d;         // per site: acc (c doubles)
+    float* site_d32;        // the fast form's acc (c floats, fp32 sums), over the same buffer
     Ranges sites;           // per plane: its sites, w0 h0 (1 + K)
     int nmono, c0, c1, nin, nout, k;
     Div31 by_sites;                 // division by the sites per pixel, 1 + K
     Div31 by_w0[MAX_PLANES];        // division by each plane's width
+    const double* tab;              // the fast gathers' slot weights of an interior texel (k_gather_table)
 };
 
 NNTC_HD void footprint_range(int ti, int n0, int n1, int& lo, int& hi)
@@ -335,7 +337,7 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
         }
 #pragma unroll
         for (int aa = 0; aa < C; aa++)
-            a.site_d[gs * C + aa] = (double)accf[aa];
+            a.site_d32[gs * C + aa] = accf[aa];
         return;
     }
     double acc[C], hq[NH];
@@ -805,7 +807,7 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
             monomials<CS>(sv, mono);
 #pragma unroll
             for (int aa = 0; aa < C; aa++)
-                gacc[aa] = fma((double)w_self, a.site_d[s * C + aa], gacc[aa]);
+                gacc[aa] = fma((double)w_self, (double)a.site_d32[s * C + aa], gacc[aa]);
 #pragma unroll
             for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
 #pragma unroll
@@ -830,7 +832,6 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
     const long long gtx = L.texels.start[p] + ti;
     const long long sn = L.sm.sn;
     const long long opos = stencil_pos(L.sm, p, w1, tx, ty);
-    double* out_h = L.stencil + opos;
     float* out_f = L.stencil32 + opos;
     for (int e = sub; e < C * C; e += G)
     {
@@ -845,7 +846,6 @@ __global__ __launch_bounds__(128) void k_gather_fast1(AssembleArgs a)
 #pragma unroll
             for (int i = 0; i < NM; i++)
                 v = fma(sums[slot][i], mat[i], v);
-            out_h[(slot * C * C + e) * sn] = v;
             out_f[(slot * C * C + e) * sn] = (float)v;
         }
     }
@@ -871,6 +871,189 @@ void launch_gather_fast1(int cs, long long texels, hipStream_t s, const Assemble
     case 3: hipLaunchKernelGGL((k_gather_fast1<C, 3, G>), grid, dim3(128), 0, s, a); break;
     default: hipLaunchKernelGGL((k_gather_fast1<C, 4, G>), grid, dim3(128), 0, s, a); break;
     }
+}
+
+// The fast level-0 gather's weights of an interior texel: away from the plane's edges no tap is clamped, and a site's
+// taps relative to the texel (so its weight w_self and its five slot scales) depend only on the site's place in the
+// 3x3-pixel footprint and on the pixel parity that turns its sub-texel offset - the same bits for every such texel,
+// since p + d and its floor are exact. So there are four classes, the texel's (x, y) parity. Class c lists, in the walk's
+// order (pixel row, pixel, site), the sites that put weight on the texel: entry e = [code, w_self, five scales] with
+// code = 64 oy + 8 ox + j (the pixel's offset from the footprint's corner, and the site), and the list's length at
+// tab[GATHER_TAB_MAX * GATHER_TAB_STRIDE + c]. Computed by the functions the general path calls, on a texel in the
+// middle of a plane large enough that nothing clamps.
+constexpr int GATHER_TAB_STRIDE = 2 + STENCIL_BLOCKS;
+constexpr int GATHER_TAB_MAX = 4 * 9 * 5;   // four classes of up to 45 sites (K <= 4)
+
+__global__ __launch_bounds__(64) void k_gather_table(int k, double* tab)
+{
+    const int cls = threadIdx.x;
+    if (cls >= 4)
+        return;
+    const int pp = 1 + k;
+    const int tx = 16 + (cls & 1), ty = 16 + (cls >> 1);
+    int n = 0;
+    for (int oy = 0; oy < 3; oy++)
+        for (int ox = 0; ox < 3; ox++)
+            for (int j = 0; j < pp; j++)
+            {
+                const Tap t = site_taps(k, j, tx - 1 + ox, ty - 1 + oy, 64, 64, 64, 64, true);
+                double scale[STENCIL_BLOCKS];
+                float w_self;
+                if (!slot_scales(t, tx, ty, scale, w_self))
+                    continue;
+                double* out = tab + (size_t)(cls * 45 + n) * GATHER_TAB_STRIDE;
+                out[0] = (double)(64 * oy + 8 * ox + j);
+                out[1] = (double)w_self;
+                for (int q = 0; q < STENCIL_BLOCKS; q++)
+                    out[2 + q] = scale[q];
+                n++;
+            }
+    tab[GATHER_TAB_MAX * GATHER_TAB_STRIDE + cls] = (double)n;
+}
+
+// The fast form's level-0 gather over LDS tiles (C <= 2), one texel per lane: a workgroup takes a GT_W x GT_H tile of
+// texels, copies its footprint's site terms (the pixels of the tile and a one-pixel ring: rows of consecutive sites, so
+// the copy is coalesced) into LDS, and each lane walks its texel's 3x3 pixels' sites there, in the order (pixel row,
+// pixel, site), interior texels taking their sites and weights from their class's list and the others from their taps. The per-site sums
+// are fp32 values (k_site_terms' fast form), so LDS holds them as floats without loss.
+constexpr int GT_W = 32, GT_H = 8;
+
+template <int C>
+__global__ __launch_bounds__(256) void k_gather_tile0(AssembleArgs a, Ranges tiles)
+{
+    constexpr int NH = C * (C + 1) / 2;
+    constexpr int BOX = (GT_W + 2) * (GT_H + 2) * 5;   // the footprint's sites at K <= 4
+    __shared__ float s_f[BOX * NH];
+    __shared__ float s_d[BOX * C];
+    const LatentView& L = a.L;
+    const int pp = 1 + a.k;
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int p = range_plane(tiles, tile);
+    const PlaneGeom& g = L.planes.p[p];
+    const int w0 = g.w0, h0 = g.h0;
+    const int t = tile - (int)tiles.start[p];
+    const int tiles_x = (w0 + GT_W - 1) / GT_W;
+    const int tx0 = (t % tiles_x) * GT_W, ty0 = (t / tiles_x) * GT_H;
+    const int bx0 = max(0, tx0 - 1), by0 = max(0, ty0 - 1);
+    const int bw = min(w0 - 1, tx0 + GT_W) - bx0 + 1, bh = min(h0 - 1, ty0 + GT_H) - by0 + 1;
+    const int row_sites = bw * pp;
+    const long long site0 = a.sites.start[p];
+    {
+        // Box row r holds sites [((by0 + r) w0 + bx0) pp, + row_sites) of the plane, at LDS site r row_sites. Eight
+        // loads in flight per lane before their LDS stores.
+        const Div16 by_rowf(row_sites * NH), by_rowd(row_sites * C);
+        const float* gf = a.site_f + (site0 + ((long long)by0 * w0 + bx0) * pp) * NH;
+        const float* gd = a.site_d32 + (site0 + ((long long)by0 * w0 + bx0) * pp) * C;
+        const long long stride_f = (long long)w0 * pp * NH, stride_d = (long long)w0 * pp * C;
+        const int nf = bh * row_sites * NH, nd = bh * row_sites * C;
+        for (int i0 = threadIdx.x; i0 < nf || i0 < nd; i0 += 8 * 256)
+        {
+            float vf[8], vd[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+            {
+                const int i = i0 + u * 256;
+                if (i < nf)
+                {
+                    const int r = (int)by_rowf((unsigned)i);
+                    vf[u] = gf[r * stride_f + (i - r * row_sites * NH)];
+                }
+                if (i < nd)
+                {
+                    const int r = (int)by_rowd((unsigned)i);
+                    vd[u] = gd[r * stride_d + (i - r * row_sites * C)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++)
+            {
+                const int i = i0 + u * 256;
+                if (i < nf)
+                    s_f[i] = vf[u];
+                if (i < nd)
+                    s_d[i] = vd[u];
+            }
+        }
+    }
+    __syncthreads();
+
+    // Wavefront w takes the texels of parity class c = w ((x, y) parity (c & 1, c >> 1)): 16 columns x 4 rows of the
+    // tile. One class per wavefront makes the interior list the same for all its lanes (scalar reads), and the class's
+    // texels are the contiguous ones of the stencil's colour-major layout.
+    const int cls = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6), lane = (int)threadIdx.x & 63;
+    const int tx = tx0 + 2 * (lane & 15) + (cls & 1), ty = ty0 + 2 * (lane >> 4) + (cls >> 1);
+    if (tx >= w0 || ty >= h0)
+        return;
+    double hs[STENCIL_BLOCKS][NH], gacc[C];
+#pragma unroll
+    for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+#pragma unroll
+        for (int e = 0; e < NH; e++)
+            hs[sl][e] = 0.0;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        gacc[i] = 0.0;
+    auto add = [&](int site, double ws, const double* scale) {
+        const float* f = s_f + site * NH;
+        const float* d = s_d + site * C;
+#pragma unroll
+        for (int aa = 0; aa < C; aa++)
+            gacc[aa] = fma(ws, (double)d[aa], gacc[aa]);
+#pragma unroll
+        for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+#pragma unroll
+            for (int q = 0; q < NH; q++)
+                hs[sl][q] = fma(scale[sl], (double)f[q], hs[sl][q]);
+    };
+    if (tx >= 2 && tx <= w0 - 3 && ty >= 2 && ty <= h0 - 3)
+    {
+        const double* ct = a.tab + cls * 45 * GATHER_TAB_STRIDE;
+        const int n = (int)a.tab[GATHER_TAB_MAX * GATHER_TAB_STRIDE + cls];
+        const int corner = ((ty - 1 - by0) * bw + (tx - 1 - bx0)) * pp;
+        for (int v = 0; v < n; v++)
+        {
+            const double* e = ct + v * GATHER_TAB_STRIDE;
+            const int code = (int)e[0];
+            add(corner + ((code >> 6) * bw + ((code >> 3) & 7)) * pp + (code & 7), e[1], e + 2);
+        }
+    }
+    else
+    {
+        int px_lo, px_hi, py_lo, py_hi;
+        footprint_range(tx, w0, w0, px_lo, px_hi);
+        footprint_range(ty, h0, h0, py_lo, py_hi);
+        for (int py = py_lo; py <= py_hi; py++)
+            for (int px = px_lo; px <= px_hi; px++)
+                for (int j = 0; j < pp; j++)
+                {
+                    const Tap tp = site_taps(a.k, j, px, py, w0, h0, w0, h0, true);
+                    double scale[STENCIL_BLOCKS];
+                    float w_self;
+                    if (slot_scales(tp, tx, ty, scale, w_self))
+                        add(((py - by0) * bw + (px - bx0)) * pp + j, (double)w_self, scale);
+                }
+    }
+
+    // The five owned blocks, entry (i, jj) from the triangle's (min, max) element, and the gradient.
+    const long long sn = L.sm.sn;
+    const long long opos = stencil_pos(L.sm, p, w0, tx, ty);
+    float* out_f = L.stencil32 + opos;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+#pragma unroll
+        for (int jj = 0; jj < C; jj++)
+        {
+            const int lo = i < jj ? i : jj, hi = i < jj ? jj : i;
+            const int tri = lo * C - lo * (lo - 1) / 2 + (hi - lo);
+            const int e = i * C + jj;
+#pragma unroll
+            for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
+                out_f[(sl * C * C + e) * sn] = (float)hs[sl][tri];
+        }
+    const long long gtx = L.texels.start[p] + (long long)ty * w0 + tx;
+#pragma unroll
+    for (int ch = 0; ch < C; ch++)
+        L.grad[gtx * C + ch] = gacc[ch];
 }
 
 // The fast form's level-0 gather: each site stores its own block Q^T diag(cw) Q (upper triangle, fp32) in place of
@@ -927,7 +1110,7 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
             hv[e] = a.site_f[s * NH + e];
 #pragma unroll
         for (int aa = 0; aa < C; aa++)
-            gacc[aa] = fma((double)w_self, a.site_d[s * C + aa], gacc[aa]);
+            gacc[aa] = fma((double)w_self, (double)a.site_d32[s * C + aa], gacc[aa]);
 #pragma unroll
         for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
 #pragma unroll
@@ -950,7 +1133,6 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
     // (min, max) element, picked by selects (a register array indexed by a lane's value would live in scratch).
     const long long sn = L.sm.sn;
     const long long opos = stencil_pos(L.sm, p, w0, tx, ty);
-    double* out_h = L.stencil + opos;
     float* out_f = L.stencil32 + opos;
     for (int e = sub; e < C * C; e += G)
     {
@@ -964,7 +1146,6 @@ __global__ __launch_bounds__(128) void k_gather_direct(AssembleArgs a)
 #pragma unroll
             for (int q = 1; q < NH; q++)
                 v = tri == q ? hs[sl][q] : v;
-            out_h[(sl * C * C + e) * sn] = v;
             out_f[(sl * C * C + e) * sn] = (float)v;
         }
     }
@@ -1016,7 +1197,7 @@ __global__ __launch_bounds__(256) void k_diag_fast(ChunkArgs a)
     {
         const long long t = i / c - t0;
         const int q = (int)(i % c);
-        const double x = a.L.stencil[(q * c + q) * a.L.sm.sn + stencil_pos(a.L.sm, p, w, (int)(t % w), (int)(t / w))];
+        const double x = a.L.stencil32[(q * c + q) * a.L.sm.sn + stencil_pos(a.L.sm, p, w, (int)(t % w), (int)(t / w))];
         sum += x;
         lo = x < lo ? x : lo;
         hi = x > hi ? x : hi;
@@ -1278,6 +1459,7 @@ __device__ __forceinline__ void apply_precond(const float* minv, const double* r
 
 // The start of the iteration, per texel: the block inverse of (H_tt + lambda I) kept as floats (the identity where
 // it is singular), delta = 0, r = -grad, z = Minv r, p = z.
+template <bool FAST>
 __global__ __launch_bounds__(256) void k_cg_start(LatentView L)
 {
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1288,12 +1470,14 @@ __global__ __launch_bounds__(256) void k_cg_start(LatentView L)
     const int c = L.c;
     const long long t = gt - L.texels.start[p];
     const int w = L.pw[p];
-    const double* dg = L.stencil + stencil_pos(L.sm, p, w, (int)(t % w), (int)(t / w));
+    const long long pos = stencil_pos(L.sm, p, w, (int)(t % w), (int)(t / w));
     const long long sn = L.sm.sn;
     double a[MAX_CH * MAX_CH], inv[MAX_CH * MAX_CH];
+    // The fast form's gathers build only the fp32 stencil.
     for (int i = 0; i < c; i++)
         for (int j = 0; j < c; j++)
-            a[i * c + j] = dg[(i * c + j) * sn] + (i == j ? lambda : 0.0);
+            a[i * c + j] = (FAST ? (double)L.stencil32[(i * c + j) * sn + pos] : L.stencil[(i * c + j) * sn + pos]) +
+                           (i == j ? lambda : 0.0);
     if (!invert_block(a, c, inv))
         for (int i = 0; i < c; i++)
             for (int j = 0; j < c; j++)
@@ -2192,6 +2376,7 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     a.k = k;
     a.site_f = c.site_f.p;
     a.site_d = c.site_d.p;
+    a.site_d32 = reinterpret_cast<float*>(c.site_d.p);
     a.sites.n = c.planes.n;
     a.by_sites = Div31((unsigned int)(1 + k));
     long long acc = 0;
@@ -2233,7 +2418,25 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
         // The exact form gathers a texel per lane in the reference's order; the fast form spreads a level-1 texel's
         // footprint (125 sites at K = 4) over eight lanes and level 0's per-site blocks (45 sites) over two.
-        if (direct)
+        if (direct && L.c <= 2)
+        {
+            Ranges tiles;
+            tiles.n = c.planes.n;
+            long long nt = 0;
+            for (int p = 0; p < c.planes.n; p++)
+            {
+                tiles.start[p] = nt;
+                nt += (long long)((c.g(p).w0 + GT_W - 1) / GT_W) * ((c.g(p).h0 + GT_H - 1) / GT_H);
+            }
+            tiles.start[c.planes.n] = nt;
+            hipLaunchKernelGGL(k_gather_table, dim3(1), dim3(64), 0, c.stream, k, c.gather_tab.p);
+            a.tab = c.gather_tab.p;
+            if (L.c == 1)
+                hipLaunchKernelGGL(k_gather_tile0<1>, dim3((unsigned)nt), dim3(256), 0, c.stream, a, tiles);
+            else
+                hipLaunchKernelGGL(k_gather_tile0<2>, dim3((unsigned)nt), dim3(256), 0, c.stream, a, tiles);
+        }
+        else if (direct)
         {
             constexpr int G0 = 2;
             const dim3 grid(blocks_for(n * G0, 128));
@@ -2310,7 +2513,10 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             hipLaunchKernelGGL(k_dot_fast, dim3((unsigned)nch), dim3(DOT_FAST_THREADS), 0, c.stream, ch, x0, y0, x1,
                                y1);
     };
-    hipLaunchKernelGGL(k_cg_start, dim3(tb), dim3(256), 0, c.stream, L);
+    if (c.form == NNTC_FORM_EXACT)
+        hipLaunchKernelGGL(k_cg_start<false>, dim3(tb), dim3(256), 0, c.stream, L);
+    else
+        hipLaunchKernelGGL(k_cg_start<true>, dim3(tb), dim3(256), 0, c.stream, L);
     dot(L.r, L.r, L.r, L.z, 16.0 * (double)n * L.c);
     hipLaunchKernelGGL(k_fold_start, dim3(1), dim3(64), 0, c.stream, ch);
     HIP_OK(hipGetLastError());

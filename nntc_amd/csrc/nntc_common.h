@@ -465,6 +465,16 @@ struct Div31
     NNTC_HD unsigned int div(unsigned int n) const { return (unsigned int)(((unsigned long long)n * m) >> s); }
 };
 
+// Division by a workgroup-uniform divisor d of numerators below 2^16: q = floor(n m / 2^32) with m = ceil(2^32 / d)
+// is exact, since the error n (m d - 2^32) / (d 2^32) stays below 1 / d for n, d < 2^16. A multiply-high in place of the
+// compiler's general 32-bit division sequence in the staging loops' index arithmetic.
+struct Div16
+{
+    unsigned int d, m;
+    __device__ explicit Div16(unsigned int dd) : d(dd), m(dd > 1 ? (unsigned int)((0x100000000ull + dd - 1) / dd) : 0u) {}
+    __device__ unsigned int operator()(unsigned int n) const { return d > 1 ? __umulhi(n, m) : n; }
+};
+
 struct Planes
 {
     int n;

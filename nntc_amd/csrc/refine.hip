@@ -35,6 +35,7 @@ struct BcArgs
     double* delta;
     const float* prev;
     const double* stencil;  // level 0's, entry e of position s at stencil[e * sm.sn + s]
+    const float* stencil32; // its fp32 copy, the one the fast form's gathers build
     StencilMap sm;
     const double* grad;
     const float* lo0;
@@ -175,11 +176,11 @@ struct BlockPos
     }
 };
 
-// Texel u of a block, as its lane sees the block.
-template <int C0>
+// Texel u of a block, as its lane sees the block; ST is the stencil's element type (double, or the fast form's float).
+template <int C0, class ST>
 struct BlockLane
 {
-    const double* S;
+    const ST* S;
     long long sn;
     BlockPos bp;
     int u, ux, uy;
@@ -227,7 +228,7 @@ struct BlockLane
     __device__ __forceinline__ double a2(int i, int dx, int dy, int j) const
     {
         const int k = (dy + 1) * 3 + dx + 1;
-        const double* e;
+        const ST* e;
         if (dx == 0 && dy == 0)
             e = S + (long long)(i * C0 + j) * sn + pu;
         else if (dy == 1 || (dy == 0 && dx == 1))
@@ -237,7 +238,7 @@ struct BlockLane
             const long long pv = ((nbmask >> k) & 1u) ? bp.at(ux + dx, uy + dy) : pu;
             e = S + (long long)((dy == 0 ? 1 : 3 - dx) * C0 * C0 + j * C0 + i) * sn + pv;
         }
-        return *e * keep(k);
+        return (double)*e * keep(k);
     }
 
 };
@@ -333,8 +334,8 @@ __device__ __forceinline__ float level0_value(float lo, float hi, int r0, int r1
 // linear term and diagonal broadcast over the row, and on a strict decrease the move added into the rows of T's
 // neighbours. Lane u's entry (u, ii; T, c) sits in its LDS column at e[k * C0 * 64 * C0 + ii] for T = u + offset k: an
 // offset that differs from lane to lane, which in registers would take a select per slot.
-template <int C0, int T>
-__device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, const double (&a1)[C0], double (&uv)[C0],
+template <int C0, int T, class BL>
+__device__ __forceinline__ void selector_steps(const BL& L, int c, const double (&a1)[C0], double (&uv)[C0],
                                                const double* e, double diag, const float (&pal)[8], float lo, float hi,
                                                int r0, int r1, unsigned long long& sc, double& taken)
 {
@@ -385,8 +386,8 @@ __device__ __forceinline__ void selector_steps(const BlockLane<C0>& L, int c, co
 
 // The lane's entries a2(i, neighbour K, j) of the initial uv = a2 dv, e[j][i]. The geometry is made opaque first, so
 // the loads cannot be issued before this point.
-template <int C0, int K>
-__device__ __forceinline__ void uv_entries(BlockLane<C0>& L, double (&e)[C0][C0])
+template <int C0, int K, class BL>
+__device__ __forceinline__ void uv_entries(BL& L, double (&e)[C0][C0])
 {
     constexpr int dx = K % 3 - 1, dy = K / 3 - 1;
     L.opaque();
@@ -425,8 +426,8 @@ __device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (
 
 // uv = a2 dv over the nine neighbours in increasing order (the order of the reference's row sums), neighbour K + 1's
 // entries loaded while neighbour K's terms are added: two neighbours' entries in flight, not all nine.
-template <int C0, int K>
-__device__ __forceinline__ void uv_pipeline(BlockLane<C0>& L, const double (&dv)[C0], double (&uv)[C0],
+template <int C0, int K, class BL>
+__device__ __forceinline__ void uv_pipeline(BL& L, const double (&dv)[C0], double (&uv)[C0],
                                             const double (&cur)[C0][C0], double* col)
 {
     if constexpr (K < 8)
@@ -440,7 +441,7 @@ __device__ __forceinline__ void uv_pipeline(BlockLane<C0>& L, const double (&dv)
         uv_add<C0, K>(cur, dv, uv, col);
 }
 
-template <int C0>
+template <int C0, class ST>
 __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int colour_index)
 {
     // Four blocks per wavefront, one per sixteen-lane row; a row past the colour's last block leaves as a whole (the
@@ -466,10 +467,14 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     const int w0 = g.w0, h0 = g.h0;
     const long long base = g.off0;
     const int px0 = gx * 4, py0 = gy * 4;
-    const double* S = a.stencil;
+    const ST* S;
+    if constexpr (sizeof(ST) == sizeof(float))
+        S = a.stencil32;
+    else
+        S = a.stencil;
     const long long sn = a.sm.sn;
 
-    BlockLane<C0> L;   // not const: opaque() launders its fields
+    BlockLane<C0, ST> L;   // not const: opaque() launders its fields
     L.S = S;
     L.sn = sn;
     L.bp.init(a.sm.cpos[p], w0, gx, gy);
@@ -529,7 +534,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += (S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + L.pu] * xv[j]) * fkeep;
+                        acc[i] += ((double)S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + L.pu] * xv[j]) * fkeep;
             }
             const int bx = ux - dx, by = uy - dy;
             const bool bok = !(bx >= 0 && bx < 4 && by >= 0 && by < 4) && inside_xy(bx, by);
@@ -546,7 +551,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
                 for (int i = 0; i < C0; i++)
 #pragma unroll
                     for (int j = 0; j < C0; j++)
-                        acc[i] += (S[(long long)(slot * C0 * C0 + j * C0 + i) * sn + pb] * xv[j]) * bkeep;
+                        acc[i] += ((double)S[(long long)(slot * C0 * C0 + j * C0 + i) * sn + pb] * xv[j]) * bkeep;
             }
         }
 #pragma unroll
@@ -876,6 +881,16 @@ __global__ __launch_bounds__(1024) void k_bc_se_total(const double* se, long lon
         *out = s[0];
 }
 
+// The exact form reads the fp64 stencil, the fast form the fp32 copy its gathers build (the only one they build).
+template <int C0>
+void launch_refine(bool fast, dim3 grid, hipStream_t s, const BcArgs& a, const Ranges& colour, int col)
+{
+    if (fast)
+        hipLaunchKernelGGL((k_bc_refine<C0, float>), grid, dim3(64), 0, s, a, colour, col);
+    else
+        hipLaunchKernelGGL((k_bc_refine<C0, double>), grid, dim3(64), 0, s, a, colour, col);
+}
+
 BcArgs bc_args(Ctx& c)
 {
     BcArgs a;
@@ -889,6 +904,7 @@ BcArgs bc_args(Ctx& c)
     a.delta = c.lat0.x.p;
     a.prev = c.lat0.prev.p;
     a.stencil = c.lat0.stencil.p;
+    a.stencil32 = c.lat0.stencil32.p;
     a.sm = stencil_map(c, true);
     a.grad = c.lat0.grad.p;
     a.lo0 = c.grid.p;
@@ -934,22 +950,25 @@ void bc_refine_pass(Ctx& c, double* fold)
         }
         colour[col].start[c.planes.n] = acc;
     }
+    const bool fast = c.form != NNTC_FORM_EXACT;
     for (int col = 0; col < 4; col++)
     {
         const long long n = colour[col].start[colour[col].n];
         if (n == 0)
             continue;
-        // Each block reads its sixteen texels' stencil blocks, gradients, planes and corrections once.
-        const double c0 = c.c0;
-        KernelTimer kt(c.stream, "k_bc_refine", (double)n * 16.0 * (40.0 * c0 * c0 + 8.0 * c0 + 8.0 * c0 + 9.0 * c0),
+        // Each block reads its sixteen texels' five stencil blocks (8-byte entries, 4-byte in the fast form),
+        // gradients and corrections (8 bytes a channel), and planes and previous values (4 bytes a channel), and
+        // writes its planes, corrections and indices (4 + 8 + 1 bytes a channel).
+        const double c0 = c.c0, sb = fast ? 4.0 : 8.0;
+        KernelTimer kt(c.stream, "k_bc_refine", (double)n * 16.0 * (5.0 * sb * c0 * c0 + 8.0 * c0 + 8.0 * c0 + 9.0 * c0),
                        0.0);
         const dim3 grid(blocks_for(n, 4));   // four blocks per 64-lane workgroup
         switch (c.c0)
         {
-        case 1: hipLaunchKernelGGL(k_bc_refine<1>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
-        case 2: hipLaunchKernelGGL(k_bc_refine<2>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
-        case 3: hipLaunchKernelGGL(k_bc_refine<3>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
-        default: hipLaunchKernelGGL(k_bc_refine<4>, grid, dim3(64), 0, c.stream, a, colour[col], col); break;
+        case 1: launch_refine<1>(fast, grid, c.stream, a, colour[col], col); break;
+        case 2: launch_refine<2>(fast, grid, c.stream, a, colour[col], col); break;
+        case 3: launch_refine<3>(fast, grid, c.stream, a, colour[col], col); break;
+        default: launch_refine<4>(fast, grid, c.stream, a, colour[col], col); break;
         }
         HIP_OK(hipGetLastError());
     }

@@ -39,16 +39,6 @@ __device__ __forceinline__ void level1_box(int p0, int n, int w0, int w1, int& l
     hi = min(w1 - 1, (int)floorf(xmax) + 2);
 }
 
-// Division by a workgroup-uniform divisor d of numerators below 2^16: q = floor(n m / 2^32) with m = ceil(2^32 / d)
-// is exact, since the error n (m d - 2^32) / (d 2^32) stays below 1 / d for n, d < 2^16. A multiply-high in place of the
-// compiler's general 32-bit division sequence in the staging loops' index arithmetic.
-struct Div16
-{
-    unsigned int d, m;
-    __device__ explicit Div16(unsigned int dd) : d(dd), m(dd > 1 ? (unsigned int)((0x100000000ull + dd - 1) / dd) : 0u) {}
-    __device__ unsigned int operator()(unsigned int n) const { return d > 1 ? __umulhi(n, m) : n; }
-};
-
 struct TileGeom
 {
     int w0, h0, w1, h1, nout;
