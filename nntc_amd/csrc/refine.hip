@@ -334,9 +334,9 @@ __device__ __forceinline__ float level0_value(float lo, float hi, int r0, int r1
 // linear term and diagonal broadcast over the row, and on a strict decrease the move added into the rows of T's
 // neighbours. Lane u's entry (u, ii; T, c) sits in its LDS column at e[k * C0 * 64 * C0 + ii] for T = u + offset k: an
 // offset that differs from lane to lane, which in registers would take a select per slot.
-template <int C0, int T, class BL>
+template <int C0, int T, class BL, class ST>
 __device__ __forceinline__ void selector_steps(const BL& L, int c, const double (&a1)[C0], double (&uv)[C0],
-                                               const double* e, double diag, const float (&pal)[8], float lo, float hi,
+                                               const ST* e, double diag, const float (&pal)[8], float lo, float hi,
                                                int r0, int r1, unsigned long long& sc, double& taken)
 {
     constexpr int tx = T & 3, ty = T >> 2;
@@ -370,10 +370,10 @@ __device__ __forceinline__ void selector_steps(const BL& L, int c, const double 
         const int dx = tx - L.ux, dy = ty - L.uy;
         if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1)
         {
-            const double* ek = e + ((dy + 1) * 3 + dx + 1) * C0 * 64 * C0;
+            const ST* ek = e + ((dy + 1) * 3 + dx + 1) * C0 * 64 * C0;
 #pragma unroll
             for (int ii = 0; ii < C0; ii++)
-                uv[ii] += ek[ii] * move;
+                uv[ii] += (double)ek[ii] * move;
         }
         // Opaque, so the shift is not folded into the argmin's selects as eight shifted constants (which VOP3 cannot
         // encode, so each would hold a register).
@@ -401,14 +401,14 @@ __device__ __forceinline__ void uv_entries(BL& L, double (&e)[C0][C0])
 // Neighbour K's terms of the lane's rows of uv, channel by channel, with the neighbour's correction read from its lane;
 // uv is made opaque after them, so they are added before the loads that follow are issued. The entries go to the
 // lane's LDS column, col[((K C0 + j) 64) C0 + i], where every later step reads them.
-template <int C0, int K>
-__device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (&dv)[C0], double (&uv)[C0], double* col)
+template <int C0, int K, class ST>
+__device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (&dv)[C0], double (&uv)[C0], ST* col)
 {
 #pragma unroll
     for (int j = 0; j < C0; j++)
 #pragma unroll
         for (int i = 0; i < C0; i++)
-            col[((K * C0 + j) * 64) * C0 + i] = e[j][i];
+            col[((K * C0 + j) * 64) * C0 + i] = (ST)e[j][i];
     constexpr int dx = K % 3 - 1, dy = K / 3 - 1;
     double dn[C0];
 #pragma unroll
@@ -426,9 +426,9 @@ __device__ __forceinline__ void uv_add(const double (&e)[C0][C0], const double (
 
 // uv = a2 dv over the nine neighbours in increasing order (the order of the reference's row sums), neighbour K + 1's
 // entries loaded while neighbour K's terms are added: two neighbours' entries in flight, not all nine.
-template <int C0, int K, class BL>
+template <int C0, int K, class BL, class ST>
 __device__ __forceinline__ void uv_pipeline(BL& L, const double (&dv)[C0], double (&uv)[C0],
-                                            const double (&cur)[C0][C0], double* col)
+                                            const double (&cur)[C0][C0], ST* col)
 {
     if constexpr (K < 8)
     {
@@ -453,8 +453,8 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
     // step: one column per lane, entry (u, i; neighbour k, j) at [((k C0 + j) 64 + lane) C0 + i]. A lane reads only
     // its own column, so no barrier is needed. Reading the stencil again in each step instead would miss the L2 (the
     // rows a CU's wavefronts hold exceed it) and fetch the entries from HBM once per step.
-    __shared__ double s_E[9 * C0 * 64 * C0];
-    double* const col = s_E + threadIdx.x * C0;
+    __shared__ ST s_E[9 * C0 * 64 * C0];
+    ST* const col = s_E + threadIdx.x * C0;
 
     const int p = range_plane(colour, gi);
     const PlaneGeom& g = a.planes.p[p];
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
             L.opaque();
             int lane_off = (int)threadIdx.x * C0 + c * 64 * C0;
             asm volatile("" : "+v"(lane_off));
-            const double* const ec = s_E + lane_off;   // the lane's column of a2's column channel c
+            const ST* const ec = s_E + lane_off;   // the lane's column of a2's column channel c
             const double diag = ec[4 * C0 * 64 * C0 + c];
             selector_steps<C0, 0>(L, c, a1, uv, ec, diag, pal[c], a.lo0[c], a.hi0[c], ep0[c], ep1[c], sel[c], taken);
         }
