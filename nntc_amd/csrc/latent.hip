@@ -1298,19 +1298,67 @@ __global__ __launch_bounds__(64) void k_diag_chunks(ChunkArgs a)
 }
 
 // Per plane: the chunk sums in chunk order, then lambda = ridge * sum / unknowns.
-__global__ void k_diag_fold(ChunkArgs a, double ridge)
+// acc + t[0] + t[stride] + ... (n terms) in index order, sixteen loads issued ahead of their dependent additions.
+__device__ __forceinline__ double fold_strided(const double* t, long long n, int stride, double acc)
 {
-    const int p = threadIdx.x;
-    if (p >= a.chunks.n)
-        return;
-    double sum = 0.0, lo = 1e300, hi = -1e300;
-    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+    long long i = 0;
+    for (; i + 16 <= n; i += 16)
     {
-        sum += a.partial[ch * 3];
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            v[u] = t[(i + u) * stride];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            acc += v[u];
+    }
+    for (; i < n; i++)
+        acc += t[i * stride];
+    return acc;
+}
+
+// The workgroup's (256 lanes) minimum, maximum or sum of one value per lane, through LDS; every lane gets it.
+__device__ __forceinline__ double block_reduce256(double v, int op, double* scratch)
+{
+    scratch[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1)
+    {
+        if ((int)threadIdx.x < h)
+        {
+            const double o = scratch[threadIdx.x + h], m = scratch[threadIdx.x];
+            scratch[threadIdx.x] = op == 0 ? (o < m ? o : m) : (op == 1 ? (o > m ? o : m) : m + o);
+        }
+        __syncthreads();
+    }
+    const double r = scratch[0];
+    __syncthreads();
+    return r;
+}
+
+// One 256-lane workgroup per plane: the extremes over the plane's chunks in any order (they are exact), the sum in chunk
+// order from zero (the exact form) or in this kernel's fixed order (the fast form).
+__global__ __launch_bounds__(256) void k_diag_fold(ChunkArgs a, double ridge)
+{
+    __shared__ double scratch[256];
+    const int p = blockIdx.x;
+    const long long b = a.chunks.start[p], e = a.chunks.start[p + 1];
+    double sum = 0.0, lo = 1e300, hi = -1e300;
+    for (long long ch = b + threadIdx.x; ch < e; ch += 256)
+    {
+        if (!a.exact)
+            sum += a.partial[ch * 3];
         const double l = a.partial[ch * 3 + 1], h = a.partial[ch * 3 + 2];
         lo = l < lo ? l : lo;
         hi = h > hi ? h : hi;
     }
+    lo = block_reduce256(lo, 0, scratch);
+    hi = block_reduce256(hi, 1, scratch);
+    sum = a.exact ? 0.0 : block_reduce256(sum, 2, scratch);
+    if (threadIdx.x != 0)
+        return;
+    if (a.exact)
+        sum = fold_strided(a.partial + b * 3, e - b, 3, 0.0);
     double* s = a.L.scalars + (size_t)p * SC_COUNT;
     const long long unknowns = (a.L.texels.start[p + 1] - a.L.texels.start[p]) * a.L.c;
     s[SC_DIAG_SUM] = sum;
@@ -1723,23 +1771,28 @@ __global__ __launch_bounds__(64) void k_range_chunks(ChunkArgs a)
         }
 }
 
-__global__ void k_range_fold(ChunkArgs a)
+// One 256-lane workgroup per plane: each channel's extremes over the plane's chunks (exact in any order).
+__global__ __launch_bounds__(256) void k_range_fold(ChunkArgs a)
 {
-    const int p = threadIdx.x;
-    if (p >= a.chunks.n)
-        return;
+    __shared__ double scratch[256];
+    const int p = blockIdx.x;
     double* s = a.L.scalars + (size_t)p * SC_COUNT;
     for (int q = 0; q < a.L.c; q++)
     {
         double lo = 1e300, hi = -1e300;
-        for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+        for (long long ch = a.chunks.start[p] + threadIdx.x; ch < a.chunks.start[p + 1]; ch += 256)
         {
             const double l = a.partial[ch * 8 + q], h = a.partial[ch * 8 + 4 + q];
             lo = l < lo ? l : lo;
             hi = h > hi ? h : hi;
         }
-        s[SC_LO + q] = lo;
-        s[SC_HI + q] = hi;
+        lo = block_reduce256(lo, 0, scratch);
+        hi = block_reduce256(hi, 1, scratch);
+        if (threadIdx.x == 0)
+        {
+            s[SC_LO + q] = lo;
+            s[SC_HI + q] = hi;
+        }
     }
 }
 
@@ -1788,19 +1841,32 @@ __global__ __launch_bounds__(64) void k_movement_chunks(ChunkArgs a)
     }
 }
 
-__global__ void k_movement_fold(ChunkArgs a)
+// One workgroup per plane: the three sums over the plane's chunks, in chunk order from zero (the exact form; lanes 0-2
+// take one sum each) or in this kernel's fixed order (the fast form).
+__global__ __launch_bounds__(256) void k_movement_fold(ChunkArgs a)
 {
-    const int p = threadIdx.x;
-    if (p >= a.chunks.n)
-        return;
+    __shared__ double scratch[256];
+    const int p = blockIdx.x;
+    const long long b = a.chunks.start[p], e = a.chunks.start[p + 1];
     double* s = a.L.scalars + (size_t)p * SC_COUNT;
     double moved = 0.0, d2 = 0.0, p2 = 0.0;
-    for (long long ch = a.chunks.start[p]; ch < a.chunks.start[p + 1]; ch++)
+    if (a.exact)
+    {
+        if (threadIdx.x < 3)
+            s[SC_MOVED + threadIdx.x] = fold_strided(a.partial + b * 3 + threadIdx.x, e - b, 3, 0.0);
+        return;
+    }
+    for (long long ch = b + threadIdx.x; ch < e; ch += 256)
     {
         moved += a.partial[ch * 3];
         d2 += a.partial[ch * 3 + 1];
         p2 += a.partial[ch * 3 + 2];
     }
+    moved = block_reduce256(moved, 2, scratch);
+    d2 = block_reduce256(d2, 2, scratch);
+    p2 = block_reduce256(p2, 2, scratch);
+    if (threadIdx.x != 0)
+        return;
     s[SC_MOVED] = moved;
     s[SC_DELTA2] = d2;
     s[SC_PLANE2] = p2;
@@ -2476,7 +2542,7 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         else
             hipLaunchKernelGGL(k_diag_fast, dim3((unsigned)nch), dim3(256), 0, c.stream, ch);
     }
-    hipLaunchKernelGGL(k_diag_fold, dim3(1), dim3(64), 0, c.stream, ch, ridge);
+    hipLaunchKernelGGL(k_diag_fold, dim3((unsigned)c.planes.n), dim3(256), 0, c.stream, ch, ridge);
     HIP_OK(hipGetLastError());
 }
 
@@ -2629,9 +2695,9 @@ void correction_and_probe(Ctx& c, bool level0)
     const long long nch = ch.chunks.start[ch.chunks.n];
     hipLaunchKernelGGL(k_add_correction, dim3(blocks_for(n * L.c, 256)), dim3(256), 0, c.stream, L);
     hipLaunchKernelGGL(k_range_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
-    hipLaunchKernelGGL(k_range_fold, dim3(1), dim3(64), 0, c.stream, ch);
+    hipLaunchKernelGGL(k_range_fold, dim3((unsigned)c.planes.n), dim3(256), 0, c.stream, ch);
     hipLaunchKernelGGL(k_movement_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch);
-    hipLaunchKernelGGL(k_movement_fold, dim3(1), dim3(64), 0, c.stream, ch);
+    hipLaunchKernelGGL(k_movement_fold, dim3((unsigned)c.planes.n), dim3(256), 0, c.stream, ch);
     HIP_OK(hipGetLastError());
 }
 

@@ -815,16 +815,26 @@ __global__ __launch_bounds__(1024) void k_bc_fold(Planes planes, const double* t
             double pt = 0.0, pm = 0.0;
             const int b0 = ch * 64, b1 = min(b0 + 64, count);
             int bxi = b0 % cbx, byi = b0 / cbx;
-            for (int b = b0; b < b1; b++)
+            // Sixteen blocks' values loaded ahead of their (ordered) additions.
+            for (int b = b0; b < b1; b += 16)
             {
-                const double v = tp[(long long)(cy + 2 * byi) * bx + cx + 2 * bxi];
-                const bool neg = v < 0.0;
-                pt += neg ? v : 0.0;
-                pm += neg ? 1.0 : 0.0;
-                if (++bxi == cbx)
+                double v[16];
+#pragma unroll
+                for (int u = 0; u < 16; u++)
                 {
-                    bxi = 0;
-                    byi++;
+                    v[u] = b + u < b1 ? tp[(long long)(cy + 2 * byi) * bx + cx + 2 * bxi] : 0.0;
+                    if (++bxi == cbx)
+                    {
+                        bxi = 0;
+                        byi++;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 16; u++)
+                {
+                    const bool neg = v[u] < 0.0;
+                    pt += neg ? v[u] : 0.0;
+                    pm += neg ? 1.0 : 0.0;
                 }
             }
             s_t[col][t] = pt;
