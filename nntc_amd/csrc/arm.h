@@ -147,6 +147,7 @@ struct Ctx
     DevBuf<unsigned int> counters;         // per plane, MOVED_SLOTS counters MOVED_STRIDE apart
     DevBuf<double> cg_partial;             // the fast form's CG: two partials per chunk of 256 texels
     DevBuf<double> gather_tab;             // the fast level-0 gather's interior site lists (4 x 45 entries of 7)
+    int gather_tab_k = -1;                 // the K the list was built for
     DevBuf<uint8_t> bc_ep, bc_sel;         // blocks * c0 * 2, blocks * c0 * 16
     DevBuf<uint8_t> bc_blocks;             // blocks * files * 16 (file f of block b at (b * files + f) * 16)
     DevBuf<uint8_t> k0_mirror;             // the host's Model::k0 as the bc calls read it

@@ -752,6 +752,8 @@ struct SiteWalk
     }
 };
 
+constexpr int GATHER_TAB_STRIDE = 2 + STENCIL_BLOCKS;   // an interior list's entry: code, w_self, five scales
+
 // The fast form's level-1 gather: G lanes per texel take the footprint's sites in turn (site j fastest, then pixel,
 // then pixel row), each adding its five slots' terms branch-free with fused multiply-adds; the group's sums meet in a
 // fixed butterfly (deterministic).
@@ -881,7 +883,6 @@ void launch_gather_fast1(int cs, long long texels, hipStream_t s, const Assemble
 // code = 64 oy + 8 ox + j (the pixel's offset from the footprint's corner, and the site), and the list's length at
 // tab[GATHER_TAB_MAX * GATHER_TAB_STRIDE + c]. Computed by the functions the general path calls, on a texel in the
 // middle of a plane large enough that nothing clamps.
-constexpr int GATHER_TAB_STRIDE = 2 + STENCIL_BLOCKS;
 constexpr int GATHER_TAB_MAX = 4 * 9 * 5;   // four classes of up to 45 sites (K <= 4)
 
 __global__ __launch_bounds__(64) void k_gather_table(int k, double* tab)
@@ -2495,7 +2496,11 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
                 nt += (long long)((c.g(p).w0 + GT_W - 1) / GT_W) * ((c.g(p).h0 + GT_H - 1) / GT_H);
             }
             tiles.start[c.planes.n] = nt;
-            hipLaunchKernelGGL(k_gather_table, dim3(1), dim3(64), 0, c.stream, k, c.gather_tab.p);
+            if (c.gather_tab_k != k)
+            {
+                hipLaunchKernelGGL(k_gather_table, dim3(1), dim3(64), 0, c.stream, k, c.gather_tab.p);
+                c.gather_tab_k = k;
+            }
             a.tab = c.gather_tab.p;
             if (L.c == 1)
                 hipLaunchKernelGGL(k_gather_tile0<1>, dim3((unsigned)nt), dim3(256), 0, c.stream, a, tiles);

@@ -49,16 +49,16 @@ NNTC_HD int clamp_index(int i, int n)
 
 // The hardware's bilinear addressing with clamp-to-edge, for a site given as a pixel of the w0 x h0 plane being
 // fitted, into a w x h plane: exact in float when the two extents are equal.
-// One axis of it: p + d itself on the pixel's own grid, else ((p + 0.5 + d) n) / n0 - 0.5 in fp32. When n is a power of
-// two and n0 = 2 n, (p + 0.5 + d) n is exact and the quotient is exactly half of p + 0.5 + d, so a multiply by one half
-// gives the division's bits without its instruction sequence.
+// One axis of it: p + d itself on the pixel's own grid, else ((p + 0.5 + d) n) / n0 - 0.5 in fp32. When n and n0 are
+// both powers of two, (p + 0.5 + d) n is exact and so is its quotient by n0, a power-of-two scaling of p + 0.5 + d: a
+// scaling by 2^(log2 n - log2 n0) (an exponent adjustment) gives the division's bits without its instruction sequence.
 NNTC_HD float tap_coord(int p, float d, int n0, int n)
 {
     if (n == n0)
         return (float)p + d;
     const float a = (float)p + 0.5f + d;
-    if (n0 == 2 * n && (n & (n - 1)) == 0)
-        return a * 0.5f - 0.5f;
+    if ((n & (n - 1)) == 0 && (n0 & (n0 - 1)) == 0)
+        return ldexpf(a, __builtin_ctz((unsigned)n) - __builtin_ctz((unsigned)n0)) - 0.5f;
     return (a * (float)n) / (float)n0 - 0.5f;
 }
 
