@@ -2751,10 +2751,11 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
             const long long n = colour[col].start[colour[col].n];
             if (n == 0)
                 continue;
-            // A colour's texels read their own stencil, gradient and plane and their neighbours' corrections.
-            const double cc = L.c;
+            // A colour's texels read their own stencil (five C x C blocks, 8-byte entries, 4-byte in the fast form),
+            // gradient, nine corrections (their own and their eight neighbours') and plane and index.
+            const double cc = L.c, sb = c.form == NNTC_FORM_EXACT ? 8.0 : 4.0;
             KernelTimer kt(c.stream, level0 ? "k_quant_sweep0" : "k_quant_sweep1",
-                           (double)n * (40.0 * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc), 0.0);
+                           (double)n * (5.0 * sb * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc), 0.0);
             const dim3 grid(blocks_for(n, 256));
             // The exact form reads the fp64 stencil; the fast form its fp32 copy (half the bytes).
             if (c.form == NNTC_FORM_EXACT)
