@@ -1907,8 +1907,7 @@ __device__ __forceinline__ const float* stencil_of<float>(const LatentView& L)
 // One colour of one sweep: every texel (tx, ty) of the colour, the neighbours' share of (H delta)_t once, then its
 // channels in turn, each one's exact grid argmin using the values the earlier channels have just taken.
 template <int C, class T>
-__global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour, int colour_index,
-                                                     unsigned int* changed)
+__device__ __forceinline__ void quant_sweep(LatentView L, Ranges colour, int colour_index, unsigned int* changed)
 {
     const long long gi = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool live = gi < colour.start[colour.n];
@@ -1936,6 +1935,13 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
     }
 }
 
+template <int C, class T>
+__global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour, int colour_index,
+                                                     unsigned int* changed)
+{
+    quant_sweep<C, T>(L, colour, colour_index, changed);
+}
+
 // One neighbour group of a texel's sweep: the neighbour's correction, the stencil block that couples it to the texel
 // (entry (i, j) at s[i * C + j], transposed for a backward neighbour, whose block it is) and a 0/1 weight. Every load is
 // unconditional: a neighbour outside the plane reads the texel's own entries (a valid address, finite values) and
@@ -1945,7 +1951,8 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
 template <int C, class T = double>
 struct SweepNeighbour
 {
-    double x[C], s[C * C], keep;
+    double x[C], keep;
+    T s[C * C];   // the stencil entries as stored (converted where they are used)
 
     // Group g: slot 1 + g / 2, forward for even g, backward for odd g.
     __device__ __forceinline__ void load(const LatentView& L, const T* S0, long long sn, const double* delta0,
@@ -1972,7 +1979,7 @@ struct SweepNeighbour
                 x[j] = xn[j];
 #pragma unroll
             for (int e = 0; e < C * C; e++)
-                s[e] = (double)S[(slot * C * C + e) * sn + pos];
+                s[e] = S[(slot * C * C + e) * sn + pos];
         }
         else
         {
@@ -1992,7 +1999,7 @@ struct SweepNeighbour
             for (int i = 0; i < C; i++)
 #pragma unroll
                 for (int j = 0; j < C; j++)
-                    s[i * C + j] = (double)S[(slot * C * C + j * C + i) * sn + npos];
+                    s[i * C + j] = S[(slot * C * C + j * C + i) * sn + npos];
         }
     }
 
@@ -2003,7 +2010,7 @@ struct SweepNeighbour
 #pragma unroll
             for (int j = 0; j < C; j++)
             {
-                const double term = s[i * C + j] * x[j];
+                const double term = (double)s[i * C + j] * x[j];
                 nb[i] += term * keep;
             }
     }
