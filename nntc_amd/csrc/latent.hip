@@ -1404,8 +1404,9 @@ __global__ __launch_bounds__(64) void k_dot_chunks(ChunkArgs a, const double* x0
 // The fast form's two inner products: one 256-lane workgroup per chunk, each lane's own strided values, then trees.
 constexpr int DOT_FAST_THREADS = 256;
 
-__global__ __launch_bounds__(DOT_FAST_THREADS) void k_dot_fast(ChunkArgs a, const double* x0, const double* y0,
-                                                               const double* x1, const double* y1)
+template <class V>
+__global__ __launch_bounds__(DOT_FAST_THREADS) void k_dot_fast(ChunkArgs a, const V* x0, const V* y0, const V* x1,
+                                                               const V* y1)
 {
     __shared__ double part[DOT_FAST_THREADS / 64][2];
     int p;
@@ -1416,9 +1417,9 @@ __global__ __launch_bounds__(DOT_FAST_THREADS) void k_dot_fast(ChunkArgs a, cons
     double s0 = 0.0, s1 = 0.0;
     for (long long i = vb + threadIdx.x; i < ve; i += DOT_FAST_THREADS)
     {
-        s0 += x0[i] * y0[i];
+        s0 += (double)x0[i] * (double)y0[i];
         if (x1)
-            s1 += x1[i] * y1[i];
+            s1 += (double)x1[i] * (double)y1[i];
     }
     s0 = tree_lanes(s0);
     s1 = tree_lanes(s1);
@@ -1492,18 +1493,29 @@ __device__ bool invert_block(double* a, int n, double* inv)
     return true;
 }
 
-__device__ __forceinline__ void apply_precond(const float* minv, const double* r, int c, double* z, long long t)
+template <class V>
+__device__ __forceinline__ void apply_precond(const float* minv, const V* r, int c, V* z, long long t)
 {
     const float* a = minv + t * (c * c);
-    const double* x = r + t * c;
-    double* o = z + t * c;
+    const V* x = r + t * c;
+    V* o = z + t * c;
     for (int i = 0; i < c; i++)
     {
         double acc = 0.0;
         for (int j = 0; j < c; j++)
-            acc += (double)a[i * c + j] * x[j];
-        o[i] = acc;
+            acc += (double)a[i * c + j] * (double)x[j];
+        o[i] = (V)acc;
     }
+}
+
+// The fast form's conjugate-gradient vectors r, z, p and Ap are fp32 (x, the solution, stays fp64), held in the first
+// half of the fp64 buffers; every product and sum on them is formed in fp64.
+template <bool FAST>
+using CgVec = typename std::conditional<FAST, float, double>::type;
+template <bool FAST>
+__device__ __forceinline__ CgVec<FAST>* cg_vec(double* v)
+{
+    return reinterpret_cast<CgVec<FAST>*>(v);
 }
 
 // The start of the iteration, per texel: the block inverse of (H_tt + lambda I) kept as floats (the identity where
@@ -1534,14 +1546,17 @@ __global__ __launch_bounds__(256) void k_cg_start(LatentView L)
     float* o = L.precond + gt * (c * c);
     for (int i = 0; i < c * c; i++)
         o[i] = (float)inv[i];
+    CgVec<FAST>* r = cg_vec<FAST>(L.r);
+    CgVec<FAST>* z = cg_vec<FAST>(L.z);
+    CgVec<FAST>* pv = cg_vec<FAST>(L.p);
     for (int i = 0; i < c; i++)
     {
         L.x[gt * c + i] = 0.0;
-        L.r[gt * c + i] = -L.grad[gt * c + i];
+        r[gt * c + i] = (CgVec<FAST>)(-L.grad[gt * c + i]);
     }
-    apply_precond(L.precond, L.r, c, L.z, gt);
+    apply_precond(L.precond, r, c, z, gt);
     for (int i = 0; i < c; i++)
-        L.p[gt * c + i] = L.z[gt * c + i];
+        pv[gt * c + i] = z[gt * c + i];
 }
 
 // y_t = (H_tt + lambda I) x_t + the eight neighbours' blocks, the four the texel does not own read as the transpose of
@@ -1948,14 +1963,14 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
 // weighs its terms by an opaque 0.0 (1.0 inside). x * 1.0 is x, and x * 0.0 is a zero that leaves the sum unchanged
 // (the sum starts at +0.0 and so is never -0.0); the weight is opaque so that the compiler cannot turn it back into a
 // branch around the loads, which would serialise them.
-template <int C, class T = double>
+template <int C, class T = double, class V = double>
 struct SweepNeighbour
 {
     double x[C], keep;
     T s[C * C];   // the stencil entries as stored (converted where they are used)
 
-    // Group g: slot 1 + g / 2, forward for even g, backward for odd g.
-    __device__ __forceinline__ void load(const LatentView& L, const T* S0, long long sn, const double* delta0,
+    // Group g: slot 1 + g / 2, forward for even g, backward for odd g; V is the neighbours' vector type.
+    __device__ __forceinline__ void load(const LatentView& L, const T* S0, long long sn, const V* delta0,
                                          const long long (&cbase)[4], int g, int tx, int ty, int w, int h,
                                          long long base, long long t, long long pos)
     {
@@ -1963,7 +1978,7 @@ struct SweepNeighbour
         long long z = 0;
         asm volatile("" : "+v"(z));
         const T* S = S0 + z;
-        const double* delta = delta0 + z;
+        const V* delta = delta0 + z;
         int dx, dy;
         stencil_offset(1 + g / 2, dx, dy);
         const int slot = 1 + g / 2;
@@ -1973,10 +1988,10 @@ struct SweepNeighbour
             const bool ok = fx >= 0 && fx < w && fy >= 0 && fy < h;
             keep = ok ? 1.0 : 0.0;
             asm volatile("" : "+v"(keep));
-            const double* xn = delta + (ok ? base + (long long)fy * w + fx : t) * C;
+            const V* xn = delta + (ok ? base + (long long)fy * w + fx : t) * C;
 #pragma unroll
             for (int j = 0; j < C; j++)
-                x[j] = xn[j];
+                x[j] = (double)xn[j];
 #pragma unroll
             for (int e = 0; e < C * C; e++)
                 s[e] = S[(slot * C * C + e) * sn + pos];
@@ -1991,10 +2006,10 @@ struct SweepNeighbour
             const int ncx = bxs & 1, ncy = bys & 1;
             const long long nbase = ncy ? (ncx ? cbase[3] : cbase[2]) : (ncx ? cbase[1] : cbase[0]);
             const long long npos = nbase + (long long)(bys >> 1) * ((w - ncx + 1) >> 1) + (bxs >> 1);
-            const double* xn = delta + (base + (long long)bys * w + bxs) * C;
+            const V* xn = delta + (base + (long long)bys * w + bxs) * C;
 #pragma unroll
             for (int j = 0; j < C; j++)
-                x[j] = xn[j];
+                x[j] = (double)xn[j];
 #pragma unroll
             for (int i = 0; i < C; i++)
 #pragma unroll
@@ -2175,6 +2190,26 @@ __device__ __forceinline__ double fold_partials(const FastCg& a, int p, int k, d
     return block_sum256(v, s4);
 }
 
+// p = z + beta p on the fast form's fp32 vectors.
+__global__ __launch_bounds__(256) void k_xpby_fast(LatentView L)
+{
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gt >= L.texels.start[L.texels.n])
+        return;
+    const int p = range_plane(L.texels, gt);
+    const double* s = L.scalars + (size_t)p * SC_COUNT;
+    if (s[SC_ACTIVE] == 0.0)
+        return;
+    const double beta = s[SC_BETA];
+    float* pv = cg_vec<true>(L.p);
+    const float* z = cg_vec<true>(L.z);
+    for (int i = 0; i < L.c; i++)
+    {
+        const long long e = gt * L.c + i;
+        pv[e] = (float)((double)z[e] + beta * (double)pv[e]);
+    }
+}
+
 // Ap for the chunk's texels (the mat-vec of k_matvec, its neighbour loads pipelined as in the quantised sweep), and the
 // chunk's p.Ap.
 template <int C>
@@ -2206,10 +2241,11 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
             cbase[k] = L.sm.cpos[p][k];
             asm volatile("" : "+v"(cbase[k]));
         }
+        const float* pf = cg_vec<true>(L.p);
         double xt[C], acc[C];
 #pragma unroll
         for (int j = 0; j < C; j++)
-            xt[j] = L.p[gt * C + j];
+            xt[j] = (double)pf[gt * C + j];
 #pragma unroll
         for (int i = 0; i < C; i++)
         {
@@ -2220,12 +2256,12 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
             acc[i] = sacc;
         }
         constexpr int D = C <= 2 ? 4 : 2;
-        SweepNeighbour<C, float> q[8];
+        SweepNeighbour<C, float, float> q[8];
 #pragma unroll
         for (int g = 0; g < 8 + D; g++)
         {
             if (g < 8)
-                q[g].load(L, S, sn, L.p, cbase, g, tx, ty, w, h, base, gt, pos);
+                q[g].load(L, S, sn, pf, cbase, g, tx, ty, w, h, base, gt, pos);
             if (g >= D)
             {
                 q[g - D].add_to(acc);
@@ -2237,7 +2273,7 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
 #pragma unroll
         for (int i = 0; i < C; i++)
         {
-            L.ap[gt * C + i] = acc[i];
+            cg_vec<true>(L.ap)[gt * C + i] = (float)acc[i];
             pap += xt[i] * acc[i];
         }
     }
@@ -2278,18 +2314,22 @@ __global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
     {
         const double alpha = sc[SC_ALPHA];
         const int c = L.c;
+        float* r = cg_vec<true>(L.r);
+        float* z = cg_vec<true>(L.z);
+        const float* pv = cg_vec<true>(L.p);
+        const float* ap = cg_vec<true>(L.ap);
         for (int i = 0; i < c; i++)
         {
             const long long e = gt * c + i;
-            L.x[e] += alpha * L.p[e];
-            L.r[e] += -alpha * L.ap[e];
+            L.x[e] += alpha * (double)pv[e];
+            r[e] = (float)((double)r[e] - alpha * (double)ap[e]);
         }
-        apply_precond(L.precond, L.r, c, L.z, gt);
+        apply_precond(L.precond, r, c, z, gt);
         for (int i = 0; i < c; i++)
         {
-            const double ri = L.r[gt * c + i];
+            const double ri = r[gt * c + i];
             rr += ri * ri;
-            rz += ri * L.z[gt * c + i];
+            rz += ri * (double)z[gt * c + i];
         }
     }
     rr = block_sum256(rr, s4);
@@ -2588,14 +2628,22 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
         if (ch.exact)
             hipLaunchKernelGGL(k_dot_chunks, dim3((unsigned)nch), dim3(64), 0, c.stream, ch, x0, y0, x1, y1);
         else
-            hipLaunchKernelGGL(k_dot_fast, dim3((unsigned)nch), dim3(DOT_FAST_THREADS), 0, c.stream, ch, x0, y0, x1,
-                               y1);
+            hipLaunchKernelGGL(k_dot_fast<double>, dim3((unsigned)nch), dim3(DOT_FAST_THREADS), 0, c.stream, ch, x0,
+                               y0, x1, y1);
     };
     if (c.form == NNTC_FORM_EXACT)
+    {
         hipLaunchKernelGGL(k_cg_start<false>, dim3(tb), dim3(256), 0, c.stream, L);
+        dot(L.r, L.r, L.r, L.z, 16.0 * (double)n * L.c);
+    }
     else
+    {
+        // The fast form's r and z are fp32 (CgVec).
         hipLaunchKernelGGL(k_cg_start<true>, dim3(tb), dim3(256), 0, c.stream, L);
-    dot(L.r, L.r, L.r, L.z, 16.0 * (double)n * L.c);
+        KernelTimer kt(c.stream, level0 ? "k_dot0" : "k_dot1", 8.0 * (double)n * L.c, 0.0);
+        const float *r = reinterpret_cast<const float*>(L.r), *z = reinterpret_cast<const float*>(L.z);
+        hipLaunchKernelGGL(k_dot_fast<float>, dim3((unsigned)nch), dim3(DOT_FAST_THREADS), 0, c.stream, ch, r, r, r, z);
+    }
     hipLaunchKernelGGL(k_fold_start, dim3(1), dim3(64), 0, c.stream, ch);
     HIP_OK(hipGetLastError());
 
@@ -2627,8 +2675,10 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
         if (!ch.exact)
         {
             fc.probe = (it + 1) % CG_PROBE == 0;
+            // The fast form's bytes: the fp32 stencil (20 c^2 per texel), p read and Ap written (4 c each); the update
+            // reads the preconditioner, x, p, r, Ap and writes x, r, z; p = z + beta p reads two and writes one.
             {
-                KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", b_matvec + 16.0 * nv,
+                KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", (double)n * 20.0 * cc * cc + 8.0 * nv,
                                (double)n * (2.0 * 9.0 * cc * cc));
                 switch (L.c)
                 {
@@ -2640,13 +2690,13 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             }
             hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
             {
-                KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
+                KernelTimer kt(c.stream, "k_cg_update", (double)n * 4.0 * cc * cc + 44.0 * nv, nv * (4.0 + 2.0 * cc));
                 hipLaunchKernelGGL(k_cg_update_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
             }
             hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
             {
-                KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);
-                hipLaunchKernelGGL(k_xpby, dim3(tb), dim3(256), 0, c.stream, L);
+                KernelTimer kt(c.stream, "k_xpby", 12.0 * nv, 2.0 * nv);
+                hipLaunchKernelGGL(k_xpby_fast, dim3(tb), dim3(256), 0, c.stream, L);
             }
             HIP_OK(hipGetLastError());
         }
