@@ -2048,7 +2048,9 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
     const long long sn = L.sm.sn;
     const T* S = stencil_of<T>(L);
     const long long pos = L.sm.cpos[p][colour_index] + ci;
-    double* delta = L.x;
+    // The corrections in the stencil's type: fp64, or fp32 in the fast form (in the first half of the fp64 buffer,
+    // which k_copy_prev has zeroed).
+    T* delta = reinterpret_cast<T*>(L.x);
 
     // The plane's four colour bases in the stencil's runs, read once: a neighbour's position is then arithmetic and
     // selects, not a dependent load (opaque, so the selects are not folded back into an indexed load).
@@ -2067,7 +2069,7 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
     // terms, and volatile asm statements keep their order. The terms reach each sum slot by slot, forward then
     // backward, channel by channel, as in a loop over the neighbours.
     constexpr int D = C <= 2 ? 4 : 2;
-    SweepNeighbour<C, T> q[8];
+    SweepNeighbour<C, T, T> q[8];
     double nb[C], dl[C], own[C * C], gr[C];
     float pvs[C];
     uint8_t kold[C];
@@ -2086,7 +2088,7 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
             for (int j = 0; j < C; j++)
             {
                 kold[j] = L.k[t * C + j];
-                dl[j] = delta[t * C + j];
+                dl[j] = (double)delta[t * C + j];
                 gr[j] = L.grad[t * C + j];
                 pvs[j] = L.prev[t * C + j];
             }
@@ -2132,7 +2134,7 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
         }
         L.v[idx] = value;
         dl[q] = (double)value - (double)pv;
-        delta[idx] = dl[q];
+        delta[idx] = (T)dl[q];
     }
     return moved;
 }
@@ -2808,11 +2810,11 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
             const long long n = colour[col].start[colour[col].n];
             if (n == 0)
                 continue;
-            // A colour's texels read their own stencil (five C x C blocks, 8-byte entries, 4-byte in the fast form),
-            // gradient, nine corrections (their own and their eight neighbours') and plane and index.
+            // A colour's texels read their own stencil (five C x C blocks) and nine corrections (their own and their
+            // eight neighbours'), 8-byte values or 4-byte in the fast form, and their gradient, plane and index.
             const double cc = L.c, sb = c.form == NNTC_FORM_EXACT ? 8.0 : 4.0;
             KernelTimer kt(c.stream, level0 ? "k_quant_sweep0" : "k_quant_sweep1",
-                           (double)n * (5.0 * sb * cc * cc + 8.0 * cc + 9.0 * 8.0 * cc + 9.0 * cc), 0.0);
+                           (double)n * (5.0 * sb * cc * cc + 8.0 * cc + 9.0 * sb * cc + 9.0 * cc), 0.0);
             const dim3 grid(blocks_for(n, 256));
             // The exact form reads the fp64 stencil; the fast form its fp32 copy (half the bytes).
             if (c.form == NNTC_FORM_EXACT)
