@@ -22,6 +22,7 @@
 #include "arm.h"
 
 #include <cmath>
+#include <type_traits>
 
 namespace nntc
 {
@@ -1957,6 +1958,167 @@ __global__ __launch_bounds__(256) void k_quant_sweep(LatentView L, Ranges colour
     quant_sweep<C, T>(L, colour, colour_index, changed);
 }
 
+// Lane q of a quad's double, to every lane of the quad (DPP quad_perm [q, q, q, q]).
+template <int Q>
+__device__ __forceinline__ double quad_bcast(double x)
+{
+    constexpr int ctrl = Q | (Q << 2) | (Q << 4) | (Q << 6);
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), ctrl, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), ctrl, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
+// The fast form's sweep at C >= 3 (level 1): four lanes per texel, lane i of the quad summing row i of the neighbours'
+// share (the groups in order, the columns in order: row i of the one-lane form's sums), so a colour of the small
+// level-1 planes runs four times the wavefronts, each with a quarter of the loads. The channels then move in turn:
+// lane q takes channel q and hands its correction to the quad by a DPP broadcast. With C = 3 the fourth lane mirrors
+// row 2 and writes nothing.
+template <int C>
+__global__ __launch_bounds__(256) void k_quant_sweep_rows(LatentView L, Ranges colour, int colour_index,
+                                                          unsigned int* changed)
+{
+    static_assert(C >= 3 && C <= 4, "four lanes hold at most four channel rows");
+    const long long gq = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const long long total = colour.start[colour.n];
+    const long long gi = gq >> 2;
+    const int lane_row = (int)(gq & 3), i = lane_row < C ? lane_row : C - 1;
+    // Every lane of a quad reaches the broadcasts: a quad past the end works on the last texel and writes nothing.
+    const bool live = gi < total;
+    const long long gt_i = live ? gi : total - 1;
+    const int p = range_plane(colour, gt_i);
+    const long long ci = gt_i - colour.start[p];
+    const int w = L.pw[p], h = L.ph[p];
+    const int cx = colour_index & 1, cy = colour_index >> 1;
+    const int cw1 = (w - cx + 1) / 2;
+    const int tx = cx + 2 * (int)(ci % cw1);
+    const int ty = cy + 2 * (int)(ci / cw1);
+    const long long base = L.texels.start[p];
+    const long long t = base + (long long)ty * w + tx;
+    const double lambda = L.scalars[(size_t)p * SC_COUNT + SC_LAMBDA];
+    const long long sn = L.sm.sn;
+    const float* S = L.stencil32;
+    const long long pos = L.sm.cpos[p][colour_index] + ci;
+    float* delta = reinterpret_cast<float*>(L.x);
+
+    // Row i of each neighbour group's block and the neighbour's corrections, all loads first.
+    float sr[8][C], xv[8][C];
+    double keep[8];
+#pragma unroll
+    for (int g = 0; g < 8; g++)
+    {
+        const int slot = 1 + g / 2;
+        int dx, dy;
+        stencil_offset(slot, dx, dy);
+        if (g % 2 == 0)
+        {
+            const int fx = tx + dx, fy = ty + dy;
+            const bool ok = fx >= 0 && fx < w && fy >= 0 && fy < h;
+            keep[g] = ok ? 1.0 : 0.0;
+            const float* xn = delta + (ok ? base + (long long)fy * w + fx : t) * C;
+#pragma unroll
+            for (int j = 0; j < C; j++)
+            {
+                xv[g][j] = xn[j];
+                sr[g][j] = S[(slot * C * C + i * C + j) * sn + pos];
+            }
+        }
+        else
+        {
+            const int bx = tx - dx, by = ty - dy;
+            const bool ok = bx >= 0 && bx < w && by >= 0 && by < h;
+            keep[g] = ok ? 1.0 : 0.0;
+            const int bxs = ok ? bx : tx, bys = ok ? by : ty;
+            const int ncx = bxs & 1, ncy = bys & 1;
+            const long long npos = L.sm.cpos[p][ncy * 2 + ncx] + (long long)(bys >> 1) * ((w - ncx + 1) >> 1) + (bxs >> 1);
+            const float* xn = delta + (base + (long long)bys * w + bxs) * C;
+#pragma unroll
+            for (int j = 0; j < C; j++)
+            {
+                xv[g][j] = xn[j];
+                sr[g][j] = S[(slot * C * C + j * C + i) * sn + npos];
+            }
+        }
+    }
+    float own[C];
+    double dl[C];
+#pragma unroll
+    for (int j = 0; j < C; j++)
+    {
+        own[j] = S[(i * C + j) * sn + pos];
+        dl[j] = (double)delta[t * C + j];
+    }
+    const double gr = L.grad[t * C + i];
+    const float pv = L.prev[t * C + i];
+    const int kold = L.k[t * C + i];
+    double nb = 0.0;
+#pragma unroll
+    for (int g = 0; g < 8; g++)
+#pragma unroll
+        for (int j = 0; j < C; j++)
+        {
+            const double term = (double)sr[g][j] * (double)xv[g][j];
+            nb += term * keep[g];
+        }
+
+    // The channels in order, each seeing the ones before it already moved.
+    unsigned int moved = 0;
+    const float l = L.lo[i], span = L.hi[i] - L.lo[i];
+    auto step = [&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        double mine = dl[q];
+        if (lane_row == q && live)
+        {
+            const double aq = (double)own[q] + lambda;
+            if (aq > 0.0 && span > 0.0f)
+            {
+                double rest = nb;
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    if (j != q)
+                        rest += (double)own[j] * dl[j];
+                const double bq = gr + rest;
+                const double want = (double)pv - bq / aq;
+                const int kk = level1_index(l, L.hi[i], L.bits, (float)want);
+                const float value = level1_value(l, L.hi[i], L.bits, kk);
+                const long long idx = t * C + q;
+                if (kold != kk)
+                {
+                    L.k[idx] = (uint8_t)kk;
+                    moved++;
+                }
+                L.v[idx] = value;
+                mine = (double)value - (double)pv;
+                delta[idx] = (float)mine;
+            }
+        }
+        dl[q] = quad_bcast<q>(mine);
+    };
+    step(std::integral_constant<int, 0>());
+    step(std::integral_constant<int, 1>());
+    step(std::integral_constant<int, 2>());
+    if constexpr (C == 4)
+        step(std::integral_constant<int, 3>());
+
+    // One atomic per workgroup, as in the one-lane form (a texel's moves are its quad's lanes' moves).
+    __shared__ unsigned int s_wave[4];
+    const long long g0 = ((long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x) >> 2;
+    const int pw = range_plane(colour, g0 < total ? g0 : total - 1);
+    unsigned int same = p == pw ? moved : 0u;
+    for (int off = 32; off > 0; off >>= 1)
+        same += __shfl_xor(same, off);
+    if (threadIdx.x % 64 == 0)
+        s_wave[threadIdx.x / 64] = same;
+    if (p != pw && moved)
+        atomicAdd(changed + ((size_t)p * MOVED_SLOTS + blockIdx.x % MOVED_SLOTS) * MOVED_STRIDE, moved);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        const unsigned int sum = s_wave[0] + s_wave[1] + s_wave[2] + s_wave[3];
+        if (sum)
+            atomicAdd(changed + ((size_t)pw * MOVED_SLOTS + blockIdx.x % MOVED_SLOTS) * MOVED_STRIDE, sum);
+    }
+}
+
 // One neighbour group of a texel's sweep: the neighbour's correction, the stencil block that couples it to the texel
 // (entry (i, j) at s[i * C + j], transposed for a backward neighbour, whose block it is) and a 0/1 weight. Every load is
 // unconditional: a neighbour outside the plane reads the texel's own entries (a valid address, finite values) and
@@ -2819,6 +2981,17 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
             // The exact form reads the fp64 stencil; the fast form its fp32 copy (half the bytes).
             if (c.form == NNTC_FORM_EXACT)
                 launch_sweep<double>(L.c, grid, c.stream, L, colour[col], col, c.counters.p);
+            else if (L.c >= 3)
+            {
+                // Four lanes per texel (one per channel row).
+                const dim3 grid4(blocks_for(4 * n, 256));
+                if (L.c == 3)
+                    hipLaunchKernelGGL(k_quant_sweep_rows<3>, grid4, dim3(256), 0, c.stream, L, colour[col], col,
+                                       c.counters.p);
+                else
+                    hipLaunchKernelGGL(k_quant_sweep_rows<4>, grid4, dim3(256), 0, c.stream, L, colour[col], col,
+                                       c.counters.p);
+            }
             else
                 launch_sweep<float>(L.c, grid, c.stream, L, colour[col], col, c.counters.p);
         }
