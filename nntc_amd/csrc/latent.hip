@@ -2446,6 +2446,113 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
         a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2] = pap;
 }
 
+// The mat-vec at C >= 3 (level 1) with four lanes per texel, lane i of the quad forming row i of Ap (the own block's row,
+// then the neighbour groups in order, as the one-lane form orders row i): a 1024-lane workgroup per chunk of 256
+// texels, so the chunk partials are laid out as the other fast CG kernels expect.
+template <int C>
+__global__ __launch_bounds__(1024) void k_matvec_fast_rows(FastCg a)
+{
+    static_assert(C >= 3 && C <= 4, "four lanes hold at most four rows");
+    __shared__ double s16[16];
+    long long t0, t1;
+    const int p = fast_chunk(a, t0, t1);
+    const LatentView& L = a.L;
+    double* sc = L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;   // the whole workgroup: one plane
+    const long long gt = t0 + (threadIdx.x >> 2);
+    const int lane_row = (int)(threadIdx.x & 3), i = lane_row < C ? lane_row : C - 1;
+    double pap = 0.0;
+    if (gt < t1)
+    {
+        const double lambda = sc[SC_LAMBDA];
+        const long long t = gt - L.texels.start[p];
+        const int w = L.pw[p], h = L.ph[p];
+        const int tx = (int)(t % w), ty = (int)(t / w);
+        const long long base = L.texels.start[p];
+        const long long sn = L.sm.sn;
+        const float* S = L.stencil32;
+        const long long pos = stencil_pos(L.sm, p, w, tx, ty);
+        const float* pf = cg_vec<true>(L.p);
+        float sr[8][C], xv[8][C];
+        double keep[8];
+#pragma unroll
+        for (int g = 0; g < 8; g++)
+        {
+            const int slot = 1 + g / 2;
+            int dx, dy;
+            stencil_offset(slot, dx, dy);
+            if (g % 2 == 0)
+            {
+                const int fx = tx + dx, fy = ty + dy;
+                const bool ok = fx >= 0 && fx < w && fy >= 0 && fy < h;
+                keep[g] = ok ? 1.0 : 0.0;
+                const float* xn = pf + (ok ? base + (long long)fy * w + fx : gt) * C;
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                {
+                    xv[g][j] = xn[j];
+                    sr[g][j] = S[(slot * C * C + i * C + j) * sn + pos];
+                }
+            }
+            else
+            {
+                const int bx = tx - dx, by = ty - dy;
+                const bool ok = bx >= 0 && bx < w && by >= 0 && by < h;
+                keep[g] = ok ? 1.0 : 0.0;
+                const int bxs = ok ? bx : tx, bys = ok ? by : ty;
+                const int ncx = bxs & 1, ncy = bys & 1;
+                const long long npos =
+                    L.sm.cpos[p][ncy * 2 + ncx] + (long long)(bys >> 1) * ((w - ncx + 1) >> 1) + (bxs >> 1);
+                const float* xn = pf + (base + (long long)bys * w + bxs) * C;
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                {
+                    xv[g][j] = xn[j];
+                    sr[g][j] = S[(slot * C * C + j * C + i) * sn + npos];
+                }
+            }
+        }
+        double xt[C];
+        float own[C];
+#pragma unroll
+        for (int j = 0; j < C; j++)
+        {
+            xt[j] = (double)pf[gt * C + j];
+            own[j] = S[(i * C + j) * sn + pos];
+        }
+        double acc = lambda * xt[i];
+#pragma unroll
+        for (int j = 0; j < C; j++)
+            acc += (double)own[j] * xt[j];
+#pragma unroll
+        for (int g = 0; g < 8; g++)
+#pragma unroll
+            for (int j = 0; j < C; j++)
+            {
+                const double term = (double)sr[g][j] * (double)xv[g][j];
+                acc += term * keep[g];
+            }
+        if (lane_row < C)
+        {
+            cg_vec<true>(L.ap)[gt * C + i] = (float)acc;
+            pap = xt[i] * acc;
+        }
+    }
+    // The workgroup's sum: each wave's tree, then the sixteen waves in order.
+    pap = tree_lanes(pap);
+    if (threadIdx.x % 64 == 0)
+        s16[threadIdx.x / 64] = pap;
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        double v = 0.0;
+        for (int k = 0; k < 16; k++)
+            v += s16[k];
+        a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2] = v;
+    }
+}
+
 // Per plane (one workgroup each): pAp, then alpha = rz / pAp (zero on non-positive curvature).
 __global__ __launch_bounds__(256) void k_fold_alpha_fast(FastCg a)
 {
@@ -2848,8 +2955,8 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
                 {
                 case 1: hipLaunchKernelGGL(k_matvec_fast<1>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
                 case 2: hipLaunchKernelGGL(k_matvec_fast<2>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
-                case 3: hipLaunchKernelGGL(k_matvec_fast<3>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
-                default: hipLaunchKernelGGL(k_matvec_fast<4>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+                case 3: hipLaunchKernelGGL(k_matvec_fast_rows<3>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
+                default: hipLaunchKernelGGL(k_matvec_fast_rows<4>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
                 }
             }
             hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
