@@ -1195,14 +1195,20 @@ __global__ __launch_bounds__(256) void k_diag_fast(ChunkArgs a)
     const int c = a.L.c, w = a.L.pw[p];
     const long long t0 = a.L.texels.start[p];
     double sum = 0.0, lo = 1e300, hi = -1e300;
-    for (long long i = vb + threadIdx.x; i < ve; i += 256)
+    // Texel by texel (32-bit position arithmetic: a plane's texels number fewer than 2^31), each texel's C diagonal
+    // entries in turn.
+    const int tb = (int)(vb / c - t0), te = (int)(ve / c - t0);
+    for (int t = tb + (int)threadIdx.x; t < te; t += 256)
     {
-        const long long t = i / c - t0;
-        const int q = (int)(i % c);
-        const double x = a.L.stencil32[(q * c + q) * a.L.sm.sn + stencil_pos(a.L.sm, p, w, (int)(t % w), (int)(t / w))];
-        sum += x;
-        lo = x < lo ? x : lo;
-        hi = x > hi ? x : hi;
+        const int ty = t / w, tx = t - ty * w;
+        const long long pos = stencil_pos(a.L.sm, p, w, tx, ty);
+        for (int q = 0; q < c; q++)
+        {
+            const double x = a.L.stencil32[(q * c + q) * a.L.sm.sn + pos];
+            sum += x;
+            lo = x < lo ? x : lo;
+            hi = x > hi ? x : hi;
+        }
     }
     for (int off = 32; off > 0; off >>= 1)
     {
