@@ -1772,10 +1772,11 @@ __global__ __launch_bounds__(64) void k_range_chunks(ChunkArgs a)
         lo[q] = 1e300;
         hi[q] = -1e300;
     }
-    for (long long i = vb + lane; i < ve; i += 64)
+    const int n = (int)(ve - vb);   // vb is a whole texel's first value: value o is channel o % c
+    for (int o = lane; o < n; o += 64)
     {
-        const int q = (int)(i % c);
-        const double val = (double)a.L.v[i];
+        const int q = o % c;
+        const double val = (double)a.L.v[vb + o];
         lo[q] = val < lo[q] ? val : lo[q];
         hi[q] = val > hi[q] ? val : hi[q];
     }
@@ -1829,6 +1830,36 @@ __global__ __launch_bounds__(64) void k_movement_chunks(ChunkArgs a)
     const int c = a.L.c, lane = threadIdx.x;
     const double* s = a.L.scalars + (size_t)p * SC_COUNT;
     double moved = 0.0, d2 = 0.0, p2 = 0.0;
+    if (!a.exact)
+    {
+        // The fast form: each lane's own sums over the chunk (32-bit offsets within it), one tree at the end.
+        double step[MAX_CH];
+        for (int q = 0; q < c; q++)
+            step[q] = (s[SC_HI + q] - s[SC_LO + q]) / (double)a.L.levels;
+        const int n = (int)(ve - vb);
+        for (int o = lane; o < n; o += 64)
+        {
+            const int q = o % c;   // vb is a whole texel's first value
+            double st = step[0];
+            for (int k = 1; k < c; k++)
+                st = q == k ? step[k] : st;
+            const double x = a.L.x[vb + o];
+            const double v = (double)a.L.v[vb + o];
+            moved += fabs(x) > 0.5 * st ? 1.0 : 0.0;
+            d2 += x * x;
+            p2 += v * v;
+        }
+        moved = tree_lanes(moved);
+        d2 = tree_lanes(d2);
+        p2 = tree_lanes(p2);
+        if (lane == 0)
+        {
+            a.partial[blockIdx.x * 3] = moved;
+            a.partial[blockIdx.x * 3 + 1] = d2;
+            a.partial[blockIdx.x * 3 + 2] = p2;
+        }
+        return;
+    }
     for (long long base = vb; base < ve; base += 64)
     {
         const long long i = base + lane;
