@@ -166,6 +166,8 @@ struct Ctx
     DevBuf<uint8_t> rgb8;                  // one plane's decode
     DevBuf<float> site_f;                  // per site of the chain: the other latent's sample (assembly pass 1)
     DevBuf<double> site_d;                 // per site: the residual pulled back through the decoder
+    DevBuf<float> site_target;             // the fast form's per-site source sample (nout floats), built once per K
+    int site_target_k = -1;
     DevBuf<unsigned long long> tex_sse;    // per texture, SSE_SLOTS counters: the base decode's squared byte error
 
     // Pinned host staging for the small per-call readbacks.

@@ -835,6 +835,10 @@ int nntc_hip_init_level0(nntc_hip_ctx* h, nntc_model_state* st, const float luma
 {
     return guarded([&] {
         Ctx& c = h->c;
+        // An encode starts here: the tables the fast form derives while it runs (the per-site source samples, the
+        // interior site list) are built again inside it, not carried over from an earlier encode.
+        c.site_target_k = -1;
+        c.gather_tab_k = -1;
         upload_grids(c, st);
         init_level0_kernel(c, luma, luma_channel0);
         HIP_OK(hipStreamSynchronize(c.stream));

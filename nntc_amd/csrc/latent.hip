@@ -173,6 +173,7 @@ struct AssembleArgs
     float* site_f;          // per site: the other latent's sample (cs floats)
     double* site_d;         // per site: acc (c doubles)
     float* site_d32;        // the fast form's acc (c floats, fp32 sums), over the same buffer
+    const float* site_target;   // the fast form's per-site source sample (nout floats), or null
     Ranges sites;           // per plane: its sites, w0 h0 (1 + K)
     int nmono, c0, c1, nin, nout, k;
     Div31 by_sites;                 // division by the sites per pixel, 1 + K
@@ -241,14 +242,19 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
     {
         const unsigned int pi = (unsigned int)py * (unsigned int)w0 + (unsigned int)px;
         load_texel<C0>(v0 + pi * C0, C0, sbar);
-        load_texel<MAX_NOUT>(src + pi * (unsigned int)nout, nout, target);
+        if (!FAST)
+            load_texel<MAX_NOUT>(src + pi * (unsigned int)nout, nout, target);
     }
     else
     {
         const Tap t0 = tap_pixel(px, py, dx, dy, w0, h0, w0, h0);
         sample_bilinear_v<C0, FAST>(v0, w0, C0, t0, sbar);
-        sample_bilinear_v<MAX_NOUT, FAST>(src, w0, nout, t0, target);
+        if (!FAST)
+            sample_bilinear_v<MAX_NOUT, FAST>(src, w0, nout, t0, target);
     }
+    // The fast form's source samples are fixed for the encode: read them from the per-site table.
+    if (FAST)
+        load_texel<MAX_NOUT>(a.site_target + gs * nout, nout, target);
     constexpr int C = LEVEL0 ? C0 : C1;   // the solved latent's channels
     constexpr int NH = C * (C + 1) / 2;
     if constexpr (FAST)
@@ -426,6 +432,38 @@ __global__ __launch_bounds__(256) void k_site_terms(AssembleArgs a)
 #pragma unroll
     for (int aa = 0; aa < C; aa++)
         a.site_d[gs * C + aa] = acc[aa];
+}
+
+// The fast form's per-site source samples: site j of pixel (px, py) reads the source at the pixel (j = 0) or blends it
+// with fused multiply-adds at the site's taps - what the fast site terms used to sample each time, fixed while K is.
+__global__ __launch_bounds__(256) void k_site_target(AssembleArgs a, const float* src_chain, float* out)
+{
+    const long long gs = (long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (gs >= a.sites.start[a.sites.n])
+        return;
+    const int p = range_plane(a.sites, gs);
+    const PlaneGeom& g = a.L.planes.p[p];
+    const unsigned int si = (unsigned int)(gs - a.sites.start[p]);
+    const unsigned int per_pixel = (unsigned int)(1 + a.k);
+    const unsigned int pixel = a.by_sites.div(si);
+    const int j = (int)(si - pixel * per_pixel);
+    const unsigned int pyu = a.by_w0[p].div(pixel);
+    const int px = (int)(pixel - pyu * (unsigned int)g.w0), py = (int)pyu;
+    const int nout = a.nout;
+    const float* src = src_chain + g.off0 * nout;
+    float target[MAX_NOUT];
+    if (j == 0)
+        load_texel<MAX_NOUT>(src + ((unsigned int)py * (unsigned int)g.w0 + (unsigned int)px) * (unsigned int)nout,
+                             nout, target);
+    else
+    {
+        float dx, dy;
+        site_delta(a.k, j, px, py, dx, dy);
+        const Tap t0 = tap_pixel(px, py, dx, dy, g.w0, g.h0, g.w0, g.h0);
+        sample_bilinear_v<MAX_NOUT, true>(src, g.w0, nout, t0, target);
+    }
+    for (int r = 0; r < nout; r++)
+        out[gs * nout + r] = target[r];
 }
 
 template <bool LEVEL0, int C0, bool DIRECT, bool FAST>
@@ -2822,6 +2860,21 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
                        0.0);
         const dim3 grid(blocks_for(acc, 256));
+        a.site_target = nullptr;
+        if (c.form != NNTC_FORM_EXACT)
+        {
+            if (c.site_target.n < (size_t)acc * c.nout)
+            {
+                c.site_target.alloc((size_t)acc * c.nout);
+                c.site_target_k = -1;
+            }
+            if (c.site_target_k != k)
+            {
+                hipLaunchKernelGGL(k_site_target, grid, dim3(256), 0, c.stream, a, c.src.p, c.site_target.p);
+                c.site_target_k = k;
+            }
+            a.site_target = c.site_target.p;
+        }
         if (level0 && direct)
             launch_site_terms<true, true, true>(c.c0, c.c1, grid, c.stream, a);
         else if (level0)
