@@ -173,7 +173,7 @@ struct GramArgs
     const float* v0;
     const float* v1;
     int nout, k, mm, tri, entries;
-    double* partial;       // tile * entries + e
+    double* partial;       // e * tiles + tile (entry-major)
 };
 
 // x as bf16 rounded to nearest even (the hardware conversion; finite inputs), and the bf16 of what it leaves.
@@ -327,7 +327,8 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
             jj = MM + e2 / MM;     // target, always in a block at or right of the feature's
         }
         const int bi = i / 32, bj = jj / 32;
-        a.partial[(size_t)tile * a.entries + e] = bi <= bj ? s_g[i * DP + jj] : s_g[jj * DP + i];
+        // Entry-major (entry e of every tile contiguous), so the reduction reads each entry's column coalesced.
+        a.partial[(size_t)e * a.tiles.start[a.tiles.n] + tile] = bi <= bj ? s_g[i * DP + jj] : s_g[jj * DP + i];
     }
 }
 
@@ -343,7 +344,7 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* partial, cons
     {
         double sum = 0.0;
         for (long long t = tiles.start[p] + tid; t < tiles.start[p + 1]; t += 256)
-            sum += partial[(size_t)t * entries + e];
+            sum += partial[(size_t)e * tiles.start[tiles.n] + t];
         for (int off = 32; off > 0; off >>= 1)
             sum += __shfl_xor(sum, off);
         if (tid % 64 == 0)
