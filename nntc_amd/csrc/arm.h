@@ -267,6 +267,8 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations);
 void correction_and_probe(Ctx& c, bool level0);
 void quant_sweeps(Ctx& c, bool level0, int sweeps);
 void copy_prev_zero_delta(Ctx& c, bool level0);
+// The fast form's per-site source samples (site s's nout values at [s nout]), built when K changes or an encode starts.
+const float* site_targets(Ctx& c, int k);
 // init.cu counterparts (init.hip)
 void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0);
 void init_level1_box(Ctx& c);

@@ -2860,21 +2860,7 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
                        0.0);
         const dim3 grid(blocks_for(acc, 256));
-        a.site_target = nullptr;
-        if (c.form != NNTC_FORM_EXACT)
-        {
-            if (c.site_target.n < (size_t)acc * c.nout)
-            {
-                c.site_target.alloc((size_t)acc * c.nout);
-                c.site_target_k = -1;
-            }
-            if (c.site_target_k != k)
-            {
-                hipLaunchKernelGGL(k_site_target, grid, dim3(256), 0, c.stream, a, c.src.p, c.site_target.p);
-                c.site_target_k = k;
-            }
-            a.site_target = c.site_target.p;
-        }
+        a.site_target = c.form != NNTC_FORM_EXACT ? site_targets(c, k) : nullptr;
         if (level0 && direct)
             launch_site_terms<true, true, true>(c.c0, c.c1, grid, c.stream, a);
         else if (level0)
@@ -3143,6 +3129,37 @@ static void launch_sweep(int ch, dim3 grid, hipStream_t s, const LatentView& L, 
     case 3: hipLaunchKernelGGL((k_quant_sweep<3, T>), grid, dim3(256), 0, s, L, colour, col, counters); break;
     default: hipLaunchKernelGGL((k_quant_sweep<4, T>), grid, dim3(256), 0, s, L, colour, col, counters); break;
     }
+}
+
+const float* site_targets(Ctx& c, int k)
+{
+    AssembleArgs a {};
+    a.L.planes = c.planes;
+    a.nout = c.nout;
+    a.k = k;
+    a.sites.n = c.planes.n;
+    a.by_sites = Div31((unsigned int)(1 + k));
+    long long acc = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        a.sites.start[p] = acc;
+        acc += c.g(p).n0 * (1 + k);
+        a.by_w0[p] = Div31((unsigned int)c.g(p).w0);
+    }
+    a.sites.start[c.planes.n] = acc;
+    if (c.site_target.n < (size_t)acc * c.nout)
+    {
+        c.site_target.alloc((size_t)acc * c.nout);
+        c.site_target_k = -1;
+    }
+    if (c.site_target_k != k)
+    {
+        hipLaunchKernelGGL(k_site_target, dim3(blocks_for(acc, 256)), dim3(256), 0, c.stream, a, c.src.p,
+                           c.site_target.p);
+        HIP_OK(hipGetLastError());
+        c.site_target_k = k;
+    }
+    return c.site_target.p;
 }
 
 void quant_sweeps(Ctx& c, bool level0, int sweeps)
