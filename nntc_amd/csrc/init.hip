@@ -291,6 +291,80 @@ __global__ __launch_bounds__(256) void k_seed_channel(const float* resid, long l
     v0[t * c0 + channel] = (float)(p < -1.0 ? -1.0 : (p > 1.0 ? 1.0 : p));
 }
 
+// The fast form's covariance partials (nout <= 12): one pass over the rows, each lane accumulating the sums and the
+// upper triangle of the products of its rows in fp64 registers, then wave trees and the four waves in order; block b
+// writes its entries (the full matrix, mirrored) where k_cov_partial would. Every row is read once, where the
+// reference-order kernel reads each row once per matrix row.
+constexpr int COV_FAST_NOUT = 12;
+
+__global__ __launch_bounds__(256) void k_cov_partial_fast(const float* data, long long texels, int nout, double* partial)
+{
+    constexpr int N = COV_FAST_NOUT, NU = N * (N + 1) / 2;
+    __shared__ double s_w[4][N + NU];
+    const int b = blockIdx.x;
+    double sum[N], prod[NU];
+#pragma unroll
+    for (int i = 0; i < N; i++)
+        sum[i] = 0.0;
+#pragma unroll
+    for (int e = 0; e < NU; e++)
+        prod[e] = 0.0;
+    for (long long t = (long long)b * 256 + threadIdx.x; t < texels; t += (long long)gridDim.x * 256)
+    {
+        const float* d = data + t * nout;
+        double v[N];
+#pragma unroll
+        for (int i = 0; i < N; i++)
+        {
+            v[i] = 0.0;
+            if (i < nout)   // uniform: only the row's own values are read
+                v[i] = (double)d[i];
+        }
+        int e = 0;
+#pragma unroll
+        for (int i = 0; i < N; i++)
+        {
+            sum[i] += v[i];
+#pragma unroll
+            for (int j = i; j < N; j++, e++)
+                prod[e] = fma(v[i], v[j], prod[e]);
+        }
+    }
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+#pragma unroll
+    for (int i = 0; i < N; i++)
+    {
+        double x = sum[i];
+        for (int off = 32; off > 0; off >>= 1)
+            x += __shfl_xor(x, off);
+        if (lane == 0)
+            s_w[wave][i] = x;
+    }
+#pragma unroll
+    for (int e = 0; e < NU; e++)
+    {
+        double x = prod[e];
+        for (int off = 32; off > 0; off >>= 1)
+            x += __shfl_xor(x, off);
+        if (lane == 0)
+            s_w[wave][N + e] = x;
+    }
+    __syncthreads();
+    const int entries = nout + nout * nout;
+    for (int q = threadIdx.x; q < entries; q += 256)
+    {
+        int k;
+        if (q < nout)
+            k = q;
+        else
+        {
+            const int r = (q - nout) / nout, c = (q - nout) % nout, i = r < c ? r : c, j = r < c ? c : r;
+            k = N + i * N - i * (i - 1) / 2 + (j - i);
+        }
+        partial[(size_t)b * entries + q] = ((s_w[0][k] + s_w[1][k]) + s_w[2][k]) + s_w[3][k];
+    }
+}
+
 // acc[e] (from zero when `first`) plus weight x partial[b entries + e] for b = 0 .. SITE_BLOCKS - 1, in that order: the
 // host fold of each plane's workgroup partials into the raw sums, kept on the device so the chain's planes need one
 // readback.
@@ -449,8 +523,12 @@ void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi)
 void cov_accumulate(Ctx& c, const float* data, long long texels, double weight, bool first)
 {
     const int entries = c.nout + c.nout * c.nout;
-    hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, c.nout + 1), dim3(256), 0, c.stream, data, texels, c.nout,
-                       c.cov_partial.p);
+    if (c.form != NNTC_FORM_EXACT && c.nout <= COV_FAST_NOUT)
+        hipLaunchKernelGGL(k_cov_partial_fast, dim3(SITE_BLOCKS), dim3(256), 0, c.stream, data, texels, c.nout,
+                           c.cov_partial.p);
+    else
+        hipLaunchKernelGGL(k_cov_partial, dim3(SITE_BLOCKS, c.nout + 1), dim3(256), 0, c.stream, data, texels, c.nout,
+                           c.cov_partial.p);
     hipLaunchKernelGGL(k_cov_fold, dim3(1), dim3(256), 0, c.stream, (const double*)c.cov_partial.p, entries, weight,
                        first ? 1 : 0, c.cov_partial.p + (size_t)SITE_BLOCKS * entries);
     HIP_OK(hipGetLastError());
