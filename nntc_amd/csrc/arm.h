@@ -145,7 +145,7 @@ struct Ctx
     DevBuf<float> resid;                   // base texels * nout
     DevBuf<double> cov_partial;            // SITE_BLOCKS * entries
     DevBuf<unsigned int> counters;         // per plane, MOVED_SLOTS counters MOVED_STRIDE apart
-    DevBuf<double> cg_partial;             // the fast form's CG: two partials per chunk of 256 texels
+    DevBuf<double> cg_partial;             // the fast form's CG: three partials per chunk of 256 texels
     DevBuf<double> gather_tab;             // the fast level-0 gather's interior site lists (4 x 45 entries of 7)
     int gather_tab_k = -1;                 // the K the list was built for
     DevBuf<uint8_t> bc_ep, bc_sel;         // blocks * c0 * 2, blocks * c0 * 16

@@ -717,7 +717,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         c.resid.alloc((size_t)o0 * nout);
         c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout) + (nout + nout * nout));
         c.counters.alloc((size_t)md->nplanes * MOVED_SLOTS * MOVED_STRIDE);
-        c.cg_partial.alloc(2 * ((size_t)std::max(o0, o1) / 256 + MAX_PLANES + 1));
+        c.cg_partial.alloc(3 * ((size_t)std::max(o0, o1) / 256 + MAX_PLANES + 1));
         c.gather_tab.alloc(4 * 9 * 5 * 7 + 4);
         c.bc_ep.alloc((size_t)ob * c0 * 2);
         c.bc_sel.alloc((size_t)ob * c0 * 16);

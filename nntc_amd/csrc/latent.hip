@@ -2381,11 +2381,14 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
 // ---------------------------------------------------------------------------------------------------------------------
 //
 // The mat-vec and the update each produce the iteration's inner products as workgroup partials over chunks of
-// FAST_CG_CHUNK texels of one plane, and a one-workgroup-per-plane fold derives the scalars the next launch reads:
-// alpha after the mat-vec; |r|^2, r.z, beta, the residual and, at a probe iteration, the stopping bar after the update.
-// The two inner-product launches are gone, and with them two passes over the vectors. (The folds stay launches of
-// their own: a plane's last workgroup folding in place would need an agent-scope release and acquire per workgroup,
-// which on a chip of eight L2 caches writes back and invalidates them.)
+// FAST_CG_CHUNK texels of one plane. Alpha comes from the plane's p.Ap partials, folded in one fixed order: where a
+// plane has at most 256 chunks (level 1), every workgroup of the update folds them itself (one L2 load per lane);
+// where it has more, a one-workgroup-per-plane launch folds them first. After the update a
+// one-workgroup-per-plane fold derives |r|^2, r.z, beta, the residual and, at a probe iteration, the stopping bar, which
+// p = z + beta p and the next iteration read. The two inner-product launches are gone, and with them two passes over the
+// vectors. (The remaining fold stays a launch of its own: a plane's last workgroup folding in place would need an
+// agent-scope release and acquire per workgroup, which on a chip of eight L2 caches writes back and invalidates them,
+// and the update's workgroups cannot each fold it, since the fold rewrites r.z, which they read.)
 
 constexpr int FAST_CG_CHUNK = 256;
 
@@ -2393,7 +2396,8 @@ struct FastCg
 {
     LatentView L;
     Ranges chunks;            // per plane: its chunks of FAST_CG_CHUNK texels
-    double* partial;          // two per chunk
+    double* partial;          // two per chunk: the update's |r|^2 and r.z
+    double* pap;              // one per chunk: the mat-vec's p.Ap
     int probe;                // the fold after the update applies the stopping bar
 };
 
@@ -2426,6 +2430,15 @@ __device__ __forceinline__ double fold_partials(const FastCg& a, int p, int k, d
     double v = 0.0;
     for (long long ch = a.chunks.start[p] + threadIdx.x; ch < a.chunks.start[p + 1]; ch += 256)
         v += a.partial[(size_t)ch * 2 + k];
+    return block_sum256(v, s4);
+}
+
+// The plane's p.Ap, folded in the same order by the workgroup that asks.
+__device__ __forceinline__ double fold_pap(const FastCg& a, int p, double* s4)
+{
+    double v = 0.0;
+    for (long long ch = a.chunks.start[p] + threadIdx.x; ch < a.chunks.start[p + 1]; ch += 256)
+        v += a.pap[ch];
     return block_sum256(v, s4);
 }
 
@@ -2518,7 +2531,7 @@ __global__ __launch_bounds__(256) void k_matvec_fast(FastCg a)
     }
     pap = block_sum256(pap, s4);
     if (threadIdx.x == 0)
-        a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2] = pap;
+        a.pap[xcd_block(blockIdx.x, gridDim.x)] = pap;
 }
 
 // The mat-vec at C >= 3 (level 1) with four lanes per texel, lane i of the quad forming row i of Ap (the own block's row,
@@ -2624,7 +2637,7 @@ __global__ __launch_bounds__(1024) void k_matvec_fast_rows(FastCg a)
         double v = 0.0;
         for (int k = 0; k < 16; k++)
             v += s16[k];
-        a.partial[(size_t)xcd_block(blockIdx.x, gridDim.x) * 2] = v;
+        a.pap[xcd_block(blockIdx.x, gridDim.x)] = v;
     }
 }
 
@@ -2636,7 +2649,7 @@ __global__ __launch_bounds__(256) void k_fold_alpha_fast(FastCg a)
     double* sc = a.L.scalars + (size_t)p * SC_COUNT;
     if (sc[SC_ACTIVE] == 0.0)
         return;
-    const double pap = fold_partials(a, p, 0, s4);
+    const double pap = fold_pap(a, p, s4);
     if (threadIdx.x == 0)
     {
         sc[SC_PAP] = pap;
@@ -2644,7 +2657,10 @@ __global__ __launch_bounds__(256) void k_fold_alpha_fast(FastCg a)
     }
 }
 
-// x += alpha p, r -= alpha Ap, z = Minv r for the chunk's texels, and the chunk's |r|^2 and r.z.
+// x += alpha p, r -= alpha Ap, z = Minv r for the chunk's texels, and the chunk's |r|^2 and r.z. With FOLD the
+// workgroup first folds the plane's pAp and alpha itself (the plane's first chunk records them), as
+// k_fold_alpha_fast does.
+template <bool FOLD>
 __global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
 {
     __shared__ double s4[4];
@@ -2654,11 +2670,23 @@ __global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
     double* sc = L.scalars + (size_t)p * SC_COUNT;
     if (sc[SC_ACTIVE] == 0.0)
         return;
+    double alpha;
+    if constexpr (FOLD)
+    {
+        const double pap = fold_pap(a, p, s4);
+        alpha = pap > 0.0 ? sc[SC_RZ] / pap : 0.0;
+        if (threadIdx.x == 0 && xcd_block(blockIdx.x, gridDim.x) == a.chunks.start[p])
+        {
+            sc[SC_PAP] = pap;
+            sc[SC_ALPHA] = alpha;
+        }
+    }
+    else
+        alpha = sc[SC_ALPHA];
     const long long gt = t0 + threadIdx.x;
     double rr = 0.0, rz = 0.0;
     if (gt < t1)
     {
-        const double alpha = sc[SC_ALPHA];
         const int c = L.c;
         float* r = cg_vec<true>(L.r);
         float* z = cg_vec<true>(L.z);
@@ -3012,9 +3040,14 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             acc += (L.texels.start[p + 1] - L.texels.start[p] + FAST_CG_CHUNK - 1) / FAST_CG_CHUNK;
         }
         fc.chunks.start[np] = acc;
-        if ((size_t)acc * 2 > c.cg_partial.n)
+        if ((size_t)acc * 3 > c.cg_partial.n)
             throw Refused { "latent: CG partials exceed the workspace" };
+        fc.pap = c.cg_partial.p + 2 * acc;
     }
+    long long most_chunks = 0;
+    for (int p = 0; p < np; p++)
+        most_chunks = std::max(most_chunks, fc.chunks.start[p + 1] - fc.chunks.start[p]);
+    const bool fold_in_update = most_chunks <= 256;
     fc.partial = c.cg_partial.p;
     const unsigned fgrid = (unsigned)fc.chunks.start[np];
     for (int it = 0; it < CG_MAX_ITERATIONS; it++)
@@ -3035,10 +3068,14 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
                 default: hipLaunchKernelGGL(k_matvec_fast_rows<4>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
                 }
             }
-            hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
+            if (!fold_in_update)
+                hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
             {
                 KernelTimer kt(c.stream, "k_cg_update", (double)n * 4.0 * cc * cc + 44.0 * nv, nv * (4.0 + 2.0 * cc));
-                hipLaunchKernelGGL(k_cg_update_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
+                if (fold_in_update)
+                    hipLaunchKernelGGL(k_cg_update_fast<true>, dim3(fgrid), dim3(256), 0, c.stream, fc);
+                else
+                    hipLaunchKernelGGL(k_cg_update_fast<false>, dim3(fgrid), dim3(256), 0, c.stream, fc);
             }
             hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
             {
