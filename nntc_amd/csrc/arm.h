@@ -58,6 +58,7 @@ enum Scalar
     SC_RHS2,
     SC_RZ,
     SC_RZ_NEW,
+    SC_RZ_PREV,   // the fast form at level 1: r.z before the update, which the fused p = z + beta p divides by
     SC_PAP,
     SC_RR,
     SC_RESID,

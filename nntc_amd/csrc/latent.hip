@@ -2381,14 +2381,14 @@ __device__ __forceinline__ unsigned int sweep_texel(const LatentView& L, const R
 // ---------------------------------------------------------------------------------------------------------------------
 //
 // The mat-vec and the update each produce the iteration's inner products as workgroup partials over chunks of
-// FAST_CG_CHUNK texels of one plane. Alpha comes from the plane's p.Ap partials, folded in one fixed order: where a
-// plane has at most 256 chunks (level 1), every workgroup of the update folds them itself (one L2 load per lane);
-// where it has more, a one-workgroup-per-plane launch folds them first. After the update a
-// one-workgroup-per-plane fold derives |r|^2, r.z, beta, the residual and, at a probe iteration, the stopping bar, which
-// p = z + beta p and the next iteration read. The two inner-product launches are gone, and with them two passes over the
-// vectors. (The remaining fold stays a launch of its own: a plane's last workgroup folding in place would need an
-// agent-scope release and acquire per workgroup, which on a chip of eight L2 caches writes back and invalidates them,
-// and the update's workgroups cannot each fold it, since the fold rewrites r.z, which they read.)
+// FAST_CG_CHUNK texels of one plane, and each fold reads a plane's partials in one fixed order. Where a plane has at
+// most 256 chunks (level 1: one L2 load per lane), the consumers fold them themselves: every workgroup of the update
+// folds p.Ap into alpha, and every workgroup of p = z + beta p folds |r|^2 and r.z into beta, the residual and, at a
+// probe iteration, the stopping bar (the plane's first chunk records the scalars; r.z before the update is kept apart
+// from the r.z being written, so no workgroup reads a scalar another rewrites). Where a plane has more chunks, a
+// one-workgroup-per-plane launch folds each before its consumer. The two inner-product launches are gone, and with them
+// two passes over the vectors. (A plane's last workgroup folding in place would need an agent-scope release and acquire
+// per workgroup, which on a chip of eight L2 caches writes back and invalidates them.)
 
 constexpr int FAST_CG_CHUNK = 256;
 
@@ -2679,6 +2679,7 @@ __global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
         {
             sc[SC_PAP] = pap;
             sc[SC_ALPHA] = alpha;
+            sc[SC_RZ_PREV] = sc[SC_RZ];
         }
     }
     else
@@ -2737,6 +2738,52 @@ __global__ __launch_bounds__(256) void k_fold_beta_fast(FastCg a)
         sc[SC_RESID] = rhs2 > 0.0 ? sqrt((rr_t > 0.0 ? rr_t : 0.0) / rhs2) : 0.0;
         if (a.probe && !(sc[SC_RESID] >= CG_TOLERANCE))
             sc[SC_ACTIVE] = 0.0;
+    }
+}
+
+// k_fold_beta_fast then p = z + beta p, for the chunk's texels, with every workgroup folding the plane's |r|^2 and r.z
+// itself (the plane's first chunk records the scalars). r.z before the update comes from SC_RZ_PREV, which no
+// workgroup of this launch writes. Whether the plane still iterates is read once per workgroup: the first chunk may stop
+// the plane during the launch, and every other workgroup then reaches the same stop from the same fold.
+__global__ __launch_bounds__(256) void k_xpby_fold_fast(FastCg a)
+{
+    __shared__ double s4[4];
+    __shared__ int s_active;
+    long long t0, t1;
+    const int p = fast_chunk(a, t0, t1);
+    const LatentView& L = a.L;
+    double* sc = L.scalars + (size_t)p * SC_COUNT;
+    if (threadIdx.x == 0)
+        s_active = sc[SC_ACTIVE] != 0.0;
+    __syncthreads();
+    if (!s_active)
+        return;
+    const double rr_t = fold_partials(a, p, 0, s4);
+    const double rz_t = fold_partials(a, p, 1, s4);
+    const double rz_old = sc[SC_RZ_PREV];
+    const double beta = rz_old != 0.0 ? rz_t / rz_old : 0.0;
+    const double rhs2 = sc[SC_RHS2];
+    const double resid = rhs2 > 0.0 ? sqrt((rr_t > 0.0 ? rr_t : 0.0) / rhs2) : 0.0;
+    const bool stop = a.probe && !(resid >= CG_TOLERANCE);
+    if (threadIdx.x == 0 && xcd_block(blockIdx.x, gridDim.x) == a.chunks.start[p])
+    {
+        sc[SC_RR] = rr_t;
+        sc[SC_RZ_NEW] = rz_t;
+        sc[SC_BETA] = beta;
+        sc[SC_RZ] = rz_t;
+        sc[SC_RESID] = resid;
+        if (stop)
+            sc[SC_ACTIVE] = 0.0;
+    }
+    const long long gt = t0 + threadIdx.x;
+    if (stop || gt >= t1)
+        return;
+    float* pv = cg_vec<true>(L.p);
+    const float* z = cg_vec<true>(L.z);
+    for (int i = 0; i < L.c; i++)
+    {
+        const long long e = gt * L.c + i;
+        pv[e] = (float)((double)z[e] + beta * (double)pv[e]);
     }
 }
 
@@ -3077,10 +3124,14 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
                 else
                     hipLaunchKernelGGL(k_cg_update_fast<false>, dim3(fgrid), dim3(256), 0, c.stream, fc);
             }
-            hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
+            if (!fold_in_update)
+                hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
             {
                 KernelTimer kt(c.stream, "k_xpby", 12.0 * nv, 2.0 * nv);
-                hipLaunchKernelGGL(k_xpby_fast, dim3(tb), dim3(256), 0, c.stream, L);
+                if (fold_in_update)
+                    hipLaunchKernelGGL(k_xpby_fold_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
+                else
+                    hipLaunchKernelGGL(k_xpby_fast, dim3(tb), dim3(256), 0, c.stream, L);
             }
             HIP_OK(hipGetLastError());
         }
