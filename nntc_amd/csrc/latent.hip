@@ -2423,23 +2423,37 @@ __device__ __forceinline__ double block_sum256(double v, double* s4)
     return ((s4[0] + s4[1]) + s4[2]) + s4[3];
 }
 
+// Lane t's sum of the values at chunks t, t + 256, ... of [c0, c1) (chunk ch at base[ch * stride]), added in that
+// order with four loads in flight.
+__device__ __forceinline__ double lane_fold256(const double* base, int stride, long long c0, long long c1)
+{
+    double v = 0.0;
+    long long ch = c0 + threadIdx.x;
+    for (; ch + 3 * 256 < c1; ch += 4 * 256)
+    {
+        const double v0 = base[(size_t)ch * stride], v1 = base[(size_t)(ch + 256) * stride];
+        const double v2 = base[(size_t)(ch + 512) * stride], v3 = base[(size_t)(ch + 768) * stride];
+        v += v0;
+        v += v1;
+        v += v2;
+        v += v3;
+    }
+    for (; ch < c1; ch += 256)
+        v += base[(size_t)ch * stride];
+    return v;
+}
+
 // The plane's partials (component k) folded by one workgroup: lane l adds chunks l, l + 256, ... in order, then the
 // workgroup's sum - a fixed order, the same every run.
 __device__ __forceinline__ double fold_partials(const FastCg& a, int p, int k, double* s4)
 {
-    double v = 0.0;
-    for (long long ch = a.chunks.start[p] + threadIdx.x; ch < a.chunks.start[p + 1]; ch += 256)
-        v += a.partial[(size_t)ch * 2 + k];
-    return block_sum256(v, s4);
+    return block_sum256(lane_fold256(a.partial + k, 2, a.chunks.start[p], a.chunks.start[p + 1]), s4);
 }
 
 // The plane's p.Ap, folded in the same order by the workgroup that asks.
 __device__ __forceinline__ double fold_pap(const FastCg& a, int p, double* s4)
 {
-    double v = 0.0;
-    for (long long ch = a.chunks.start[p] + threadIdx.x; ch < a.chunks.start[p + 1]; ch += 256)
-        v += a.pap[ch];
-    return block_sum256(v, s4);
+    return block_sum256(lane_fold256(a.pap, 1, a.chunks.start[p], a.chunks.start[p + 1]), s4);
 }
 
 // p = z + beta p on the fast form's fp32 vectors.
