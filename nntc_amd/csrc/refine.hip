@@ -920,8 +920,20 @@ __global__ __launch_bounds__(1024) void k_bc_fold_fast(Planes planes, const doub
 __global__ __launch_bounds__(1024) void k_bc_se_total(const double* se, long long n, double* out)
 {
     __shared__ double s[1024];
+    // Lane t adds elements t, t + 1024, ... in that order, eight loads in flight.
     double acc = 0.0;
-    for (long long i = threadIdx.x; i < n; i += 1024)
+    long long i = threadIdx.x;
+    for (; i + 7 * 1024 < n; i += 8 * 1024)
+    {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            v[u] = se[i + u * 1024];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            acc += v[u];
+    }
+    for (; i < n; i += 1024)
         acc += se[i];
     s[threadIdx.x] = acc;
     __syncthreads();
