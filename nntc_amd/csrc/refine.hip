@@ -882,7 +882,23 @@ __global__ __launch_bounds__(1024) void k_bc_fold_fast(Planes planes, const doub
     const int count = ((g.w0 + 3) / 4) * ((g.h0 + 3) / 4);
     const double* const tp = taken + g.offb;
     double pt[4] = { 0.0, 0.0, 0.0, 0.0 }, pm[4] = { 0.0, 0.0, 0.0, 0.0 };
-    for (int b = t; b < count; b += 4 * 1024)
+    int b = t;
+    // Sixteen loads in flight; accumulator u % 4 takes the same values in the same order as the loop below.
+    for (; b + 15 * 1024 < count; b += 16 * 1024)
+    {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+            v[u] = tp[b + u * 1024];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+        {
+            const bool neg = v[u] < 0.0;
+            pt[u & 3] += neg ? v[u] : 0.0;
+            pm[u & 3] += neg ? 1.0 : 0.0;
+        }
+    }
+    for (; b < count; b += 4 * 1024)
     {
         double v[4];
 #pragma unroll
