@@ -135,12 +135,20 @@ typedef struct nntc_bc_refine_report
 #define NNTC_RANGE_MINMAX 0
 #define NNTC_RANGE_PCT 1
 
-/* Execution forms. EXACT replays the reference CPU backend's reduction orders (fixed 4096-element chunks folded in
- * chunk order, the 256-site decoder tiles, the covariance grid) so that every result is the reference's bits. FAST
- * keeps every per-texel kernel identical but folds the dot products, the objective and the diagonal with wavefront
- * tree reductions, which changes last bits only. */
+/* Execution forms. Not in the reference's seam: the reference has one form.
+ *   EXACT  replays the reference CPU backend's reduction orders (fixed 4096-element chunks folded in chunk order, the
+ *          256-site decoder tiles, the covariance grid) so that every result is the reference's bits; conjugate
+ *          gradients in fp64 to the reference's 1e-10 relative residual (src/solve_level1.cu:1257).
+ *   PLAIN  one kernel per operation with wavefront-tree reductions, the assembled fp32 stencil, fp32 conjugate gradients
+ *          to a 1e-6 relative residual (at most 200 iterations), and the decoder Gram in fp64 on the vector ALUs
+ *          (the reference's 512 site blocks x 256-site tiles, src/solve_decoder.cu:87-161).
+ *   FAST   the chosen form: PLAIN with the decoder Gram as bf16 hi/lo split products on the matrix cores (fp32
+ *          accumulation, fp64 across tiles), the level-0 conjugate gradients on the matrix-free decode adjoint for
+ *          planes of 4096^2 and above, and each probe group of conjugate-gradient iterations replayed from a captured
+ *          HIP graph. */
 #define NNTC_FORM_EXACT 0
 #define NNTC_FORM_FAST 1
+#define NNTC_FORM_PLAIN 2
 
 typedef struct nntc_hip_ctx nntc_hip_ctx;
 

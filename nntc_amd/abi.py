@@ -20,6 +20,7 @@ RANGE_MINMAX = 0
 RANGE_PCT = 1
 FORM_EXACT = 0
 FORM_FAST = 1
+FORM_PLAIN = 2
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 HIP_LIB = os.path.join(HERE, "libnntc_hip.so")

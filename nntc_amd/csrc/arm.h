@@ -142,6 +142,7 @@ struct Ctx
     DevBuf<double> partial;                // chunk partials of the fixed-order reductions
     DevBuf<double> ls_partial;             // nplanes * entries * SITE_BLOCKS
     DevBuf<double> ls_sum;                 // entries
+    DevBuf<double> dec_io;                 // the device decoder solve's incoming decoder, result and status
     DevBuf<double> obj;                    // nplanes * 3
     DevBuf<float> resid;                   // base texels * nout
     DevBuf<double> cov_partial;            // SITE_BLOCKS * entries
@@ -260,7 +261,12 @@ void decode_pass(Ctx& c, int plane, uint8_t* out);
 void residual_pass(Ctx& c, int plane, float* resid);
 void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out);
 // decoder.hip
-void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc);
+// Block (a)'s normal equations into c.ls_sum (and, when to_host, into acc).
+void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc, bool to_host = true);
+// The fast form's solve of the normal equations in c.ls_sum on the device (fp64 Cholesky, one refinement step, the
+// ridge ladder and acceptance rule): the rung taken (3: refused), whether x holds a decoder, and x (nout x (nin + 1)).
+void decoder_solve_device(Ctx& c, const nntc_model_state* st, double inv_norm, bool norm_ok, const double ladder[3],
+                          double accept_rise, int& taken, bool& wrote, std::vector<float>& x);
 // latent.hip
 void build_block_matrices(Ctx& c, bool level0);
 void assemble(Ctx& c, bool level0, int k, double ridge);

@@ -713,6 +713,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
             tiles += objective_tiles(c.planes.p[p]);
         c.ls_partial.alloc((size_t)entries * (size_t)std::max<long long>(md->nplanes * SITE_BLOCKS, tiles));
         c.ls_sum.alloc((size_t)entries + MAX_PLANES);
+        c.dec_io.alloc(1024);
         c.obj.alloc((size_t)md->nplanes * 3);
         c.resid.alloc((size_t)o0 * nout);
         c.cov_partial.alloc((size_t)SITE_BLOCKS * (nout + nout * nout) + (nout + nout * nout));
@@ -820,7 +821,7 @@ size_t nntc_hip_device_memory(const nntc_hip_ctx* h)
 int nntc_hip_set_form(nntc_hip_ctx* h, int form)
 {
     return guarded([&] {
-        if (form != NNTC_FORM_EXACT && form != NNTC_FORM_FAST)
+        if (form != NNTC_FORM_EXACT && form != NNTC_FORM_FAST && form != NNTC_FORM_PLAIN)
             throw Refused { "unknown form" };
         h->c.form = form;
     });
@@ -979,11 +980,12 @@ int nntc_hip_solve_decoder(nntc_hip_ctx* h, nntc_model_state* st, int k, const d
         const int planes = c.planes.n;
         HIP_OK(hipEventRecord(c.ev0, c.stream));
         std::vector<double> acc;
-        ls_accumulate(c, k, per_site, acc);
-        const double ms = c.ms_since(c.ev0);
+        const bool on_device = c.form == NNTC_FORM_FAST;
+        ls_accumulate(c, k, per_site, acc, !on_device);
+        const double ms = on_device ? 0.0 : c.ms_since(c.ev0);
 
         double trace = 0.0;
-        for (int i = 0; i < mm; i++)
+        for (int i = 0; i < mm && !on_device; i++)
             trace += acc[(size_t)tri_index(i, i, mm)];
         const double mean_diag = trace > 0.0 ? trace / mm : 1.0;
         const bool had_previous = st->has_decoder != 0;
@@ -1003,6 +1005,29 @@ int nntc_hip_solve_decoder(nntc_hip_ctx* h, nntc_model_state* st, int k, const d
             den += per_site[i] * (double)c.g(i).w0 * (double)c.g(i).h0 * (double)(1 + k);
         const bool norm_ok = sum_cw * den > 0.0;
         const double inv_norm = norm_ok ? 1.0 / (sum_cw * den) : 0.0;
+        if (on_device)
+        {
+            // The fast form: the ladder solved on the device from the fp64 Gram there; one readback of the result.
+            int rung = 3;
+            bool wrote = false;
+            std::vector<float> x;
+            decoder_solve_device(c, st, inv_norm, norm_ok, RIDGE_LADDER, LS_ACCEPT_RISE, rung, wrote, x);
+            g_ridge_tally[rung < 3 ? rung : RIDGE_REFUSED]++;
+            if (wrote)
+            {
+                st->has_decoder = 1;
+                for (int cc = 0; cc < nout; cc++)
+                {
+                    for (int i = 0; i < nin; i++)
+                        st->w[(size_t)cc * nin + i] = x[(size_t)cc * mm + i];
+                    st->b[cc] = x[(size_t)cc * mm + nin];
+                }
+            }
+            upload_decoder(c, st);
+            if (ms_out)
+                *ms_out = c.ms_since(c.ev0);
+            return;
+        }
         auto qform = [&](const std::vector<double>& x, double& err) {
             double q = 0.0, e = 0.0;
             for (int cc = 0; cc < nout; cc++)

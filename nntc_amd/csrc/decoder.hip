@@ -148,18 +148,50 @@ __global__ __launch_bounds__(64) void k_ls_reduce(const double* partial, const d
         out[e] = total;
 }
 
+// The plain form's fold of the same partials: one 256-lane workgroup per entry, each lane's strided blocks of every
+// plane (times the plane's weight) summed in registers, then a tree.
+__global__ __launch_bounds__(256) void k_ls_reduce_tree(const double* partial, const double* omega, int planes,
+                                                        int entries, double* out)
+{
+    __shared__ double s_w[4];
+    const int e = blockIdx.x;
+    double acc = 0.0;
+    for (int p = 0; p < planes; p++)
+    {
+        const double* src = partial + ((size_t)p * entries + e) * SITE_BLOCKS;
+        double part = 0.0;
+        for (int b = threadIdx.x; b < SITE_BLOCKS; b += 256)
+            part += src[b];
+        acc += omega[p] * part;
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        acc += __shfl_xor(acc, off);
+    if ((threadIdx.x & 63) == 0)
+        s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[e] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
+}
+
 // ---------------------------------------------------------------------------------------------------------------------
 // The fast form: the Gram on the matrix cores, BF16 hi/lo split products accumulated in fp32
 // ---------------------------------------------------------------------------------------------------------------------
 //
 // With u = [phi; 1; t] (D = mm + nout values per site), every normal-equation entry is an entry of the Gram
 // G = sum u u^T: A = G[0:mm, 0:mm], rhs = G[mm + c, i]. One 256-lane workgroup per pixel tile (tile_stage.h) builds the
-// u of its 1 + K sites per pixel, splits each value into bf16 hi + lo (x - hi is exact in fp32, so hi + lo carries 16
-// of x's 24 mantissa bits), stores them feature-major in LDS, and each wavefront multiplies its 64 sites' slice on the
-// matrix cores: G_tile += hi hi^T + hi lo^T + lo hi^T, v_mfma_f32_32x32x16_bf16 per 32 x 32 block of G and 16
-// sites, accumulated in fp32 over the wavefront's 320 sites. The four wavefronts' blocks are added in double, in order,
-// and the tile's entries written; a second kernel folds the tiles of each plane in order, times the plane's weight.
-// Roofline: 2 S D^2 MFMA flops per pass (three split products), bounded by the feature pass that feeds them.
+// u of its 1 + K sites per pixel and splits each value exactly into three bf16 parts, x = h + m + l (h = bf16(x),
+// m = bf16(x - h), l = bf16(x - h - m); each difference is exact in fp32, so the three parts carry all 24 of x's
+// mantissa bits). The parts are stored feature-major in LDS and each wavefront multiplies its 64 sites' slice on the
+// matrix cores, v_mfma_f32_32x32x16_bf16 per 32 x 32 block of G and 16 sites, keeping every product down to 2^-16 of
+// h h^T: h h^T + h m^T + m h^T + m m^T + h l^T + l h^T (the products of two bf16 values are exact; the matrix core adds
+// them in fp32). On a diagonal block the mirrored pairs are one product and its transpose, so a diagonal block takes
+// four products, A += h h^T + m m^T and S += h m^T + h l^T, and G = A + S + S^T at the end. The fp32 accumulators are
+// flushed into fp64 registers after every 64 sites (one round of the wavefront), so fp32 rounding acts on at most 64
+// terms. The four wavefronts' blocks are added in double, in order, and the tile's entries written; a second kernel
+// folds the tiles of each plane, times the plane's weight. (With two parts and three products - 16 bits of each
+// feature - the decoder's small coefficients moved by up to 2 % of their size from the fp64 solve; with three parts and
+// the per-round flush they stay within 1e-4.)
+// Roofline: 2 S D^2 per product; the useful flops are the upper triangle and rhs, 2 S (mm (mm + 1) / 2 + nout mm).
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef short short8 __attribute__((ext_vector_type(8)));
@@ -176,15 +208,18 @@ struct GramArgs
     double* partial;       // e * tiles + tile (entry-major)
 };
 
-// x as bf16 rounded to nearest even (the hardware conversion; finite inputs), and the bf16 of what it leaves.
-__device__ __forceinline__ void split_bf16(float x, unsigned short& hi, unsigned short& lo)
+// x = h + m + l in bf16, each part the nearest-even bf16 of what the parts before it leave (the hardware conversion;
+// finite inputs). x - h and x - h - m are exact in fp32.
+__device__ __forceinline__ void split3_bf16(float x, unsigned short& h, unsigned short& m, unsigned short& l)
 {
-    hi = __builtin_bit_cast(unsigned short, (__bf16)x);
-    const float rest = x - __uint_as_float((unsigned int)hi << 16);
-    lo = __builtin_bit_cast(unsigned short, (__bf16)rest);
+    h = __builtin_bit_cast(unsigned short, (__bf16)x);
+    const float r1 = x - __uint_as_float((unsigned int)h << 16);
+    m = __builtin_bit_cast(unsigned short, (__bf16)r1);
+    const float r2 = r1 - __uint_as_float((unsigned int)m << 16);
+    l = __builtin_bit_cast(unsigned short, (__bf16)r2);
 }
 
-// Site s of feature f in the hi / lo tables. A matrix-core operand read takes one 16-byte group of eight sites from
+// Site s of feature f in a part's table. A matrix-core operand read takes one 16-byte group of eight sites from
 // each of 32 features; with rows of 512 bytes every such group would sit in the same LDS banks, so the groups of a row
 // are XOR-swizzled by the feature and the 32 reads spread over the banks.
 __device__ __forceinline__ int gram_slot(int f, int s)
@@ -198,28 +233,39 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
     constexpr int NIN = C1 + C0 + C0 * C1, MM = NIN + 1;
     constexpr int DP = 32 * DB;                 // features padded to whole 32-blocks
     constexpr int NT = DB * (DB + 1) / 2;       // the blocks of G's upper block triangle
+    constexpr int TABLE = DP * 256;             // bf16 values per part table
     __shared__ float s_l0[tile_l0_floats<C0>()];
     __shared__ float s_l1[TILE_L1_FLOATS];
-    __shared__ __align__(16) unsigned short s_hi[DP * 256], s_lo[DP * 256];   // [feature][site], gram_slot order
-    __shared__ double s_g[DP * DP];
+    // The three part tables [feature][site] (gram_slot order); after the last round the first 8 DP^2 bytes hold G in
+    // double (the tables are no longer read then).
+    __shared__ __align__(16) unsigned short s_tab[3 * TABLE];
+    static_assert(DP * DP * sizeof(double) <= 3 * TABLE * sizeof(unsigned short), "G fits over the tables");
+    unsigned short* s_h = s_tab;
+    unsigned short* s_m = s_tab + TABLE;
+    unsigned short* s_lo = s_tab + 2 * TABLE;
+    double* s_g = reinterpret_cast<double*>(s_tab);
 
     const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
     const int p = range_plane(a.tiles, tile);
     const int nout = a.nout, tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const TileGeom tg = stage_tile<C0, C1>(a.planes.p[p], tile - (int)a.tiles.start[p], nout, a.src, a.v0, a.v1,
                                            s_l0, s_l1);
-    for (int i = tid; i < DP * DP; i += 256)
-        s_g[i] = 0.0;
-    __syncthreads();
-
     const int px = tg.tx0 + tid % TILE_W, py = tg.ty0 + tid / TILE_W;
     const bool live = px < tg.w0 && py < tg.h0;
-    floatx16 acc[NT];
+    // acc[t]: block t's h h^T + m m^T (diagonal) or all six products (off the diagonal); sacc[b]: diagonal block b's
+    // h m^T + h l^T. acc64 / sacc64: their fp64 sums over the rounds.
+    floatx16 acc[NT], sacc[DB];
+    double acc64[NT][16], sacc64[DB][16];
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
         for (int r = 0; r < 16; r++)
-            acc[t][r] = 0.0f;
+            acc64[t][r] = 0.0;
+#pragma unroll
+    for (int b = 0; b < DB; b++)
+#pragma unroll
+        for (int r = 0; r < 16; r++)
+            sacc64[b][r] = 0.0;
 
     // Rows past the last target (the padding of G to whole 32-blocks) stay zero in every round.
     const int D = MM + nout;
@@ -227,16 +273,17 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
     for (int f = 0; f < DP; f++)
         if (f >= D)
         {
-            s_hi[gram_slot(f, tid)] = 0;
+            s_h[gram_slot(f, tid)] = 0;
+            s_m[gram_slot(f, tid)] = 0;
             s_lo[gram_slot(f, tid)] = 0;
         }
     for (int j = 0; j <= a.k; j++)
     {
-        // ---- this round's site of every lane, as a column of bf16 hi and lo values (zeros past the plane)
+        // ---- this round's site of every lane, as a column of its three bf16 parts (zeros past the plane)
         {
             float phi[NIN], target[MAX_NOUT];
             if (live)
-                tile_site<C0, C1>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
+                tile_site<C0, C1, false>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
 #pragma unroll
             for (int f = 0; f < DP; f++)
             {
@@ -252,25 +299,37 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
                     else if (f - MM < nout && f - MM < MAX_NOUT)
                         x = target[f - MM];
                 }
-                unsigned short hi, lo;
-                split_bf16(x, hi, lo);
-                s_hi[gram_slot(f, tid)] = hi;
-                s_lo[gram_slot(f, tid)] = lo;
+                unsigned short h, m, l;
+                split3_bf16(x, h, m, l);
+                s_h[gram_slot(f, tid)] = h;
+                s_m[gram_slot(f, tid)] = m;
+                s_lo[gram_slot(f, tid)] = l;
             }
         }
         __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc[t][r] = 0.0f;
+#pragma unroll
+        for (int b = 0; b < DB; b++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                sacc[b][r] = 0.0f;
         // ---- the wavefront's 64 sites, 16 per product; lane (feature l % 32, sites 8 (l / 32) .. + 7)
 #pragma unroll
         for (int ks = 0; ks < 4; ks++)
         {
             const int site0 = 64 * wave + 16 * ks + 8 * (lane / 32);
-            bf16x8 hv[DB], lv[DB];
+            bf16x8 hv[DB], mv[DB], lv[DB];
 #pragma unroll
             for (int b = 0; b < DB; b++)
             {
-                const int f = 32 * b + lane % 32;
-                hv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_hi + gram_slot(f, site0)));
-                lv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_lo + gram_slot(f, site0)));
+                const int slot = gram_slot(32 * b + lane % 32, site0);
+                hv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_h + slot));
+                mv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_m + slot));
+                lv[b] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const short8*>(s_lo + slot));
             }
             int t = 0;
 #pragma unroll
@@ -278,16 +337,44 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
 #pragma unroll
                 for (int bj = bi; bj < DB; bj++, t++)
                 {
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], hv[bj], acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], lv[bj], acc[t], 0, 0, 0);
-                    acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lv[bi], hv[bj], acc[t], 0, 0, 0);
+                    if (bi == bj)
+                    {
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], hv[bj], acc[t], 0, 0, 0);
+                        sacc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], mv[bj], sacc[bi], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(mv[bi], mv[bj], acc[t], 0, 0, 0);
+                        sacc[bi] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], lv[bj], sacc[bi], 0, 0, 0);
+                    }
+                    else
+                    {
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], hv[bj], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], mv[bj], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(mv[bi], hv[bj], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(mv[bi], mv[bj], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(hv[bi], lv[bj], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lv[bi], hv[bj], acc[t], 0, 0, 0);
+                    }
                 }
         }
+        // ---- the round's 64 sites into the fp64 sums
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                acc64[t][r] += (double)acc[t][r];
+#pragma unroll
+        for (int b = 0; b < DB; b++)
+#pragma unroll
+            for (int r = 0; r < 16; r++)
+                sacc64[b][r] += (double)sacc[b][r];
         __syncthreads();
     }
 
     // ---- the four wavefronts' blocks added into G in double, wavefront by wavefront; lane l, value r holds
-    // row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of its 32 x 32 block
+    // row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of its 32 x 32 block. A diagonal block adds S at its place,
+    // then (a second sweep, so that no two lanes of one wavefront add to one entry at once) S^T at the mirrored place.
+    for (int i = tid; i < DP * DP; i += 256)
+        s_g[i] = 0.0;
+    __syncthreads();
     for (int w = 0; w < 4; w++)
     {
         if (wave == w)
@@ -302,8 +389,24 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
                     {
                         const int m = 32 * bi + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
                         const int n = 32 * bj + lane % 32;
-                        s_g[m * DP + n] += (double)acc[t][r];
+                        s_g[m * DP + n] += bi == bj ? acc64[t][r] + sacc64[bi][r] : acc64[t][r];
                     }
+        }
+        __syncthreads();
+    }
+    for (int w = 0; w < 4; w++)
+    {
+        if (wave == w)
+        {
+#pragma unroll
+            for (int b = 0; b < DB; b++)
+#pragma unroll
+                for (int r = 0; r < 16; r++)
+                {
+                    const int m = 32 * b + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                    const int n = 32 * b + lane % 32;
+                    s_g[n * DP + m] += sacc64[b][r];
+                }
         }
         __syncthreads();
     }
@@ -390,7 +493,7 @@ void launch_gram_tiles(int c0, int c1, int db, unsigned grid, hipStream_t s, con
 }
 
 // The fast form's accumulation: tiled MFMA Gram, then the per-plane weighted fold.
-void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
+void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc, bool to_host)
 {
     GramArgs a;
     a.planes = c.planes;
@@ -433,19 +536,226 @@ void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& ac
     hipLaunchKernelGGL(k_gram_reduce, dim3(a.entries), dim3(256), 0, c.stream, (const double*)a.partial,
                        (const double*)d_omega, a.tiles, a.entries, c.ls_sum.p);
     HIP_OK(hipGetLastError());
+    if (!to_host)
+        return;
     const double* h = read_back(c, (const double*)c.ls_sum.p, (size_t)a.entries);
     acc.assign(h, h + a.entries);
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// The fast form's decoder solve on the device: fp64 Cholesky with one step of iterative refinement
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// The reference solves [A + ridge I | rhs] by Gauss-Jordan on the host for each rung of its ridge ladder and accepts
+// the first rung whose candidate does not raise the objective's quadratic Q beyond its rounding bound
+// (src/solve_decoder.cu:267-458). The fast form keeps the ladder, the rounding to float and the acceptance rule, and
+// solves each rung on the device from the fp64 Gram it accumulated: one 256-lane workgroup factors A + ridge I = L L^T
+// (right-looking, a column per step), solves every output's system by forward and back substitution, and refines each
+// solution once against the same fp64 system (r = rhs - (A + ridge I) x, x += (L L^T)^-1 r), which removes the
+// factorisation's own rounding. A rung whose factorisation meets a pivot that is not positive and finite is skipped,
+// as the reference skips a singular elimination.
+
+constexpr int DS_MM = NNTC_MAX_NIN + 1, DS_NOUT = NNTC_MAX_NOUT;
+
+struct DecoderSolveArgs
+{
+    const double* acc;         // tri + nout mm: A's upper triangle row by row, then rhs (output c, feature i)
+    const double* x_prev;      // nout x mm: the incoming decoder (W row, then b), when had_previous
+    const float* cw;
+    int mm, nout, tri, had_previous, norm_ok;
+    double inv_norm, accept_rise;
+    double ladder[3];
+    float* x_out;              // nout x mm: the decoder the call leaves (accepted, or the constant fallback)
+    int* status;               // rung taken (0..2), 3 when every rung was refused; [1]: 1 when x_out holds a decoder
+};
+
+// sum_c cw_c Q_c(x) and its rounding bound, Q_c(x) = x_c^T A x_c - 2 x_c . rhs_c (x_c at x + c DS_MM); lanes < nout
+// take an output each and every lane adds them in output order.
+__device__ void ds_qform(const DecoderSolveArgs& a, const double* sA, const double* sR, const double* x, double* s_q,
+                         double* s_e, double& q, double& err)
+{
+    const int mm = a.mm, tid = threadIdx.x;
+    if (tid < a.nout)
+    {
+        const double* xc = x + tid * DS_MM;
+        double qc = 0.0, ec = 0.0;
+        for (int i = 0; i < mm; i++)
+        {
+            for (int j = 0; j < mm; j++)
+            {
+                const double v = sA[i * DS_MM + j];
+                qc += xc[i] * v * xc[j];
+                ec += fabs(xc[i]) * fabs(v) * fabs(xc[j]);
+            }
+            qc -= 2.0 * xc[i] * sR[tid * DS_MM + i];
+            ec += 2.0 * fabs(xc[i]) * fabs(sR[tid * DS_MM + i]);
+        }
+        s_q[tid] = (double)a.cw[tid] * qc;
+        s_e[tid] = (double)a.cw[tid] * ec;
+    }
+    __syncthreads();
+    double qq = 0.0, ee = 0.0;
+    for (int c = 0; c < a.nout; c++)
+    {
+        qq += s_q[c];
+        ee += s_e[c];
+    }
+    q = qq;
+    err = 2.220446049250313e-16 * (double)mm * ee * a.inv_norm;
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
+{
+    __shared__ double sA[DS_MM * DS_MM], sL[DS_MM * DS_MM], sR[DS_NOUT * DS_MM], sX[DS_NOUT * DS_MM];
+    __shared__ double sP[DS_NOUT * DS_MM], s_q[DS_NOUT], s_e[DS_NOUT];
+    __shared__ int s_ok;
+    const int mm = a.mm, nout = a.nout, tid = threadIdx.x;
+    for (int e = tid; e < mm * mm; e += 256)
+    {
+        const int i = e / mm, j = e % mm, lo = i < j ? i : j, hi = i < j ? j : i;
+        sA[i * DS_MM + j] = a.acc[lo * mm - lo * (lo - 1) / 2 + (hi - lo)];
+    }
+    for (int e = tid; e < nout * mm; e += 256)
+    {
+        sR[(e / mm) * DS_MM + e % mm] = a.acc[a.tri + e];
+        sP[(e / mm) * DS_MM + e % mm] = a.had_previous ? a.x_prev[e] : 0.0;
+    }
+    __syncthreads();
+    double trace = 0.0;
+    for (int i = 0; i < mm; i++)
+        trace += sA[i * DS_MM + i];
+    const double mean_diag = trace > 0.0 ? trace / mm : 1.0;
+    double q_prev = 0.0, err_prev = 0.0;
+    if (a.had_previous)
+        ds_qform(a, sA, sR, sP, s_q, s_e, q_prev, err_prev);
+    int taken = 3;
+    for (int step = 0; step < 3 && taken == 3; step++)
+    {
+        const double ridge = a.ladder[step] * mean_diag;
+        for (int e = tid; e < mm * mm; e += 256)
+        {
+            const int i = e / mm, j = e % mm;
+            sL[i * DS_MM + j] = sA[i * DS_MM + j] + (i == j ? ridge : 0.0);
+        }
+        if (tid == 0)
+            s_ok = 1;
+        __syncthreads();
+        // ---- L L^T = A + ridge I in the lower triangle of sL (right-looking: column j, then the trailing update)
+        for (int j = 0; j < mm; j++)
+        {
+            if (tid == 0)
+            {
+                const double d = sL[j * DS_MM + j];
+                if (!(d > 0.0) || !isfinite(d))
+                    s_ok = 0;
+                sL[j * DS_MM + j] = d > 0.0 ? sqrt(d) : 1.0;
+            }
+            __syncthreads();
+            const double piv = sL[j * DS_MM + j];
+            if (tid > j && tid < mm)
+                sL[tid * DS_MM + j] /= piv;
+            __syncthreads();
+            const int rest = mm - j - 1;
+            for (int e = tid; e < rest * rest; e += 256)
+            {
+                const int i = j + 1 + e / rest, k = j + 1 + e % rest;
+                if (k <= i)
+                    sL[i * DS_MM + k] -= sL[i * DS_MM + j] * sL[k * DS_MM + j];
+            }
+            __syncthreads();
+        }
+        if (!s_ok)
+            continue;
+        // ---- each output's solve, refined once: lanes < nout take an output each
+        if (tid < nout)
+        {
+            double* x = sX + tid * DS_MM;
+            double r[DS_MM];
+            for (int i = 0; i < mm; i++)
+                r[i] = sR[tid * DS_MM + i];
+            for (int pass = 0; pass < 2; pass++)
+            {
+                // forward: L y = r, then back: L^T z = y (z overwrites r)
+                for (int i = 0; i < mm; i++)
+                {
+                    double v = r[i];
+                    for (int k = 0; k < i; k++)
+                        v -= sL[i * DS_MM + k] * r[k];
+                    r[i] = v / sL[i * DS_MM + i];
+                }
+                for (int i = mm - 1; i >= 0; i--)
+                {
+                    double v = r[i];
+                    for (int k = i + 1; k < mm; k++)
+                        v -= sL[k * DS_MM + i] * r[k];
+                    r[i] = v / sL[i * DS_MM + i];
+                }
+                if (pass == 0)
+                {
+                    for (int i = 0; i < mm; i++)
+                        x[i] = r[i];
+                    // the residual of the solve against the fp64 system
+                    for (int i = 0; i < mm; i++)
+                    {
+                        double v = sR[tid * DS_MM + i] - ridge * x[i];
+                        for (int k = 0; k < mm; k++)
+                            v -= sA[i * DS_MM + k] * x[k];
+                        r[i] = v;
+                    }
+                }
+                else
+                    for (int i = 0; i < mm; i++)
+                        x[i] = (double)(float)(x[i] + r[i]);
+            }
+        }
+        __syncthreads();
+        int finite = 1;
+        for (int e = 0; e < nout * mm; e++)
+            finite = finite && isfinite(sX[(e / mm) * DS_MM + e % mm]);
+        if (!finite)
+            continue;
+        double q_cand = 0.0, err_cand = 0.0;
+        ds_qform(a, sA, sR, sX, s_q, s_e, q_cand, err_cand);
+        if (!a.had_previous || (a.norm_ok && (q_cand - q_prev) * a.inv_norm + err_prev + err_cand <= a.accept_rise))
+            taken = step;
+    }
+    int wrote = 0;
+    if (taken < 3)
+    {
+        for (int e = tid; e < nout * mm; e += 256)
+            a.x_out[e] = (float)sX[(e / mm) * DS_MM + e % mm];
+        wrote = 1;
+    }
+    else if (!a.had_previous)
+    {
+        // the constant fallback: every weight zero, each bias the mean target (src/solve_decoder.cu:440-452)
+        const double count = sA[(mm - 1) * DS_MM + (mm - 1)];
+        for (int e = tid; e < nout * mm; e += 256)
+        {
+            const int cc = e / mm, i = e % mm;
+            a.x_out[e] = i == mm - 1 ? (count > 0.0 ? (float)(sR[cc * DS_MM + mm - 1] / count) : 0.0f) : 0.0f;
+        }
+        wrote = 1;
+    }
+    if (tid == 0)
+    {
+        a.status[0] = taken;
+        a.status[1] = wrote;
+    }
+}
+
 }   // namespace
 
-void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
+void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc, bool to_host)
 {
-    if (c.form != NNTC_FORM_EXACT)
+    if (c.form == NNTC_FORM_FAST)
     {
-        gram_accumulate(c, k, omega, acc);
+        gram_accumulate(c, k, omega, acc, to_host);
         return;
     }
+    // The exact and the plain form: fp64 dots per site block; the exact form folds the blocks in the reference's
+    // order, the plain form by trees.
     LsArgs a;
     a.planes = c.planes;
     a.src = c.src.p;
@@ -473,11 +783,59 @@ void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc)
     // omega travels through the scalar workspace's tail: nplanes doubles uploaded with the launch.
     double* d_omega = c.ls_sum.p + a.entries;
     upload_staged(c, d_omega, omega, (size_t)c.planes.n);
-    hipLaunchKernelGGL(k_ls_reduce, dim3(a.entries), dim3(64), 0, c.stream, (const double*)a.partial,
-                       (const double*)d_omega, c.planes.n, a.entries, c.ls_sum.p);
+    if (c.form == NNTC_FORM_EXACT)
+        hipLaunchKernelGGL(k_ls_reduce, dim3(a.entries), dim3(64), 0, c.stream, (const double*)a.partial,
+                           (const double*)d_omega, c.planes.n, a.entries, c.ls_sum.p);
+    else
+        hipLaunchKernelGGL(k_ls_reduce_tree, dim3(a.entries), dim3(256), 0, c.stream, (const double*)a.partial,
+                           (const double*)d_omega, c.planes.n, a.entries, c.ls_sum.p);
     HIP_OK(hipGetLastError());
+    if (!to_host)
+        return;
     const double* h = read_back(c, (const double*)c.ls_sum.p, (size_t)a.entries);
     acc.assign(h, h + a.entries);
+}
+
+void decoder_solve_device(Ctx& c, const nntc_model_state* st, double inv_norm, bool norm_ok, const double ladder[3],
+                          double accept_rise, int& taken, bool& wrote, std::vector<float>& x)
+{
+    const int mm = c.nin + 1, nout = c.nout;
+    DecoderSolveArgs a;
+    a.acc = c.ls_sum.p;
+    a.mm = mm;
+    a.nout = nout;
+    a.tri = mm * (mm + 1) / 2;
+    a.cw = c.cw.p;
+    a.had_previous = st->has_decoder != 0;
+    a.norm_ok = norm_ok ? 1 : 0;
+    a.inv_norm = inv_norm;
+    a.accept_rise = accept_rise;
+    for (int i = 0; i < 3; i++)
+        a.ladder[i] = ladder[i];
+    double* xp = c.dec_io.p;
+    a.x_prev = xp;
+    a.x_out = reinterpret_cast<float*>(xp + (size_t)NNTC_MAX_NOUT * DS_MM);
+    a.status = reinterpret_cast<int*>(a.x_out + (size_t)NNTC_MAX_NOUT * DS_MM);
+    if (a.had_previous)
+    {
+        std::vector<double> prev((size_t)nout * mm);
+        for (int cc = 0; cc < nout; cc++)
+        {
+            for (int i = 0; i < c.nin; i++)
+                prev[(size_t)cc * mm + i] = (double)st->w[(size_t)cc * c.nin + i];
+            prev[(size_t)cc * mm + c.nin] = (double)st->b[cc];
+        }
+        upload_staged(c, xp, prev.data(), prev.size());
+    }
+    hipLaunchKernelGGL(k_decoder_solve, dim3(1), dim3(256), 0, c.stream, a);
+    HIP_OK(hipGetLastError());
+    // One readback: the decoder (nout x mm floats) and the two status words after it.
+    const size_t nx = (size_t)nout * mm;
+    const float* h = read_back(c, a.x_out, (size_t)NNTC_MAX_NOUT * DS_MM + 2);
+    x.assign(h, h + nx);
+    const int* stat = reinterpret_cast<const int*>(h + (size_t)NNTC_MAX_NOUT * DS_MM);
+    taken = stat[0];
+    wrote = stat[1] != 0;
 }
 
 }   // namespace nntc

@@ -32,7 +32,8 @@ namespace
 
 constexpr int CG_PROBE = 4;
 constexpr int CG_MAX_ITERATIONS = 200;
-constexpr double CG_TOLERANCE = 1e-10;
+constexpr double CG_TOLERANCE = 1e-10;        // the exact form (the reference's bar)
+constexpr double CG_TOLERANCE_FP32 = 1e-6;     // the plain and the fast form: fp32 vectors
 
 // ---------------------------------------------------------------------------------------------------------------------
 // Host: the fixed matrices of the quadratic form (src/cpu/cpu_solve_level1.cpp:103-187)
@@ -2399,6 +2400,7 @@ struct FastCg
     double* partial;          // two per chunk: the update's |r|^2 and r.z
     double* pap;              // one per chunk: the mat-vec's p.Ap
     int probe;                // the fold after the update applies the stopping bar
+    double tol;               // the stopping bar: relative residual below tol (the plain and the fast form: 1e-6)
 };
 
 // The workgroup's chunk: its plane and its texels [t0, t1) (chain indices).
@@ -2750,7 +2752,7 @@ __global__ __launch_bounds__(256) void k_fold_beta_fast(FastCg a)
         sc[SC_RZ] = rz_t;
         const double rhs2 = sc[SC_RHS2];
         sc[SC_RESID] = rhs2 > 0.0 ? sqrt((rr_t > 0.0 ? rr_t : 0.0) / rhs2) : 0.0;
-        if (a.probe && !(sc[SC_RESID] >= CG_TOLERANCE))
+        if (a.probe && !(sc[SC_RESID] >= a.tol))
             sc[SC_ACTIVE] = 0.0;
     }
 }
@@ -2778,7 +2780,7 @@ __global__ __launch_bounds__(256) void k_xpby_fold_fast(FastCg a)
     const double beta = rz_old != 0.0 ? rz_t / rz_old : 0.0;
     const double rhs2 = sc[SC_RHS2];
     const double resid = rhs2 > 0.0 ? sqrt((rr_t > 0.0 ? rr_t : 0.0) / rhs2) : 0.0;
-    const bool stop = a.probe && !(resid >= CG_TOLERANCE);
+    const bool stop = a.probe && !(resid >= a.tol);
     if (threadIdx.x == 0 && xcd_block(blockIdx.x, gridDim.x) == a.chunks.start[p])
     {
         sc[SC_RR] = rr_t;
@@ -3092,6 +3094,7 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     // The fast form: the inner products inside the mat-vec and the update, folded per plane by a workgroup each.
     FastCg fc;
     fc.L = L;
+    fc.tol = CG_TOLERANCE_FP32;
     fc.chunks.n = np;
     {
         long long acc = 0;

@@ -4,7 +4,7 @@
 //   nntc_hip_encode a.png [b.png ...] [--c0 N] [--c1 N] [--rounds R] [--tol X] [--k K] [--mips 0|1] [--mip-min S]
 //                   [--mip-weight pixels|sqrt|uniform] [--mip-mix X] [--q1-start R0] [--q1-sweeps N]
 //                   [--q1-range minmax|pct] [--ridge X] [--bc-refine N] [--bc-refine-after N] [--bc-outer N]
-//                   [--init0-scope base|chain] [--weights W,..] [--rgb-weights R,G,B] [--form exact|fast]
+//                   [--init0-scope base|chain] [--weights W,..] [--rgb-weights R,G,B] [--form exact|plain|fast]
 //                   [--quiet] [-o PREFIX]
 //
 // Options the reference has and this path does not build (--l0 palette, --init pca, --init0 texture|luma, named mip
@@ -130,9 +130,9 @@ int main(int argc, char** argv)
         else if (a == "--form")
         {
             const std::string v = next();
-            if (v != "exact" && v != "fast")
-                refuse("takes exact or fast");
-            o.form = v == "exact" ? NNTC_FORM_EXACT : NNTC_FORM_FAST;
+            if (v != "exact" && v != "fast" && v != "plain")
+                refuse("takes exact, plain or fast");
+            o.form = v == "exact" ? NNTC_FORM_EXACT : v == "plain" ? NNTC_FORM_PLAIN : NNTC_FORM_FAST;
         }
         else if (a == "--quiet")
             o.quiet = 1;

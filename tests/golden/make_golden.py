@@ -3,10 +3,12 @@
 Runs oracle/_ref/nntc_encode (the reference CPU encoder, built from /root/reference by oracle/Makefile) on a workload
 from nntc_amd.synth and records, under tests/golden/<workload>/:
     report.json   per-texture psnr, centre psnr, E shipped, every round's three E values, the decoder (W, b) and both
-                  grids from the asset's JSON, the sha256 of each .dds, and the reference's own wall clock and threads
+                  grids from the asset's JSON, the sha256 of each .dds, and the reference's own wall clock and threads;
+                  with --sse, each texture's integer sum of squared byte errors at the base level (from the
+                  reference's own reconstruction PNGs, written by its --png 1), so PSNR is known to full precision
     *.dds         the level-0 and level-1 files themselves (small workloads only)
 
-    python tests/golden/make_golden.py plain_cpu_4x256 [--rounds 8] [--threads 8]
+    python tests/golden/make_golden.py plain_cpu_4x256 [--rounds 8] [--threads 8] [--sse] [--keep-dir DIR]
 """
 
 from __future__ import annotations
@@ -21,6 +23,8 @@ import subprocess
 import sys
 import tempfile
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
@@ -54,6 +58,10 @@ def main():
     ap.add_argument("--rounds", type=int, default=8)
     ap.add_argument("--threads", type=int, default=8)
     ap.add_argument("--keep-dds", action="store_true", help="store the .dds files themselves")
+    ap.add_argument("--keep-json", action="store_true", help="store the asset's _nntc.json itself")
+    ap.add_argument("--sse", action="store_true",
+                    help="run with --png 1 and record each texture's integer squared-error sum at the base level")
+    ap.add_argument("--keep-dir", help="copy the whole reference asset (dds + json) into this directory")
     a = ap.parse_args()
     spec = synth.WORKLOADS[a.workload]
     tex = synth.workload(a.workload)
@@ -63,7 +71,7 @@ def main():
         paths = synth.write_pngs(tex, os.path.join(tmp, "m"))
         prefix = os.path.join(tmp, "out") + os.sep   # a directory: the asset is named after the first input
         cmd = [REF_BIN, *paths, "--c0", str(spec["c0"]), "--c1", str(spec["c1"]), "--rounds", str(a.rounds),
-               "--tol", "0", "--png", "0", "--backend", "cpu", "-j", str(a.threads), "-o", prefix]
+               "--tol", "0", "--png", "1" if a.sse else "0", "--backend", "cpu", "-j", str(a.threads), "-o", prefix]
         t0 = time.time()
         r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
         wall = time.time() - t0
@@ -83,6 +91,23 @@ def main():
             if a.keep_dds:
                 shutil.copy(os.path.join(od, f), os.path.join(out_dir, "ref" + key))
         js = json.load(open(os.path.join(od, "m_t0_nntc.json")))
+        if a.keep_json:
+            shutil.copy(os.path.join(od, "m_t0_nntc.json"), os.path.join(out_dir, "ref_nntc.json"))
+        if a.keep_dir:
+            os.makedirs(a.keep_dir, exist_ok=True)
+            for f in os.listdir(od):
+                if f.endswith(".dds") or f.endswith(".json"):
+                    shutil.copy(os.path.join(od, f), os.path.join(a.keep_dir, f))
+        if a.sse:
+            from PIL import Image
+
+            rep["sse"] = []
+            for t, src in enumerate(tex):
+                name = f"m_t0_recon_t{t}_M0.png" if len(tex) > 1 else "m_t0_recon_M0.png"
+                rec = np.asarray(Image.open(os.path.join(od, name)).convert("RGB"), dtype=np.int64)
+                d = rec - src.astype(np.int64)
+                rep["sse"].append(int((d * d).sum()))
+            rep["sse_samples"] = int(tex[0].size)
         rep["decoder"] = {"weights": js["decoder"]["layers"][0]["weights"], "bias": js["decoder"]["layers"][0]["bias"]}
         rep["grids"] = [t["dequantise"] for t in js["textures"]]
     json.dump(rep, open(os.path.join(out_dir, "report.json"), "w"), indent=1)

@@ -13,7 +13,7 @@ import os
 import numpy as np
 import pytest
 
-from nntc_amd import encoder, synth
+from nntc_amd import abi, encoder, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -27,6 +27,28 @@ def golden(name):
     return json.load(open(p))
 
 
+def texture_sse(e, tex):
+    """Each texture's sum of squared byte errors of the encoder's base decode against the source (the quantity the
+    reference's psnr line is made of, src/image.h:387-407)."""
+    h, w = tex[0].shape[:2]
+    pl = e.desc.planes[0]
+    rec = e.recon().reshape(pl.h0, pl.w0, -1)[:h, :w].astype(np.int64)
+    out = []
+    for t, src in enumerate(tex):
+        d = rec[:, :, 3 * t: 3 * t + 3] - src.astype(np.int64)
+        out.append(int((d * d).sum()))
+    return out
+
+
+def reference_psnr(g):
+    """The reference's per-texture PSNR to full precision, from its integer error sums (tests/golden/make_golden.py
+    --sse), or None for a fixture made without them."""
+    if "sse" not in g:
+        return None
+    n = g["sse_samples"]
+    return [10.0 * np.log10(255.0 * 255.0 / (s / n)) for s in g["sse"]]
+
+
 def encode(name, tmp_path, **kw):
     spec = synth.WORKLOADS[name]
     tex = synth.workload(name)
@@ -37,12 +59,17 @@ def encode(name, tmp_path, **kw):
     return e, rep, prefix
 
 
-@pytest.mark.parametrize("name", ["plain_cpu_4x256", "plain_gpu_4x1024"])
+@pytest.mark.parametrize("name", ["plain_cpu_4x256", "plain_gpu_4x1024", "precision_6x2048", "differentiation_4x4096"])
 def test_encode_exact_matches_reference(name, tmp_path):
+    """The exact form reproduces the reference CPU encoder: every round's E, the shipped E, the decoder's bits, the
+    per-texture error sums and the .dds bytes. The 2048^2 workload covers six textures (nout 18) and four level-0
+    channels (two BC5 files); the 4096^2 one three level-0 channels (a BC5 and a BC4 file) on ten planes."""
     g = golden(name)
     print(f"encode {name}", flush=True)
     e, rep, prefix = encode(name, tmp_path)
     print(f"  {rep.seconds:.3f} s, {rep.mpix / rep.seconds:.3f} Mpix/s", flush=True)
+    if "sse" in g:
+        assert texture_sse(e, synth.workload(name)) == g["sse"]
     assert rep.rounds_used == g["rounds_used"]
     for r, (ea, eb, ec) in enumerate(g["rounds"]):
         # The reference prints E to 7 significant digits; the exact form's E is the reference's double.
@@ -93,3 +120,29 @@ def test_fast_form_within_psnr_bar(name, tmp_path):
     assert r1.e_shipped == r2.e_shipped
     for a, b in zip(k1, k2):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("form", [abi.FORM_FAST, abi.FORM_PLAIN], ids=["fast", "plain"])
+@pytest.mark.parametrize("name", ["plain_gpu_4x1024", "precision_6x2048", "differentiation_4x4096"])
+def test_form_psnr_against_reference(name, form, tmp_path):
+    """The chosen (fast) and the plain form against the reference CPU encoder on the benchmark workloads: decoded PSNR
+    within 0.05 dB of the reference's on every texture (its integer error sums, so the bar is read to full precision),
+    and the written asset's level-1 values on the grid. These workloads have more than 256 chunks of 256 texels per
+    level-0 plane, so the fast conjugate gradients fold their partials in their own launches (the path a small plane
+    does not take)."""
+    g = golden(name)
+    ref = reference_psnr(g)
+    if ref is None:
+        pytest.skip(f"{name}: the fixture has no error sums")
+    print(f"form {form} {name}", flush=True)
+    e, rep, prefix = encode(name, tmp_path, form=form)
+    tex = synth.workload(name)
+    sse = texture_sse(e, tex)
+    n = tex[0].size
+    ours = [10.0 * np.log10(255.0 * 255.0 / (x / n)) for x in sse]
+    gap = [ours[t] - ref[t] for t in range(len(tex))]
+    print(f"  {rep.seconds:.3f} s; psnr gap per texture (dB): {', '.join(f'{x:+.5f}' for x in gap)}", flush=True)
+    for t in range(len(tex)):
+        assert abs(rep.psnr[t] - ours[t]) < 1e-9
+        assert abs(gap[t]) <= 0.05, f"texture {t}: {ours[t]:.5f} vs the reference's {ref[t]:.5f} dB"
+    assert rep.rounds_used == g["rounds_used"]

@@ -18,11 +18,13 @@ def material(name):
     if name in synth.WORKLOADS:
         spec = synth.WORKLOADS[name]
         return synth.workload(name), spec["c0"], spec["c1"]
-    textures, w, h, c0, c1 = {
-        "small_2x100x76_c3c3": (2, 100, 76, 3, 3),
-        "small_1x64x40_c1c2": (1, 64, 40, 1, 2),
+    textures, w, h, c0, c1, smooth = {
+        "small_2x100x76_c3c3": (2, 100, 76, 3, 3, 3),
+        "small_1x64x40_c1c2": (1, 64, 40, 1, 2, 3),
+        # six textures (nout 18) and four level-0 channels (two BC5 files), as the 2048^2 precision workload
+        "small_6x100x76_c4c4": (6, 100, 76, 4, 4, 5),
     }[name]
-    tex = synth.make_material(textures, (w, h), 32, 3, 1.0, config=9)
+    tex = synth.make_material(textures, (w, h), 32, smooth, 1.0, config=9)
     return tex, c0, c1
 
 
@@ -54,7 +56,8 @@ def compare(a, b, step, i):
                                      f"{bad[0]}: {x[bad[0]]!r} vs {y[bad[0]]!r}")
 
 
-@pytest.mark.parametrize("name", ["small_1x64x40_c1c2", "small_2x100x76_c3c3", "plain_cpu_4x256"])
+@pytest.mark.parametrize("name", ["small_1x64x40_c1c2", "small_2x100x76_c3c3", "small_6x100x76_c4c4",
+                                  "plain_cpu_4x256"])
 def test_lockstep_exact(name, hip_arm, ref_arm):
     print(f"lockstep {name}", flush=True)
     h, r, per_site = sessions(name, hip_arm, ref_arm)
