@@ -46,7 +46,7 @@ typedef struct nntc_encode_options
     int32_t bc_refine_after;     /* --bc-refine-after 0 */
     int32_t bc_outer;            /* --bc-outer 2 */
     int32_t init0_chain;         /* --init0-scope chain */
-    int32_t form;                /* NNTC_FORM_EXACT or NNTC_FORM_FAST (not a reference option) */
+    int32_t form;                /* NNTC_FORM_EXACT, NNTC_FORM_PLAIN or NNTC_FORM_FAST (not a reference option) */
     int32_t quiet;               /* --quiet */
     double weights[6];           /* --weights (per texture; 1 when unset) */
     double rgb_weights[3];       /* --rgb-weights 1,1,1 */
@@ -65,6 +65,12 @@ nntc_encoder* nntc_encoder_create(const nntc_encode_options* o, int textures, in
 /* The same from image files (PNG, 8-bit RGB / RGBA / grey), as `nntc_encode a.png b.png ...` reads them. */
 nntc_encoder* nntc_encoder_create_files(const nntc_encode_options* o, int textures, const char* const* paths,
                                         const char* arm_lib, const char* arm_prefix);
+
+/* The same from textures already in device memory: rgb_device[t] is texture t's width * height * 3 bytes (row-major
+ * RGB) on the gfx950 device. The packed source and its mip chain are built there (nntc_hip_device_create_rgb8), so no
+ * source crosses PCIe; the gfx950 arm only. */
+nntc_encoder* nntc_encoder_create_device(const nntc_encode_options* o, int textures, int width, int height,
+                                         const uint8_t* const* rgb_device);
 
 /* One whole encode on the resident device. Returns 0, or nonzero with the reason in nntc_encode_last_error. */
 int nntc_encoder_run(nntc_encoder* e);

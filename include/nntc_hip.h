@@ -162,6 +162,24 @@ int nntc_hip_device_select(int index, nntc_device_info* info);
  * source_chain[p] holds planes[p].w0 * planes[p].h0 * nout floats. Returns NULL on failure. */
 nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* model, const float* const* source_chain);
 
+/* device_create for textures already in device memory: no counterpart in the reference's seam, whose caller builds the
+ * source chain on the host (src/main.cpp:1792-1895) and hands it to be::device_create. Here the packed source plane
+ * (every texture's bytes times 1/255, edge-padded to the base plane's size: src/image.h:298-331) and its box mip chain
+ * (src/image.h:343-366) are built on the device from rgb8_device[t], texture t's width * height * 3 bytes (row-major
+ * RGB) in device memory, with the same float arithmetic, so the chain holds the bits device_create would be handed.
+ * model->planes[0] must be (width, height) rounded up to multiples of 4. Returns NULL on failure. */
+nntc_hip_ctx* nntc_hip_device_create_rgb8(const nntc_model_desc* model, const uint8_t* const* rgb8_device, int width,
+                                          int height);
+
+/* The exact-copy test of the box init's channel order (src/main.cpp:1938-1960) on the device's source chain: copies[s]
+ * (nout entries) has bit e set when channel e < s holds the same value as channel s at every texel of every plane.
+ * No counterpart in the reference's seam (its caller compares the host chain). */
+int nntc_hip_source_copies(nntc_hip_ctx* ctx, uint32_t* copies);
+
+/* One plane of the source chain as the device holds it (w0 * h0 * nout floats); n and count as the other downloads.
+ * No counterpart in the reference's seam. */
+int nntc_hip_source_download(nntc_hip_ctx* ctx, int plane, float* out, size_t n, size_t* count);
+
 /* Replaces be::device_destroy (src/backend.h:88); NULL is ignored. */
 void nntc_hip_device_destroy(nntc_hip_ctx* ctx);
 

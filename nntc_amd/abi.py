@@ -431,3 +431,36 @@ def make_desc(textures: int, c0: int, c1: int, planes, cw=None, bits1: int = 8) 
     for i, v in enumerate(cw):
         d.cw[i] = v
     return d
+
+
+class DeviceBytes:
+    """A host array copied into device memory through the HIP runtime the arm runs on (the runtime libnntc_hip.so
+    loaded, found by its soname), for entry points that take device pointers. Freed by close()."""
+
+    def __init__(self, data: np.ndarray):
+        C.CDLL(HIP_LIB, mode=C.RTLD_GLOBAL)   # the arm first, so the soname below resolves to its runtime
+        self.rt = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
+        self.rt.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.rt.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.rt.hipFree.argtypes = [C.c_void_p]
+        self.host = np.ascontiguousarray(data)
+        self.ptr = C.c_void_p(0)
+        if self.rt.hipMalloc(C.byref(self.ptr), self.host.nbytes) != 0:
+            raise ArmError("hipMalloc failed")
+        if self.rt.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1) != 0:   # hipMemcpyHostToDevice
+            raise ArmError("hipMemcpy failed")
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value)
+
+    def close(self):
+        if self.ptr.value:
+            self.rt.hipFree(self.ptr)
+            self.ptr = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -276,6 +276,10 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps);
 void copy_prev_zero_delta(Ctx& c, bool level0);
 // The fast form's per-site source samples (site s's nout values at [s nout]), built when K changes or an encode starts.
 const float* site_targets(Ctx& c, int k);
+// The source chain built on the device from the textures' RGB8 bytes in device memory (init.hip).
+void build_source_chain(Ctx& c, const uint8_t* const* rgb8_device, int width, int height);
+// copies[s] bit e (e < s): source channel e is an exact copy of channel s over the whole chain.
+void channel_copies(Ctx& c, unsigned int* copies);
 // init.cu counterparts (init.hip)
 void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0);
 void init_level1_box(Ctx& c);

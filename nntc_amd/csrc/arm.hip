@@ -622,11 +622,14 @@ int nntc_hip_device_select(int index, nntc_device_info* info)
     return 1;
 }
 
-nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* const* source_chain)
+namespace
 {
-    nntc_hip_ctx* h = new nntc_hip_ctx();
-    Ctx& c = h->c;
-    const int st = guarded([&] {
+
+// Everything device_create does but the source chain's upload: the plane table, the stream, every buffer, the
+// zeroed state and the starting grids.
+void create_model(Ctx& c, const nntc_model_desc* md)
+{
+    {
         if (!md || md->nplanes < 1 || md->nplanes > MAX_PLANES || md->c0 < 1 || md->c0 > MAX_CH || md->c1 < 1 ||
             md->c1 > MAX_CH || md->nout < 3 || md->nout > MAX_NOUT)
             throw Refused { "model descriptor out of range" };
@@ -749,11 +752,7 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
                        L.z.bytes() + L.p.bytes() + L.ap.bytes() + L.prev.bytes() + L.mono.bytes();
         }
 
-        // The source chain, the one upload before any block; the decoder starts at zero, the grids at the state
-        // device_create is handed (src/cpu/cpu_model.cpp:208-237).
-        for (int p = 0; p < md->nplanes; p++)
-            HIP_OK(hipMemcpyAsync(c.src.p + c.g(p).off0 * nout, source_chain[p],
-                                  (size_t)c.g(p).n0 * nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
+        // The decoder starts at zero, the grids at the state device_create is handed (src/cpu/cpu_model.cpp:208-237).
         HIP_OK(hipMemcpyAsync(c.cw.p, md->cw, nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
         HIP_OK(hipMemsetAsync(c.weights.p, 0, c.weights.bytes(), c.stream));
         HIP_OK(hipMemsetAsync(c.bias.p, 0, c.bias.bytes(), c.stream));
@@ -776,6 +775,21 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
             s.hi0[q] = 1.0f;
         }
         upload_grids(c, &s);
+    }
+}
+
+}   // namespace
+
+nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* const* source_chain)
+{
+    nntc_hip_ctx* h = new nntc_hip_ctx();
+    Ctx& c = h->c;
+    const int st = guarded([&] {
+        create_model(c, md);
+        // The source chain, the one upload before any block.
+        for (int p = 0; p < md->nplanes; p++)
+            HIP_OK(hipMemcpyAsync(c.src.p + c.g(p).off0 * c.nout, source_chain[p],
+                                  (size_t)c.g(p).n0 * c.nout * sizeof(float), hipMemcpyHostToDevice, c.stream));
         HIP_OK(hipStreamSynchronize(c.stream));
     });
     if (st)
@@ -784,6 +798,57 @@ nntc_hip_ctx* nntc_hip_device_create(const nntc_model_desc* md, const float* con
         return nullptr;
     }
     return h;
+}
+
+nntc_hip_ctx* nntc_hip_device_create_rgb8(const nntc_model_desc* md, const uint8_t* const* rgb8_device, int width,
+                                          int height)
+{
+    nntc_hip_ctx* h = new nntc_hip_ctx();
+    Ctx& c = h->c;
+    const int st = guarded([&] {
+        if (!md || !rgb8_device || width < 1 || height < 1 || md->nout < 3 || md->nout > MAX_NOUT)
+            throw Refused { "device_create_rgb8: no textures" };
+        const int wp = (width + 3) / 4 * 4, hp = (height + 3) / 4 * 4;
+        if (md->nplanes < 1 || md->planes[0].w0 != wp || md->planes[0].h0 != hp)
+            throw Refused { "device_create_rgb8: the base plane is not the textures padded to whole 4x4 blocks" };
+        for (int t = 0; t < md->nout / 3; t++)
+            if (!rgb8_device[t])
+                throw Refused { "device_create_rgb8: a texture pointer is null" };
+        create_model(c, md);
+        build_source_chain(c, rgb8_device, width, height);
+        HIP_OK(hipStreamSynchronize(c.stream));
+    });
+    if (st)
+    {
+        nntc_hip_device_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+int nntc_hip_source_copies(nntc_hip_ctx* h, uint32_t* copies)
+{
+    return guarded([&] {
+        unsigned int m[NNTC_MAX_NOUT] = {};
+        channel_copies(h->c, m);
+        for (int s = 0; s < h->c.nout; s++)
+            copies[s] = m[s];
+    });
+}
+
+int nntc_hip_source_download(nntc_hip_ctx* h, int plane, float* out, size_t n, size_t* count)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        if (plane < 0 || plane >= c.planes.n)
+            throw Refused { "source_download: no such plane" };
+        const size_t total = (size_t)c.g(plane).n0 * c.nout;
+        if (count)
+            *count = total;
+        const size_t m = std::min(n, total);
+        if (m)
+            HIP_OK(hipMemcpy(out, c.src.p + c.g(plane).off0 * c.nout, m * sizeof(float), hipMemcpyDeviceToHost));
+    });
 }
 
 void nntc_hip_device_destroy(nntc_hip_ctx* h)

@@ -380,6 +380,75 @@ __global__ __launch_bounds__(256) void k_cov_fold(const double* partial, int ent
     }
 }
 
+// ---- the source chain built on the device from the textures' bytes (nntc_hip_device_create_rgb8)
+
+struct Rgb8Textures
+{
+    const uint8_t* t[NNTC_MAX_NOUT / 3];
+};
+
+// The packed base plane (src/image.h:298-331, src/main.cpp:1865-1884): texel (x, y) of the padded wp x hp plane takes
+// the source pixel (min(x, width - 1), min(y, height - 1)) of every texture, each byte times the float 1/255.
+__global__ __launch_bounds__(256) void k_pack_base(Rgb8Textures tex, int textures, int width, int height, int wp,
+                                                   long long n, float* base)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const int y = (int)(i / wp), x = (int)(i - (long long)y * wp);
+    const int sx = x < width ? x : width - 1, sy = y < height ? y : height - 1;
+    const size_t at = ((size_t)sy * width + sx) * 3;
+    const float inv = 1.0f / 255.0f;
+    const int nout = 3 * textures;
+    for (int t = 0; t < textures; t++)
+        for (int c = 0; c < 3; c++)
+            base[(size_t)i * nout + 3 * t + c] = (float)tex.t[t][at + c] * inv;
+}
+
+// image_box (src/image.h:343-366): 0.25 ((a + b) + (c + d)) of the 2x2 block, the last row or column repeated on a
+// one-wide plane; one lane per output value.
+__global__ __launch_bounds__(256) void k_box_halve(const float* in, int w, int h, int nc, float* out, int ow, int oh)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)ow * oh * nc)
+        return;
+    const int k = (int)(i % nc);
+    const long long px = i / nc;
+    const int y = (int)(px / ow), x = (int)(px - (long long)y * ow);
+    const int xa = 2 * x < w ? 2 * x : w - 1, xb = 2 * x + 1 < w ? 2 * x + 1 : w - 1;
+    const int ya = 2 * y < h ? 2 * y : h - 1, yb = 2 * y + 1 < h ? 2 * y + 1 : h - 1;
+    const float a = in[((size_t)ya * w + xa) * nc + k], b = in[((size_t)ya * w + xb) * nc + k];
+    const float cc = in[((size_t)yb * w + xa) * nc + k], d = in[((size_t)yb * w + xb) * nc + k];
+    out[i] = 0.25f * ((a + b) + (cc + d));
+}
+
+// differs[s] bit e (e < s): channel s and channel e of the base plane differ somewhere. Each lane ORs its pixels'
+// comparisons, each workgroup folds its lanes' masks and adds them with one atomic OR per channel.
+__global__ __launch_bounds__(256) void k_channel_differs(const float* base, long long n, int nc, unsigned int* differs)
+{
+    __shared__ unsigned int s_m[NNTC_MAX_NOUT];
+    if (threadIdx.x < NNTC_MAX_NOUT)
+        s_m[threadIdx.x] = 0u;
+    __syncthreads();
+    unsigned int m[NNTC_MAX_NOUT];
+    for (int s = 0; s < NNTC_MAX_NOUT; s++)
+        m[s] = 0u;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    {
+        const float* v = base + (size_t)i * nc;
+        for (int s = 1; s < NNTC_MAX_NOUT; s++)
+            for (int e = 0; e < s; e++)
+                if (s < nc && v[s] != v[e])
+                    m[s] |= 1u << e;
+    }
+    for (int s = 1; s < NNTC_MAX_NOUT; s++)
+        if (m[s])
+            atomicOr(&s_m[s], m[s]);
+    __syncthreads();
+    if (threadIdx.x < NNTC_MAX_NOUT && s_m[threadIdx.x])
+        atomicOr(&differs[threadIdx.x], s_m[threadIdx.x]);
+}
+
 }   // namespace
 
 void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0)
@@ -580,6 +649,42 @@ void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const 
     hipLaunchKernelGGL(k_seed_channel, dim3(blocks_for(g.n0, 256)), dim3(256), 0, c.stream, resid, g.n0, c.nout, dir,
                        inv, c.c0, channel, c.v0.p + g.off0 * c.c0);
     HIP_OK(hipGetLastError());
+}
+
+}   // namespace nntc
+
+namespace nntc
+{
+
+void build_source_chain(Ctx& c, const uint8_t* const* rgb8_device, int width, int height)
+{
+    Rgb8Textures tex {};
+    const int textures = c.nout / 3;
+    for (int t = 0; t < textures; t++)
+        tex.t[t] = rgb8_device[t];
+    const PlaneGeom& g0 = c.g(0);
+    hipLaunchKernelGGL(k_pack_base, dim3(blocks_for(g0.n0, 256)), dim3(256), 0, c.stream, tex, textures, width, height,
+                       g0.w0, g0.n0, c.src.p);
+    for (int p = 1; p < c.planes.n; p++)
+    {
+        const PlaneGeom &a = c.g(p - 1), &b = c.g(p);
+        hipLaunchKernelGGL(k_box_halve, dim3(blocks_for(b.n0 * c.nout, 256)), dim3(256), 0, c.stream,
+                           (const float*)(c.src.p + a.off0 * c.nout), a.w0, a.h0, c.nout, c.src.p + b.off0 * c.nout,
+                           b.w0, b.h0);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+void channel_copies(Ctx& c, unsigned int* copies)
+{
+    unsigned int* d = c.work_u.p;   // the radix-select histograms' area, free outside a grid fit
+    HIP_OK(hipMemsetAsync(d, 0, NNTC_MAX_NOUT * sizeof(unsigned int), c.stream));
+    hipLaunchKernelGGL(k_channel_differs, dim3(1024), dim3(256), 0, c.stream, (const float*)c.src.p, c.g(0).n0, c.nout,
+                       d);
+    HIP_OK(hipGetLastError());
+    const unsigned int* h = read_back(c, (const unsigned int*)d, NNTC_MAX_NOUT);
+    for (int s = 0; s < c.nout; s++)
+        copies[s] = ~h[s] & ((1u << s) - 1u);
 }
 
 }   // namespace nntc

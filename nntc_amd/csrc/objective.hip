@@ -274,32 +274,49 @@ struct DecodeArgs
 // One pixel decoded at its centre in the decoder's own fp32 arithmetic (src/objective.cu:53-75): saturated and
 // rounded half away from zero to a byte, or, for the residual (src/init.cu:1024-1045), left unsaturated and
 // differenced against the target.
+// Pixel idx's features at its centre, and output r of the decoder on them (before saturation).
+__device__ __forceinline__ void decode_phi(const DecodeArgs& a, long long idx, float* phi)
+{
+    const int px = (int)(idx % a.w0), py = (int)(idx / a.w0);
+    float s[MAX_CH], c[MAX_CH];
+    for (int i = 0; i < a.c0; i++)
+        s[i] = a.v0[idx * a.c0 + i];
+    const Tap t = tap_pixel(px, py, 0.0f, 0.0f, a.w0, a.h0, a.w1, a.h1);
+    sample_bilinear(a.v1, a.w1, a.c1, t, c);
+    build_phi(s, a.c0, c, a.c1, phi);
+}
+
+__device__ __forceinline__ float decode_row(const DecodeArgs& a, const float* phi, int r)
+{
+    float acc = a.bias[r];
+    const float* row = a.weights + (size_t)r * a.nin;
+    for (int i = 0; i < a.nin; i++)
+        acc += row[i] * phi[i];
+    return acc;
+}
+
+// The decoded byte of a pre-saturation output: saturated, then rounded half away from zero.
+__device__ __forceinline__ uint8_t decode_byte(float acc)
+{
+    acc = acc < 0.0f ? 0.0f : (acc > 1.0f ? 1.0f : acc);
+    return (uint8_t)(int)roundf(acc * 255.0f);
+}
+
 template <bool RESIDUAL>
 __global__ __launch_bounds__(256) void k_decode(DecodeArgs a)
 {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= (long long)a.w0 * a.h0)
         return;
-    const int px = (int)(idx % a.w0), py = (int)(idx / a.w0);
-    float s[MAX_CH], c[MAX_CH], phi[MAX_NIN];
-    for (int i = 0; i < a.c0; i++)
-        s[i] = a.v0[idx * a.c0 + i];
-    const Tap t = tap_pixel(px, py, 0.0f, 0.0f, a.w0, a.h0, a.w1, a.h1);
-    sample_bilinear(a.v1, a.w1, a.c1, t, c);
-    build_phi(s, a.c0, c, a.c1, phi);
+    float phi[MAX_NIN];
+    decode_phi(a, idx, phi);
     for (int r = 0; r < a.nout; r++)
     {
-        float acc = a.bias[r];
-        const float* row = a.weights + (size_t)r * a.nin;
-        for (int i = 0; i < a.nin; i++)
-            acc += row[i] * phi[i];
+        const float acc = decode_row(a, phi, r);
         if (RESIDUAL)
             a.resid[idx * a.nout + r] = acc - a.src[idx * a.nout + r];
         else
-        {
-            acc = acc < 0.0f ? 0.0f : (acc > 1.0f ? 1.0f : acc);
-            a.out[idx * a.nout + r] = (uint8_t)(int)roundf(acc * 255.0f);
-        }
+            a.out[idx * a.nout + r] = decode_byte(acc);
     }
 }
 

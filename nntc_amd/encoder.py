@@ -60,6 +60,9 @@ def lib():
         L.nntc_encoder_create_files.restype = C.c_void_p
         L.nntc_encoder_create_files.argtypes = [C.POINTER(Options), C.c_int, C.POINTER(C.c_char_p), C.c_char_p,
                                                 C.c_char_p]
+        L.nntc_encoder_create_device.restype = C.c_void_p
+        L.nntc_encoder_create_device.argtypes = [C.POINTER(Options), C.c_int, C.c_int, C.c_int,
+                                                 C.POINTER(C.c_void_p)]
         L.nntc_encoder_run.argtypes = [C.c_void_p]
         L.nntc_encoder_destroy.argtypes = [C.c_void_p]
         L.nntc_encode_last_error.restype = C.c_char_p
@@ -85,12 +88,18 @@ def options(**kw) -> Options:
 
 class Encoder:
     def __init__(self, textures: list[np.ndarray] | None = None, files: list[str] | None = None,
-                 arm_lib: str | None = None, arm_prefix: str | None = None, **opts):
+                 arm_lib: str | None = None, arm_prefix: str | None = None, device=None, **opts):
+        """textures: host RGB8 arrays (H, W, 3); files: PNG paths; device: (width, height, [device pointers of each
+        texture's W*H*3 bytes]) for textures already in GPU memory (the caller keeps them alive)."""
         L = lib()
         self.opts = options(**opts)
         a = arm_lib.encode() if arm_lib else None
         p = arm_prefix.encode() if arm_prefix else None
-        if files is not None:
+        if device is not None:
+            w, h, ptrs_dev = device
+            arr = (C.c_void_p * len(ptrs_dev))(*[C.c_void_p(int(x)) for x in ptrs_dev])
+            self.h = L.nntc_encoder_create_device(C.byref(self.opts), len(ptrs_dev), w, h, arr)
+        elif files is not None:
             arr = (C.c_char_p * len(files))(*[f.encode() for f in files])
             self.h = L.nntc_encoder_create_files(C.byref(self.opts), len(files), arr, a, p)
         else:

@@ -25,6 +25,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace nntc_host
@@ -455,38 +456,21 @@ void run_encode(Encoder& E)
 // Encoder: the host half
 // ---------------------------------------------------------------------------------------------------------------------
 
-void Encoder::setup(int textures, int width, int height, const uint8_t* const* rgb)
+// The plane layout (src/mips.cpp:25-40), the mip weights and the model descriptor of textures x width x height.
+void Encoder::describe(int textures, int width, int height)
 {
     if (textures < 1 || textures > 6)
         throw Failure { "one to six textures" };
     if (opts.c0 < 1 || opts.c0 > 4 || opts.c1 < 1 || opts.c1 > 4)
         throw Failure { "--c0 and --c1 take 1..4" };
+    if (width < 1 || height < 1)
+        throw Failure { "an empty texture" };
     const int nout = 3 * textures;
     source_w = width;
     source_h = height;
     const int wp = (width + 3) / 4 * 4, hp = (height + 3) / 4 * 4;
-    // The packed source, padded by edge replication (src/image.h:298-331, src/main.cpp:1865-1884).
-    std::vector<float> base((size_t)wp * hp * nout);
-    const float inv = 1.0f / 255.0f;
-    for (int y = 0; y < hp; y++)
-    {
-        const int sy = std::min(y, height - 1);
-        for (int x = 0; x < wp; x++)
-        {
-            const int sx = std::min(x, width - 1);
-            for (int t = 0; t < textures; t++)
-                for (int c = 0; c < 3; c++)
-                    base[((size_t)y * wp + x) * nout + 3 * t + c] =
-                        (float)rgb[t][((size_t)sy * width + sx) * 3 + c] * inv;
-        }
-    }
-    // The plane layout and the box chain (src/mips.cpp:25-70, src/image.h:335-366).
     std::vector<nntc_plane_size> planes;
     plane_layout(wp, hp, opts.mips != 0, opts.mip_min, planes);
-    chain.assign(planes.size(), std::vector<float>());
-    chain[0] = std::move(base);
-    for (size_t i = 1; i < planes.size(); i++)
-        box_halve(chain[i - 1], planes[i - 1].w0, planes[i - 1].h0, nout, chain[i], planes[i].w0, planes[i].h0);
     mip_weights(planes, opts.mip_weight, opts.mip_mix, opts.k, per_site);
 
     std::memset(&desc, 0, sizeof(desc));
@@ -509,12 +493,60 @@ void Encoder::setup(int textures, int width, int height, const uint8_t* const* r
         const double rs = opts.rgb_weights[0] + opts.rgb_weights[1] + opts.rgb_weights[2];
         desc.cw[c] = (float)(opts.weights[c / 3] * opts.rgb_weights[c % 3] * 3.0 / rs);
     }
-    std::vector<const float*> ptrs(planes.size());
-    for (size_t i = 0; i < planes.size(); i++)
-        ptrs[i] = chain[i].data();
+}
+
+void Encoder::setup(int textures, int width, int height, const uint8_t* const* rgb)
+{
+    describe(textures, width, height);
+    const int nout = desc.nout, wp = desc.planes[0].w0, hp = desc.planes[0].h0, np = desc.nplanes;
+    // The packed source, padded by edge replication (src/image.h:298-331, src/main.cpp:1865-1884), then the box chain
+    // (src/image.h:343-366), each plane's rows shared among host threads (every value is computed on its own, so the
+    // split changes no bits).
+    chain.assign((size_t)np, std::vector<float>());
+    chain[0].resize((size_t)wp * hp * nout);
+    const float inv = 1.0f / 255.0f;
+    float* base = chain[0].data();
+    parallel_rows(hp, [&](int y0, int y1) {
+        for (int y = y0; y < y1; y++)
+        {
+            const int sy = std::min(y, height - 1);
+            for (int x = 0; x < wp; x++)
+            {
+                const int sx = std::min(x, width - 1);
+                for (int t = 0; t < textures; t++)
+                    for (int c = 0; c < 3; c++)
+                        base[((size_t)y * wp + x) * nout + 3 * t + c] =
+                            (float)rgb[t][((size_t)sy * width + sx) * 3 + c] * inv;
+            }
+        }
+    });
+    for (int i = 1; i < np; i++)
+    {
+        const nntc_plane_size &a = desc.planes[i - 1], &b = desc.planes[i];
+        chain[(size_t)i].resize((size_t)b.w0 * b.h0 * nout);
+        parallel_rows(b.h0, [&](int y0, int y1) {
+            box_halve_rows(chain[(size_t)i - 1], a.w0, a.h0, nout, chain[(size_t)i], b.w0, y0, y1);
+        });
+    }
+    std::vector<const float*> ptrs((size_t)np);
+    for (int i = 0; i < np; i++)
+        ptrs[(size_t)i] = chain[(size_t)i].data();
     ctx = seam.device_create(&desc, ptrs.data());
     if (!ctx)
         throw Failure { std::string("device_create: ") + seam.last_error() };
+    if (opts.form != NNTC_FORM_EXACT)
+        check(seam.set_form(ctx, opts.form), "set_form");
+}
+
+void Encoder::setup_device(int textures, int width, int height, const uint8_t* const* rgb_device)
+{
+    if (seam.name != "hip")
+        throw Failure { "textures in device memory need the gfx950 arm" };
+    describe(textures, width, height);
+    ctx = nntc_hip_device_create_rgb8(&desc, rgb_device, width, height);
+    if (!ctx)
+        throw Failure { std::string("device_create_rgb8: ") + nntc_hip_last_error() };
+    chain_on_device = true;
     if (opts.form != NNTC_FORM_EXACT)
         check(seam.set_form(ctx, opts.form), "set_form");
 }
@@ -526,10 +558,13 @@ void Encoder::box_reorder()
 {
     const int nout = desc.nout, c1 = desc.c1;
     std::vector<int> order, copies;
+    uint32_t dev_copies[NNTC_MAX_NOUT] = {};
+    if (chain_on_device)
+        check(nntc_hip_source_copies(ctx, dev_copies), "source_copies");
     for (int s = 0; s < nout; s++)
     {
-        bool copy = false;
-        for (int e = 0; e < s && !copy; e++)
+        bool copy = chain_on_device && dev_copies[s] != 0;
+        for (int e = 0; e < s && !copy && !chain_on_device; e++)
         {
             bool same = true;
             for (size_t i = 0; i < chain.size() && same; i++)
@@ -551,6 +586,17 @@ void Encoder::box_reorder()
         plain = plain && chosen[(size_t)c] == c;
     if (plain)
         return;
+    if (chain_on_device && chain.empty())
+    {
+        // The reordered init reads the chain on the host: fetch it once from the device.
+        chain.assign((size_t)desc.nplanes, std::vector<float>());
+        for (int i = 0; i < desc.nplanes; i++)
+        {
+            chain[(size_t)i].resize((size_t)desc.planes[i].w0 * desc.planes[i].h0 * nout);
+            check(nntc_hip_source_download(ctx, i, chain[(size_t)i].data(), chain[(size_t)i].size(), nullptr),
+                  "source_download");
+        }
+    }
     for (int i = 0; i < desc.nplanes; i++)
     {
         const nntc_plane_size& pl = desc.planes[i];
@@ -685,29 +731,69 @@ nntc_encoder* nntc_encoder_create(const nntc_encode_options* o, int textures, in
     return nullptr;
 }
 
+nntc_encoder* nntc_encoder_create_device(const nntc_encode_options* o, int textures, int width, int height,
+                                         const uint8_t* const* rgb_device)
+{
+    Encoder* e = new Encoder();
+    try
+    {
+        e->opts = *o;
+        if (!e->seam.load(nullptr, nullptr))
+            throw Failure { g_error };
+        if (!rgb_device)
+            throw Failure { "no textures" };
+        e->setup_device(textures, width, height, rgb_device);
+        return reinterpret_cast<nntc_encoder*>(e);
+    }
+    catch (const Failure& f)
+    {
+        g_error = f.what;
+    }
+    delete e;
+    return nullptr;
+}
+
 nntc_encoder* nntc_encoder_create_files(const nntc_encode_options* o, int textures, const char* const* paths,
                                         const char* arm_lib, const char* arm_prefix)
 {
+    if (textures < 1 || textures > 6)
+    {
+        g_error = "one to six textures";
+        return nullptr;
+    }
+    // The textures are read side by side, one host thread each (PNG decoding is sequential within a file).
     std::vector<std::vector<uint8_t>> images((size_t)textures);
+    std::vector<int> ws((size_t)textures, 0), hs((size_t)textures, 0), alphas((size_t)textures, 0);
+    std::vector<std::string> whys((size_t)textures);
+    std::vector<char> ok((size_t)textures, 0);
+    {
+        std::vector<std::thread> readers;
+        for (int t = 0; t < textures; t++)
+            readers.emplace_back([&, t] {
+                bool alpha = false;
+                ok[(size_t)t] = load_png_rgb(paths[t], images[(size_t)t], ws[(size_t)t], hs[(size_t)t], alpha,
+                                             whys[(size_t)t]);
+                alphas[(size_t)t] = alpha;
+            });
+        for (std::thread& r : readers)
+            r.join();
+    }
     int w0 = 0, h0 = 0;
     for (int t = 0; t < textures; t++)
     {
-        int w = 0, h = 0;
-        bool alpha = false;
-        std::string why;
-        if (!load_png_rgb(paths[t], images[(size_t)t], w, h, alpha, why))
+        if (!ok[(size_t)t])
         {
-            g_error = std::string("cannot read '") + paths[t] + "': " + why;
+            g_error = std::string("cannot read '") + paths[t] + "': " + whys[(size_t)t];
             return nullptr;
         }
-        if (alpha)
+        if (alphas[(size_t)t])
             std::printf("WARNING: %s has an alpha channel; it is ignored (the tool encodes 24-bit RGB)\n", paths[t]);
         if (t == 0)
         {
-            w0 = w;
-            h0 = h;
+            w0 = ws[0];
+            h0 = hs[0];
         }
-        else if (w != w0 || h != h0)
+        else if (ws[(size_t)t] != w0 || hs[(size_t)t] != h0)
         {
             g_error = std::string("'") + paths[t] + "' is not the size of '" + paths[0] + "'";
             return nullptr;

@@ -10,10 +10,12 @@
 
 #include <zlib.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace nntc_host
 {
@@ -258,6 +260,43 @@ void box_halve(const std::vector<float>& in, int w, int h, int nc, std::vector<f
             for (int k = 0; k < nc; k++)
                 o[k] = 0.25f * ((a[k] + b[k]) + (c[k] + d[k]));
         }
+}
+
+void box_halve_rows(const std::vector<float>& in, int w, int h, int nc, std::vector<float>& out, int ow, int y0,
+                    int y1)
+{
+    for (int y = y0; y < y1; y++)
+        for (int x = 0; x < ow; x++)
+        {
+            const int xa = 2 * x < w ? 2 * x : w - 1, xb = 2 * x + 1 < w ? 2 * x + 1 : w - 1;
+            const int ya = 2 * y < h ? 2 * y : h - 1, yb = 2 * y + 1 < h ? 2 * y + 1 : h - 1;
+            const float* a = &in[((size_t)ya * w + xa) * nc];
+            const float* b = &in[((size_t)ya * w + xb) * nc];
+            const float* c = &in[((size_t)yb * w + xa) * nc];
+            const float* d = &in[((size_t)yb * w + xb) * nc];
+            float* o = &out[((size_t)y * ow + x) * nc];
+            for (int k = 0; k < nc; k++)
+                o[k] = 0.25f * ((a[k] + b[k]) + (c[k] + d[k]));
+        }
+}
+
+void parallel_rows(int rows, const std::function<void(int, int)>& body)
+{
+    const int hw = (int)std::thread::hardware_concurrency();
+    const int n = std::max(1, std::min({ hw > 0 ? hw : 1, 16, rows / 64 }));
+    if (n <= 1)
+    {
+        body(0, rows);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (int i = 0; i < n; i++)
+    {
+        const int y0 = (int)((long long)rows * i / n), y1 = (int)((long long)rows * (i + 1) / n);
+        pool.emplace_back([&body, y0, y1] { body(y0, y1); });
+    }
+    for (std::thread& t : pool)
+        t.join();
 }
 
 // build_mip_weights (src/mips.cpp:141-169): kind 0 pixels, 1 sqrt, 2 uniform.

@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -98,7 +99,10 @@ struct Encoder
 
     void check(int status, const char* what);
     void print_call_times();
+    bool chain_on_device = false;            // the chain was built on the device; `chain` is fetched if needed
+    void describe(int textures, int width, int height);
     void setup(int textures, int width, int height, const uint8_t* const* rgb);
+    void setup_device(int textures, int width, int height, const uint8_t* const* rgb_device);
     void box_reorder();
     void verify_level1_on_grid();
     void decode_base();
@@ -111,6 +115,11 @@ bool load_png_rgb(const char* path, std::vector<uint8_t>& rgb, int& w, int& h, b
 bool save_png_rgb(const char* path, const uint8_t* rgb, int w, int h);
 void plane_layout(int width, int height, bool mips, int mip_min, std::vector<nntc_plane_size>& planes);
 void box_halve(const std::vector<float>& in, int w, int h, int nc, std::vector<float>& out, int ow, int oh);
+// Rows [y0, y1) of box_halve's output, which is already sized ow x oh x nc.
+void box_halve_rows(const std::vector<float>& in, int w, int h, int nc, std::vector<float>& out, int ow, int y0,
+                    int y1);
+// body(y0, y1) over [0, rows) in contiguous pieces, one host thread each (up to the hardware's threads, 16 at most).
+void parallel_rows(int rows, const std::function<void(int, int)>& body);
 void mip_weights(const std::vector<nntc_plane_size>& planes, int kind, double mix, int k, std::vector<double>& per_site);
 
 // level1_value (src/model.h:159-168), for the host's on-grid check.

@@ -6,6 +6,7 @@ workload (tests/golden/make_golden.py). The exact form must reproduce them: the 
 coefficients and grids, the same per-texture psnr, the same E in every round.
 """
 
+import ctypes as C
 import hashlib
 import json
 import os
@@ -13,7 +14,7 @@ import os
 import numpy as np
 import pytest
 
-from nntc_amd import abi, encoder, synth
+from nntc_amd import abi, chain, encoder, synth
 
 pytestmark = pytest.mark.gpu
 
@@ -146,3 +147,80 @@ def test_form_psnr_against_reference(name, form, tmp_path):
         assert abs(rep.psnr[t] - ours[t]) < 1e-9
         assert abs(gap[t]) <= 0.05, f"texture {t}: {ours[t]:.5f} vs the reference's {ref[t]:.5f} dB"
     assert rep.rounds_used == g["rounds_used"]
+
+
+def _source_planes(lib, ctx, desc):
+    f = lib.nntc_hip_source_download
+    f.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_size_t)]
+    out = []
+    for p in range(desc.nplanes):
+        pl = desc.planes[p]
+        a = np.zeros(pl.w0 * pl.h0 * desc.nout, np.float32)
+        assert f(ctx, p, a.ctypes.data_as(C.POINTER(C.c_float)), a.size, None) == 0
+        out.append(a)
+    return out
+
+
+@pytest.mark.parametrize("name", ["small_2x100x76", "plain_cpu_4x256"])
+def test_device_resident_source_chain(name, hip_arm, tmp_path):
+    """nntc_hip_device_create_rgb8 builds, from the textures' bytes in device memory, the source chain the host builds
+    (the packed, edge-padded base and every box-filtered plane, bit for bit), and an exact-form encode from
+    device-resident textures writes the reference's .dds bytes."""
+    spec = synth.WORKLOADS[name]
+    tex = synth.workload(name)
+    print(f"device-resident source {name}", flush=True)
+    h, w = tex[0].shape[:2]
+    planes, ch = chain.source_chain(tex)
+    desc = abi.make_desc(len(tex), spec["c0"], spec["c1"], planes, chain.output_weights(len(tex)))
+    host = abi.Session(hip_arm, desc, [c.reshape(-1) for c in ch])
+    dev = [abi.DeviceBytes(t) for t in tex]
+    lib = hip_arm.lib
+    lib.nntc_hip_device_create_rgb8.restype = C.c_void_p
+    lib.nntc_hip_device_create_rgb8.argtypes = [C.POINTER(abi.ModelDesc), C.POINTER(C.c_void_p), C.c_int, C.c_int]
+    ptrs = (C.c_void_p * len(dev))(*[C.c_void_p(d.address) for d in dev])
+    ctx = lib.nntc_hip_device_create_rgb8(C.byref(desc), ptrs, w, h)
+    assert ctx, lib.nntc_hip_last_error()
+    try:
+        for p, (a, b) in enumerate(zip(_source_planes(lib, host.ctx, desc), _source_planes(lib, ctx, desc))):
+            np.testing.assert_array_equal(a, b, err_msg=f"source plane {p}")
+            np.testing.assert_array_equal(a, ch[p].reshape(-1), err_msg=f"source plane {p} against the host chain")
+    finally:
+        hip_arm.fn["device_destroy"](ctx)
+    host.close()
+    e = encoder.Encoder(device=(w, h, [d.address for d in dev]), c0=spec["c0"], c1=spec["c1"], rounds=8, tol=0.0,
+                        quiet=1)
+    rep = e.run()
+    prefix = str(tmp_path / "m")
+    e.write(prefix)
+    g = golden(name)
+    for suffix, digest in g["dds"].items():
+        assert hashlib.sha256(open(prefix + suffix, "rb").read()).hexdigest() == digest
+    assert float(f"{rep.e_shipped:.9e}") == g["e_shipped"]
+    e.close()
+
+
+def test_device_resident_channel_copies(hip_arm, tmp_path):
+    """The box init takes exact copies of earlier source channels last (src/main.cpp:1938-2027). With texture 1 a copy
+    of texture 0, the device's copy test finds channels 3-5 copies of 0-2, and the encode from device-resident textures
+    (which fetches the chain for the reordered init) equals the encode from host textures."""
+    base = synth.make_material(2, (64, 40), 32, 3, 1.0, config=9)
+    tex = [base[0], base[0].copy(), base[1]]
+    h, w = tex[0].shape[:2]
+    dev = [abi.DeviceBytes(t) for t in tex]
+    kw = dict(c0=2, c1=4, rounds=4, tol=0.0, quiet=1)
+    a = encoder.Encoder(tex, **kw)
+    ra = a.run()
+    b = encoder.Encoder(device=(w, h, [d.address for d in dev]), **kw)
+    rb = b.run()
+    assert ra.e_shipped == rb.e_shipped
+    for p in range(a.desc.nplanes):
+        np.testing.assert_array_equal(a.level1_indices(p), b.level1_indices(p))
+        np.testing.assert_array_equal(a.level0_blocks(p, 0), b.level0_blocks(p, 0))
+    lib = hip_arm.lib
+    planes, ch = chain.source_chain(tex)
+    desc = abi.make_desc(len(tex), 2, 4, planes, chain.output_weights(len(tex)))
+    s = abi.Session(hip_arm, desc, [c.reshape(-1) for c in ch])
+    m = (C.c_uint32 * 18)()
+    lib.nntc_hip_source_copies.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
+    assert lib.nntc_hip_source_copies(s.ctx, m) == 0
+    assert [m[i] for i in range(9)] == [0, 0, 0, 1, 2, 4, 0, 0, 0]
