@@ -260,6 +260,17 @@ int nntc_hip_decode_plane(nntc_hip_ctx* ctx, int plane, uint8_t* rgb8, size_t n)
  * sums' exact values. */
 int nntc_hip_texture_sse(nntc_hip_ctx* ctx, int width, int height, unsigned long long* sse);
 
+/* No counterpart in the reference: nntc_hip_texture_sse for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). It allocates nothing and does not wait: the decode and the
+ * error sums run in one pass with no decoded plane stored, the per-texture counters live in a fixed 48 KiB device array
+ * of the library (one per GPU, held until the process ends), and the sums are written to sse_device, `textures`
+ * unsigned 64-bit values in device memory, by work enqueued on `stream` (NULL: the context's stream). The inputs are
+ * the context's latents and decoder as its stream leaves them. Limits: at most 6 textures (the counters' capacity) and
+ * an extent within the base plane; beyond them it fails with a message. Calls that overlap on one GPU race on the
+ * shared counters: a caller issues them one after another (on one stream, or with its own ordering). */
+int nntc_hip_texture_sse_noalloc(nntc_hip_ctx* ctx, int width, int height, unsigned long long* sse_device,
+                                 void* stream);
+
 /* Replaces be::objective_eval (src/backend.h:133). per_site has nplanes entries. */
 int nntc_hip_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* r);
 

@@ -454,6 +454,15 @@ class DeviceBytes:
     def address(self) -> int:
         return int(self.ptr.value)
 
+    def read(self) -> np.ndarray:
+        """The device bytes back in an array shaped and typed like the one uploaded, after every queued device
+        operation has finished."""
+        self.rt.hipDeviceSynchronize()
+        out = np.empty_like(self.host)
+        if self.rt.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) != 0:   # hipMemcpyDeviceToHost
+            raise ArmError("hipMemcpy failed")
+        return out
+
     def close(self):
         if self.ptr.value:
             self.rt.hipFree(self.ptr)

@@ -128,6 +128,7 @@ struct Ctx
     nntc_model_desc md {};
     int nin = 0, nout = 0, c0 = 0, c1 = 0;
     int form = NNTC_FORM_EXACT;
+    int device = -1;                       // the HIP device the context was created on
     Planes planes {};
     long long n0_total = 0, n1_total = 0, nb_total = 0;
     hipStream_t stream = nullptr;
@@ -260,6 +261,11 @@ void objective_pass(Ctx& c, int k, double* sums_out /* nplanes * 3 */);
 void decode_pass(Ctx& c, int plane, uint8_t* out);
 void residual_pass(Ctx& c, int plane, float* resid);
 void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out);
+// The allocation-free variant's scratch address on `device` (the current device), taken once per device.
+void noalloc_setup(int device);
+// texture_sse_pass without its plane buffer, with counters in the file-scope scratch; enqueued on `stream`, the sums
+// written to sse_device (device memory, one per texture).
+void texture_sse_noalloc_pass(Ctx& c, int width, int height, unsigned long long* sse_device, hipStream_t stream);
 // decoder.hip
 // Block (a)'s normal equations into c.ls_sum (and, when to_host, into acc).
 void ls_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& acc, bool to_host = true);

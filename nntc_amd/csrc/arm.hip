@@ -668,6 +668,8 @@ void create_model(Ctx& c, const nntc_model_desc* md)
         c.n0_total = o0;
         c.n1_total = o1;
         c.nb_total = ob;
+        HIP_OK(hipGetDevice(&c.device));
+        noalloc_setup(c.device);
         HIP_OK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         HIP_OK(hipEventCreate(&c.ev0));
         HIP_OK(hipEventCreate(&c.ev1));
@@ -1395,6 +1397,15 @@ int nntc_hip_texture_sse(nntc_hip_ctx* h, int width, int height, unsigned long l
         if (width < 0 || height < 0 || width > c.g(0).w0 || height > c.g(0).h0)
             throw Refused { "texture_sse: the extent exceeds the base plane" };
         texture_sse_pass(c, width, height, sse);
+    });
+}
+
+int nntc_hip_texture_sse_noalloc(nntc_hip_ctx* h, int width, int height, unsigned long long* sse_device,
+                                 void* stream)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        texture_sse_noalloc_pass(c, width, height, sse_device, stream ? (hipStream_t)stream : c.stream);
     });
 }
 

@@ -20,6 +20,8 @@
 #include "arm.h"
 #include "tile_stage.h"
 
+#include <mutex>
+
 namespace nntc
 {
 
@@ -371,6 +373,82 @@ __global__ __launch_bounds__(256) void k_texture_sse(const uint8_t* rgb8, const 
     }
 }
 
+// ---- the allocation-free texture error sums (nntc_hip_texture_sse_noalloc)
+//
+// The decode and the error sums in one pass, so no decoded plane is stored: every lane decodes its pixels of the
+// top-left width x height at their centres (decode_phi / decode_row / decode_byte: k_decode's arithmetic, so the bytes
+// are decode_plane's) and adds each texture's squared byte error; a workgroup's sums go to SSE_SLOTS counters per
+// texture in the file-scope scratch below, which a second kernel folds into the caller's output.
+
+// The scratch: SSE_SLOTS counters, SSE_STRIDE apart, for each of at most NNTC_MAX_NOUT / 3 textures - 48 KiB, held on
+// every device the library runs on until the process ends.
+constexpr int NOALLOC_TEXTURES = NNTC_MAX_NOUT / 3;
+constexpr size_t NOALLOC_SCRATCH = (size_t)NOALLOC_TEXTURES * SSE_SLOTS * SSE_STRIDE;
+__device__ __align__(256) unsigned long long scratch[NOALLOC_SCRATCH];
+
+__global__ __launch_bounds__(256) void k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        p[i] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_decode_sse(DecodeArgs a, int width, int height, unsigned long long* slots)
+{
+    __shared__ unsigned long long s_part[4][NOALLOC_TEXTURES];
+    const int textures = a.nout / 3;
+    const long long n = (long long)width * height;
+    unsigned long long s[NOALLOC_TEXTURES];
+#pragma unroll
+    for (int t = 0; t < NOALLOC_TEXTURES; t++)
+        s[t] = 0;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    {
+        const int x = (int)(i % width), y = (int)(i / width);
+        const long long idx = (long long)y * a.w0 + x;
+        float phi[MAX_NIN];
+        decode_phi(a, idx, phi);
+#pragma unroll
+        for (int t = 0; t < NOALLOC_TEXTURES; t++)
+            if (t < textures)
+                for (int ch = 0; ch < 3; ch++)
+                {
+                    const int r = 3 * t + ch;
+                    const float f = a.src[idx * a.nout + r];
+                    const int want = (int)lroundf(fminf(1.0f, fmaxf(0.0f, f)) * 255.0f);
+                    const int d = want - (int)decode_byte(decode_row(a, phi, r));
+                    s[t] += (unsigned long long)(d * d);
+                }
+    }
+#pragma unroll
+    for (int t = 0; t < NOALLOC_TEXTURES; t++)
+    {
+        for (int off = 32; off > 0; off >>= 1)
+            s[t] += __shfl_xor(s[t], off);
+        if (threadIdx.x % 64 == 0)
+            s_part[threadIdx.x / 64][t] = s[t];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < textures)
+    {
+        const int t = threadIdx.x;
+        const unsigned long long sum = s_part[0][t] + s_part[1][t] + s_part[2][t] + s_part[3][t];
+        if (sum)
+            atomicAdd(slots + ((size_t)t * SSE_SLOTS + blockIdx.x % SSE_SLOTS) * SSE_STRIDE, sum);
+    }
+}
+
+__global__ void k_fold_sse(const unsigned long long* slots, int textures, unsigned long long* out)
+{
+    const int t = threadIdx.x;
+    if (t >= textures)
+        return;
+    unsigned long long sum = 0;
+    for (int k = 0; k < SSE_SLOTS; k++)
+        sum += slots[((size_t)t * SSE_SLOTS + k) * SSE_STRIDE];
+    out[t] = sum;
+}
+
 }   // namespace
 
 void texture_sse_pass(Ctx& c, int width, int height, unsigned long long* sse_out)
@@ -531,6 +609,54 @@ static DecodeArgs decode_args(Ctx& c, int plane)
     a.out = nullptr;
     a.resid = nullptr;
     return a;
+}
+
+// The scratch's device address on each device, taken once per device when a context is first created on it (with that
+// device current). The array's name is a host-side stand-in; kernels take this address.
+namespace
+{
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex g_noalloc_lock;
+unsigned long long* g_noalloc_scratch[NOALLOC_DEVICES] = {};
+}   // namespace
+
+void noalloc_setup(int device)
+{
+    if (device < 0 || device >= NOALLOC_DEVICES)
+        throw Refused { "texture_sse_noalloc: device index beyond the scratch table" };
+    std::lock_guard<std::mutex> hold(g_noalloc_lock);
+    if (g_noalloc_scratch[device])
+        return;
+    void* ptr = nullptr;
+    HIP_OK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_noalloc_scratch[device] = static_cast<unsigned long long*>(ptr);
+}
+
+void texture_sse_noalloc_pass(Ctx& c, int width, int height, unsigned long long* sse_device, hipStream_t stream)
+{
+    const int textures = c.nout / 3;
+    if (textures > NOALLOC_TEXTURES || (size_t)textures * SSE_SLOTS * SSE_STRIDE > NOALLOC_SCRATCH)
+        throw Refused { "texture_sse_noalloc: more textures than the scratch holds (6)" };
+    if (width < 0 || height < 0 || width > c.g(0).w0 || height > c.g(0).h0)
+        throw Refused { "texture_sse_noalloc: the extent exceeds the base plane" };
+    if (!sse_device)
+        throw Refused { "texture_sse_noalloc: no output" };
+    unsigned long long* slots = nullptr;
+    {
+        std::lock_guard<std::mutex> hold(g_noalloc_lock);
+        slots = c.device >= 0 && c.device < NOALLOC_DEVICES ? g_noalloc_scratch[c.device] : nullptr;
+    }
+    if (!slots)
+        throw Refused { "texture_sse_noalloc: the scratch of this context's device was not set up" };
+    const int n = (int)((size_t)textures * SSE_SLOTS * SSE_STRIDE);
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)blocks_for(n, 256)), dim3(256), 0, stream, slots, n);
+    DecodeArgs a = decode_args(c, 0);
+    const unsigned grid = (unsigned)std::max<long long>(1, std::min<long long>(1024, blocks_for((long long)width * height,
+                                                                                                 256)));
+    hipLaunchKernelGGL(k_decode_sse, dim3(grid), dim3(256), 0, stream, a, width, height, slots);
+    hipLaunchKernelGGL(k_fold_sse, dim3(1), dim3(64), 0, stream, (const unsigned long long*)slots, textures,
+                       sse_device);
+    HIP_OK(hipGetLastError());
 }
 
 void decode_pass(Ctx& c, int plane, uint8_t* out)
