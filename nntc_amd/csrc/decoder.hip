@@ -185,9 +185,9 @@ __global__ __launch_bounds__(256) void k_ls_reduce_tree(const double* partial, c
 // matrix cores, v_mfma_f32_32x32x16_bf16 per 32 x 32 block of G and 16 sites, keeping every product down to 2^-16 of
 // h h^T: h h^T + h m^T + m h^T + m m^T + h l^T + l h^T (the products of two bf16 values are exact; the matrix core adds
 // them in fp32). On a diagonal block the mirrored pairs are one product and its transpose, so a diagonal block takes
-// four products, A += h h^T + m m^T and S += h m^T + h l^T, and G = A + S + S^T at the end. The fp32 accumulators are
-// flushed into fp64 registers after every 64 sites (one round of the wavefront), so fp32 rounding acts on at most 64
-// terms. The four wavefronts' blocks are added in double, in order, and the tile's entries written; a second kernel
+// four products, A += h h^T + m m^T and S += h m^T + h l^T, and G = A + S + S^T at the end. The leading part's fp32
+// accumulator is flushed into fp64 registers after every 64 sites (one round of the wavefront), so fp32 rounding acts
+// on at most 64 of its terms. The four wavefronts' blocks are added in double, in order, and the tile's entries written; a second kernel
 // folds the tiles of each plane, times the plane's weight. (With two parts and three products - 16 bits of each
 // feature - the decoder's small coefficients moved by up to 2 % of their size from the fp64 solve; with three parts and
 // the per-round flush they stay within 1e-4.)
@@ -227,23 +227,38 @@ __device__ __forceinline__ int gram_slot(int f, int s)
     return f * 256 + ((((s >> 3) ^ (f & 31)) << 3) | (s & 7));
 }
 
+// The workgroup's LDS (dynamic, sized by the launch): the three part tables [feature][site] (gram_slot order; after the
+// last round the first 8 DP^2 bytes hold G in double, the tables being no longer read then), then the staged level 1,
+// then the staged level 0 at nout + C0 floats a pixel (odd), so a workload of 12 outputs keeps the one-block kernel
+// within half the CU's LDS and two workgroups share a CU.
+template <int DB>
+constexpr size_t gram_table_bytes()
+{
+    return (size_t)3 * 32 * DB * 256 * sizeof(unsigned short);
+}
+inline size_t gram_lds_bytes(int db, int nout, int c0)
+{
+    return (size_t)3 * 32 * db * 256 * sizeof(unsigned short) + (size_t)TILE_L1_FLOATS * sizeof(float) +
+           (size_t)TILE_L0_PIX * tile_l0_stride(nout, c0) * sizeof(float);
+}
+
+// Two workgroups a CU for the one-block form (registers held to 256 a lane); the two-block form's tables take most of
+// the LDS, one workgroup a CU.
 template <int C0, int C1, int DB>
-__global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
+__global__ __launch_bounds__(256, DB == 1 ? 2 : 1) void k_gram_tiles(GramArgs a)
 {
     constexpr int NIN = C1 + C0 + C0 * C1, MM = NIN + 1;
     constexpr int DP = 32 * DB;                 // features padded to whole 32-blocks
     constexpr int NT = DB * (DB + 1) / 2;       // the blocks of G's upper block triangle
     constexpr int TABLE = DP * 256;             // bf16 values per part table
-    __shared__ float s_l0[tile_l0_floats<C0>()];
-    __shared__ float s_l1[TILE_L1_FLOATS];
-    // The three part tables [feature][site] (gram_slot order); after the last round the first 8 DP^2 bytes hold G in
-    // double (the tables are no longer read then).
-    __shared__ __align__(16) unsigned short s_tab[3 * TABLE];
-    static_assert(DP * DP * sizeof(double) <= 3 * TABLE * sizeof(unsigned short), "G fits over the tables");
-    unsigned short* s_h = s_tab;
-    unsigned short* s_m = s_tab + TABLE;
-    unsigned short* s_lo = s_tab + 2 * TABLE;
-    double* s_g = reinterpret_cast<double*>(s_tab);
+    extern __shared__ __align__(16) unsigned char s_gram_lds[];
+    static_assert(DP * DP * sizeof(double) <= gram_table_bytes<DB>(), "G fits over the tables");
+    unsigned short* s_h = reinterpret_cast<unsigned short*>(s_gram_lds);
+    unsigned short* s_m = s_h + TABLE;
+    unsigned short* s_lo = s_h + 2 * TABLE;
+    double* s_g = reinterpret_cast<double*>(s_gram_lds);
+    float* s_l1 = reinterpret_cast<float*>(s_gram_lds + gram_table_bytes<DB>());
+    float* s_l0 = s_l1 + TILE_L1_FLOATS;
 
     const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
     const int p = range_plane(a.tiles, tile);
@@ -252,10 +267,11 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
                                            s_l0, s_l1);
     const int px = tg.tx0 + tid % TILE_W, py = tg.ty0 + tid / TILE_W;
     const bool live = px < tg.w0 && py < tg.h0;
-    // acc[t]: block t's h h^T + m m^T (diagonal) or all six products (off the diagonal); sacc[b]: diagonal block b's
-    // h m^T + h l^T. acc64 / sacc64: their fp64 sums over the rounds.
+    // acc[t]: block t's h h^T + m m^T (diagonal) or all six products (off the diagonal), flushed into acc64 (fp64)
+    // after every round; sacc[b]: diagonal block b's h m^T + h l^T, 2^-8 of the block's size, kept in fp32 over the
+    // tile's 320 sites (its rounding then sits 2^-8 below the flushed part's).
     floatx16 acc[NT], sacc[DB];
-    double acc64[NT][16], sacc64[DB][16];
+    double acc64[NT][16];
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -265,7 +281,7 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
     for (int b = 0; b < DB; b++)
 #pragma unroll
         for (int r = 0; r < 16; r++)
-            sacc64[b][r] = 0.0;
+            sacc[b][r] = 0.0f;
 
     // Rows past the last target (the padding of G to whole 32-blocks) stay zero in every round.
     const int D = MM + nout;
@@ -312,11 +328,6 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
 #pragma unroll
             for (int r = 0; r < 16; r++)
                 acc[t][r] = 0.0f;
-#pragma unroll
-        for (int b = 0; b < DB; b++)
-#pragma unroll
-            for (int r = 0; r < 16; r++)
-                sacc[b][r] = 0.0f;
         // ---- the wavefront's 64 sites, 16 per product; lane (feature l % 32, sites 8 (l / 32) .. + 7)
 #pragma unroll
         for (int ks = 0; ks < 4; ks++)
@@ -361,11 +372,6 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
 #pragma unroll
             for (int r = 0; r < 16; r++)
                 acc64[t][r] += (double)acc[t][r];
-#pragma unroll
-        for (int b = 0; b < DB; b++)
-#pragma unroll
-            for (int r = 0; r < 16; r++)
-                sacc64[b][r] += (double)sacc[b][r];
         __syncthreads();
     }
 
@@ -389,7 +395,7 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
                     {
                         const int m = 32 * bi + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
                         const int n = 32 * bj + lane % 32;
-                        s_g[m * DP + n] += bi == bj ? acc64[t][r] + sacc64[bi][r] : acc64[t][r];
+                        s_g[m * DP + n] += bi == bj ? acc64[t][r] + (double)sacc[bi][r] : acc64[t][r];
                     }
         }
         __syncthreads();
@@ -405,7 +411,7 @@ __global__ __launch_bounds__(256) void k_gram_tiles(GramArgs a)
                 {
                     const int m = 32 * b + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
                     const int n = 32 * b + lane % 32;
-                    s_g[n * DP + m] += sacc64[b][r];
+                    s_g[n * DP + m] += (double)sacc[b][r];
                 }
         }
         __syncthreads();
@@ -463,10 +469,11 @@ __global__ __launch_bounds__(256) void k_gram_reduce(const double* partial, cons
 template <int C0, int C1>
 void launch_gram_tiles_db(int db, unsigned grid, hipStream_t s, const GramArgs& a)
 {
+    const size_t lds = gram_lds_bytes(db, a.nout, C0);
     if (db == 1)
-        hipLaunchKernelGGL((k_gram_tiles<C0, C1, 1>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_gram_tiles<C0, C1, 1>), dim3(grid), dim3(256), lds, s, a);
     else
-        hipLaunchKernelGGL((k_gram_tiles<C0, C1, 2>), dim3(grid), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_gram_tiles<C0, C1, 2>), dim3(grid), dim3(256), lds, s, a);
 }
 
 template <int C0>
