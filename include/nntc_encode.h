@@ -50,6 +50,8 @@ typedef struct nntc_encode_options
     int32_t quiet;               /* --quiet */
     double weights[6];           /* --weights (per texture; 1 when unset) */
     double rgb_weights[3];       /* --rgb-weights 1,1,1 */
+    int32_t matrix_free;         /* the fast form's level-0 operator: -1 by size, 0 assembled, 1 matrix-free
+                                    (nntc_hip_set_matrix_free; not a reference option) */
 } nntc_encode_options;
 
 void nntc_encode_default_options(nntc_encode_options* o);

@@ -207,6 +207,20 @@ int nntc_hip_quantise_level0(nntc_hip_ctx* ctx, nntc_model_state* state, int pol
 int nntc_hip_freeze_level0_grid(nntc_hip_ctx* ctx, nntc_model_state* state, int policy);
 int nntc_hip_snap_level0_on_grid(nntc_hip_ctx* ctx, nntc_model_state* state);
 
+/* The fast form's level-0 operator (no counterpart in the reference: it has one). -1 (default): the conjugate
+ * gradients of block (c') apply the matrix-free operator - the decode linearised at every site and taken back through
+ * its adjoint - when the base plane has 4096^2 texels or more, the assembled fp32 stencil below that; 0: always the
+ * assembled stencil; 1: always the matrix-free operator. The quantised sweeps and the BC refinement always read the
+ * assembled stencil. */
+int nntc_hip_set_matrix_free(nntc_hip_ctx* ctx, int mode);
+
+/* Block (c')'s operator applied once, for checks (no counterpart in the reference): level 0's system is assembled for
+ * the state, as solve_level0_cont_all assembles it, and out = (H + lambda I) p on every plane, from the form's
+ * assembled stencil (mode 0) or the fast form's matrix-free operator (mode 1). p and out hold every plane's level-0
+ * values (the chain's n0 * c0, interleaved like the planes). */
+int nntc_hip_level0_apply(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge, int mode,
+                          const double* p, double* out);
+
 /* Replaces be::stencil_monomials (src/backend.h:102): 1 + c + c (c + 1) / 2. */
 int nntc_hip_stencil_monomials(int c0);
 
@@ -270,6 +284,29 @@ int nntc_hip_texture_sse(nntc_hip_ctx* ctx, int width, int height, unsigned long
  * shared counters: a caller issues them one after another (on one stream, or with its own ordering). */
 int nntc_hip_texture_sse_noalloc(nntc_hip_ctx* ctx, int width, int height, unsigned long long* sse_device,
                                  void* stream);
+
+/* ---- The decode and its adjoint on one plane (no counterpart in the reference's seam; its arms never differentiate the
+ * decode). The sites of a w0 x h0 level-0 plane are numbered s = (py w0 + px)(1 + k) + j for sub-texel site j of pixel
+ * (px, py) (src/sample.cuh:87-149); at each the level-0 sample s0 (the pixel's own texel at j = 0, else the bilinear
+ * blend of its clamped taps) and the level-1 sample s1 of the w1 x h1 plane give phi = [s1][s0][s0 s1] and
+ * out = W phi + b (src/sample.cuh:74-85, src/objective.cu:53-75 before saturation). Every array is device memory,
+ * double when fp64 is nonzero, else float: v0 (w0 h0 c0), v1 (w1 h1 c1), W (nout x nin, nin = c1 + c0 + c0 c1), b
+ * (nout), out and g (sites x nout). The work runs on `stream` (NULL: the null stream) and the call returns when it is
+ * done. */
+typedef struct nntc_sites_desc
+{
+    int32_t w0, h0, w1, h1;
+    int32_t c0, c1, nout, k;
+} nntc_sites_desc;
+
+int nntc_hip_decode_sites(const nntc_sites_desc* d, int fp64, const void* v0, const void* v1, const void* w,
+                          const void* b, void* out, void* stream);
+
+/* The adjoint of nntc_hip_decode_sites at (v0, v1, W, b) for the cotangent g: dv0, dv1, dW and db are the gradients of
+ * sum_s,r g_sr out_sr (the transposed bilinear sampling into both latent planes, and the decoder's gradient), each
+ * written whole, every sum in a fixed order. */
+int nntc_hip_decode_adjoint(const nntc_sites_desc* d, int fp64, const void* v0, const void* v1, const void* w,
+                            const void* g, void* dv0, void* dv1, void* dw, void* db, void* stream);
 
 /* Replaces be::objective_eval (src/backend.h:133). per_site has nplanes entries. */
 int nntc_hip_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* r);

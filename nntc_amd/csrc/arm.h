@@ -129,6 +129,8 @@ struct Ctx
     int nin = 0, nout = 0, c0 = 0, c1 = 0;
     int form = NNTC_FORM_EXACT;
     int device = -1;                       // the HIP device the context was created on
+    int matrix_free = -1;                  // the fast level-0 operator: -1 by size (4096^2 and up), 0 assembled, 1 matrix-free
+    int assembled_k = -1;                  // the K of the last assembly
     Planes planes {};
     long long n0_total = 0, n1_total = 0, nb_total = 0;
     hipStream_t stream = nullptr;
@@ -277,6 +279,9 @@ void decoder_solve_device(Ctx& c, const nntc_model_state* st, double inv_norm, b
 void build_block_matrices(Ctx& c, bool level0);
 void assemble(Ctx& c, bool level0, int k, double ridge);
 void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations);
+// (H + lambda I) p for level 0's last assembled system, every plane: mode 0 the form's assembled stencil, mode 1 the
+// fast form's matrix-free operator. p and out hold n0_total * c0 values.
+void level0_apply(Ctx& c, int mode, const double* p_host, double* out_host);
 void correction_and_probe(Ctx& c, bool level0);
 void quant_sweeps(Ctx& c, bool level0, int sweeps);
 void copy_prev_zero_delta(Ctx& c, bool level0);

@@ -233,6 +233,17 @@ int guarded(F&& f)
     return 1;
 }
 
+}   // namespace
+
+// guarded() for the other source files of the arm.
+int guarded_call(const std::function<void()>& f)
+{
+    return guarded(f);
+}
+
+namespace
+{
+
 inline int tri_index(int i, int j, int mm)
 {
     return i * mm - i * (i - 1) / 2 + (j - i);
@@ -891,6 +902,29 @@ int nntc_hip_set_form(nntc_hip_ctx* h, int form)
         if (form != NNTC_FORM_EXACT && form != NNTC_FORM_FAST && form != NNTC_FORM_PLAIN)
             throw Refused { "unknown form" };
         h->c.form = form;
+    });
+}
+
+int nntc_hip_set_matrix_free(nntc_hip_ctx* h, int mode)
+{
+    return guarded([&] {
+        if (mode < -1 || mode > 1)
+            throw Refused { "set_matrix_free takes -1, 0 or 1" };
+        h->c.matrix_free = mode;
+    });
+}
+
+int nntc_hip_level0_apply(nntc_hip_ctx* h, const nntc_model_state* st, int k, double ridge, int mode, const double* p,
+                          double* out)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        if (mode != 0 && mode != 1)
+            throw Refused { "level0_apply: mode 0 (assembled) or 1 (matrix-free)" };
+        upload_decoder(c, st);
+        build_block_matrices(c, true);
+        assemble(c, true, k, ridge);
+        level0_apply(c, mode, p, out);
     });
 }
 

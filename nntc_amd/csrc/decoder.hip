@@ -293,6 +293,7 @@ __global__ __launch_bounds__(256, DB == 1 ? 2 : 1) void k_gram_tiles(GramArgs a)
             s_m[gram_slot(f, tid)] = 0;
             s_lo[gram_slot(f, tid)] = 0;
         }
+    __syncthreads();   // the staged tile, written by every lane, is read by every lane's sites below
     for (int j = 0; j <= a.k; j++)
     {
         // ---- this round's site of every lane, as a column of its three bf16 parts (zeros past the plane)
@@ -639,6 +640,8 @@ __global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
     int taken = 3;
     for (int step = 0; step < 3 && taken == 3; step++)
     {
+        // Every lane has read the previous rung's s_ok and sL before they are written again.
+        __syncthreads();
         const double ridge = a.ladder[step] * mean_diag;
         for (int e = tid; e < mm * mm; e += 256)
         {

@@ -20,6 +20,8 @@
 // the stencil of its texels and their neighbours' corrections.
 
 #include "arm.h"
+#include "adjoint.h"
+#include "tile_stage.h"
 
 #include <cmath>
 #include <type_traits>
@@ -182,14 +184,7 @@ struct AssembleArgs
     const double* tab;              // the fast gathers' slot weights of an interior texel (k_gather_table)
 };
 
-NNTC_HD void footprint_range(int ti, int n0, int n1, int& lo, int& hi)
-{
-    const double r = (double)n0 / (double)n1;
-    lo = (int)ceil(((double)ti - 0.5) * r - 1.0);
-    hi = (int)ceil(((double)ti + 1.5) * r) - 1;
-    lo = lo < 0 ? 0 : lo;
-    hi = hi > n0 - 1 ? n0 - 1 : hi;
-}
+// (footprint_range: adjoint.h)
 
 // The monomials of the form in the one order the matrices and the sums use: 1, s_i, then s_i s_j (i <= j).
 template <int CS>
@@ -686,10 +681,7 @@ void launch_gather(int c, int cs, long long texels, hipStream_t s, const Assembl
 // The fast form's view of a site's taps: a tap texel's weight separates into a column weight times a row weight (both
 // taps of a clamped pair add up at the one texel), so the weights of a texel's five owned slots are products of five
 // column and row weights - no merge of the four taps and no test per tap and slot.
-__device__ __forceinline__ float tap_weight(int x, int x0, int x1, float f)
-{
-    return (x0 == x ? 1.0f - f : 0.0f) + (x1 == x ? f : 0.0f);
-}
+// (tap_weight: adjoint.h)
 
 // w_self (the site's weight at texel (tx, ty)) times its weight at each owned slot's texel; false when the texel holds no
 // weight of the site.
@@ -710,21 +702,7 @@ __device__ __forceinline__ bool slot_scales(const Tap& t, int tx, int ty, double
     return true;
 }
 
-// The taps of site j of pixel (px, py) into a w x h plane; level 0's centre site reads its own texel with weight 1.
-__device__ __forceinline__ Tap site_taps(int k, int j, int px, int py, int w0, int h0, int w, int h, bool nearest_centre)
-{
-    if (nearest_centre && j == 0)
-    {
-        Tap t;
-        t.x0 = t.x1 = px;
-        t.y0 = t.y1 = py;
-        t.fx = t.fy = 0.0f;
-        return t;
-    }
-    float dx, dy;
-    site_delta(k, j, px, py, dx, dy);
-    return tap_pixel(px, py, dx, dy, w0, h0, w, h);
-}
+// (site_taps: adjoint.h)
 
 // One DPP move of a double (both halves) under the data-parallel-primitive control CTRL, within the row.
 template <int CTRL>
@@ -2657,6 +2635,262 @@ __global__ __launch_bounds__(1024) void k_matvec_fast_rows(FastCg a)
     }
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// The matrix-free level-0 operator: (H + lambda I) p from the decode's adjoint instead of the assembled stencil
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// Level 0's system is H = J^T C J over the sites (C = diag(cw)): at a site the decode is affine in the level-0 sample,
+// d out_r / d s0 = m_r (adjoint.h's level0_row of the site's level-1 sample), and the sample is the bilinear map of
+// its taps. So (H p)_t = sum over the sites s whose taps reach t of w_st y_s, y_s = sum_r cw_r m_r (m_r . p(s)) with
+// p(s) the bilinear sample of p at s - the decode linearised at the site (J p), weighted, and taken back through the
+// decode's adjoint (J^T), without forming H. One 256-lane workgroup per MF_TW x MF_TH tile of texels: it stages p over
+// the tile and a two-texel halo, level 1 over the level-1 texels the halo's sites tap, and the decoder; every site of
+// the tile's pixels and their one-pixel halo forms its y_s into LDS; each lane then gathers its texel's 3 x 3 pixels'
+// sites in pixel-row, pixel, site order (the assembly's order) and adds lambda p_t. Bytes per launch: p read once
+// (about 1.5 times with the halo), Ap written once, level 1 from the cache; the decoder's rows are recomputed per site
+// (C0 C1 + 2 C0 + 1 multiply-adds per output), so the kernel is bound by the vector ALUs, not HBM - the assembled
+// stencil it replaces is 5 C0^2 fp32 values per texel, read on every iteration. p . Ap is summed per chunk afterwards
+// (k_pap_chunks), so the fast conjugate gradients' folds read the partials they read from the assembled mat-vec.
+constexpr int MF_TW = 32, MF_TH = 8;
+constexpr int MF_PW = MF_TW + 4, MF_PH = MF_TH + 4;              // p: the tile and a two-texel halo
+constexpr int MF_SW = MF_TW + 2, MF_SH = MF_TH + 2;              // sites: the tile's pixels and a one-pixel halo
+constexpr int MF_L1W = MF_SW + 6, MF_L1H = MF_SH + 6;            // level 1 tapped by those sites (w1 <= w0)
+
+struct MatFreeArgs
+{
+    FastCg fc;
+    Ranges tiles;                // per plane: its tiles
+    const float* v1;             // level 1 of the chain
+    long long off1[MAX_PLANES];  // each plane's first level-1 texel
+    const float* weights;
+    const float* cw;
+    const double* tab;           // the interior classes' site lists (k_gather_table) for this K
+    int nout, nin, k;
+};
+
+template <int C0, int C1>
+__global__ __launch_bounds__(256) void k_adjoint_matvec0(MatFreeArgs a)
+{
+    // A lane's sites of the tile's pixels and halo: at most ceil(MF_SW MF_SH 5 / 256).
+    constexpr int SPL = (MF_SW * MF_SH * 5 + 255) / 256;
+    __shared__ float s_w[NNTC_MAX_NOUT * NNTC_MAX_NIN + NNTC_MAX_NOUT];
+    __shared__ float s_p[MF_PW * MF_PH * C0];
+    __shared__ float s_l1[MF_L1W * MF_L1H * C1];
+    __shared__ float s_y[MF_SW * MF_SH * 5 * C0];
+    __shared__ int s_code[GATHER_TAB_MAX];       // the interior classes' site lists (k_gather_table): code, weight
+    __shared__ float s_wt[GATHER_TAB_MAX];
+    __shared__ int s_len[4];
+    const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
+    const int p = range_plane(a.tiles, tile);
+    const LatentView& L = a.fc.L;
+    const double* sc = L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;   // the whole workgroup: one plane
+    const PlaneGeom& g = L.planes.p[p];
+    const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
+    const int tiles_x = (w0 + MF_TW - 1) / MF_TW;
+    const int tl = tile - (int)a.tiles.start[p];
+    const int tx0 = (tl % tiles_x) * MF_TW, ty0 = (tl / tiles_x) * MF_TH;
+    const int tid = threadIdx.x, nout = a.nout, nin = a.nin, per = 1 + a.k;
+    const float* pf = cg_vec<true>(L.p) + L.texels.start[p] * C0;
+
+    // ---- stage the decoder, the interior site lists, p (clamped into the plane) and level 1
+    for (int e = tid; e < nout * nin; e += 256)
+        s_w[e] = a.weights[e];
+    if (tid < nout)
+        s_w[NNTC_MAX_NOUT * NNTC_MAX_NIN + tid] = a.cw[tid];
+    for (int e = tid; e < GATHER_TAB_MAX; e += 256)
+    {
+        s_code[e] = (int)a.tab[e * GATHER_TAB_STRIDE];
+        s_wt[e] = (float)a.tab[e * GATHER_TAB_STRIDE + 1];
+    }
+    if (tid < 4)
+        s_len[tid] = (int)a.tab[GATHER_TAB_MAX * GATHER_TAB_STRIDE + tid];
+    for (int e = tid; e < MF_PW * MF_PH * C0; e += 256)
+    {
+        const int i = e % C0, b = e / C0;
+        const int x = clamp_index(tx0 - 2 + b % MF_PW, w0), y = clamp_index(ty0 - 2 + b / MF_PW, h0);
+        s_p[e] = pf[((long long)y * w0 + x) * C0 + i];
+    }
+    int lx0, lx1, ly0, ly1;
+    level1_box(tx0 - 1, MF_SW, w0, w1, lx0, lx1);
+    level1_box(ty0 - 1, MF_SH, h0, h1, ly0, ly1);
+    const int lw = min(lx1 - lx0 + 1, MF_L1W), lh = min(ly1 - ly0 + 1, MF_L1H);
+    const float* v1 = a.v1 + a.off1[p] * C1;
+    for (int e = tid; e < lw * lh * C1; e += 256)
+    {
+        const int q = e % C1, b = e / C1;
+        s_l1[e] = v1[((long long)(ly0 + b / lw) * w1 + (lx0 + b % lw)) * C1 + q];
+    }
+    __syncthreads();
+
+    // ---- y_s for every site of the tile's pixels and their halo. Each lane takes sites tid, tid + 256, ...: first
+    // their samples of p and of level 1, then the decoder's rows one output at a time (each row read once a lane, for
+    // all its sites).
+    const int nsites = MF_SW * MF_SH * per;
+    float bp[SPL][C0], cbar[SPL][C1], y[SPL][C0];
+    bool live[SPL];
+#pragma unroll
+    for (int u = 0; u < SPL; u++)
+    {
+        const int q = tid + 256 * u;
+        const int j = q % per, hp = q / per;
+        const int px = tx0 - 1 + hp % MF_SW, py = ty0 - 1 + hp / MF_SW;
+        live[u] = q < nsites && px >= 0 && px < w0 && py >= 0 && py < h0;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+        {
+            y[u][i] = 0.0f;
+            bp[u][i] = 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < C1; c++)
+            cbar[u][c] = 0.0f;
+        if (!live[u])
+            continue;
+        {
+            const Tap t0 = site_taps(a.k, j, px, py, w0, h0, w0, h0, true);
+            const int xa = t0.x0 - (tx0 - 2), xb = t0.x1 - (tx0 - 2);
+            const int ya = t0.y0 - (ty0 - 2), yb = t0.y1 - (ty0 - 2);
+            const float w00 = (1.0f - t0.fx) * (1.0f - t0.fy), w10 = t0.fx * (1.0f - t0.fy);
+            const float w01 = (1.0f - t0.fx) * t0.fy, w11 = t0.fx * t0.fy;
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                bp[u][i] = fmaf(w11, s_p[(yb * MF_PW + xb) * C0 + i],
+                                fmaf(w01, s_p[(yb * MF_PW + xa) * C0 + i],
+                                     fmaf(w10, s_p[(ya * MF_PW + xb) * C0 + i], w00 * s_p[(ya * MF_PW + xa) * C0 + i])));
+        }
+        {
+            const Tap t1 = site_taps(a.k, j, px, py, w0, h0, w1, h1, false);
+            const int xa = min(max(t1.x0 - lx0, 0), lw - 1), xb = min(max(t1.x1 - lx0, 0), lw - 1);
+            const int ya = min(max(t1.y0 - ly0, 0), lh - 1), yb = min(max(t1.y1 - ly0, 0), lh - 1);
+            const float w00 = (1.0f - t1.fx) * (1.0f - t1.fy), w10 = t1.fx * (1.0f - t1.fy);
+            const float w01 = (1.0f - t1.fx) * t1.fy, w11 = t1.fx * t1.fy;
+#pragma unroll
+            for (int c = 0; c < C1; c++)
+                cbar[u][c] = fmaf(w11, s_l1[(yb * lw + xb) * C1 + c],
+                                  fmaf(w01, s_l1[(yb * lw + xa) * C1 + c],
+                                       fmaf(w10, s_l1[(ya * lw + xb) * C1 + c], w00 * s_l1[(ya * lw + xa) * C1 + c])));
+        }
+    }
+    const float* cw = s_w + NNTC_MAX_NOUT * NNTC_MAX_NIN;
+    for (int r = 0; r < nout; r++)
+    {
+        const float* row = s_w + r * nin;
+        float rb[C0], rq[C0][C1];
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+        {
+            rb[i] = row[C1 + i];
+#pragma unroll
+            for (int q = 0; q < C1; q++)
+                rq[i][q] = row[C1 + C0 + i * C1 + q];
+        }
+        const float wr = cw[r];
+#pragma unroll
+        for (int u = 0; u < SPL; u++)
+        {
+            // d out_r / d s0 at the site (level0_row's arithmetic), then the site's share of J^T C J p.
+            float m[C0];
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+            {
+                float qq = rb[i];
+#pragma unroll
+                for (int q = 0; q < C1; q++)
+                    qq = fmaf(cbar[u][q], rq[i][q], qq);
+                m[i] = qq;
+            }
+            float d = 0.0f;
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                d = fmaf(m[i], bp[u][i], d);
+            const float t = wr * d;
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                y[u][i] = fmaf(t, m[i], y[u][i]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < SPL; u++)
+    {
+        const int q = tid + 256 * u;
+        if (q < MF_SW * MF_SH * 5)
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                s_y[q * C0 + i] = y[u][i];
+    }
+    __syncthreads();
+
+    // ---- each lane's texel: its sites' y weighted by their taps' weight there (an interior texel's sites and weights
+    // from its class's list, the others from their taps), then lambda p
+    const int tx = tx0 + tid % MF_TW, ty = ty0 + tid / MF_TW;
+    if (tx >= w0 || ty >= h0)
+        return;
+    float acc[C0];
+    const float lambda = (float)sc[SC_LAMBDA];
+#pragma unroll
+    for (int i = 0; i < C0; i++)
+        acc[i] = lambda * s_p[((ty - ty0 + 2) * MF_PW + (tx - tx0 + 2)) * C0 + i];
+    if (tx >= 2 && tx <= w0 - 3 && ty >= 2 && ty <= h0 - 3)
+    {
+        const int cls = (tx & 1) + 2 * (ty & 1);
+        const int n = s_len[cls];
+        const int corner = ((ty - ty0) * MF_SW + (tx - tx0)) * per;   // pixel (tx - 1, ty - 1) in the site box
+        for (int v = 0; v < n; v++)
+        {
+            const int code = s_code[cls * 45 + v];
+            const float wt = s_wt[cls * 45 + v];
+            const int q = corner + ((code >> 6) * MF_SW + ((code >> 3) & 7)) * per + (code & 7);
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                acc[i] = fmaf(wt, s_y[q * C0 + i], acc[i]);
+        }
+    }
+    else
+    {
+        int xl, xh, yl, yh;
+        footprint_range(tx, w0, w0, xl, xh);
+        footprint_range(ty, h0, h0, yl, yh);
+        for (int py = yl; py <= yh; py++)
+            for (int px = xl; px <= xh; px++)
+                for (int j = 0; j < per; j++)
+                {
+                    const Tap t0 = site_taps(a.k, j, px, py, w0, h0, w0, h0, true);
+                    const float wt = tap_weight(tx, t0.x0, t0.x1, t0.fx) * tap_weight(ty, t0.y0, t0.y1, t0.fy);
+                    if (wt == 0.0f)
+                        continue;
+                    const int q = ((py - (ty0 - 1)) * MF_SW + (px - (tx0 - 1))) * per + j;
+#pragma unroll
+                    for (int i = 0; i < C0; i++)
+                        acc[i] = fmaf(wt, s_y[q * C0 + i], acc[i]);
+                }
+    }
+    float* ap = cg_vec<true>(L.ap) + (L.texels.start[p] + (long long)ty * w0 + tx) * C0;
+#pragma unroll
+    for (int i = 0; i < C0; i++)
+        ap[i] = acc[i];
+}
+
+// p . Ap per chunk of FAST_CG_CHUNK texels of the active planes, into the partials the fast folds read.
+__global__ __launch_bounds__(256) void k_pap_chunks(FastCg a)
+{
+    __shared__ double s4[4];
+    long long t0, t1;
+    const int p = fast_chunk(a, t0, t1);
+    const LatentView& L = a.L;
+    if (L.scalars[(size_t)p * SC_COUNT + SC_ACTIVE] == 0.0)
+        return;
+    const float* pv = cg_vec<true>(L.p);
+    const float* ap = cg_vec<true>(L.ap);
+    double v = 0.0;
+    for (long long e = t0 * L.c + threadIdx.x; e < t1 * L.c; e += 256)
+        v += (double)pv[e] * (double)ap[e];
+    v = block_sum256(v, s4);
+    if (threadIdx.x == 0)
+        a.pap[xcd_block(blockIdx.x, gridDim.x)] = v;
+}
+
 // Per plane (one workgroup each): pAp, then alpha = rz / pAp (zero on non-positive curvature).
 __global__ __launch_bounds__(256) void k_fold_alpha_fast(FastCg a)
 {
@@ -2929,6 +3163,7 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     a.site_d = c.site_d.p;
     a.site_d32 = reinterpret_cast<float*>(c.site_d.p);
     a.sites.n = c.planes.n;
+    c.assembled_k = k;
     a.by_sites = Div31((unsigned int)(1 + k));
     long long acc = 0;
     for (int p = 0; p < c.planes.n; p++)
@@ -3048,6 +3283,100 @@ __global__ __launch_bounds__(64) void k_mark_active(double* scalars, int np)
 
 }   // namespace
 
+namespace
+{
+
+template <int C0>
+void launch_adjoint_matvec0_c1(int c1, unsigned grid, hipStream_t s, const MatFreeArgs& a)
+{
+    switch (c1)
+    {
+    case 1: hipLaunchKernelGGL((k_adjoint_matvec0<C0, 1>), dim3(grid), dim3(256), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_adjoint_matvec0<C0, 2>), dim3(grid), dim3(256), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_adjoint_matvec0<C0, 3>), dim3(grid), dim3(256), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_adjoint_matvec0<C0, 4>), dim3(grid), dim3(256), 0, s, a); break;
+    }
+}
+
+void launch_adjoint_matvec0(int c0, int c1, unsigned grid, hipStream_t s, const MatFreeArgs& a)
+{
+    switch (c0)
+    {
+    case 1: launch_adjoint_matvec0_c1<1>(c1, grid, s, a); break;
+    case 2: launch_adjoint_matvec0_c1<2>(c1, grid, s, a); break;
+    case 3: launch_adjoint_matvec0_c1<3>(c1, grid, s, a); break;
+    default: launch_adjoint_matvec0_c1<4>(c1, grid, s, a); break;
+    }
+}
+
+// Whether the fast form's level-0 conjugate gradients run on the matrix-free operator: by default on a base plane of
+// 4096^2 texels and more, where the assembled stencil is several GB read on every iteration.
+bool matrix_free_on(const Ctx& c)
+{
+    if (c.form != NNTC_FORM_FAST)
+        return false;
+    if (c.matrix_free >= 0)
+        return c.matrix_free != 0;
+    return (long long)c.g(0).w0 * c.g(0).h0 >= 4096ll * 4096ll;
+}
+
+// The fast conjugate gradients' launch arguments: the chunks of FAST_CG_CHUNK texels per plane and their partials.
+FastCg make_fast_cg(Ctx& c, const LatentView& L)
+{
+    const int np = c.planes.n;
+    FastCg fc {};
+    fc.L = L;
+    fc.tol = CG_TOLERANCE_FP32;
+    fc.chunks.n = np;
+    long long acc = 0;
+    for (int p = 0; p < np; p++)
+    {
+        fc.chunks.start[p] = acc;
+        acc += (L.texels.start[p + 1] - L.texels.start[p] + FAST_CG_CHUNK - 1) / FAST_CG_CHUNK;
+    }
+    fc.chunks.start[np] = acc;
+    if ((size_t)acc * 3 > c.cg_partial.n)
+        throw Refused { "latent: CG partials exceed the workspace" };
+    fc.partial = c.cg_partial.p;
+    fc.pap = c.cg_partial.p + 2 * acc;
+    return fc;
+}
+
+MatFreeArgs matfree_args(Ctx& c, const FastCg& fc, unsigned& grid)
+{
+    MatFreeArgs a {};
+    a.fc = fc;
+    a.tiles.n = c.planes.n;
+    long long nt = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        a.tiles.start[p] = nt;
+        nt += (long long)((c.g(p).w0 + MF_TW - 1) / MF_TW) * ((c.g(p).h0 + MF_TH - 1) / MF_TH);
+        a.off1[p] = c.g(p).off1;
+        if (c.g(p).w1 > c.g(p).w0 || c.g(p).h1 > c.g(p).h0)
+            throw Refused { "matrix-free operator: level 1 larger than level 0" };
+    }
+    a.tiles.start[c.planes.n] = nt;
+    a.v1 = c.v1.p;
+    a.weights = c.weights.p;
+    a.cw = c.cw.p;
+    a.nout = c.nout;
+    a.nin = c.nin;
+    a.k = c.assembled_k;
+    if (a.k < 0 || a.k > 4)
+        throw Refused { "matrix-free operator: the system was not assembled" };
+    if (c.gather_tab_k != a.k)
+    {
+        hipLaunchKernelGGL(k_gather_table, dim3(1), dim3(64), 0, c.stream, a.k, c.gather_tab.p);
+        c.gather_tab_k = a.k;
+    }
+    a.tab = c.gather_tab.p;
+    grid = (unsigned)nt;
+    return a;
+}
+
+}   // namespace
+
 void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
 {
     LatentView L = make_view(c, level0);
@@ -3092,28 +3421,26 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     const double b_matvec = (double)n * 40.0 * cc * cc + 16.0 * nv;
     const double b_update = (double)n * 4.0 * cc * cc + 56.0 * nv;
     // The fast form: the inner products inside the mat-vec and the update, folded per plane by a workgroup each.
-    FastCg fc;
-    fc.L = L;
-    fc.tol = CG_TOLERANCE_FP32;
-    fc.chunks.n = np;
-    {
-        long long acc = 0;
-        for (int p = 0; p < np; p++)
-        {
-            fc.chunks.start[p] = acc;
-            acc += (L.texels.start[p + 1] - L.texels.start[p] + FAST_CG_CHUNK - 1) / FAST_CG_CHUNK;
-        }
-        fc.chunks.start[np] = acc;
-        if ((size_t)acc * 3 > c.cg_partial.n)
-            throw Refused { "latent: CG partials exceed the workspace" };
-        fc.pap = c.cg_partial.p + 2 * acc;
-    }
+    FastCg fc = make_fast_cg(c, L);
     long long most_chunks = 0;
     for (int p = 0; p < np; p++)
         most_chunks = std::max(most_chunks, fc.chunks.start[p + 1] - fc.chunks.start[p]);
     const bool fold_in_update = most_chunks <= 256;
     fc.partial = c.cg_partial.p;
     const unsigned fgrid = (unsigned)fc.chunks.start[np];
+    // The fast form's level 0 on the matrix-free operator (the decode's adjoint) in place of the assembled stencil.
+    const bool mf = level0 && !ch.exact && matrix_free_on(c);
+    MatFreeArgs mfa {};
+    unsigned mf_grid = 0;
+    double mf_bytes = 0.0, mf_flops = 0.0;
+    if (mf)
+    {
+        mfa = matfree_args(c, fc, mf_grid);
+        // p read (with the halo, about 1.5 times) and Ap written; the flops of y_s per site and the gather per texel.
+        const double sites = (double)c.n0_total * (1 + mfa.k);
+        mf_bytes = 8.0 * nv;
+        mf_flops = sites * c.nout * 2.0 * (c.c0 * c.c1 + 2.0 * c.c0 + 1.0) + sites * 3.4 * 2.0 * c.c0;
+    }
     for (int it = 0; it < CG_MAX_ITERATIONS; it++)
     {
         if (!ch.exact)
@@ -3121,6 +3448,15 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             fc.probe = (it + 1) % CG_PROBE == 0;
             // The fast form's bytes: the fp32 stencil (20 c^2 per texel), p read and Ap written (4 c each); the update
             // reads the preconditioner, x, p, r, Ap and writes x, r, z; p = z + beta p reads two and writes one.
+            if (mf)
+            {
+                {
+                    KernelTimer kt(c.stream, "k_adjoint_matvec0", mf_bytes, mf_flops);
+                    launch_adjoint_matvec0(c.c0, c.c1, mf_grid, c.stream, mfa);
+                }
+                hipLaunchKernelGGL(k_pap_chunks, dim3(fgrid), dim3(256), 0, c.stream, fc);
+            }
+            else
             {
                 KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", (double)n * 20.0 * cc * cc + 8.0 * nv,
                                (double)n * (2.0 * 9.0 * cc * cc));
@@ -3199,6 +3535,59 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
         if (!any)
             break;
     }
+}
+
+void level0_apply(Ctx& c, int mode, const double* p_host, double* out_host)
+{
+    LatentView L = make_view(c, true);
+    const long long n = L.texels.start[L.texels.n];
+    const size_t nv = (size_t)n * L.c;
+    const int np = c.planes.n;
+    hipLaunchKernelGGL(k_mark_active, dim3(1), dim3(64), 0, c.stream, c.scalars.p, np);
+    const bool fast = c.form != NNTC_FORM_EXACT;
+    if (mode == 1 && c.form != NNTC_FORM_FAST)
+        throw Refused { "level0_apply: the matrix-free operator is the fast form's" };
+    if (fast)
+    {
+        std::vector<float> pf(nv);
+        for (size_t i = 0; i < nv; i++)
+            pf[i] = (float)p_host[i];
+        upload_staged(c, reinterpret_cast<float*>(L.p), pf.data(), nv);
+        FastCg fc = make_fast_cg(c, L);
+        const unsigned fgrid = (unsigned)fc.chunks.start[np];
+        if (mode == 1)
+        {
+            unsigned grid = 0;
+            MatFreeArgs a = matfree_args(c, fc, grid);
+            launch_adjoint_matvec0(c.c0, c.c1, grid, c.stream, a);
+        }
+        else
+            switch (L.c)
+            {
+            case 1: hipLaunchKernelGGL(k_matvec_fast<1>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+            case 2: hipLaunchKernelGGL(k_matvec_fast<2>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+            case 3: hipLaunchKernelGGL(k_matvec_fast_rows<3>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
+            default: hipLaunchKernelGGL(k_matvec_fast_rows<4>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
+            }
+        HIP_OK(hipGetLastError());
+        const float* h = read_back(c, (const float*)reinterpret_cast<float*>(L.ap), nv);
+        for (size_t i = 0; i < nv; i++)
+            out_host[i] = (double)h[i];
+        return;
+    }
+    upload_staged(c, L.p, p_host, nv);
+    const unsigned tb = blocks_for(n, 256);
+    switch (L.c)
+    {
+    case 1: hipLaunchKernelGGL(k_matvec<1>, dim3(tb), dim3(256), 0, c.stream, L); break;
+    case 2: hipLaunchKernelGGL(k_matvec<2>, dim3(tb), dim3(256), 0, c.stream, L); break;
+    case 3: hipLaunchKernelGGL(k_matvec<3>, dim3(tb), dim3(256), 0, c.stream, L); break;
+    default: hipLaunchKernelGGL(k_matvec<4>, dim3(tb), dim3(256), 0, c.stream, L); break;
+    }
+    HIP_OK(hipGetLastError());
+    const double* h = read_back(c, (const double*)L.ap, nv);
+    for (size_t i = 0; i < nv; i++)
+        out_host[i] = h[i];
 }
 
 void correction_and_probe(Ctx& c, bool level0)

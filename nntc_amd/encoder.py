@@ -26,7 +26,7 @@ class Options(C.Structure):
         ("ridge", C.c_double), ("mips", C.c_int32), ("mip_min", C.c_int32), ("mip_weight", C.c_int32),
         ("mip_mix", C.c_double), ("bc_refine", C.c_int32), ("bc_refine_after", C.c_int32),
         ("bc_outer", C.c_int32), ("init0_chain", C.c_int32), ("form", C.c_int32), ("quiet", C.c_int32),
-        ("weights", C.c_double * 6), ("rgb_weights", C.c_double * 3),
+        ("weights", C.c_double * 6), ("rgb_weights", C.c_double * 3), ("matrix_free", C.c_int32),
     ]
 
 

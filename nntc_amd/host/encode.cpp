@@ -536,6 +536,8 @@ void Encoder::setup(int textures, int width, int height, const uint8_t* const* r
         throw Failure { std::string("device_create: ") + seam.last_error() };
     if (opts.form != NNTC_FORM_EXACT)
         check(seam.set_form(ctx, opts.form), "set_form");
+    if (opts.matrix_free != -1 && seam.name == "hip")
+        check(nntc_hip_set_matrix_free(ctx, opts.matrix_free), "set_matrix_free");
 }
 
 void Encoder::setup_device(int textures, int width, int height, const uint8_t* const* rgb_device)
@@ -549,6 +551,8 @@ void Encoder::setup_device(int textures, int width, int height, const uint8_t* c
     chain_on_device = true;
     if (opts.form != NNTC_FORM_EXACT)
         check(seam.set_form(ctx, opts.form), "set_form");
+    if (opts.matrix_free != -1 && seam.name == "hip")
+        check(nntc_hip_set_matrix_free(ctx, opts.matrix_free), "set_matrix_free");
 }
 
 // The box init takes the source channels with every exact copy of an earlier channel moved to the back; when that
@@ -704,6 +708,7 @@ void nntc_encode_default_options(nntc_encode_options* o)
         o->weights[t] = 1.0;
     for (int c = 0; c < 3; c++)
         o->rgb_weights[c] = 1.0;
+    o->matrix_free = -1;
 }
 
 const char* nntc_encode_last_error(void)
