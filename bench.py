@@ -230,7 +230,27 @@ def psnr_of(e, textures):
 
 
 def kernels_brief(table):
-    return {k: {"launches": v["launches"], "ms": round(v["ms"], 3)} for k, v in table.items()}
+    """Per kernel over one job: launches, milliseconds, and the algorithmic bytes and flops of one launch."""
+    return {k: {"launches": v["launches"], "ms": round(v["ms"], 3),
+                "bytes_per_launch": round(v["bytes"] / max(1, v["launches"])),
+                "flops_per_launch": round(v["flops"] / max(1, v["launches"]))} for k, v in table.items()}
+
+
+def copy_calibration(runtime, n=256 << 20):
+    """One device-to-device copy of n bytes (the runtime's copy kernel reads n and writes n): under rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE it gives the factor between those counters' readings and bytes moved."""
+    a, b = C.c_void_p(), C.c_void_p()
+    runtime.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    runtime.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    runtime.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    runtime.hipFree.argtypes = [C.c_void_p]
+    assert runtime.hipMalloc(C.byref(a), n) == 0 and runtime.hipMalloc(C.byref(b), n) == 0
+    assert runtime.hipMemset(a, 1, n) == 0
+    assert runtime.hipMemcpy(b, a, n, 3) == 0   # hipMemcpyDeviceToDevice
+    assert runtime.hipDeviceSynchronize() == 0
+    runtime.hipFree(a)
+    runtime.hipFree(b)
+    print(json.dumps({"copy_calibration_bytes": n}))
 
 
 def main():
@@ -243,6 +263,8 @@ def main():
     ap.add_argument("--forms", choices=["all", "chosen", "plain", "exact"], default="all",
                     help="time every form (default) or one; a single form is for profiler runs")
     ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device (the known-size kernel rocprofv3 counters are scaled by)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the encoder shards by material across GPUs with no exchange, and this "
@@ -257,6 +279,9 @@ def main():
     # the timed region.
     runtime = C.CDLL("libamdhip64.so.7", mode=C.RTLD_GLOBAL)
     hip.nntc_hip_device_synchronize = runtime.hipDeviceSynchronize
+    if a.copy_calibration:
+        copy_calibration(runtime)
+        return
     spec = synth.WORKLOADS[a.workload]
     textures = synth.workload(a.workload)
     gpath = os.path.join(ROOT, "tests", "golden", a.workload, "report.json")

@@ -739,6 +739,461 @@ __global__ __launch_bounds__(64) void k_bc_refine(BcArgs a, Ranges colour, int c
         a.taken[gb] = taken;
 }
 
+// ---------------------------------------------------------------------------------------------------------------------
+// The fast form's refinement: the same lanes, stencil reads and decisions, in fp32
+// ---------------------------------------------------------------------------------------------------------------------
+//
+// k_bc_refine_fast takes the reference's decisions (src/cpu/cpu_refine_bc.cpp:138-384: the least-squares endpoint step,
+// nine candidates weighed again exactly, two Gauss-Seidel passes of 8-way selector argmins, a move only on a strict
+// decrease) with three changes of arithmetic and schedule that only the exact form's bit-identity forbids:
+//   - fp32 throughout (the stencil is the fast form's fp32 copy already), the 2 x 2 endpoint solve in fp64;
+//   - a sum over the block is a butterfly across the row's sixteen lanes (quad xor 1, xor 2, half-row and row
+//     mirrors), which leaves the same value on every lane, so the row still agrees on every decision;
+//   - a selector step weighs its eight candidates one per lane (lane u and u + 8 weigh selector u & 7) and finds the
+//     winner by a three-level max over a packed key: the decrease's fp32 bits with its low three bits replaced by
+//     7 - selector, so the largest decrease wins and a tie goes to the lower selector, as the sequential scan takes it.
+// The LDS column holds each lane's five forward blocks only (center, right, down-left, down, down-right), half the
+// nine a full column takes: a2(u, i; v, j) for v behind u is v's forward entry (j, i) toward u, read from v's column.
+// A texel outside the block or the plane has its entries zeroed in its own column, so every read that reaches past the
+// block lands on a zero.
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float x)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xF, 0xF, false));
+}
+
+template <int CTRL>
+__device__ __forceinline__ unsigned int dpp_u(unsigned int x)
+{
+    return (unsigned int)__builtin_amdgcn_update_dpp(0, (int)x, CTRL, 0xF, 0xF, false);
+}
+
+// Lane K of each sixteen-lane row, to the whole row (DPP row_newbcast).
+template <int K>
+__device__ __forceinline__ float row_bcast_f(float x)
+{
+    return dpp_f<0x150 + K>(x);
+}
+
+// Lane i + OFF of lane i's row; a lane whose source falls outside the row reads zero.
+template <int OFF>
+__device__ __forceinline__ float row_from_f(float x)
+{
+    if constexpr (OFF == 0)
+        return x;
+    else
+        return dpp_f<(OFF > 0 ? 0x100 + OFF : 0x110 - OFF)>(x);
+}
+
+// The sum of x over the row's sixteen lanes, the same value on every lane.
+__device__ __forceinline__ float row_sum_f(float x)
+{
+    x += dpp_f<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+    x += dpp_f<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+    x += dpp_f<0x141>(x);   // row_half_mirror
+    x += dpp_f<0x140>(x);   // row_mirror
+    return x;
+}
+
+// The largest key over each eight-lane half of the row, on all eight lanes.
+__device__ __forceinline__ unsigned int oct_max(unsigned int k)
+{
+    k = max(k, dpp_u<0xB1>(k));
+    k = max(k, dpp_u<0x4E>(k));
+    k = max(k, dpp_u<0x141>(k));
+    return k;
+}
+
+// Direction k = (dy + 1) 3 + dx + 1 of the nine neighbours, as a lane offset in the row, and the forward slot of the
+// opposite direction (the slot of the neighbour's block that holds the pair) for k < 4.
+constexpr int nb_off(int k)
+{
+    return 4 * (k / 3 - 1) + (k % 3 - 1);
+}
+
+// x of the lane at direction k from this one (k a constant once the loop over the neighbours is unrolled).
+__device__ __forceinline__ float from_dir(float x, int k)
+{
+    switch (k)
+    {
+    case 0: return row_from_f<nb_off(0)>(x);
+    case 1: return row_from_f<nb_off(1)>(x);
+    case 2: return row_from_f<nb_off(2)>(x);
+    case 3: return row_from_f<nb_off(3)>(x);
+    case 5: return row_from_f<nb_off(5)>(x);
+    case 6: return row_from_f<nb_off(6)>(x);
+    case 7: return row_from_f<nb_off(7)>(x);
+    case 8: return row_from_f<nb_off(8)>(x);
+    default: return x;
+    }
+}
+
+__device__ __forceinline__ float bc4_weight8f(int sel)
+{
+    return sel == 0 ? 1.0f : (sel == 1 ? 0.0f : (float)(8 - sel) / 7.0f);
+}
+
+// Selector step T of channel c, then the steps after it. nbm bit T: texel T is u or one of its eight neighbours.
+template <int C0, int T>
+__device__ __forceinline__ void fast_selector_steps(const float* s_F, int lane, int rowbase, int u, unsigned int nbm,
+                                                    int c, float a1c, float (&uv)[C0], float diag, float palk,
+                                                    int& selu, float& cur, float& taken)
+{
+    // The lane's entries a2(u, ii; T, c): its own forward block toward T (T at or ahead of u in the row), or T's
+    // forward block toward u read from T's column. The forward slot of lane offset o = |T - u| is o - (o >= 3); a lane
+    // that is not T's neighbour reads a valid entry and weighs it by zero. The geometry is made opaque first, so that
+    // the step's addresses are formed here, not all sixteen at once.
+    asm volatile("" : "+v"(lane), "+v"(rowbase), "+v"(u), "+v"(nbm));
+    const int off = T - u;
+    const bool fwd = off >= 0;
+    const int ao = fwd ? off : -off;
+    const int kf = min(4, ao - (ao >= 3 ? 1 : 0));
+    const int at = kf * (C0 * 64 * C0) + (fwd ? (lane * C0 + c * 64 * C0) : ((rowbase + T) * C0 + c));
+    const int stride = fwd ? 1 : 64 * C0;
+    float e[C0];
+#pragma unroll
+    for (int ii = 0; ii < C0; ii++)
+        e[ii] = s_F[at + ii * stride];
+    const bool nb = (nbm >> T) & 1u;
+
+    const float b = row_bcast_f<T>(a1c + uv[c]);
+    const float aa = row_bcast_f<T>(diag);
+    const float now = row_bcast_f<T>(cur);
+    const float d = palk - now;
+    const float de = d * fmaf(aa, d, 2.0f * b);
+    const unsigned int kk = (unsigned int)lane & 7u;
+    unsigned int key = de < 0.0f ? ((__float_as_uint(de) & ~7u) | (7u - kk)) : 0u;
+    key = oct_max(key);
+    const bool win = key != 0u;
+    const int wk = 7 - (int)(key & 7u);
+    // The winner's palette value from the row's lane wk (lane j of the row holds selector j & 7's value).
+    const float pw = __int_as_float(__builtin_amdgcn_ds_bpermute((rowbase + wk) << 2, __float_as_int(palk)));
+    const float move = win ? pw - now : 0.0f;
+    const float mv = nb ? move : 0.0f;
+    taken += win ? __uint_as_float(key & ~7u) : 0.0f;
+#pragma unroll
+    for (int ii = 0; ii < C0; ii++)
+        uv[ii] = fmaf(e[ii], mv, uv[ii]);
+    const bool mine = win && u == T;
+    selu = mine ? wk : selu;
+    cur = mine ? pw : cur;
+    // Each step's updates made here, not deferred to the channel's end with every step's key held until then.
+    asm volatile("" : "+v"(selu), "+v"(taken), "+v"(cur));
+    if constexpr (T < 15)
+        fast_selector_steps<C0, T + 1>(s_F, lane, rowbase, u, nbm, c, a1c, uv, diag, palk, selu, cur, taken);
+}
+
+// Channels C..C0-1 of one Gauss-Seidel pass (a template over the channel: the channel indexes the lane's arrays, which a
+// loop the compiler leaves rolled would put in scratch).
+template <int C0, int C>
+__device__ __forceinline__ void fast_selector_pass(const float* s_F, int lane, int rowbase, int u, unsigned int nbm,
+                                                   const float (&a1)[C0], float (&uv)[C0], const float (&diag)[C0],
+                                                   const float (&palk)[C0], int (&selu)[C0], float (&cur)[C0],
+                                                   float& taken)
+{
+    fast_selector_steps<C0, 0>(s_F, lane, rowbase, u, nbm, C, a1[C], uv, diag[C], palk[C], selu[C], cur[C], taken);
+    if constexpr (C + 1 < C0)
+        fast_selector_pass<C0, C + 1>(s_F, lane, rowbase, u, nbm, a1, uv, diag, palk, selu, cur, taken);
+}
+
+template <int C0>
+__global__ __launch_bounds__(64) void k_bc_refine_fast(BcArgs a, Ranges colour, int colour_index)
+{
+    const long long gi = (long long)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 4);
+    if (gi >= colour.start[colour.n])
+        return;
+    // Lane l's forward blocks: entry (kf, j, i) = a2(u, i; u + d(kf), j) at [((kf C0 + j) 64 + l) C0 + i].
+    __shared__ float s_F[STENCIL_BLOCKS * C0 * 64 * C0];
+    const int lane = (int)threadIdx.x, rowbase = lane & ~15;
+
+    const int p = range_plane(colour, gi);
+    const PlaneGeom& g = a.planes.p[p];
+    const long long ci = gi - colour.start[p];
+    const int bxn = (g.w0 + 3) / 4;
+    const int cx = colour_index & 1, cy = colour_index >> 1;
+    const int cbx = (bxn - cx + 1) / 2;
+    const int gx = cx + 2 * (int)(ci % cbx);
+    const int gy = cy + 2 * (int)(ci / cbx);
+    const int w0 = g.w0, h0 = g.h0;
+    const long long base = g.off0;
+    const int px0 = gx * 4, py0 = gy * 4;
+    const float* S = a.stencil32;
+    const long long sn = a.sm.sn;
+
+    BlockLane<C0, float> L;
+    L.S = S;
+    L.sn = sn;
+    L.bp.init(a.sm.cpos[p], w0, gx, gy);
+    const int u = lane & 15, ux = u & 3, uy = u >> 2;
+    L.u = u;
+    L.ux = ux;
+    L.uy = uy;
+    {
+        const int nx = min(4, w0 - px0), ny = min(4, h0 - py0);
+        const unsigned int row = (1u << nx) - 1u;
+        L.inside = (row | row << 4 | row << 8 | row << 12) & ((1u << (4 * ny)) - 1u);
+    }
+    const bool in_u = L.in(u);
+    L.pu = in_u ? L.bp.at(ux, uy) : L.bp.at(0, 0);
+    L.init_neighbours();
+    auto inside_xy = [=](int lx, int ly) {
+        return px0 + lx >= 0 && px0 + lx < w0 && py0 + ly >= 0 && py0 + ly < h0;
+    };
+    auto tix = [=](int lx, int ly) { return base + (long long)(py0 + ly) * w0 + (px0 + lx); };
+    const long long tu = tix(ux, uy);
+
+    // ---- slot by slot: the lane's forward block (raw, its entries toward texels outside the block feed the linear
+    // term a1), the backward term of the neighbour behind it, and the block masked to the lane's column (zero where
+    // the pair is not inside the block and the plane)
+    // A neighbour that adds nothing reads a texel the plane holds (the lane's own, or the block's first for a lane
+    // outside the plane) and weighs it by zero.
+    const long long tsafe = in_u ? tu : tix(0, 0);
+    float a1[C0];
+#pragma unroll
+    for (int i = 0; i < C0; i++)
+        a1[i] = in_u ? (float)a.grad[tu * C0 + i] : 0.0f;
+#pragma unroll
+    for (int slot = 0; slot < STENCIL_BLOCKS; slot++)
+    {
+        // The geometry made opaque per slot: the slot's loads are issued here, not all five slots' at once.
+        L.opaque();
+        float Fs[C0][C0];
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+#pragma unroll
+            for (int j = 0; j < C0; j++)
+                Fs[i][j] = S[(long long)(slot * C0 * C0 + i * C0 + j) * sn + L.pu];
+        if (slot > 0)
+        {
+            int dx, dy;
+            stencil_offset(slot, dx, dy);
+            const int fx = ux + dx, fy = uy + dy;
+            const bool fok = in_u && !(fx >= 0 && fx < 4 && fy >= 0 && fy < 4) && inside_xy(fx, fy);
+            {
+                const double* xn = a.delta + (fok ? tix(fx, fy) : tsafe) * C0;
+                const float keep = fok ? 1.0f : 0.0f;
+#pragma unroll
+                for (int j = 0; j < C0; j++)
+                {
+                    const float xv = (float)xn[j] * keep;
+#pragma unroll
+                    for (int i = 0; i < C0; i++)
+                        a1[i] = fmaf(Fs[i][j], xv, a1[i]);
+                }
+            }
+            const int bx = ux - dx, by = uy - dy;
+            const bool bok = in_u && !(bx >= 0 && bx < 4 && by >= 0 && by < 4) && inside_xy(bx, by);
+            {
+                const double* xn = a.delta + (bok ? tix(bx, by) : tsafe) * C0;
+                const long long pb = bok ? L.bp.at(bx, by) : L.pu;
+                const float keep = bok ? 1.0f : 0.0f;
+#pragma unroll
+                for (int j = 0; j < C0; j++)
+                {
+                    const float xv = (float)xn[j] * keep;
+#pragma unroll
+                    for (int i = 0; i < C0; i++)
+                        a1[i] = fmaf(S[(long long)(slot * C0 * C0 + j * C0 + i) * sn + pb], xv, a1[i]);
+                }
+            }
+        }
+        const bool held = (L.nbmask >> (slot + 4)) & 1u;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+#pragma unroll
+            for (int j = 0; j < C0; j++)
+                s_F[((slot * C0 + j) * 64 + lane) * C0 + i] = held ? Fs[i][j] : 0.0f;
+    }
+    // The columns are read across lanes from here on.
+    __syncthreads();
+
+    // ---- uv = a2 dv: the forward blocks' terms with the neighbours' corrections pulled from their lanes, and the
+    // backward ones as the neighbours' own forward products pulled from theirs
+    float dv[C0], uv[C0];
+#pragma unroll
+    for (int j = 0; j < C0; j++)
+    {
+        dv[j] = in_u ? a.v0[tu * C0 + j] - a.prev[tu * C0 + j] : 0.0f;
+        uv[j] = 0.0f;
+    }
+    const float* own = s_F + lane * C0;   // entry (kf, j, i) of the lane's column at own[(kf C0 + j) 64 C0 + i]
+#pragma unroll
+    for (int k = 4; k < 9; k++)
+#pragma unroll
+        for (int j = 0; j < C0; j++)
+        {
+            const float x = from_dir(dv[j], k);
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                uv[i] = fmaf(own[((k - 4) * C0 + j) * 64 * C0 + i], x, uv[i]);
+        }
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+        {
+            float w = 0.0f;
+#pragma unroll
+            for (int j = 0; j < C0; j++)
+                w = fmaf(own[((4 - k) * C0 + i) * 64 * C0 + j], dv[j], w);
+            uv[i] += from_dir(w, k);
+        }
+
+    const long long gb = g.offb + (long long)gy * bxn + gx;
+    int ep0[C0], ep1[C0], selu[C0];
+#pragma unroll
+    for (int c = 0; c < C0; c++)
+    {
+        ep0[c] = a.ep[(gb * C0 + c) * 2];
+        ep1[c] = a.ep[(gb * C0 + c) * 2 + 1];
+        selu[c] = a.sel[(gb * C0 + c) * 16 + u] & 7;
+    }
+    float taken = 0.0f;
+
+    // ---- (i) the endpoints of each channel
+#pragma unroll
+    for (int c = 0; c < C0; c++)
+    {
+        const float lo = a.lo0[c], hi = a.hi0[c];
+        const float stp = (hi - lo) / 255.0f;
+        // a2(u, i; neighbour k, c) for the nine neighbours: the lane's own column ahead of it, the neighbour's behind
+        float E[9][C0];
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+        {
+            const int v = rowbase + ((u + nb_off(k)) & 15);
+#pragma unroll
+            for (int i = 0; i < C0; i++)
+                E[k][i] = k >= 4 ? s_F[(((k - 4) * C0 + c) * 64 + lane) * C0 + i]
+                                 : s_F[(((4 - k) * C0 + i) * 64 + v) * C0 + c];
+        }
+        const float aw = bc4_weight8f(selu[c]);
+        float s1 = 0.0f, s2 = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+        {
+            s1 += E[k][c];
+            s2 = fmaf(E[k][c], from_dir(aw, k), s2);
+        }
+        const float pp = stp * aw, qq = stp * (1.0f - aw);
+        const float ap = stp * s2, aq = stp * (s1 - s2);
+        const float b = a1[c] + uv[c];
+        const float bp_ = row_sum_f(b * pp), bq = row_sum_f(b * qq);
+        const float pap = row_sum_f(pp * ap), paq = row_sum_f(pp * aq), qaq = row_sum_f(qq * aq);
+        const int r0 = ep0[c], r1 = ep1[c];
+        const double det = (double)pap * qaq - (double)paq * paq;
+        double d0 = 0.0, d1 = 0.0;
+        if (det > 1e-6 * (double)pap * qaq)
+        {
+            d0 = ((double)paq * bq - (double)qaq * bp_) / det;
+            d1 = ((double)paq * bp_ - (double)pap * bq) / det;
+        }
+        const int c0r = (int)floor((double)r0 + d0 + 0.5), c1r = (int)floor((double)r1 + d1 + 0.5);
+        float best = 0.0f;
+        int best0 = r0, best1 = r1;
+#pragma unroll
+        for (int e0 = -1; e0 <= 1; e0++)
+#pragma unroll
+            for (int e1 = -1; e1 <= 1; e1++)
+            {
+                const int x0 = min(255, max(0, c0r + e0)), x1 = min(255, max(0, c1r + e1));
+                if (x0 <= x1)
+                    continue;
+                const float f0 = (float)(x0 - r0), f1 = (float)(x1 - r1);
+                const float de = 2.0f * (bp_ * f0 + bq * f1) + pap * f0 * f0 + 2.0f * paq * f0 * f1 + qaq * f1 * f1;
+                if (de < best)
+                {
+                    best = de;
+                    best0 = x0;
+                    best1 = x1;
+                }
+            }
+        if (best0 == r0 && best1 == r1)
+            continue;
+        const float dlt = level0_value(lo, hi, best0, best1, selu[c]) - level0_value(lo, hi, r0, r1, selu[c]);
+        float dn[9];
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            dn[k] = from_dir(dlt, k);
+        float quad = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; k++)
+            quad = fmaf(E[k][c], dn[k], quad);
+        const float exact = row_sum_f(dlt * (2.0f * (a1[c] + uv[c]) + quad));
+        if (!(exact < 0.0f))
+            continue;
+        taken += exact;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
+        {
+            float add = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 9; k++)
+                add = fmaf(E[k][i], dn[k], add);
+            uv[i] += add;
+        }
+        ep0[c] = best0;
+        ep1[c] = best1;
+    }
+
+    // ---- (ii) the selectors, two Gauss-Seidel passes of 8-way argmins
+    // nbm bit t: texel t of the block is the lane's texel or one of its eight neighbours.
+    unsigned int nbm = 0;
+#pragma unroll
+    for (int t = 0; t < 16; t++)
+    {
+        const int ddx = (t & 3) - ux, ddy = (t >> 2) - uy;
+        nbm |= (ddx >= -1 && ddx <= 1 && ddy >= -1 && ddy <= 1 ? 1u : 0u) << t;
+    }
+    float pal[C0][8], cur[C0], diag[C0], palk[C0];
+#pragma unroll
+    for (int c = 0; c < C0; c++)
+    {
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            pal[c][k] = level0_value(a.lo0[c], a.hi0[c], ep0[c], ep1[c], k);
+        cur[c] = pal[c][0];
+        palk[c] = pal[c][0];
+#pragma unroll
+        for (int k = 1; k < 8; k++)
+        {
+            cur[c] = selu[c] == k ? pal[c][k] : cur[c];
+            palk[c] = (lane & 7) == k ? pal[c][k] : palk[c];
+        }
+        diag[c] = s_F[(c * 64 + lane) * C0 + c];
+    }
+#pragma unroll 1
+    for (int pass = 0; pass < 2; pass++)
+        fast_selector_pass<C0, 0>(s_F, lane, rowbase, u, nbm, a1, uv, diag, palk, selu, cur, taken);
+
+    // ---- what the block leaves behind
+#pragma unroll
+    for (int c = 0; c < C0; c++)
+    {
+        if (u == 0)
+        {
+            a.ep[(gb * C0 + c) * 2] = (uint8_t)ep0[c];
+            a.ep[(gb * C0 + c) * 2 + 1] = (uint8_t)ep1[c];
+        }
+        a.sel[(gb * C0 + c) * 16 + u] = (uint8_t)selu[c];
+    }
+    if (in_u)
+    {
+#pragma unroll
+        for (int c = 0; c < C0; c++)
+        {
+            const float byte = bc4_entry8(ep0[c], ep1[c], selu[c]);
+            const long long idx = tu * C0 + c;
+            a.v0[idx] = cur[c];
+            a.k0[idx] = bc0_index(byte);
+            a.delta[idx] = (double)cur[c] - (double)a.prev[idx];
+        }
+    }
+    if (u == 0)
+        a.taken[gb] = (double)taken;
+}
+
 // The .dds blocks of every file of every plane (bc4_pack, shared/bc_pack.h:51-60), and each block's squared error in
 // byte units against the mirror's exact index values over the texels inside the plane
 // (src/export.cpp:283-292).
@@ -963,11 +1418,14 @@ __global__ __launch_bounds__(1024) void k_bc_se_total(const double* se, long lon
         *out = s[0];
 }
 
-// The exact form reads the fp64 stencil, the fast form the fp32 copy its gathers build (the only one they build).
+// The exact form reads the fp64 stencil in fp64; the plain form the fp32 copy its gathers build (the only one they
+// build) in fp64 arithmetic and the exact form's order; the fast form the fp32 copy in fp32 (k_bc_refine_fast).
 template <int C0>
-void launch_refine(bool fast, dim3 grid, hipStream_t s, const BcArgs& a, const Ranges& colour, int col)
+void launch_refine(int form, dim3 grid, hipStream_t s, const BcArgs& a, const Ranges& colour, int col)
 {
-    if (fast)
+    if (form == NNTC_FORM_FAST)
+        hipLaunchKernelGGL((k_bc_refine_fast<C0>), grid, dim3(64), 0, s, a, colour, col);
+    else if (form == NNTC_FORM_PLAIN)
         hipLaunchKernelGGL((k_bc_refine<C0, float>), grid, dim3(64), 0, s, a, colour, col);
     else
         hipLaunchKernelGGL((k_bc_refine<C0, double>), grid, dim3(64), 0, s, a, colour, col);
@@ -1047,10 +1505,10 @@ void bc_refine_pass(Ctx& c, double* fold)
         const dim3 grid(blocks_for(n, 4));   // four blocks per 64-lane workgroup
         switch (c.c0)
         {
-        case 1: launch_refine<1>(fast, grid, c.stream, a, colour[col], col); break;
-        case 2: launch_refine<2>(fast, grid, c.stream, a, colour[col], col); break;
-        case 3: launch_refine<3>(fast, grid, c.stream, a, colour[col], col); break;
-        default: launch_refine<4>(fast, grid, c.stream, a, colour[col], col); break;
+        case 1: launch_refine<1>(c.form, grid, c.stream, a, colour[col], col); break;
+        case 2: launch_refine<2>(c.form, grid, c.stream, a, colour[col], col); break;
+        case 3: launch_refine<3>(c.form, grid, c.stream, a, colour[col], col); break;
+        default: launch_refine<4>(c.form, grid, c.stream, a, colour[col], col); break;
         }
         HIP_OK(hipGetLastError());
     }
