@@ -107,13 +107,18 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
     if (e.source_w != d.planes[0].w0 || e.source_h != d.planes[0].h0)
         js += ", \"padded\": true, \"padded_width\": " + std::to_string(d.planes[0].w0) + ", \"padded_height\": " +
               std::to_string(d.planes[0].h0);
-    if (!e.inputs.empty())
+    // What each input was (informational): the driver reads every input as linear RGB8 with the box chain and edge
+    // clamping, so only the name and the two weights vary. Textures handed over as arrays have no file name.
+    js += ", \"inputs\": [";
+    for (int t = 0; t < d.textures; t++)
     {
-        js += ", \"inputs\": [";
-        for (size_t t = 0; t < e.inputs.size(); t++)
-            js += std::string(t ? ", " : "") + "{ \"file\": \"" + base_name(e.inputs[t]) + "\" }";
-        js += "]";
+        const std::string file = (size_t)t < e.inputs.size() ? base_name(e.inputs[(size_t)t]) : std::string();
+        js += std::string(t ? ", " : "") + "{ \"file\": \"" + file + "\", \"type\": \"\", \"filter\": \"default\", "
+              "\"srgb\": false, \"edge\": \"clamp\", \"normal_map\": false, \"weight\": " + num(e.opts.weights[t]) +
+              ", \"rgb_weights\": [" + num(e.opts.rgb_weights[0]) + "," + num(e.opts.rgb_weights[1]) + "," +
+              num(e.opts.rgb_weights[2]) + "] }";
     }
+    js += "]";
     js += " },\n";
     js += "  \"decode\": { \"width\": " + std::to_string(d.planes[0].w0) + ", \"height\": " +
           std::to_string(d.planes[0].h0) + ", \"textures_out\": " + std::to_string(d.textures) + " },\n";
