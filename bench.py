@@ -24,7 +24,9 @@ Three forms on the same inputs, W untimed and K timed steps each:
             1e-6 (at most 200 iterations), and the decoder Gram in fp64;
     chosen  the plain form with the decoder Gram on the matrix cores (bf16 three-part split products), the device's
             fp64 Cholesky decoder solve, the matrix-free decode-adjoint conjugate gradients at level 0 on 4096^2
-            planes and graph-replayed iterations.
+            planes, each conjugate-gradient probe period (4 iterations) replayed from a captured HIP graph, and the
+            sweeps' launch form per shape picked by the launch-shape tuner (graphs_last_job and tuned_launch_forms
+            in the chosen entry say what ran).
 value is the chosen form's Mpix/s, valid when its decoded PSNR is within 0.05 dB of the reference's on every texture
 (the reference's integer error sums are in the fixture). The roofline object describes the chosen step's dominant
 kernel: its algorithmic bytes or flops per launch over its average launch time, taken live from HIP events on the
@@ -73,6 +75,38 @@ def kernel_stats(hip):
         out[name.value.decode()] = {"launches": n.value, "ms": ms.value, "bytes": by.value, "flops": fl.value}
         i += 1
     return out
+
+
+def tuning_table(hip):
+    """The launch-shape tuner's choices so far in this process: per shape, the form taken and each form's trial ms."""
+    f = hip.nntc_hip_tuning_entry
+    ip = C.POINTER(C.c_int)
+    f.argtypes = [C.c_int, ip, ip, ip, ip, ip, C.POINTER(C.c_double), ip]
+    out = []
+    i = 0
+    while True:
+        fam, ch, bk, choice, n = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        forms, ms = (C.c_int * 4)(), (C.c_double * 4)()
+        if f(i, C.byref(fam), C.byref(ch), C.byref(bk), C.byref(choice), forms, ms, C.byref(n)) != 0:
+            return out
+        out.append({"family": ["sweep_lanes"][fam.value], "channels": ch.value, "texels_le": 1 << bk.value,
+                    "choice": choice.value, "trial_ms": {str(forms[j]): round(ms[j], 3) for j in range(n.value)}})
+        i += 1
+
+
+def graph_counts(hip, e):
+    """How many CG probe periods the encoder's context captured as HIP graphs, and how many it replayed."""
+    from nntc_amd import encoder
+
+    ctx_of = encoder.lib().nntc_encoder_context
+    ctx_of.restype = C.c_void_p
+    ctx_of.argtypes = [C.c_void_p]
+    f = hip.nntc_hip_graph_counts
+    f.argtypes = [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+    caps, runs = C.c_longlong(), C.c_longlong()
+    if f(ctx_of(e.h), C.byref(caps), C.byref(runs)) != 0:
+        return None
+    return {"captured": caps.value, "replayed": runs.value}
 
 
 def measured_traffic(kernel, workload):
@@ -313,6 +347,9 @@ def main():
                  "psnr_gap_db": max(abs(ps[t] - ref_psnr[t]) for t in range(ntex)) if ref_psnr else None,
                  "resident_run_s": rep.seconds, "dominant": dom,
                  "roofline": roofline(dom, live, table, a.workload), "kernels_per_step": kernels_brief(table)}
+            if name == "chosen":
+                r["graphs_last_job"] = graph_counts(hip, e)
+                r["tuned_launch_forms"] = tuning_table(hip)
             if name == "exact" and golden:
                 r["bit_exact_to_reference"] = all(
                     hashlib.sha256(open(prefix + sfx, "rb").read()).hexdigest() == d for sfx, d in golden["dds"].items())

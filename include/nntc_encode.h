@@ -52,6 +52,10 @@ typedef struct nntc_encode_options
     double rgb_weights[3];       /* --rgb-weights 1,1,1 */
     int32_t matrix_free;         /* the fast form's level-0 operator: -1 by size, 0 assembled, 1 matrix-free
                                     (nntc_hip_set_matrix_free; not a reference option) */
+    int32_t sweep_lanes;         /* the fast sweeps at three or four channels: -1 the tuner's choice, 1 or 4 lanes
+                                    a texel (nntc_hip_set_sweep_lanes; not a reference option) */
+    int32_t graphs;              /* the fast conjugate gradients' probe periods as HIP graphs: -1 or 1 on, 0 off
+                                    (nntc_hip_set_graphs; not a reference option) */
 } nntc_encode_options;
 
 void nntc_encode_default_options(nntc_encode_options* o);
@@ -78,6 +82,10 @@ nntc_encoder* nntc_encoder_create_device(const nntc_encode_options* o, int textu
 int nntc_encoder_run(nntc_encoder* e);
 
 void nntc_encoder_destroy(nntc_encoder* e);
+
+/* The gfx950 arm's context the encoder drives (NULL for another arm), for the arm's own queries
+ * (nntc_hip_device_memory, nntc_hip_graph_counts, ...). Owned by the encoder. */
+nntc_hip_ctx* nntc_encoder_context(const nntc_encoder* e);
 
 const char* nntc_encode_last_error(void);
 

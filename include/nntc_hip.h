@@ -214,6 +214,28 @@ int nntc_hip_snap_level0_on_grid(nntc_hip_ctx* ctx, nntc_model_state* state);
  * assembled stencil. */
 int nntc_hip_set_matrix_free(nntc_hip_ctx* ctx, int mode);
 
+/* The fast form's quantised sweeps at three or four channels (no counterpart in the reference): -1 (default) takes
+ * the launch-shape tuner's choice (below), 1 gives each texel one lane, 4 gives each texel four lanes (one channel row
+ * each). Both forms take the same decisions from the same sums and write the same bits. */
+int nntc_hip_set_sweep_lanes(nntc_hip_ctx* ctx, int lanes);
+
+/* The fast form's conjugate gradients as HIP graphs (no counterpart in the reference, whose iterations are launched
+ * one kernel at a time): -1 (default) or 1, each probe period of CG_PROBE = 4 iterations is captured once per set of
+ * launch arguments and replayed; 0, every kernel launched directly. The launches and their order are the same either
+ * way. The counts say how many periods were captured and how many replayed on this context. */
+int nntc_hip_set_graphs(nntc_hip_ctx* ctx, int mode);
+int nntc_hip_graph_counts(const nntc_hip_ctx* ctx, long long* captures, long long* launches);
+
+/* The launch-shape tuner's table (no counterpart in the reference). Where a kernel family has launch forms that write
+ * the same bits (today: the fast sweeps' one or four lanes a texel), the first call of a shape - latent channels and
+ * texel count to a power of two - on a device times each form once on that call's own state, restoring what the
+ * trials moved, and every later call of the shape in the process takes the fastest. Entry i (0 up): the family (0 the
+ * sweep lanes), the channels, the bucket (texels <= 2^bucket), the form chosen, and each form tried with its time in
+ * ms; nonzero past the last entry. The reset empties the table, so the next calls tune again. */
+int nntc_hip_tuning_entry(int i, int* family, int* channels, int* bucket, int* choice, int* forms, double* ms,
+                          int* nforms);
+void nntc_hip_tuning_reset(void);
+
 /* Block (c')'s operator applied once, for checks (no counterpart in the reference): level 0's system is assembled for
  * the state, as solve_level0_cont_all assembles it, and out = (H + lambda I) p on every plane, from the form's
  * assembled stencil (mode 0) or the fast form's matrix-free operator (mode 1). p and out hold every plane's level-0
@@ -227,7 +249,8 @@ int nntc_hip_stencil_monomials(int c0);
 /* Replaces be::solve_decoder (src/backend.h:104): block (a). per_site has nplanes entries. Writes state's decoder. */
 int nntc_hip_solve_decoder(nntc_hip_ctx* ctx, nntc_model_state* state, int k, const double* per_site, double* ms);
 
-/* Replaces be::decoder_ridge_tally (src/backend.h:105): the rungs the process's block (a) calls ended on. */
+/* Replaces be::decoder_ridge_tally (src/backend.h:105): the rungs the block (a) calls made from this host thread
+ * ended on (the reference counts per process; per thread here, so encodes driven from other threads do not mix in). */
 void nntc_hip_decoder_ridge_tally(long long out[4]);
 
 /* Replace be::solve_level1_all, sweep_level1_all, solve_level0_cont_all, sweep_level0_cont_all
@@ -311,7 +334,7 @@ int nntc_hip_decode_adjoint(const nntc_sites_desc* d, int fp64, const void* v0, 
 /* Replaces be::objective_eval (src/backend.h:133). per_site has nplanes entries. */
 int nntc_hip_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* r);
 
-/* Replace be::objective_total_ms and be::objective_passes (src/backend.h:134-135). */
+/* Replace be::objective_total_ms and be::objective_passes (src/backend.h:134-135), counted per host thread. */
 double nntc_hip_objective_total_ms(void);
 long long nntc_hip_objective_passes(void);
 

@@ -12,6 +12,7 @@
 #include "nntc_hip.h"
 
 #include <string>
+#include <mutex>
 #include <vector>
 
 namespace nntc
@@ -104,6 +105,7 @@ struct Profiler
     std::string only;                   // when set, the one kernel timed
     std::vector<std::pair<std::string, KernelStat>> stats;
     std::vector<hipEvent_t> pool;
+    std::recursive_mutex lock;          // contexts driven from several host threads time into one table
     KernelStat& get(const char* name);
     hipEvent_t event();
     void settle();                      // read every pending pair (synchronizes on the last)
@@ -112,6 +114,7 @@ struct Profiler
 };
 
 Profiler& profiler();
+bool profiled(const char* name);   // would a KernelTimer of this name record events now
 
 // RAII bracket around one launch: `KernelTimer t(c.stream, "k_matvec", bytes, flops); hipLaunchKernelGGL(...);`
 struct KernelTimer
@@ -131,6 +134,17 @@ struct Ctx
     int device = -1;                       // the HIP device the context was created on
     int matrix_free = -1;                  // the fast level-0 operator: -1 by size (4096^2 and up), 0 assembled, 1 matrix-free
     int assembled_k = -1;                  // the K of the last assembly
+    int sweep_lanes = -1;                  // the fast sweeps at C >= 3: -1 the tuner's choice, 1 or 4 lanes a texel
+    int graphs = -1;                       // the fast CG's probe periods as HIP graphs: -1 / 1 on, 0 off
+    // The fast CG's instantiated probe periods, by their launch arguments (kept for the context's lifetime: an
+    // encode's calls of one latent repeat the same launches on the same buffers).
+    struct CgGraph
+    {
+        std::vector<unsigned char> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<CgGraph> cg_graphs;
+    long long graph_launches = 0, graph_captures = 0;
     Planes planes {};
     long long n0_total = 0, n1_total = 0, nb_total = 0;
     hipStream_t stream = nullptr;
@@ -312,5 +326,27 @@ inline unsigned int blocks_for(long long n, int threads)
 {
     return (unsigned int)((n + threads - 1) / threads);
 }
+
+inline int ilog2_ceil(long long n)
+{
+    int b = 0;
+    while ((1ll << b) < n)
+        b++;
+    return b;
+}
+
+// ---- the launch-shape tuner (arm.hip): for a kernel family with several launch forms that write the same bits, the
+// form each shape takes on each device, picked by timing the forms on the first call of that shape.
+enum TuneFamily
+{
+    TUNE_SWEEP_LANES = 0,   // the fast quantised sweeps at C >= 3: one lane or four lanes a texel
+    TUNE_FAMILIES
+};
+struct TuneKey
+{
+    int device, family, channels, bucket;   // bucket: the texel count's power of two, rounded up
+};
+int tuned_choice(const TuneKey& k);          // 0 when the shape has not been tuned
+void record_choice(const TuneKey& k, int choice, const int* forms, const double* ms, int nforms);
 
 }   // namespace nntc

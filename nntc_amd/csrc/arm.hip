@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 namespace nntc
 {
@@ -86,6 +87,7 @@ hipEvent_t Profiler::event()
 
 void Profiler::settle()
 {
+    std::lock_guard<std::recursive_mutex> g(lock);
     for (auto& kv : stats)
     {
         KernelStat& st = kv.second;
@@ -104,6 +106,7 @@ void Profiler::settle()
 
 void Profiler::reset()
 {
+    std::lock_guard<std::recursive_mutex> g(lock);
     settle();
     stats.clear();
 }
@@ -114,10 +117,22 @@ Profiler::~Profiler()
         (void)hipEventDestroy(e);
 }
 
+bool profiled(const char* name)
+{
+    Profiler& p = profiler();
+    if (!p.on)
+        return false;
+    std::lock_guard<std::recursive_mutex> g(p.lock);
+    return p.only.empty() || p.only == name;
+}
+
 KernelTimer::KernelTimer(hipStream_t s, const char* name, double bytes, double flops) : stream(s)
 {
     Profiler& p = profiler();
-    if (!p.on || (!p.only.empty() && p.only != name))
+    if (!p.on)
+        return;
+    std::lock_guard<std::recursive_mutex> g(p.lock);
+    if (!p.only.empty() && p.only != name)
         return;
     st = &p.get(name);
     st->launches++;
@@ -132,6 +147,7 @@ KernelTimer::~KernelTimer()
     if (!st)
         return;
     Profiler& p = profiler();
+    std::lock_guard<std::recursive_mutex> g(p.lock);
     hipEvent_t e = p.event();
     if (hipEventRecord(e, stream) == hipSuccess)
     {
@@ -196,9 +212,11 @@ namespace
 {
 
 thread_local std::string g_error;
-double g_objective_ms = 0.0;
-long long g_objective_calls = 0;
-long long g_ridge_tally[4] = { 0, 0, 0, 0 };
+// The reference's process-global counters (src/backend.h:105, 134-135), kept per host thread: the driver reads them
+// before and after an encode, and each encode runs on one thread, so encodes on other threads do not mix in.
+thread_local double g_objective_ms = 0.0;
+thread_local long long g_objective_calls = 0;
+thread_local long long g_ridge_tally[4] = { 0, 0, 0, 0 };
 
 const double RIDGE_LADDER[] = { 1e-9, 1e-12, 0.0 };
 const double LS_ACCEPT_RISE = 1e-12;
@@ -594,6 +612,49 @@ void blocks_from_device(Ctx& c, double* pack_psnr, size_t* pack_texels)
 
 }   // namespace nntc
 
+// ---- the launch-shape tuner's table: one entry per (device, family, channels, bucket), for the process's lifetime.
+namespace nntc
+{
+namespace
+{
+struct TuneEntry
+{
+    TuneKey key;
+    int choice;
+    int forms[4];
+    double ms[4];
+    int nforms;
+};
+std::mutex g_tune_lock;
+std::vector<TuneEntry> g_tune;
+}   // namespace
+
+int tuned_choice(const TuneKey& k)
+{
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    for (const TuneEntry& e : g_tune)
+        if (e.key.device == k.device && e.key.family == k.family && e.key.channels == k.channels &&
+            e.key.bucket == k.bucket)
+            return e.choice;
+    return 0;
+}
+
+void record_choice(const TuneKey& k, int choice, const int* forms, const double* ms, int nforms)
+{
+    std::lock_guard<std::mutex> g(g_tune_lock);
+    TuneEntry e {};
+    e.key = k;
+    e.choice = choice;
+    e.nforms = nforms < 4 ? nforms : 4;
+    for (int i = 0; i < e.nforms; i++)
+    {
+        e.forms[i] = forms[i];
+        e.ms[i] = ms[i];
+    }
+    g_tune.push_back(e);
+}
+}   // namespace nntc
+
 using namespace nntc;
 
 struct nntc_hip_ctx
@@ -885,6 +946,9 @@ void nntc_hip_device_destroy(nntc_hip_ctx* h)
         (void)hipEventDestroy(c.ev2);
     for (hipEvent_t e : c.pass_ev)
         (void)hipEventDestroy(e);
+    for (Ctx::CgGraph& g : c.cg_graphs)
+        if (g.exec)
+            (void)hipGraphExecDestroy(g.exec);
     hipStream_t s = c.stream;
     delete h;
     if (s)
@@ -911,6 +975,58 @@ int nntc_hip_set_matrix_free(nntc_hip_ctx* h, int mode)
         if (mode < -1 || mode > 1)
             throw Refused { "set_matrix_free takes -1, 0 or 1" };
         h->c.matrix_free = mode;
+    });
+}
+
+int nntc_hip_tuning_entry(int i, int* family, int* channels, int* bucket, int* choice, int* forms, double* ms,
+                          int* nforms)
+{
+    std::lock_guard<std::mutex> g(nntc::g_tune_lock);
+    if (i < 0 || (size_t)i >= nntc::g_tune.size())
+        return 1;
+    const nntc::TuneEntry& e = nntc::g_tune[(size_t)i];
+    *family = e.key.family;
+    *channels = e.key.channels;
+    *bucket = e.key.bucket;
+    *choice = e.choice;
+    *nforms = e.nforms;
+    for (int f = 0; f < e.nforms; f++)
+    {
+        forms[f] = e.forms[f];
+        ms[f] = e.ms[f];
+    }
+    return 0;
+}
+
+void nntc_hip_tuning_reset(void)
+{
+    std::lock_guard<std::mutex> g(nntc::g_tune_lock);
+    nntc::g_tune.clear();
+}
+
+int nntc_hip_set_sweep_lanes(nntc_hip_ctx* h, int lanes)
+{
+    return guarded([&] {
+        if (lanes != -1 && lanes != 1 && lanes != 4)
+            throw Refused { "set_sweep_lanes takes -1, 1 or 4" };
+        h->c.sweep_lanes = lanes;
+    });
+}
+
+int nntc_hip_set_graphs(nntc_hip_ctx* h, int mode)
+{
+    return guarded([&] {
+        if (mode < -1 || mode > 1)
+            throw Refused { "set_graphs takes -1, 0 or 1" };
+        h->c.graphs = mode;
+    });
+}
+
+int nntc_hip_graph_counts(const nntc_hip_ctx* h, long long* captures, long long* launches)
+{
+    return guarded([&] {
+        *captures = h->c.graph_captures;
+        *launches = h->c.graph_launches;
     });
 }
 
@@ -1606,7 +1722,9 @@ void nntc_hip_profile_enable(int on)
 
 void nntc_hip_profile_only(const char* kernel)
 {
-    profiler().only = kernel ? kernel : "";
+    Profiler& p = profiler();
+    std::lock_guard<std::recursive_mutex> g(p.lock);
+    p.only = kernel ? kernel : "";
 }
 
 void nntc_hip_profile_reset(void)
@@ -1618,6 +1736,7 @@ int nntc_hip_kernel_stats(int index, char* name_out, size_t name_len, long long*
                           double* flops)
 {
     Profiler& p = profiler();
+    std::lock_guard<std::recursive_mutex> g(p.lock);
     if (guarded([&] { p.settle(); }) != 0)
         return 1;
     if (index < 0 || (size_t)index >= p.stats.size())

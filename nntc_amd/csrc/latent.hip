@@ -24,6 +24,7 @@
 #include "tile_stage.h"
 
 #include <cmath>
+#include <functional>
 #include <type_traits>
 
 namespace nntc
@@ -3375,6 +3376,48 @@ MatFreeArgs matfree_args(Ctx& c, const FastCg& fc, unsigned& grid)
     return a;
 }
 
+// The launch arguments a captured graph was made from, as bytes (values and buffer addresses).
+struct GraphKey
+{
+    std::vector<unsigned char> bytes;
+    template <class T>
+    GraphKey& add(const T& v)
+    {
+        const unsigned char* b = reinterpret_cast<const unsigned char*>(&v);
+        bytes.insert(bytes.end(), b, b + sizeof(T));
+        return *this;
+    }
+};
+
+// The context's graph for `key`, captured from `body`'s launches on the context's stream when it has none yet. The
+// capture is thread-local, so contexts driven by other host threads keep launching while one captures.
+hipGraphExec_t period_graph(Ctx& c, const std::vector<unsigned char>& key, const std::function<void()>& body)
+{
+    for (const Ctx::CgGraph& g : c.cg_graphs)
+        if (g.key == key)
+            return g.exec;
+    hipGraph_t graph = nullptr;
+    HIP_OK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    try
+    {
+        body();
+    }
+    catch (...)
+    {
+        if (hipStreamEndCapture(c.stream, &graph) == hipSuccess && graph)
+            (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIP_OK(hipStreamEndCapture(c.stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t made = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIP_OK(made);
+    c.cg_graphs.push_back(Ctx::CgGraph { key, exec });
+    c.graph_captures++;
+    return exec;
+}
+
 }   // namespace
 
 void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
@@ -3441,52 +3484,82 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
         mf_bytes = 8.0 * nv;
         mf_flops = sites * c.nout * 2.0 * (c.c0 * c.c1 + 2.0 * c.c0 + 1.0) + sites * 3.4 * 2.0 * c.c0;
     }
+    // One iteration of the fast form's conjugate gradients: the operator (and p.Ap), alpha, the update (with |r|^2 and
+    // r.z), beta and p = z + beta p; the fold after the update applies the stopping bar on every CG_PROBE-th.
+    auto fast_iteration = [&](int it) {
+        fc.probe = (it + 1) % CG_PROBE == 0;
+        // The fast form's bytes: the fp32 stencil (20 c^2 per texel), p read and Ap written (4 c each); the update
+        // reads the preconditioner, x, p, r, Ap and writes x, r, z; p = z + beta p reads two and writes one.
+        if (mf)
+        {
+            {
+                KernelTimer kt(c.stream, "k_adjoint_matvec0", mf_bytes, mf_flops);
+                launch_adjoint_matvec0(c.c0, c.c1, mf_grid, c.stream, mfa);
+            }
+            hipLaunchKernelGGL(k_pap_chunks, dim3(fgrid), dim3(256), 0, c.stream, fc);
+        }
+        else
+        {
+            KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", (double)n * 20.0 * cc * cc + 8.0 * nv,
+                           (double)n * (2.0 * 9.0 * cc * cc));
+            switch (L.c)
+            {
+            case 1: hipLaunchKernelGGL(k_matvec_fast<1>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+            case 2: hipLaunchKernelGGL(k_matvec_fast<2>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
+            case 3: hipLaunchKernelGGL(k_matvec_fast_rows<3>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
+            default: hipLaunchKernelGGL(k_matvec_fast_rows<4>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
+            }
+        }
+        if (!fold_in_update)
+            hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
+        {
+            KernelTimer kt(c.stream, "k_cg_update", (double)n * 4.0 * cc * cc + 44.0 * nv, nv * (4.0 + 2.0 * cc));
+            if (fold_in_update)
+                hipLaunchKernelGGL(k_cg_update_fast<true>, dim3(fgrid), dim3(256), 0, c.stream, fc);
+            else
+                hipLaunchKernelGGL(k_cg_update_fast<false>, dim3(fgrid), dim3(256), 0, c.stream, fc);
+        }
+        if (!fold_in_update)
+            hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
+        {
+            KernelTimer kt(c.stream, "k_xpby", 12.0 * nv, 2.0 * nv);
+            if (fold_in_update)
+                hipLaunchKernelGGL(k_xpby_fold_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
+            else
+                hipLaunchKernelGGL(k_xpby_fast, dim3(tb), dim3(256), 0, c.stream, L);
+        }
+        HIP_OK(hipGetLastError());
+    };
+    // The fast form runs each probe period - CG_PROBE iterations, the same launches on the same buffers every time -
+    // as one HIP graph, captured on the first call that meets these launch arguments and replayed by every later
+    // period and call. Not while the profiler times one of its kernels (its events are host-side per launch).
+    hipGraphExec_t period = nullptr;
+    if (c.form == NNTC_FORM_FAST && c.graphs != 0 && !profiled(mf ? "k_adjoint_matvec0" : (level0 ? "k_matvec0" : "k_matvec1")) &&
+        !profiled("k_cg_update") && !profiled("k_xpby"))
+    {
+        GraphKey key;
+        key.add(level0).add(mf).add(fold_in_update).add(fgrid).add(tb).add(mf_grid).add(mfa.k).add(np);
+        key.add(L.x).add(L.r).add(L.z).add(L.p).add(L.ap).add(L.precond).add(L.stencil32).add(L.scalars);
+        key.add(fc.partial).add(fc.pap).add(mfa.tab).add(mfa.v1).add(mfa.weights).add(mfa.cw);
+        for (int p = 0; p <= np; p++)
+            key.add(L.texels.start[p]).add(fc.chunks.start[p]);
+        period = period_graph(c, key.bytes, [&] {
+            for (int i = 0; i < CG_PROBE; i++)
+                fast_iteration(i);
+        });
+    }
     for (int it = 0; it < CG_MAX_ITERATIONS; it++)
     {
         if (!ch.exact)
         {
-            fc.probe = (it + 1) % CG_PROBE == 0;
-            // The fast form's bytes: the fp32 stencil (20 c^2 per texel), p read and Ap written (4 c each); the update
-            // reads the preconditioner, x, p, r, Ap and writes x, r, z; p = z + beta p reads two and writes one.
-            if (mf)
+            if (period)
             {
-                {
-                    KernelTimer kt(c.stream, "k_adjoint_matvec0", mf_bytes, mf_flops);
-                    launch_adjoint_matvec0(c.c0, c.c1, mf_grid, c.stream, mfa);
-                }
-                hipLaunchKernelGGL(k_pap_chunks, dim3(fgrid), dim3(256), 0, c.stream, fc);
+                HIP_OK(hipGraphLaunch(period, c.stream));
+                c.graph_launches++;
+                it += CG_PROBE - 1;
             }
             else
-            {
-                KernelTimer kt(c.stream, level0 ? "k_matvec0" : "k_matvec1", (double)n * 20.0 * cc * cc + 8.0 * nv,
-                               (double)n * (2.0 * 9.0 * cc * cc));
-                switch (L.c)
-                {
-                case 1: hipLaunchKernelGGL(k_matvec_fast<1>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
-                case 2: hipLaunchKernelGGL(k_matvec_fast<2>, dim3(fgrid), dim3(256), 0, c.stream, fc); break;
-                case 3: hipLaunchKernelGGL(k_matvec_fast_rows<3>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
-                default: hipLaunchKernelGGL(k_matvec_fast_rows<4>, dim3(fgrid), dim3(1024), 0, c.stream, fc); break;
-                }
-            }
-            if (!fold_in_update)
-                hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
-            {
-                KernelTimer kt(c.stream, "k_cg_update", (double)n * 4.0 * cc * cc + 44.0 * nv, nv * (4.0 + 2.0 * cc));
-                if (fold_in_update)
-                    hipLaunchKernelGGL(k_cg_update_fast<true>, dim3(fgrid), dim3(256), 0, c.stream, fc);
-                else
-                    hipLaunchKernelGGL(k_cg_update_fast<false>, dim3(fgrid), dim3(256), 0, c.stream, fc);
-            }
-            if (!fold_in_update)
-                hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
-            {
-                KernelTimer kt(c.stream, "k_xpby", 12.0 * nv, 2.0 * nv);
-                if (fold_in_update)
-                    hipLaunchKernelGGL(k_xpby_fold_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
-                else
-                    hipLaunchKernelGGL(k_xpby_fast, dim3(tb), dim3(256), 0, c.stream, L);
-            }
-            HIP_OK(hipGetLastError());
+                fast_iteration(it);
         }
         else
         {
@@ -3656,10 +3729,73 @@ const float* site_targets(Ctx& c, int k)
     return c.site_target.p;
 }
 
+// One colour of one sweep in the fast form at C >= 3: one lane a texel, or four (one channel row each). The two forms
+// take the same decisions from the same sums and write the same bits.
+static void launch_sweep_fast(int lanes, long long n, hipStream_t s, const LatentView& L, const Ranges& colour, int col,
+                              unsigned int* counters)
+{
+    if (lanes == 1)
+    {
+        launch_sweep<float>(L.c, dim3(blocks_for(n, 256)), s, L, colour, col, counters);
+        return;
+    }
+    const dim3 grid4(blocks_for(4 * n, 256));
+    if (L.c == 3)
+        hipLaunchKernelGGL(k_quant_sweep_rows<3>, grid4, dim3(256), 0, s, L, colour, col, counters);
+    else
+        hipLaunchKernelGGL(k_quant_sweep_rows<4>, grid4, dim3(256), 0, s, L, colour, col, counters);
+}
+
+// The lanes a texel of the fast sweeps at C >= 3, from the launch-shape tuner: the first call of a shape (latent,
+// channels, texel count to a power of two) times one sweep of each form on the call's own state, restoring the values,
+// indices and corrections it moves after each trial, and every later call of that shape on the device takes the
+// faster. The forms write the same bits, so the choice changes the time and nothing else.
+static int sweep_lanes_for(Ctx& c, const LatentView& L, const Ranges colour[4])
+{
+    if (c.sweep_lanes > 0)
+        return c.sweep_lanes;
+    const long long n = L.texels.start[L.texels.n];
+    const TuneKey key { c.device, TUNE_SWEEP_LANES, L.c, ilog2_ceil(n) };
+    const int known = tuned_choice(key);
+    if (known > 0)
+        return known;
+    const size_t nv = (size_t)n * L.c;
+    const size_t bytes = nv * (sizeof(float) + sizeof(uint8_t) + sizeof(double));
+    DevBuf<uint8_t> save;
+    save.alloc(bytes);
+    uint8_t* sv = save.p;
+    uint8_t* sk = sv + nv * sizeof(float);
+    uint8_t* sx = sk + nv;
+    HIP_OK(hipMemcpyAsync(sv, L.v, nv * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(sk, L.k, nv, hipMemcpyDeviceToDevice, c.stream));
+    HIP_OK(hipMemcpyAsync(sx, L.x, nv * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    const int forms[2] = { 1, 4 };
+    double ms[2];
+    for (int f = 0; f < 2; f++)
+    {
+        HIP_OK(hipEventRecord(c.ev0, c.stream));
+        for (int col = 0; col < 4; col++)
+        {
+            const long long nc = colour[col].start[colour[col].n];
+            if (nc > 0)
+                launch_sweep_fast(forms[f], nc, c.stream, L, colour[col], col, c.counters.p);
+        }
+        HIP_OK(hipGetLastError());
+        ms[f] = c.ms_since(c.ev0);
+        HIP_OK(hipMemcpyAsync(L.v, sv, nv * sizeof(float), hipMemcpyDeviceToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(L.k, sk, nv, hipMemcpyDeviceToDevice, c.stream));
+        HIP_OK(hipMemcpyAsync(L.x, sx, nv * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    }
+    HIP_OK(hipStreamSynchronize(c.stream));
+    save.release();
+    const int choice = ms[0] <= ms[1] ? forms[0] : forms[1];
+    record_choice(key, choice, forms, ms, 2);
+    return choice;
+}
+
 void quant_sweeps(Ctx& c, bool level0, int sweeps)
 {
     LatentView L = make_view(c, level0);
-    HIP_OK(hipMemsetAsync(c.counters.p, 0, c.counters.bytes(), c.stream));
     Ranges colour[4];
     for (int col = 0; col < 4; col++)
     {
@@ -3674,34 +3810,30 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
         }
         colour[col].start[c.planes.n] = acc;
     }
+    const bool fast_rows = c.form != NNTC_FORM_EXACT && L.c >= 3;
+    const int lanes = fast_rows && sweeps > 0 ? sweep_lanes_for(c, L, colour) : 1;
+    // The counters start at zero after any trials of the tuner.
+    HIP_OK(hipMemsetAsync(c.counters.p, 0, c.counters.bytes(), c.stream));
     for (int sw = 0; sw < sweeps; sw++)
         for (int col = 0; col < 4; col++)
         {
             const long long n = colour[col].start[colour[col].n];
             if (n == 0)
                 continue;
-            // A colour's texels read their own stencil (five C x C blocks) and nine corrections (their own and their
-            // eight neighbours'), 8-byte values or 4-byte in the fast form, and their gradient, plane and index.
+            // A colour's texels read their own stencil blocks and their four backward neighbours' (the transposes
+            // of the blocks those own towards the texel: nine C x C blocks in all, each read by this texel alone in
+            // this launch), nine corrections (their own and their eight neighbours'), 8-byte values or 4-byte in the
+            // fast form, and their gradient, plane and index.
             const double cc = L.c, sb = c.form == NNTC_FORM_EXACT ? 8.0 : 4.0;
             KernelTimer kt(c.stream, level0 ? "k_quant_sweep0" : "k_quant_sweep1",
-                           (double)n * (5.0 * sb * cc * cc + 8.0 * cc + 9.0 * sb * cc + 9.0 * cc), 0.0);
-            const dim3 grid(blocks_for(n, 256));
+                           (double)n * (9.0 * sb * cc * cc + 8.0 * cc + 9.0 * sb * cc + 9.0 * cc), 0.0);
             // The exact form reads the fp64 stencil; the fast form its fp32 copy (half the bytes).
             if (c.form == NNTC_FORM_EXACT)
-                launch_sweep<double>(L.c, grid, c.stream, L, colour[col], col, c.counters.p);
-            else if (L.c >= 3)
-            {
-                // Four lanes per texel (one per channel row).
-                const dim3 grid4(blocks_for(4 * n, 256));
-                if (L.c == 3)
-                    hipLaunchKernelGGL(k_quant_sweep_rows<3>, grid4, dim3(256), 0, c.stream, L, colour[col], col,
-                                       c.counters.p);
-                else
-                    hipLaunchKernelGGL(k_quant_sweep_rows<4>, grid4, dim3(256), 0, c.stream, L, colour[col], col,
-                                       c.counters.p);
-            }
+                launch_sweep<double>(L.c, dim3(blocks_for(n, 256)), c.stream, L, colour[col], col, c.counters.p);
+            else if (fast_rows)
+                launch_sweep_fast(lanes, n, c.stream, L, colour[col], col, c.counters.p);
             else
-                launch_sweep<float>(L.c, grid, c.stream, L, colour[col], col, c.counters.p);
+                launch_sweep<float>(L.c, dim3(blocks_for(n, 256)), c.stream, L, colour[col], col, c.counters.p);
         }
     HIP_OK(hipGetLastError());
 }

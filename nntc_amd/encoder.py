@@ -27,6 +27,8 @@ class Options(C.Structure):
         ("mip_mix", C.c_double), ("bc_refine", C.c_int32), ("bc_refine_after", C.c_int32),
         ("bc_outer", C.c_int32), ("init0_chain", C.c_int32), ("form", C.c_int32), ("quiet", C.c_int32),
         ("weights", C.c_double * 6), ("rgb_weights", C.c_double * 3), ("matrix_free", C.c_int32),
+        ("sweep_lanes", C.c_int32),
+        ("graphs", C.c_int32),
     ]
 
 

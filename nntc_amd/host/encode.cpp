@@ -538,6 +538,10 @@ void Encoder::setup(int textures, int width, int height, const uint8_t* const* r
         check(seam.set_form(ctx, opts.form), "set_form");
     if (opts.matrix_free != -1 && seam.name == "hip")
         check(nntc_hip_set_matrix_free(ctx, opts.matrix_free), "set_matrix_free");
+    if (opts.sweep_lanes != -1 && seam.name == "hip")
+        check(nntc_hip_set_sweep_lanes(ctx, opts.sweep_lanes), "set_sweep_lanes");
+    if (opts.graphs != -1 && seam.name == "hip")
+        check(nntc_hip_set_graphs(ctx, opts.graphs), "set_graphs");
 }
 
 void Encoder::setup_device(int textures, int width, int height, const uint8_t* const* rgb_device)
@@ -553,6 +557,10 @@ void Encoder::setup_device(int textures, int width, int height, const uint8_t* c
         check(seam.set_form(ctx, opts.form), "set_form");
     if (opts.matrix_free != -1 && seam.name == "hip")
         check(nntc_hip_set_matrix_free(ctx, opts.matrix_free), "set_matrix_free");
+    if (opts.sweep_lanes != -1 && seam.name == "hip")
+        check(nntc_hip_set_sweep_lanes(ctx, opts.sweep_lanes), "set_sweep_lanes");
+    if (opts.graphs != -1 && seam.name == "hip")
+        check(nntc_hip_set_graphs(ctx, opts.graphs), "set_graphs");
 }
 
 // The box init takes the source channels with every exact copy of an earlier channel moved to the back; when that
@@ -709,6 +717,8 @@ void nntc_encode_default_options(nntc_encode_options* o)
     for (int c = 0; c < 3; c++)
         o->rgb_weights[c] = 1.0;
     o->matrix_free = -1;
+    o->sweep_lanes = -1;
+    o->graphs = -1;
 }
 
 const char* nntc_encode_last_error(void)
@@ -836,6 +846,12 @@ int nntc_encoder_run(nntc_encoder* h)
 void nntc_encoder_destroy(nntc_encoder* h)
 {
     delete reinterpret_cast<Encoder*>(h);
+}
+
+nntc_hip_ctx* nntc_encoder_context(const nntc_encoder* h)
+{
+    const Encoder* e = reinterpret_cast<const Encoder*>(h);
+    return e->seam.name == "hip" ? e->ctx : nullptr;
 }
 
 int nntc_encoder_report(const nntc_encoder* h, nntc_encode_report* r)
