@@ -270,6 +270,49 @@ def kernels_brief(table):
                 "flops_per_launch": round(v["flops"] / max(1, v["launches"]))} for k, v in table.items()}
 
 
+LIBRARY_WORKERS = 4
+
+
+def library_figures(hip, workers=LIBRARY_WORKERS):
+    """The batching workload: 64 seeded materials of uneven sizes (each axis a multiple of 4 in [128, 2048]), 1 to 6
+    textures and four layouts (synth.library_spec), textures resident in device memory, chosen form, 8 rounds. One
+    untimed pass of the whole library (it also lets the launch-shape tuner meet every shape), then the library
+    timed one material at a time and `workers` at a time (nntc_encode_library: a host thread, a context and a stream
+    per material in flight). Mpix/s = the sum of W * H * T over the materials / seconds; the two passes' per-material
+    shipped objectives must be equal."""
+    from nntc_amd import abi, encoder, synth
+
+    specs = synth.library_spec()
+    textures = synth.library(specs)
+    mpix = sum(s["size"][0] * s["size"][1] * s["textures"] for s in specs) / 1e6
+    dev = [[abi.DeviceBytes(t) for t in tex] for tex in textures]
+    kw = dict(rounds=8, tol=0.0, quiet=1, form=FORMS["chosen"])
+    try:
+        mats = [dict(device=(s["size"][0], s["size"][1], [d.address for d in ds]), c0=s["c0"], c1=s["c1"])
+                for s, ds in zip(specs, dev)]
+        encoder.encode_library(mats, workers=workers, **kw)
+        out = {"materials": len(specs), "mpix": mpix, "workers": workers,
+               "layouts": {f"c0 {c0}/c1 {c1}": sum(1 for s in specs if (s["c0"], s["c1"]) == (c0, c1))
+                           for c0, c1 in synth.LIBRARY_LAYOUTS}}
+        shipped = {}
+        for w in (1, workers):
+            hip.nntc_hip_device_synchronize()
+            t0 = time.perf_counter()
+            reps = encoder.encode_library(mats, workers=w, **kw)
+            hip.nntc_hip_device_synchronize()
+            dt = time.perf_counter() - t0
+            shipped[w] = [r.e_shipped for r in reps]
+            out["one_at_a_time" if w == 1 else "side_by_side"] = {"seconds": dt, "mpix_s": mpix / dt}
+            print(f"# library, {w} worker(s): {mpix / dt:.2f} Mpix/s ({dt:.2f} s)", file=sys.stderr, flush=True)
+        out["side_by_side_over_one_at_a_time"] = out["side_by_side"]["mpix_s"] / out["one_at_a_time"]["mpix_s"]
+        out["identical_results"] = shipped[1] == shipped[workers]
+        return out
+    finally:
+        for ds in dev:
+            for d in ds:
+                d.close()
+
+
 def copy_calibration(runtime, n=256 << 20):
     """One device-to-device copy of n bytes (the runtime's copy kernel reads n and writes n): under rocprofv3 --pmc
     FETCH_SIZE / WRITE_SIZE it gives the factor between those counters' readings and bytes moved."""
@@ -361,6 +404,7 @@ def main():
             print(f"# {name}: {r['mpix_s']:.2f} Mpix/s resident, {r['end_to_end_mpix_s']:.2f} end to end, psnr gap "
                   f"{r['psnr_gap_db']}", file=sys.stderr, flush=True)
         extra = None
+        library = None
         if not a.no_extra:
             ex_spec = synth.WORKLOADS[EXTRA]
             ex_tex = synth.workload(EXTRA)
@@ -373,6 +417,7 @@ def main():
                                "roofline": roofline(dom, live, table, EXTRA)}
                 job.close()
             extra["chosen_over_plain"] = extra["chosen"]["mpix_s"] / extra["plain"]["mpix_s"]
+            library = library_figures(hip)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
 
@@ -412,6 +457,7 @@ def main():
                                "of the host-built chain included",
                        **{k: round(out[k]["end_to_end_mpix_s"], 4) for k in ("exact", "plain", "chosen")}},
         "extra": extra,
+        "library": library,
         "roofline": ch["roofline"],
         "cpu_baseline": cpu,
         "reference_full_workload": ({"seconds": golden["seconds"], "threads": golden["threads"],

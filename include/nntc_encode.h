@@ -118,6 +118,26 @@ int nntc_encoder_desc(const nntc_encoder* e, nntc_model_desc* d);
 /* Write the asset: PREFIX_lat0.dds (or _lat0a/_lat0b), PREFIX_lat1.dds and PREFIX_nntc.json (src/export.cpp:470). */
 int nntc_encoder_write(const nntc_encoder* e, const char* prefix);
 
+/* ---- a library of materials (no counterpart in the reference, which encodes one material per process) */
+typedef struct nntc_material
+{
+    int32_t textures, width, height;   /* 1..6 textures of width x height */
+    int32_t c0, c1;                    /* the layout (the options' c0 and c1 are replaced by these) */
+    int32_t on_device;                 /* rgb[t] is in gfx950 device memory (as nntc_encoder_create_device), else host */
+    const uint8_t* const* rgb;         /* texture t: width * height * 3 bytes, row-major RGB */
+    const char* prefix;                /* the asset's prefix (nntc_encoder_write), or NULL to write nothing */
+} nntc_material;
+
+/* Encodes `count` materials on the gfx950 arm, `workers` at a time: each worker is a host thread that owns one
+ * encoder at a time (its own context, so its own stream), takes the next material from a queue ordered largest first
+ * (width * height * textures), encodes it with the options `o`, writes its asset and frees it. The launches of up to
+ * `workers` materials are therefore in flight together, and one material's host decisions overlap the others'
+ * kernels. Every material runs the launches an encode of it alone runs, in the same order on its own stream: its
+ * outputs are that encode's bits. reports[i] (may be NULL) receives material i's report and status[i] (may be NULL)
+ * 0 or nonzero. Returns how many materials failed; nntc_encode_last_error names the first. */
+int nntc_encode_library(const nntc_material* materials, int count, const nntc_encode_options* o, int workers,
+                        nntc_encode_report* reports, int32_t* status);
+
 #ifdef __cplusplus
 }
 #endif

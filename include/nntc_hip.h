@@ -334,6 +334,23 @@ int nntc_hip_decode_adjoint(const nntc_sites_desc* d, int fp64, const void* v0, 
 /* Replaces be::objective_eval (src/backend.h:133). per_site has nplanes entries. */
 int nntc_hip_objective_eval(nntc_hip_ctx* ctx, int k, const double* per_site, nntc_objective* r);
 
+/* Replaces be::objective_check_host (src/backend.h:136; src/objective.cu:317-380): the --check twin of
+ * objective_eval, E by brute force on the host in fp64 from the device's current planes and the state's decoder - every
+ * site sampled by the bilinear clamp rule, decoded and weighed - sharing nothing with the device's passes but the
+ * model's definition. A checker for small planes; the encode never calls it. */
+int nntc_hip_objective_check_host(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, const double* per_site,
+                                  nntc_objective* r);
+
+/* Replaces be::solve_level1_dense_host (src/backend.h:124; src/solve_level1.cu:1502-1688): the --check twin of block
+ * (b)'s solve on plane `plane`. Level 1's Gauss-Newton system at c_prev (the plane's w1 * h1 * c1 level-1 values) is
+ * scattered into a dense matrix on the host in fp64, site by site, and (H + lambda I) delta = -grad is solved by
+ * Cholesky into delta (n_delta >= w1 * h1 * c1). *solved is 0, and delta untouched, when the plane has more than
+ * max_unknowns unknowns or the factorisation meets a pivot that is not positive (lambda 0 on a singular H), as the
+ * reference returns false. */
+int nntc_hip_solve_level1_dense_host(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, int plane, double lambda,
+                                     const float* c_prev, size_t n_prev, double* delta, size_t n_delta,
+                                     int max_unknowns, int* solved);
+
 /* Replace be::objective_total_ms and be::objective_passes (src/backend.h:134-135), counted per host thread. */
 double nntc_hip_objective_total_ms(void);
 long long nntc_hip_objective_passes(void);

@@ -162,6 +162,9 @@ SIGNATURES = {
     "decode_plane": (C.c_int, [C.c_void_p, C.c_int, _u8p, C.c_size_t]),
     "texture_sse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_ulonglong)]),
     "objective_eval": (C.c_int, [C.c_void_p, C.c_int, _f64p, _P(Objective)]),
+    "objective_check_host": (C.c_int, [C.c_void_p, _P(ModelState), C.c_int, _f64p, _P(Objective)]),
+    "solve_level1_dense_host": (C.c_int, [C.c_void_p, _P(ModelState), C.c_int, C.c_int, C.c_double, _f32p,
+                                          C.c_size_t, _f64p, C.c_size_t, C.c_int, _P(C.c_int)]),
     "objective_total_ms": (C.c_double, []),
     "objective_passes": (C.c_longlong, []),
     "bc_pack_prepare": (C.c_int, [C.c_void_p, _P(ModelState), C.c_int, C.c_int, _P(_u8p), _f64p, _P(C.c_size_t),
@@ -313,6 +316,22 @@ class Session:
         ps = np.ascontiguousarray(per_site, dtype=np.float64)
         self.call("objective_eval", k, _ptr(ps, C.c_double), C.byref(o))
         return o
+
+    def objective_check(self, k, per_site):
+        """The --check twin of objective(): E by brute force on the host."""
+        o = Objective()
+        ps = np.ascontiguousarray(per_site, dtype=np.float64)
+        self.call("objective_check_host", C.byref(self.state), k, _ptr(ps, C.c_double), C.byref(o))
+        return o
+
+    def level1_dense(self, k, plane, lam, c_prev, max_unknowns=1024):
+        """The --check twin of block (b) on one plane: the dense solve at c_prev, or None when it declines."""
+        prev = np.ascontiguousarray(c_prev, dtype=np.float32)
+        out = np.zeros(prev.size, dtype=np.float64)
+        solved = C.c_int(0)
+        self.call("solve_level1_dense_host", C.byref(self.state), k, plane, lam, _ptr(prev, C.c_float), prev.size,
+                  _ptr(out, C.c_double), out.size, max_unknowns, C.byref(solved))
+        return out if solved.value else None
 
     def solve_decoder(self, k, per_site):
         ps = np.ascontiguousarray(per_site, dtype=np.float64)

@@ -23,8 +23,8 @@ HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(ROOT, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["arm", "objective", "decoder", "latent", "init", "refine", "adjoint"]
-HOST_SOURCES = ["encode", "image", "writer"]
+HIP_SOURCES = ["arm", "objective", "decoder", "latent", "init", "refine", "adjoint", "twins"]
+HOST_SOURCES = ["encode", "image", "writer", "library"]
 
 # -ffp-contract=off: no fused multiply-add anywhere, as the reference builds (CMakeLists.txt:183-187, 206-208).
 # The packed-fp32 feature is switched off for the device: the compiler's own packed FP32 forms are not used.

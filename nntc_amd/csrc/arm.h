@@ -315,6 +315,10 @@ void cov_read(Ctx& c, std::vector<double>& raw);
 double proj_peak_chain(Ctx& c, const float* resid, const double* e, const double* mu);
 void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const double* mu, double inv, int channel);
 unsigned long long level1_off_grid(Ctx& c, const float* lo, const float* hi);
+// twins.hip: the --check twins (host, fp64, from the device's current planes)
+void objective_check_host(Ctx& c, const nntc_model_state* st, int k, double* acc /* nplanes * 3 */);
+bool solve_level1_dense_host(Ctx& c, const nntc_model_state* st, int k, int plane, double lambda,
+                             const float* c_prev, std::vector<double>& delta, int max_unknowns);
 // refine.hip
 void bc_decode(Ctx& c);
 void bc_seed(Ctx& c);

@@ -642,6 +642,11 @@ int tuned_choice(const TuneKey& k)
 void record_choice(const TuneKey& k, int choice, const int* forms, const double* ms, int nforms)
 {
     std::lock_guard<std::mutex> g(g_tune_lock);
+    // Two contexts may tune one shape at once (a library of materials): the first to finish keeps the entry.
+    for (const TuneEntry& t : g_tune)
+        if (t.key.device == k.device && t.key.family == k.family && t.key.channels == k.channels &&
+            t.key.bucket == k.bucket)
+            return;
     TuneEntry e {};
     e.key = k;
     e.choice = choice;
@@ -1559,6 +1564,41 @@ int nntc_hip_texture_sse_noalloc(nntc_hip_ctx* h, int width, int height, unsigne
     });
 }
 
+}   // extern "C"
+
+namespace
+{
+
+// E and its report from the per-plane sums [weighted SE, centre SE, fractional SE] (src/objective.cu:233-279).
+void objective_report(const Ctx& c, int k, const double* per_site, const std::vector<double>& acc,
+                      nntc_objective* r)
+{
+    double sum_cw = 0.0;
+    for (int q = 0; q < c.nout; q++)
+        sum_cw += (double)c.md.cw[q];
+    double num = 0.0, den = 0.0;
+    *r = nntc_objective {};
+    r->nplanes = c.planes.n;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        const double pixels = (double)c.g(p).w0 * (double)c.g(p).h0;
+        const double sites = pixels * (double)(1 + k);
+        r->e_plane[p] = acc[(size_t)3 * p] / (sum_cw * sites);
+        r->centre_plane[p] = acc[(size_t)3 * p + 1] / (pixels * (double)c.nout);
+        num += per_site[p] * acc[(size_t)3 * p];
+        den += per_site[p] * sites;
+    }
+    r->e = den > 0.0 ? num / (sum_cw * den) : 0.0;
+    const double base = (double)c.g(0).w0 * (double)c.g(0).h0 * (double)c.nout;
+    r->centre_mse = acc[1] / base;
+    r->sampled_mse = k > 0 ? acc[2] / (base * (double)k) : 0.0;
+}
+
+}   // namespace
+
+extern "C"
+{
+
 int nntc_hip_objective_eval(nntc_hip_ctx* h, int k, const double* per_site, nntc_objective* r)
 {
     return guarded([&] {
@@ -1567,28 +1607,45 @@ int nntc_hip_objective_eval(nntc_hip_ctx* h, int k, const double* per_site, nntc
         std::vector<double> acc((size_t)c.planes.n * 3);
         objective_pass(c, k, acc.data());
         const double ms = now_ms() - t0;
-        double sum_cw = 0.0;
-        for (int q = 0; q < c.nout; q++)
-            sum_cw += (double)c.md.cw[q];
-        double num = 0.0, den = 0.0;
-        *r = nntc_objective {};
-        r->nplanes = c.planes.n;
-        for (int p = 0; p < c.planes.n; p++)
-        {
-            const double pixels = (double)c.g(p).w0 * (double)c.g(p).h0;
-            const double sites = pixels * (double)(1 + k);
-            r->e_plane[p] = acc[(size_t)3 * p] / (sum_cw * sites);
-            r->centre_plane[p] = acc[(size_t)3 * p + 1] / (pixels * (double)c.nout);
-            num += per_site[p] * acc[(size_t)3 * p];
-            den += per_site[p] * sites;
-        }
-        r->e = den > 0.0 ? num / (sum_cw * den) : 0.0;
-        const double base = (double)c.g(0).w0 * (double)c.g(0).h0 * (double)c.nout;
-        r->centre_mse = acc[1] / base;
-        r->sampled_mse = k > 0 ? acc[2] / (base * (double)k) : 0.0;
+        objective_report(c, k, per_site, acc, r);
         r->ms = ms;
         g_objective_ms += ms;
         g_objective_calls++;
+    });
+}
+
+int nntc_hip_objective_check_host(nntc_hip_ctx* h, const nntc_model_state* st, int k, const double* per_site,
+                                  nntc_objective* r)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        if (!st->has_decoder)
+            throw Refused { "objective_check_host: the state has no decoder" };
+        std::vector<double> acc((size_t)c.planes.n * 3);
+        objective_check_host(c, st, k, acc.data());
+        objective_report(c, k, per_site, acc, r);
+        r->ms = 0.0;
+    });
+}
+
+int nntc_hip_solve_level1_dense_host(nntc_hip_ctx* h, const nntc_model_state* st, int k, int plane, double lambda,
+                                     const float* c_prev, size_t n_prev, double* delta, size_t n_delta,
+                                     int max_unknowns, int* solved)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        check_plane(c, plane);
+        if (!st->has_decoder)
+            throw Refused { "solve_level1_dense_host: the state has no decoder" };
+        const size_t nunk = (size_t)c.g(plane).n1 * c.c1;
+        if (n_prev != nunk)
+            throw Refused { "solve_level1_dense_host: c_prev must hold the plane's level-1 values" };
+        std::vector<double> d;
+        *solved = solve_level1_dense_host(c, st, k, plane, lambda, c_prev, d, max_unknowns) ? 1 : 0;
+        if (*solved && n_delta < d.size())
+            throw Refused { "solve_level1_dense_host: delta is shorter than the plane's unknowns" };
+        for (size_t i = 0; i < d.size() && *solved; i++)
+            delta[i] = d[i];
     });
 }
 

@@ -45,6 +45,13 @@ class Report(C.Structure):
     ]
 
 
+class Material(C.Structure):
+    _fields_ = [
+        ("textures", C.c_int32), ("width", C.c_int32), ("height", C.c_int32), ("c0", C.c_int32), ("c1", C.c_int32),
+        ("on_device", C.c_int32), ("rgb", C.c_void_p), ("prefix", C.c_char_p),
+    ]
+
+
 _lib = None
 
 
@@ -77,6 +84,8 @@ def lib():
                                                  C.POINTER(C.c_size_t)]
         L.nntc_encoder_recon.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), C.c_size_t, C.POINTER(C.c_size_t)]
         L.nntc_encoder_write.argtypes = [C.c_void_p, C.c_char_p]
+        L.nntc_encode_library.argtypes = [C.POINTER(Material), C.c_int, C.POINTER(Options), C.c_int,
+                                          C.POINTER(Report), C.POINTER(C.c_int32)]
     return _lib
 
 
@@ -167,3 +176,35 @@ class Encoder:
             self.close()
         except Exception:
             pass
+
+
+def encode_library(materials: list[dict], workers: int = 4, **opts) -> list[Report]:
+    """Encode a library of materials side by side (include/nntc_encode.h, nntc_encode_library). Each material is a
+    dict with c0, c1 and either `textures` (host RGB8 arrays) or `device` = (width, height, [device pointers]), and
+    optionally `prefix` (where to write its asset). Returns the reports in the materials' order; raises if any
+    material failed."""
+    L = lib()
+    o = options(**opts)
+    arr = (Material * len(materials))()
+    keep = []
+    for i, m in enumerate(materials):
+        if "device" in m:
+            w, h, ptrs = m["device"]
+            rgb = (C.c_void_p * len(ptrs))(*[C.c_void_p(int(x)) for x in ptrs])
+            on_device = 1
+        else:
+            tex = [np.ascontiguousarray(t, dtype=np.uint8) for t in m["textures"]]
+            h, w = tex[0].shape[:2]
+            rgb = (C.c_void_p * len(tex))(*[t.ctypes.data for t in tex])
+            keep.append(tex)
+            on_device = 0
+        keep.append(rgb)
+        prefix = m.get("prefix")
+        arr[i] = Material(len(rgb), w, h, m["c0"], m["c1"], on_device, C.cast(rgb, C.c_void_p),
+                          prefix.encode() if prefix else None)
+    reports = (Report * len(materials))()
+    status = (C.c_int32 * len(materials))()
+    failed = L.nntc_encode_library(arr, len(materials), C.byref(o), workers, reports, status)
+    if failed:
+        raise RuntimeError(f"{failed} of {len(materials)} materials failed: {L.nntc_encode_last_error().decode()}")
+    return list(reports)

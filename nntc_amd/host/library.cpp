@@ -10,7 +10,7 @@
 // others' kernels. Each material runs the launches an encode of it alone runs, in the same order on its own stream, so
 // its outputs are that encode's bits.
 
-#include "nntc_encode.h"
+#include "nntc_host.h"
 
 #include <algorithm>
 #include <atomic>
@@ -84,6 +84,6 @@ extern "C" int nntc_encode_library(const nntc_material* materials, int count, co
     for (std::thread& t : pool)
         t.join();
     if (!first_error.empty())
-        nntc_encode_set_error(first_error.c_str());
+        nntc_host::g_error = first_error;
     return failed.load();
 }

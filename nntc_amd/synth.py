@@ -104,3 +104,36 @@ def write_pngs(textures: list[np.ndarray], prefix: str) -> list[str]:
         Image.fromarray(t, mode="RGB").save(p)
         paths.append(p)
     return paths
+
+
+# The batching workload's layouts (c0, c1).
+LIBRARY_LAYOUTS = [(2, 4), (3, 4), (4, 4), (1, 2)]
+
+
+def library_spec(count: int = 64, config: int = 5) -> list[dict]:
+    """A library of `count` seeded materials of uneven shapes: per material, width and height uniform over the
+    multiples of 4 in [128, 2048] (so not square in general), 1 to 6 textures, and a layout drawn from
+    LIBRARY_LAYOUTS; the textures themselves are make_material's at base period 64 with 3 smooth fields and a full
+    detail field, seeded by (config, material)."""
+    rng = np.random.Generator(np.random.PCG64(material_seed(config, 0xFFFF)))
+    out = []
+    for m in range(count):
+        w = 4 * int(rng.integers(32, 513))
+        h = 4 * int(rng.integers(32, 513))
+        t = int(rng.integers(1, 7))
+        c0, c1 = LIBRARY_LAYOUTS[int(rng.integers(0, len(LIBRARY_LAYOUTS)))]
+        out.append(dict(textures=t, size=(w, h), period=64, smooth=3, detail=1.0, c0=c0, c1=c1, config=config,
+                        material=m))
+    return out
+
+
+def library(specs: list[dict], threads: int = 8) -> list[list[np.ndarray]]:
+    """The textures of every material of `specs` (library_spec), made on `threads` host threads."""
+    import concurrent.futures as cf
+
+    def one(s):
+        return make_material(s["textures"], s["size"], s["period"], s["smooth"], s["detail"], s["config"],
+                             s["material"])
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        return list(ex.map(one, specs))

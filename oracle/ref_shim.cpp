@@ -579,6 +579,22 @@ int nntc_ref_objective_check_host(nntc_hip_ctx* ctx, const nntc_model_state* st,
     return 0;
 }
 
+int nntc_ref_solve_level1_dense_host(nntc_hip_ctx* ctx, const nntc_model_state* st, int k, int plane, double lambda,
+                                     const float* c_prev, size_t n_prev, double* delta, size_t n_delta,
+                                     int max_unknowns, int* solved)
+{
+    REF_CTX;
+    state_to_model(st, m);
+    std::vector<float> prev(c_prev, c_prev + n_prev);
+    std::vector<double> d;
+    *solved = be::solve_level1_dense_host(r->d, m, k, plane, lambda, prev, d, max_unknowns) ? 1 : 0;
+    if (*solved && n_delta < d.size())
+        return 1;
+    for (size_t i = 0; i < d.size() && *solved; i++)
+        delta[i] = d[i];
+    return 0;
+}
+
 double nntc_ref_objective_total_ms(void)
 {
     return be::objective_total_ms();
