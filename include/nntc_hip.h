@@ -142,10 +142,12 @@ typedef struct nntc_bc_refine_report
  *   PLAIN  one kernel per operation with wavefront-tree reductions, the assembled fp32 stencil, fp32 conjugate gradients
  *          to a 1e-6 relative residual (at most 200 iterations), and the decoder Gram in fp64 on the vector ALUs
  *          (the reference's 512 site blocks x 256-site tiles, src/solve_decoder.cu:87-161).
- *   FAST   the chosen form: PLAIN with the decoder Gram as bf16 hi/lo split products on the matrix cores (fp32
- *          accumulation, fp64 across tiles), the level-0 conjugate gradients on the matrix-free decode adjoint for
- *          planes of 4096^2 and above, and each probe group of conjugate-gradient iterations replayed from a captured
- *          HIP graph. */
+ *   FAST   the chosen form: PLAIN with the decoder Gram as bf16 three-part split products on the matrix cores (fp32
+ *          accumulation within a wavefront's sites, fp64 beyond), the decoder's normal equations solved on the device
+ *          (fp64 Cholesky, one refinement step), the level-0 conjugate gradients on the matrix-free decode adjoint for
+ *          planes of 4096^2 and above, each probe group of conjugate-gradient iterations replayed from a captured HIP
+ *          graph, fp32 arithmetic in the BC refinement, and the sweeps' launch form at three or four channels taken
+ *          from the launch-shape tuner. */
 #define NNTC_FORM_EXACT 0
 #define NNTC_FORM_FAST 1
 #define NNTC_FORM_PLAIN 2

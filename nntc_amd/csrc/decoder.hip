@@ -219,6 +219,15 @@ __device__ __forceinline__ void split3_bf16(float x, unsigned short& h, unsigned
     l = __builtin_bit_cast(unsigned short, (__bf16)r2);
 }
 
+// Orders a wavefront's LDS accesses before and after it (writes then reads of other lanes' words, and back), without
+// waiting for the rest of the workgroup.
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Site s of feature f in a part's table. A matrix-core operand read takes one 16-byte group of eight sites from
 // each of 32 features; with rows of 512 bytes every such group would sit in the same LDS banks, so the groups of a row
 // are XOR-swizzled by the feature and the 32 reads spread over the banks.
@@ -323,7 +332,11 @@ __global__ __launch_bounds__(256, DB == 1 ? 2 : 1) void k_gram_tiles(GramArgs a)
                 s_lo[gram_slot(f, tid)] = l;
             }
         }
-        __syncthreads();
+        // A wavefront reads back only the site columns its own lanes wrote (sites 64 wave .. 64 wave + 63), so the
+        // rounds need no workgroup barrier: a wavefront-scope fence orders its LDS writes before its reads (and, at
+        // the end of the round, its reads before the next round's writes), and the four wavefronts run their rounds
+        // out of step, one's feature pass beside another's products.
+        wave_lds_fence();
 #pragma unroll
         for (int t = 0; t < NT; t++)
 #pragma unroll
@@ -373,8 +386,9 @@ __global__ __launch_bounds__(256, DB == 1 ? 2 : 1) void k_gram_tiles(GramArgs a)
 #pragma unroll
             for (int r = 0; r < 16; r++)
                 acc64[t][r] += (double)acc[t][r];
-        __syncthreads();
+        wave_lds_fence();
     }
+    __syncthreads();   // G below overlays every wavefront's tables
 
     // ---- the four wavefronts' blocks added into G in double, wavefront by wavefront; lane l, value r holds
     // row 8 (r / 4) + 4 (l / 32) + r % 4, column l % 32 of its 32 x 32 block. A diagonal block adds S at its place,

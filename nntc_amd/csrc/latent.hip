@@ -3810,7 +3810,8 @@ void quant_sweeps(Ctx& c, bool level0, int sweeps)
         }
         colour[col].start[c.planes.n] = acc;
     }
-    const bool fast_rows = c.form != NNTC_FORM_EXACT && L.c >= 3;
+    // The plain form takes one lane a texel; the fast form at three or four channels the tuner's launch form.
+    const bool fast_rows = c.form == NNTC_FORM_FAST && L.c >= 3;
     const int lanes = fast_rows && sweeps > 0 ? sweep_lanes_for(c, L, colour) : 1;
     // The counters start at zero after any trials of the tuner.
     HIP_OK(hipMemsetAsync(c.counters.p, 0, c.counters.bytes(), c.stream));
