@@ -931,11 +931,13 @@ __global__ __launch_bounds__(64) void k_gather_table(int k, double* tab)
     tab[GATHER_TAB_MAX * GATHER_TAB_STRIDE + cls] = (double)n;
 }
 
-// The fast form's level-0 gather over LDS tiles (C <= 2), one texel per lane: a workgroup takes a GT_W x GT_H tile of
-// texels, copies its footprint's site terms (the pixels of the tile and a one-pixel ring: rows of consecutive sites, so
-// the copy is coalesced) into LDS, and each lane walks its texel's 3x3 pixels' sites there, in the order (pixel row,
-// pixel, site), interior texels taking their sites and weights from their class's list and the others from their taps. The per-site sums
-// are fp32 values (k_site_terms' fast form), so LDS holds them as floats without loss.
+// The fast form's level-0 gather over LDS tiles (C <= 3: the footprint's terms fill 61 KB at C = 3, and four
+// channels' would pass the 64 KB a workgroup's static LDS holds), one texel per lane: a workgroup takes a GT_W x GT_H
+// tile of texels, copies its footprint's site terms (the pixels of the tile and a one-pixel ring: rows of consecutive
+// sites, so the copy is coalesced) into LDS, and each lane walks its texel's 3x3 pixels' sites there, in the order
+// (pixel row, pixel, site), interior texels taking their sites and weights from their class's list and the others
+// from their taps. The per-site sums are fp32 values (k_site_terms' fast form), so LDS holds them as floats without
+// loss.
 constexpr int GT_W = 32, GT_H = 8;
 
 template <int C>
@@ -3175,6 +3177,8 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     }
     a.sites.start[c.planes.n] = acc;
     const int cs = level0 ? c.c1 : c.c0;   // the other latent's channels: the sample each site stores
+    // Bytes of a site's pulled-back residual and of a stencil entry: fp64 in the exact form, fp32 in the others.
+    const double sd = c.form == NNTC_FORM_EXACT ? 8.0 : 4.0, se = sd;
     // The fast form's level 0 stores each site's own block (its triangle) instead of the sample.
     const bool direct = level0 && c.form != NNTC_FORM_EXACT;
     const int per_site_f = direct ? L.c * (L.c + 1) / 2 : cs;
@@ -3182,10 +3186,13 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
         throw Refused { "assembly: the site buffer holds K <= 4" };
     const long long n = L.texels.start[L.texels.n];
     {
-        // Pass 1 reads the three planes once and writes (4 cs + 8 c) bytes per site.
-        const double in = 4.0 * c.n0_total * (c.nout + c.c0) + 4.0 * c.n1_total * c.c1;
-        KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1", in + (double)acc * (4.0 * cs + 8.0 * L.c),
-                       0.0);
+        // Pass 1 reads both latents once and the source (the exact form: the source plane once; the others: each
+        // site's entry of the per-site source table) and writes (4 per_site_f + sd c) bytes per site: the sample or
+        // the site's own block, and the residual's pull-back in fp64 (the exact form) or fp32.
+        const double in = 4.0 * c.n0_total * c.c0 + 4.0 * c.n1_total * c.c1 +
+                          (c.form == NNTC_FORM_EXACT ? 4.0 * c.n0_total : 4.0 * (double)acc) * c.nout;
+        KernelTimer kt(c.stream, level0 ? "k_site_terms0" : "k_site_terms1",
+                       in + (double)acc * (4.0 * per_site_f + sd * L.c), 0.0);
         const dim3 grid(blocks_for(acc, 256));
         a.site_target = c.form != NNTC_FORM_EXACT ? site_targets(c, k) : nullptr;
         if (level0 && direct)
@@ -3200,13 +3207,14 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
     HIP_OK(hipGetLastError());
     {
         // Pass 2 reads every site's terms once per texel it reaches (4 for level 1's sites, about 3.4 for level 0's)
-        // and writes the stencil and gradient, 8 (5 c^2 + c) bytes per texel; the algorithmic traffic counts each
-        // site once.
-        const double out = 8.0 * n * (5.0 * L.c * L.c + L.c);
-        KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * cs + 8.0 * L.c) + out, 0.0);
+        // and writes the stencil (fp64 in the exact form, the fp32 copy in the others) and the fp64 gradient; the
+        // algorithmic traffic counts each site once.
+        const double out = n * (se * 5.0 * L.c * L.c + 8.0 * L.c);
+        KernelTimer kt(c.stream, level0 ? "k_gather0" : "k_gather1", (double)acc * (4.0 * per_site_f + sd * L.c) + out,
+                       0.0);
         // The exact form gathers a texel per lane in the reference's order; the fast form spreads a level-1 texel's
         // footprint (125 sites at K = 4) over eight lanes and level 0's per-site blocks (45 sites) over two.
-        if (direct && L.c <= 2)
+        if (direct && L.c <= 3)
         {
             Ranges tiles;
             tiles.n = c.planes.n;
@@ -3225,8 +3233,10 @@ void assemble(Ctx& c, bool level0, int k, double ridge)
             a.tab = c.gather_tab.p;
             if (L.c == 1)
                 hipLaunchKernelGGL(k_gather_tile0<1>, dim3((unsigned)nt), dim3(256), 0, c.stream, a, tiles);
-            else
+            else if (L.c == 2)
                 hipLaunchKernelGGL(k_gather_tile0<2>, dim3((unsigned)nt), dim3(256), 0, c.stream, a, tiles);
+            else
+                hipLaunchKernelGGL(k_gather_tile0<3>, dim3((unsigned)nt), dim3(256), 0, c.stream, a, tiles);
         }
         else if (direct)
         {
