@@ -161,11 +161,8 @@ template <int C0, int C1>
 __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
 {
     constexpr int NIN = C1 + C0 + C0 * C1;
-    constexpr int NINP = (NIN + 3) & ~3;   // decoder rows padded to whole float4s
     __shared__ float s_l0[tile_l0_floats<C0>()];
     __shared__ float s_l1[TILE_L1_FLOATS];
-    __shared__ __align__(16) float s_w[MAX_NOUT * ((MAX_NIN + 3) & ~3)];
-    __shared__ float s_b[MAX_NOUT], s_cw[MAX_NOUT];
     __shared__ double part[4][3];
 
     const int tile = (int)xcd_block(blockIdx.x, gridDim.x);
@@ -173,16 +170,12 @@ __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
     const int nout = a.nout, tid = threadIdx.x;
     const TileGeom tg = stage_tile<C0, C1>(a.planes.p[p], tile - (int)a.chunks.start[p], nout, a.src, a.v0, a.v1,
                                            s_l0, s_l1);
-    for (int i = tid; i < nout * NINP; i += 256)
-    {
-        const int r = i / NINP, q = i - r * NINP;
-        s_w[i] = q < NIN ? a.weights[r * NIN + q] : 0.0f;
-    }
-    if (tid < nout)
-    {
-        s_b[tid] = a.bias[tid];
-        s_cw[tid] = a.cw[tid];
-    }
+    // The decoder is the same for every lane: its rows are read at uniform addresses, which the compiler serves with
+    // scalar loads into scalar registers, each a direct operand of the lanes' multiply-adds (no LDS copy, no vector
+    // register per weight).
+    const float* __restrict__ wts = a.weights;
+    const float* __restrict__ bias = a.bias;
+    const float* __restrict__ cw = a.cw;
     __syncthreads();
 
     // ---- one pixel per lane
@@ -193,28 +186,17 @@ __global__ __launch_bounds__(256) void k_objective_tiles(ObjArgs a)
         {
             float phi[NIN], target[MAX_NOUT];
             tile_site<C0, C1>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
-            // The decoder is re-read from LDS for every site (a broadcast read): kept in registers across the sites it
-            // would take nout x nin of them.
-            asm volatile("" ::: "memory");
             float we = 0.0f, ee = 0.0f;
 #pragma unroll
             for (int r = 0; r < MAX_NOUT; r++)
                 if (r < nout)
                 {
-                    float out = s_b[r];
-                    const float4* row = reinterpret_cast<const float4*>(s_w + r * NINP);
+                    float out = bias[r];
 #pragma unroll
-                    for (int i4 = 0; i4 < NINP / 4; i4++)
-                    {
-                        const float4 wv = row[i4];
-                        const float ww[4] = { wv.x, wv.y, wv.z, wv.w };
-#pragma unroll
-                        for (int u = 0; u < 4; u++)
-                            if (i4 * 4 + u < NIN)
-                                out = fmaf(ww[u], phi[i4 * 4 + u], out);
-                    }
+                    for (int i = 0; i < NIN; i++)
+                        out = fmaf(wts[r * NIN + i], phi[i], out);
                     const float e = out - target[r];
-                    we = fmaf(s_cw[r] * e, e, we);
+                    we = fmaf(cw[r] * e, e, we);
                     ee = fmaf(e, e, ee);
                 }
             acc_w += we;
