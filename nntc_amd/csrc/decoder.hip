@@ -292,16 +292,6 @@ __global__ __launch_bounds__(256, DB == 1 ? 2 : 1) void k_gram_tiles(GramArgs a)
         for (int r = 0; r < 16; r++)
             sacc[b][r] = 0.0f;
 
-    // Rows past the last target (the padding of G to whole 32-blocks) stay zero in every round.
-    const int D = MM + nout;
-#pragma unroll
-    for (int f = 0; f < DP; f++)
-        if (f >= D)
-        {
-            s_h[gram_slot(f, tid)] = 0;
-            s_m[gram_slot(f, tid)] = 0;
-            s_lo[gram_slot(f, tid)] = 0;
-        }
     __syncthreads();   // the staged tile, written by every lane, is read by every lane's sites below
     for (int j = 0; j <= a.k; j++)
     {
@@ -309,22 +299,22 @@ __global__ __launch_bounds__(256, DB == 1 ? 2 : 1) void k_gram_tiles(GramArgs a)
         {
             float phi[NIN], target[MAX_NOUT];
             if (live)
-                tile_site<C0, C1, false>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
+                tile_site<C0, C1>(tg, s_l0, s_l1, a.k, j, px, py, phi, target);
+            // Every feature slot is written every round (a padding slot or a lane past the plane writes zeros), its
+            // value picked by selects on compile-time slot numbers: no branch per feature.
 #pragma unroll
             for (int f = 0; f < DP; f++)
             {
-                if (f >= D)
-                    continue;
-                float x = 0.0f;
-                if (live)
-                {
-                    if (f < NIN)
-                        x = phi[f];
-                    else if (f == NIN)
-                        x = 1.0f;
-                    else if (f - MM < nout && f - MM < MAX_NOUT)
-                        x = target[f - MM];
-                }
+                float x;
+                if (f < NIN)
+                    x = phi[f];
+                else if (f == NIN)
+                    x = 1.0f;
+                else if (f - MM < MAX_NOUT)
+                    x = f - MM < nout ? target[f - MM] : 0.0f;
+                else
+                    x = 0.0f;
+                x = live ? x : 0.0f;
                 unsigned short h, m, l;
                 split3_bf16(x, h, m, l);
                 s_h[gram_slot(f, tid)] = h;

@@ -121,9 +121,7 @@ __device__ __forceinline__ TileGeom stage_tile(const PlaneGeom& g, int t, int no
 // Site j of pixel (px, py) of the staged tile: phi = [c (C1)][s (C0)][s_i c_q] and the target (nout values), fp32.
 // The blends are w00 a + w10 b + w01 e + w11 f with the last three products fused into the sum (these tiles serve the
 // fast form only), and the LDS offsets use 24-bit multiplies (every offset is far below 2^24).
-// FUSED: the blends with fused multiply-adds (the objective); otherwise in the reference's unfused order, so the
-// features are the bits the exact form's sampling gives (the decoder Gram).
-template <int C0, int C1, bool FUSED = true>
+template <int C0, int C1>
 __device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0, const float* s_l1, int k, int j,
                                           int px, int py, float* phi, float* target)
 {
@@ -144,8 +142,7 @@ __device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0,
         const float* lf = s_l1 + (rb + xb) * C1;
 #pragma unroll
         for (int q = 0; q < C1; q++)
-            phi[q] = FUSED ? fmaf(w11, lf[q], fmaf(w01, le[q], fmaf(w10, lb[q], w00 * la[q])))
-                           : w00 * la[q] + w10 * lb[q] + w01 * le[q] + w11 * lf[q];
+            phi[q] = fmaf(w11, lf[q], fmaf(w01, le[q], fmaf(w10, lb[q], w00 * la[q])));
     }
     const int nout = tg.nout, P = tg.P;
     if (j == 0)
@@ -171,13 +168,11 @@ __device__ __forceinline__ void tile_site(const TileGeom& tg, const float* s_l0,
         const float w01 = (1.0f - t0.fx) * t0.fy, w11 = t0.fx * t0.fy;
 #pragma unroll
         for (int i = 0; i < C0; i++)
-            phi[C1 + i] = FUSED ? fmaf(w11, pf[nout + i], fmaf(w01, pe[nout + i], fmaf(w10, pb[nout + i], w00 * pa[nout + i])))
-                                : w00 * pa[nout + i] + w10 * pb[nout + i] + w01 * pe[nout + i] + w11 * pf[nout + i];
+            phi[C1 + i] = fmaf(w11, pf[nout + i], fmaf(w01, pe[nout + i], fmaf(w10, pb[nout + i], w00 * pa[nout + i])));
 #pragma unroll
         for (int r = 0; r < MAX_NOUT; r++)
             if (r < nout)
-                target[r] = FUSED ? fmaf(w11, pf[r], fmaf(w01, pe[r], fmaf(w10, pb[r], w00 * pa[r])))
-                                  : w00 * pa[r] + w10 * pb[r] + w01 * pe[r] + w11 * pf[r];
+                target[r] = fmaf(w11, pf[r], fmaf(w01, pe[r], fmaf(w10, pb[r], w00 * pa[r])));
     }
 #pragma unroll
     for (int i = 0; i < C0; i++)
