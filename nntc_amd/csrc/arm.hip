@@ -1090,8 +1090,6 @@ int nntc_hip_init0_residual_channel(nntc_hip_ctx* h, nntc_model_state* st, int c
     return guarded([&] {
         Ctx& c = h->c;
         const int nout = c.nout;
-        if (c.md.init0_texture)
-            throw Refused { "--init0 texture is not built in the gfx950 arm; the default --init0 residual is" };
         if (channel < 0 || channel >= c.c0)
             throw Refused { "channel out of range" };
         upload_decoder(c, st);
@@ -1128,17 +1126,50 @@ int nntc_hip_init0_residual_channel(nntc_hip_ctx* h, nntc_model_state* st, int c
                                              mu[(size_t)rr] * mu[(size_t)s];
         for (int rr = 0; rr < nout; rr++)
             r.total_variance += cov[(size_t)rr * nout + rr];
-        // (3) the leading eigenvector, its largest component made positive.
-        std::vector<double> work = cov, vec, val;
-        jacobi_eigen(work, nout, vec, val);
-        int top = 0;
-        for (int i = 1; i < nout; i++)
-            if (val[(size_t)i] > val[(size_t)top])
-                top = i;
-        r.eigenvalue = val[(size_t)top];
-        std::vector<double> e((size_t)nout);
-        for (int rr = 0; rr < nout; rr++)
-            e[(size_t)rr] = vec[(size_t)rr * nout + top];
+        // (3) the direction, its largest component made positive: the leading eigenvector of the whole covariance
+        // (--init0 residual), or (--init0 texture, src/init.cu:1219-1260) that of the one texture's 3 x 3 block whose
+        // leading eigenvalue is the largest, every other output at zero.
+        std::vector<double> e((size_t)nout, 0.0);
+        if (c.md.init0_texture)
+        {
+            int best = 0;
+            double best_value = -1.0, best_vec[3] = { 0.0, 0.0, 0.0 };
+            for (int t = 0; t < c.md.textures; t++)
+            {
+                std::vector<double> block(9), vec, val;
+                for (int a = 0; a < 3; a++)
+                    for (int b = 0; b < 3; b++)
+                        block[(size_t)(a * 3 + b)] = cov[(size_t)(3 * t + a) * nout + (3 * t + b)];
+                jacobi_eigen(block, 3, vec, val);
+                int top = 0;
+                for (int i = 1; i < 3; i++)
+                    if (val[(size_t)i] > val[(size_t)top])
+                        top = i;
+                if (val[(size_t)top] > best_value)
+                {
+                    best_value = val[(size_t)top];
+                    best = t;
+                    for (int a = 0; a < 3; a++)
+                        best_vec[a] = vec[(size_t)(a * 3 + top)];
+                }
+            }
+            r.texture = best;
+            r.eigenvalue = best_value > 0.0 ? best_value : 0.0;
+            for (int a = 0; a < 3; a++)
+                e[(size_t)(3 * best + a)] = best_vec[a];
+        }
+        else
+        {
+            std::vector<double> work = cov, vec, val;
+            jacobi_eigen(work, nout, vec, val);
+            int top = 0;
+            for (int i = 1; i < nout; i++)
+                if (val[(size_t)i] > val[(size_t)top])
+                    top = i;
+            r.eigenvalue = val[(size_t)top];
+            for (int rr = 0; rr < nout; rr++)
+                e[(size_t)rr] = vec[(size_t)rr * nout + top];
+        }
         r.share = r.total_variance > 0.0 ? r.eigenvalue / r.total_variance : 0.0;
         int biggest = 0;
         for (int rr = 1; rr < nout; rr++)

@@ -2969,6 +2969,163 @@ __global__ __launch_bounds__(256) void k_cg_update_fast(FastCg a)
     }
 }
 
+// k_cg_update_fast<false> with one wavefront per chunk and four consecutive texels a lane, so that every vector is
+// moved in 16-byte pieces: r, p, Ap and z as float4s, x as double2s, the block inverses as float4s (a lane's four
+// texels hold 4 C floats of each fp32 vector, 4 C doubles of x and 4 C^2 floats of inverses, whole numbers of 16-byte
+// pieces when the chunk's first texel is a multiple of 4). Each element's arithmetic is k_cg_update_fast's; the
+// chunk's |r|^2 and r.z add a lane's texels in order, then the wavefront's lanes by a tree. A chunk that does not start
+// on a multiple of 4 (only where a small mip plane has an odd extent) takes the element path.
+template <int C>
+__global__ __launch_bounds__(256) void k_cg_update_vec(FastCg a)
+{
+    const int wave = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const long long chunk = (long long)xcd_block(blockIdx.x, gridDim.x) * 4 + wave;
+    if (chunk >= a.chunks.start[a.chunks.n])
+        return;   // a whole wavefront: nothing below synchronises beyond it
+    const LatentView& L = a.L;
+    const int p = range_plane(a.chunks, chunk);
+    const double* sc = L.scalars + (size_t)p * SC_COUNT;
+    if (sc[SC_ACTIVE] == 0.0)
+        return;
+    const double alpha = sc[SC_ALPHA];
+    const long long texels = L.texels.start[p + 1] - L.texels.start[p];
+    const long long off = (chunk - a.chunks.start[p]) * FAST_CG_CHUNK;
+    const long long t0 = L.texels.start[p] + off;
+    const long long t1 = L.texels.start[p] + (off + FAST_CG_CHUNK < texels ? off + FAST_CG_CHUNK : texels);
+    const long long tb = t0 + 4 * lane;
+    float* r = cg_vec<true>(L.r);
+    float* z = cg_vec<true>(L.z);
+    const float* pv = cg_vec<true>(L.p);
+    const float* ap = cg_vec<true>(L.ap);
+    double rr = 0.0, rz = 0.0;
+    if (tb + 4 <= t1 && (t0 & 3) == 0)
+    {
+        constexpr int NF = 4 * C, NM = 4 * C * C;
+        float rv[NF], pp[NF], av[NF], mv[NM], zv[NF];
+        double xv[NF];
+#pragma unroll
+        for (int q = 0; q < C; q++)
+        {
+            const float4 r4 = reinterpret_cast<const float4*>(r + tb * C)[q];
+            const float4 p4 = reinterpret_cast<const float4*>(pv + tb * C)[q];
+            const float4 a4 = reinterpret_cast<const float4*>(ap + tb * C)[q];
+            rv[4 * q] = r4.x; rv[4 * q + 1] = r4.y; rv[4 * q + 2] = r4.z; rv[4 * q + 3] = r4.w;
+            pp[4 * q] = p4.x; pp[4 * q + 1] = p4.y; pp[4 * q + 2] = p4.z; pp[4 * q + 3] = p4.w;
+            av[4 * q] = a4.x; av[4 * q + 1] = a4.y; av[4 * q + 2] = a4.z; av[4 * q + 3] = a4.w;
+        }
+#pragma unroll
+        for (int q = 0; q < 2 * C; q++)
+        {
+            const double2 x2 = reinterpret_cast<const double2*>(L.x + tb * C)[q];
+            xv[2 * q] = x2.x;
+            xv[2 * q + 1] = x2.y;
+        }
+#pragma unroll
+        for (int q = 0; q < C * C; q++)
+        {
+            const float4 m4 = reinterpret_cast<const float4*>(L.precond + tb * C * C)[q];
+            mv[4 * q] = m4.x; mv[4 * q + 1] = m4.y; mv[4 * q + 2] = m4.z; mv[4 * q + 3] = m4.w;
+        }
+#pragma unroll
+        for (int e = 0; e < NF; e++)
+        {
+            xv[e] += alpha * (double)pp[e];
+            rv[e] = (float)((double)rv[e] - alpha * (double)av[e]);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+        {
+#pragma unroll
+            for (int i = 0; i < C; i++)
+            {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < C; j++)
+                    acc += (double)mv[u * C * C + i * C + j] * (double)rv[u * C + j];
+                zv[u * C + i] = (float)acc;
+            }
+#pragma unroll
+            for (int i = 0; i < C; i++)
+            {
+                const double ri = rv[u * C + i];
+                rr += ri * ri;
+                rz += ri * (double)zv[u * C + i];
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < C; q++)
+        {
+            reinterpret_cast<float4*>(r + tb * C)[q] = make_float4(rv[4 * q], rv[4 * q + 1], rv[4 * q + 2], rv[4 * q + 3]);
+            reinterpret_cast<float4*>(z + tb * C)[q] = make_float4(zv[4 * q], zv[4 * q + 1], zv[4 * q + 2], zv[4 * q + 3]);
+        }
+#pragma unroll
+        for (int q = 0; q < 2 * C; q++)
+            reinterpret_cast<double2*>(L.x + tb * C)[q] = make_double2(xv[2 * q], xv[2 * q + 1]);
+    }
+    else
+    {
+        for (long long gt = tb; gt < tb + 4 && gt < t1; gt++)
+        {
+            for (int i = 0; i < C; i++)
+            {
+                const long long e = gt * C + i;
+                L.x[e] += alpha * (double)pv[e];
+                r[e] = (float)((double)r[e] - alpha * (double)ap[e]);
+            }
+            apply_precond(L.precond, r, C, z, gt);
+            for (int i = 0; i < C; i++)
+            {
+                const double ri = r[gt * C + i];
+                rr += ri * ri;
+                rz += ri * (double)z[gt * C + i];
+            }
+        }
+    }
+    rr = tree_lanes(rr);
+    rz = tree_lanes(rz);
+    if (lane == 0)
+    {
+        a.partial[(size_t)chunk * 2] = rr;
+        a.partial[(size_t)chunk * 2 + 1] = rz;
+    }
+}
+
+// k_xpby_fast over the flat vector, four elements (a float4) a lane; a piece that straddles two planes, or the tail,
+// takes the element path. Each element's arithmetic is k_xpby_fast's.
+__global__ __launch_bounds__(256) void k_xpby_vec(LatentView L)
+{
+    const long long n = L.texels.start[L.texels.n] * L.c;
+    const long long e0 = ((long long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) * 4;
+    if (e0 >= n)
+        return;
+    float* pv = cg_vec<true>(L.p);
+    const float* z = cg_vec<true>(L.z);
+    const int pa = range_plane(L.texels, e0 / L.c);
+    const int pb = range_plane(L.texels, (e0 + 3 < n ? e0 + 3 : n - 1) / L.c);
+    if (pa == pb && e0 + 4 <= n)
+    {
+        const double* s = L.scalars + (size_t)pa * SC_COUNT;
+        if (s[SC_ACTIVE] == 0.0)
+            return;
+        const double beta = s[SC_BETA];
+        const float4 z4 = reinterpret_cast<const float4*>(z)[e0 / 4];
+        float4 p4 = reinterpret_cast<float4*>(pv)[e0 / 4];
+        p4.x = (float)((double)z4.x + beta * (double)p4.x);
+        p4.y = (float)((double)z4.y + beta * (double)p4.y);
+        p4.z = (float)((double)z4.z + beta * (double)p4.z);
+        p4.w = (float)((double)z4.w + beta * (double)p4.w);
+        reinterpret_cast<float4*>(pv)[e0 / 4] = p4;
+        return;
+    }
+    for (long long e = e0; e < e0 + 4 && e < n; e++)
+    {
+        const double* s = L.scalars + (size_t)range_plane(L.texels, e / L.c) * SC_COUNT;
+        if (s[SC_ACTIVE] == 0.0)
+            continue;
+        pv[e] = (float)((double)z[e] + s[SC_BETA] * (double)pv[e]);
+    }
+}
+
 // Per plane (one workgroup each): |r|^2, r.z, beta, the relative residual; at a probe iteration a plane under the bar
 // (or NaN) stops.
 __global__ __launch_bounds__(256) void k_fold_beta_fast(FastCg a)
@@ -3527,7 +3684,17 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             if (fold_in_update)
                 hipLaunchKernelGGL(k_cg_update_fast<true>, dim3(fgrid), dim3(256), 0, c.stream, fc);
             else
-                hipLaunchKernelGGL(k_cg_update_fast<false>, dim3(fgrid), dim3(256), 0, c.stream, fc);
+            {
+                // One wavefront per chunk, four chunks a workgroup.
+                const dim3 vgrid((fgrid + 3) / 4);
+                switch (L.c)
+                {
+                case 1: hipLaunchKernelGGL(k_cg_update_vec<1>, vgrid, dim3(256), 0, c.stream, fc); break;
+                case 2: hipLaunchKernelGGL(k_cg_update_vec<2>, vgrid, dim3(256), 0, c.stream, fc); break;
+                case 3: hipLaunchKernelGGL(k_cg_update_vec<3>, vgrid, dim3(256), 0, c.stream, fc); break;
+                default: hipLaunchKernelGGL(k_cg_update_vec<4>, vgrid, dim3(256), 0, c.stream, fc); break;
+                }
+            }
         }
         if (!fold_in_update)
             hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
@@ -3536,7 +3703,7 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             if (fold_in_update)
                 hipLaunchKernelGGL(k_xpby_fold_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
             else
-                hipLaunchKernelGGL(k_xpby_fast, dim3(tb), dim3(256), 0, c.stream, L);
+                hipLaunchKernelGGL(k_xpby_vec, dim3(blocks_for((n * L.c + 3) / 4, 256)), dim3(256), 0, c.stream, L);
         }
         HIP_OK(hipGetLastError());
     };
