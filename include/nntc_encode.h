@@ -56,6 +56,9 @@ typedef struct nntc_encode_options
                                     a texel (nntc_hip_set_sweep_lanes; not a reference option) */
     int32_t graphs;              /* the fast conjugate gradients' probe periods as HIP graphs: -1 or 1 on, 0 off
                                     (nntc_hip_set_graphs; not a reference option) */
+    int32_t init1_pca;           /* --init pca (1) or box (0, the default) */
+    int32_t init0_texture;       /* --init0 texture (1) or residual (0, the default) */
+    int32_t init0_luma;          /* --init0 luma (1): level 0's channel 0 from the luminance, no residual seed */
 } nntc_encode_options;
 
 void nntc_encode_default_options(nntc_encode_options* o);

@@ -308,6 +308,7 @@ void channel_copies(Ctx& c, unsigned int* copies);
 // init.cu counterparts (init.hip)
 void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0);
 void init_level1_box(Ctx& c);
+void block_means(Ctx& c, int plane, float* out /* device, n1 * nout */);   // the pca init's per-texel source means
 void quantise_values(Ctx& c, bool level0, const float* in_device, int plane);
 void fit_range(Ctx& c, bool level0, int policy, float* lo, float* hi);
 void cov_accumulate(Ctx& c, const float* data, long long texels, double weight, bool first);

@@ -602,6 +602,86 @@ void pack_figures(const Ctx& c, double se, double* pack_psnr, size_t* pack_texel
         *pack_psnr = n > 0 && se > 0.0 ? 10.0 * std::log10(255.0 * 255.0 / (se / (double)n)) : 100.0;
 }
 
+// --init pca (src/init.cu:861-987): the base plane's block means of every source channel, their covariance (the
+// device's sums in the reference's k_cov_partial order, as the residual seed takes them), the host's Jacobi
+// eigensolver, the directions in order of their eigenvalues; every plane's block means projected onto the first C1
+// directions in fp64, each channel divided by the largest magnitude it reaches over the chain (clamped to [-1, 1]),
+// and written as the plane's level 1 for the grid fit that follows. The projection runs on the host, over the means
+// read back, as the reference's does, so the exact form keeps its bits.
+void init_level1_pca(Ctx& c, nntc_level1_init_report* rep)
+{
+    const int nout = c.nout, c1 = c.c1;
+    const int entries = nout + nout * nout;
+    std::vector<double> raw((size_t)entries, 0.0);
+    block_means(c, 0, c.resid.p);
+    cov_accumulate(c, c.resid.p, c.g(0).n1, 1.0, true);
+    cov_read(c, raw);
+    const double count = (double)c.g(0).n1;
+    std::vector<double> mu((size_t)nout, 0.0), cov((size_t)nout * nout, 0.0);
+    for (int r = 0; r < nout; r++)
+        mu[(size_t)r] = count > 0.0 ? raw[(size_t)r] / count : 0.0;
+    for (int r = 0; r < nout; r++)
+        for (int s = 0; s < nout; s++)
+            cov[(size_t)r * nout + s] =
+                (count > 0.0 ? raw[(size_t)(nout + r * nout + s)] / count : 0.0) - mu[(size_t)r] * mu[(size_t)s];
+    double total = 0.0;
+    for (int r = 0; r < nout; r++)
+        total += cov[(size_t)r * nout + r];
+    std::vector<double> work = cov, vec, val;
+    jacobi_eigen(work, nout, vec, val);
+    std::vector<int> order((size_t)nout);
+    for (int i = 0; i < nout; i++)
+        order[(size_t)i] = i;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return val[(size_t)a] > val[(size_t)b]; });
+    const int components = c1 < nout ? c1 : nout;
+    if (rep)
+    {
+        rep->total_variance = total;
+        rep->components = components;
+        for (int j = 0; j < components && j < MAX_CH; j++)
+            rep->eigenvalue[j] = val[(size_t)order[(size_t)j]];
+    }
+    std::vector<std::vector<float>> cont((size_t)c.planes.n);
+    std::vector<double> peak((size_t)c1, 0.0);
+    std::vector<float> means;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        const long long texels = c.g(p).n1;
+        block_means(c, p, c.resid.p);
+        means.resize((size_t)texels * nout);
+        const float* h = read_back(c, (const float*)c.resid.p, means.size());
+        std::copy(h, h + means.size(), means.begin());
+        cont[(size_t)p].assign((size_t)texels * c1, 0.0f);
+        for (long long t = 0; t < texels; t++)
+            for (int j = 0; j < components; j++)
+            {
+                const int col = order[(size_t)j];
+                double acc = 0.0;
+                for (int r = 0; r < nout; r++)
+                    acc += vec[(size_t)r * nout + col] * ((double)means[(size_t)t * nout + r] - mu[(size_t)r]);
+                cont[(size_t)p][(size_t)t * c1 + j] = (float)acc;
+                peak[(size_t)j] = std::max(peak[(size_t)j], std::fabs(acc));
+            }
+    }
+    for (int j = 0; j < c1; j++)
+    {
+        if (rep && j < MAX_CH)
+            rep->peak[j] = peak[(size_t)j];
+        if (!(peak[(size_t)j] > 0.0))
+            continue;
+        const double inv = 1.0 / peak[(size_t)j];
+        for (std::vector<float>& v : cont)
+            for (size_t t = (size_t)j; t < v.size(); t += (size_t)c1)
+            {
+                const double x = (double)v[t] * inv;
+                v[t] = (float)(x < -1.0 ? -1.0 : (x > 1.0 ? 1.0 : x));
+            }
+    }
+    for (int p = 0; p < c.planes.n; p++)
+        upload_staged(c, c.v1.p + c.g(p).off1 * c1, cont[(size_t)p].data(), cont[(size_t)p].size());
+    HIP_OK(hipStreamSynchronize(c.stream));
+}
+
 void blocks_from_device(Ctx& c, double* pack_psnr, size_t* pack_texels)
 {
     bc_pack_blocks(c, c.bc_fold.p);
@@ -1072,14 +1152,15 @@ int nntc_hip_init_level1(nntc_hip_ctx* h, nntc_model_state* st, int pca, nntc_le
 {
     return guarded([&] {
         Ctx& c = h->c;
-        if (pca)
-            throw Refused { "--init pca is not built in the gfx950 arm; the default box init is" };
         if (rep)
         {
             *rep = nntc_level1_init_report {};
-            rep->pca = 0;
+            rep->pca = pca ? 1 : 0;
         }
-        init_level1_box(c);
+        if (pca)
+            init_level1_pca(c, rep);
+        else
+            init_level1_box(c);
         fit_and_snap(c, st, false, NNTC_RANGE_MINMAX);
     });
 }

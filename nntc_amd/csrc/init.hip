@@ -459,6 +459,42 @@ void init_level0_kernel(Ctx& c, const float* luma, int luma_channel0)
     HIP_OK(hipGetLastError());
 }
 
+// The pca init's block means: every source channel's mean over level-1 texel t's footprint (box_footprint), with a
+// float accumulator added in footprint order, times the reciprocal of the count (src/init.cu:747-777).
+__global__ __launch_bounds__(256) void k_block_means(const float* src, int w0, int h0, int w1, int h1, int nout,
+                                                     float* means)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)w1 * h1)
+        return;
+    const int x = (int)(i % w1), y = (int)(i / w1);
+    int xa, xb, ya, yb;
+    box_footprint(x, y, w0, h0, w1, h1, xa, xb, ya, yb);
+    float acc[MAX_NOUT];
+    for (int c = 0; c < nout; c++)
+        acc[c] = 0.0f;
+    int n = 0;
+    for (int sy = ya; sy < yb; sy++)
+        for (int sx = xa; sx < xb; sx++)
+        {
+            const float* s = src + ((size_t)sy * w0 + sx) * nout;
+            for (int c = 0; c < nout; c++)
+                acc[c] += s[c];
+            n++;
+        }
+    const float inv = n > 0 ? 1.0f / (float)n : 0.0f;
+    for (int c = 0; c < nout; c++)
+        means[i * nout + c] = acc[c] * inv;
+}
+
+void block_means(Ctx& c, int plane, float* out)
+{
+    const PlaneGeom& g = c.g(plane);
+    hipLaunchKernelGGL(k_block_means, dim3(blocks_for(g.n1, 256)), dim3(256), 0, c.stream,
+                       (const float*)c.src.p + g.off0 * c.nout, g.w0, g.h0, g.w1, g.h1, c.nout, out);
+    HIP_OK(hipGetLastError());
+}
+
 void init_level1_box(Ctx& c)
 {
     hipLaunchKernelGGL(k_init_level1_box, dim3(blocks_for(c.n1_total, 256)), dim3(256), 0, c.stream, c.planes,

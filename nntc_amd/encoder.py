@@ -29,6 +29,7 @@ class Options(C.Structure):
         ("weights", C.c_double * 6), ("rgb_weights", C.c_double * 3), ("matrix_free", C.c_int32),
         ("sweep_lanes", C.c_int32),
         ("graphs", C.c_int32),
+        ("init1_pca", C.c_int32), ("init0_texture", C.c_int32), ("init0_luma", C.c_int32),
     ]
 
 

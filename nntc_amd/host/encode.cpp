@@ -161,10 +161,11 @@ void run_encode(Encoder& E)
     }
 
     // ---- the latents
-    cks(S.init_level0(d, &st, LUMA, 0), "init_level0");
+    cks(S.init_level0(d, &st, LUMA, E.opts.init0_luma ? 1 : 0), "init_level0");
     nntc_level1_init_report irep;
-    cks(S.init_level1(d, &st, 0, &irep), "init_level1");
-    E.box_reorder();
+    cks(S.init_level1(d, &st, E.opts.init1_pca ? 1 : 0, &irep), "init_level1");
+    if (!E.opts.init1_pca)
+        E.box_reorder();   // the box init's channel order (src/main.cpp:1938-2027); --init pca takes directions instead
 
     nntc_objective before, after, ea, eb, ec, probe;
     obj(before);
@@ -172,8 +173,9 @@ void run_encode(Encoder& E)
     ms_a += t;
     obj(after);
 
-    // ---- the residual seed, one channel at a time with a refit after each (--init0 residual)
-    for (int c = 0; c < E.desc.c0; c++)
+    // ---- the residual seed, one channel at a time with a refit after each (--init0 residual or texture; --init0
+    // luma seeds level 0's channel 0 from the luminance at init_level0 and takes no residual)
+    for (int c = 0; c < E.desc.c0 && !E.opts.init0_luma; c++)
     {
         nntc_init0_report cr;
         cks(S.init0_residual_channel(d, &st, c, ps, &cr), "init0_residual_channel");
@@ -483,7 +485,9 @@ void Encoder::describe(int textures, int width, int height)
     desc.bits1 = opts.bits1;
     desc.bc0 = 1;
     desc.l0_bc8 = 1;
-    desc.init0_residual = 1;
+    // --init0 residual or texture both seed from the residual; luma does not (src/main.cpp:1904-1906).
+    desc.init0_residual = opts.init0_luma ? 0 : 1;
+    desc.init0_texture = opts.init0_texture && !opts.init0_luma ? 1 : 0;
     desc.init0_chain = opts.init0_chain;
     desc.nplanes = (int)planes.size();
     for (size_t i = 0; i < planes.size(); i++)
@@ -719,6 +723,9 @@ void nntc_encode_default_options(nntc_encode_options* o)
     o->matrix_free = -1;
     o->sweep_lanes = -1;
     o->graphs = -1;
+    o->init1_pca = 0;
+    o->init0_texture = 0;
+    o->init0_luma = 0;
 }
 
 const char* nntc_encode_last_error(void)

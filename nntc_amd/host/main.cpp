@@ -4,11 +4,11 @@
 //   nntc_hip_encode a.png [b.png ...] [--c0 N] [--c1 N] [--rounds R] [--tol X] [--k K] [--mips 0|1] [--mip-min S]
 //                   [--mip-weight pixels|sqrt|uniform] [--mip-mix X] [--q1-start R0] [--q1-sweeps N]
 //                   [--q1-range minmax|pct] [--ridge X] [--bc-refine N] [--bc-refine-after N] [--bc-outer N]
-//                   [--init0-scope base|chain] [--weights W,..] [--rgb-weights R,G,B] [--form exact|plain|fast]
-//                   [--quiet] [-o PREFIX]
+//                   [--init box|pca] [--init0 residual|texture|luma] [--init0-scope base|chain] [--weights W,..]
+//                   [--rgb-weights R,G,B] [--form exact|plain|fast] [--quiet] [-o PREFIX]
 //
-// Options the reference has and this path does not build (--l0 palette, --init pca, --init0 texture|luma, named mip
-// filters, --backend cuda|cpu|check) are refused by name rather than ignored.
+// Options the reference has and this path does not build (--l0 palette, named mip filters, --backend cuda|cpu|check)
+// are refused by name rather than ignored.
 
 #include "nntc_encode.h"
 
@@ -145,13 +145,18 @@ int main(int argc, char** argv)
         }
         else if (a == "--init")
         {
-            if (std::string(next()) != "box")
-                refuse("pca is not built in the gfx950 arm; box, the default, is");
+            const std::string v = next();
+            if (v != "box" && v != "pca")
+                refuse("must be box or pca");
+            o.init1_pca = v == "pca";
         }
         else if (a == "--init0")
         {
-            if (std::string(next()) != "residual")
-                refuse("texture and luma are not built in the gfx950 arm; residual, the default, is");
+            const std::string v = next();
+            if (v != "residual" && v != "texture" && v != "luma")
+                refuse("must be residual, texture or luma");
+            o.init0_texture = v == "texture";
+            o.init0_luma = v == "luma";
         }
         else if (a == "--backend")
         {

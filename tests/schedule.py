@@ -28,23 +28,26 @@ class Options:
     ridge: float = 1e-4
     bc_refine: int = 4
     bc_outer: int = 2
+    pca: bool = False           # --init pca (the box init otherwise)
+    luma: bool = False          # --init0 luma (no residual seed)
 
 
 def steps(s: abi.Session, o: Options, per_site):
     nplanes = s.nplanes
-    s.init_level0(luma_channel0=False)
+    s.init_level0(luma_channel0=o.luma)
     yield "init_level0", {}
-    s.init_level1(pca=False)
-    yield "init_level1", {}
+    irep = s.init_level1(pca=o.pca)
+    yield "init_level1", {"eigenvalue": list(irep.eigenvalue), "peak": list(irep.peak),
+                          "total_variance": irep.total_variance, "components": irep.components}
     e = s.objective(o.k, per_site)
     yield "objective", {"e": e.e}
     s.solve_decoder(o.k, per_site)
     yield "solve_decoder", {}
     after = s.objective(o.k, per_site)
     yield "objective", {"e": after.e}
-    for c in range(s.desc.c0):
+    for c in range(0 if o.luma else s.desc.c0):
         rep = s.init0_residual_channel(c, per_site)
-        yield "init0_residual_channel", {"eigenvalue": rep.eigenvalue, "peak": rep.peak}
+        yield "init0_residual_channel", {"eigenvalue": rep.eigenvalue, "peak": rep.peak, "texture": rep.texture}
         seeded = s.objective(o.k, per_site)
         yield "objective", {"e": seeded.e}
         s.solve_decoder(o.k, per_site)

@@ -116,3 +116,34 @@ def test_bench_refuses_more_than_one_gpu():
                        stderr=subprocess.PIPE, text=True)
     assert r.returncode != 0
     assert "one GPU" in r.stderr
+
+
+@pytest.mark.parametrize("inits", [("pca", "texture"), ("box", "luma")], ids=["pca-texture", "box-luma"])
+@pytest.mark.parametrize("name", ["smoke_1x64x40", "small_2x100x76"])
+def test_driver_inits_match_reference_binary(name, inits, tmp_path):
+    """--init pca with --init0 texture, and --init0 luma: the driver over the reference CPU backend writes the bytes
+    the reference encoder itself writes for the same PNG files and options (both run here, live)."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "nntc_encode")
+    if not (os.path.exists(REF_LIB) and os.path.exists(ref_bin)):
+        pytest.skip("oracle/_ref is not built")
+    spec = synth.WORKLOADS[name]
+    paths = synth.write_pngs(synth.workload(name), str(tmp_path / "m"))
+    out = str(tmp_path / "ref") + os.sep
+    os.makedirs(out)
+    cmd = [ref_bin, *paths, "--c0", str(spec["c0"]), "--c1", str(spec["c1"]), "--rounds", "4", "--tol", "0",
+           "--png", "0", "--backend", "cpu", "-j", "8", "--quiet", "--init", inits[0], "--init0", inits[1], "-o", out]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_GLOBAL).nntc_ref_set_threads(8)
+    e = encoder.Encoder(files=paths, arm_lib=REF_LIB, arm_prefix="nntc_ref_", c0=spec["c0"], c1=spec["c1"], rounds=4,
+                        tol=0.0, quiet=1, init1_pca=int(inits[0] == "pca"), init0_texture=int(inits[1] == "texture"),
+                        init0_luma=int(inits[1] == "luma"))
+    e.run()
+    e.write(str(tmp_path / "drv"))
+    ref_files = sorted(f for f in os.listdir(out) if f.endswith(".dds"))
+    assert ref_files
+    for f in ref_files:
+        suffix = f[len("m_t0"):]
+        a = hashlib.sha256(open(os.path.join(out, f), "rb").read()).hexdigest()
+        b = hashlib.sha256(open(str(tmp_path / "drv") + suffix, "rb").read()).hexdigest()
+        assert a == b, f"{suffix} differs from the reference's"

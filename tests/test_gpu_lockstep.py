@@ -21,6 +21,8 @@ def material(name):
     textures, w, h, c0, c1, smooth = {
         "small_2x100x76_c3c3": (2, 100, 76, 3, 3, 3),
         "small_1x64x40_c1c2": (1, 64, 40, 1, 2, 3),
+        # more level-1 channels than the source has directions (the pca init's zero channel)
+        "small_1x64x40_c2c4": (1, 64, 40, 2, 4, 3),
         # six textures (nout 18) and four level-0 channels (two BC5 files), as the 2048^2 precision workload
         "small_6x100x76_c4c4": (6, 100, 76, 4, 4, 5),
     }[name]
@@ -28,11 +30,13 @@ def material(name):
     return tex, c0, c1
 
 
-def sessions(name, hip_arm, ref_arm):
+def sessions(name, hip_arm, ref_arm, init0_texture=False, init0_luma=False):
     tex, c0, c1 = material(name)
     planes, ch = chain.source_chain(tex)
     _, per_site = chain.mip_weights(planes)
     desc = abi.make_desc(len(tex), c0, c1, planes, chain.output_weights(len(tex)))
+    desc.init0_texture = 1 if init0_texture else 0
+    desc.init0_residual = 0 if init0_luma else 1
     flat = [c.reshape(-1) for c in ch]
     return abi.Session(hip_arm, desc, flat), abi.Session(ref_arm, desc, flat), per_site
 
@@ -61,7 +65,28 @@ def compare(a, b, step, i):
 def test_lockstep_exact(name, hip_arm, ref_arm):
     print(f"lockstep {name}", flush=True)
     h, r, per_site = sessions(name, hip_arm, ref_arm)
-    o = Options()
+    run_lockstep(h, r, per_site, Options())
+
+
+@pytest.mark.parametrize("name", ["small_2x100x76_c3c3", "small_1x64x40_c2c4"])
+def test_lockstep_exact_pca_and_texture_inits(name, hip_arm, ref_arm):
+    """--init pca (level 1 from the principal directions of the base plane's block means, src/init.cu:861-987) and
+    --init0 texture (each level-0 channel seeded along the leading direction of one texture's block of the residual's
+    covariance, src/init.cu:1219-1260), call by call against the reference; the second material has more level-1
+    channels than its source has directions, so one pca channel is all zeros."""
+    print(f"lockstep {name}, pca + init0 texture", flush=True)
+    h, r, per_site = sessions(name, hip_arm, ref_arm, init0_texture=True)
+    run_lockstep(h, r, per_site, Options(pca=True))
+
+
+def test_lockstep_exact_luma_init(hip_arm, ref_arm):
+    """--init0 luma: level 0's channel 0 from the source luminance, no residual seed (src/init.cu:389-482)."""
+    print("lockstep small_2x100x76_c3c3, init0 luma", flush=True)
+    h, r, per_site = sessions("small_2x100x76_c3c3", hip_arm, ref_arm, init0_luma=True)
+    run_lockstep(h, r, per_site, Options(luma=True))
+
+
+def run_lockstep(h, r, per_site, o):
     gh, gr = steps(h, o, per_site), steps(r, o, per_site)
     i = 0
     for (nh, ih), (nr, ir) in zip(gh, gr):
