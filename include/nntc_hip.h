@@ -266,6 +266,11 @@ int nntc_hip_solve_level0_cont_all(nntc_hip_ctx* ctx, const nntc_model_state* st
 int nntc_hip_sweep_level0_cont_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge, int sweeps,
                                    nntc_level1_report* rep, double* ms);
 
+/* Replaces be::solve_level0_all (src/backend.h:127): --l0 palette's block (c), the exact per-texel search over the
+ * palette in four colour passes on every plane. rep has nplanes entries (the block's time on the base plane's). */
+int nntc_hip_solve_level0_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, int sweeps,
+                              nntc_level0_report* rep, double* ms);
+
 /* Replaces be::assemble_level0_for_refine (src/backend.h:112). */
 int nntc_hip_assemble_level0_for_refine(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double* ms);
 

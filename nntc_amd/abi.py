@@ -81,6 +81,11 @@ class Init0Report(C.Structure):
                 ("nout", C.c_int32), ("texture", C.c_int32)]
 
 
+class Level0Report(C.Structure):
+    _fields_ = [("moved", C.c_int64 * 4), ("moved_total", C.c_int64), ("ms", C.c_double), ("joint", C.c_int32),
+                ("states", C.c_int64)]
+
+
 class Level1Report(C.Structure):
     _fields_ = [
         ("iterations", C.c_int32),
@@ -163,6 +168,7 @@ SIGNATURES = {
     "texture_sse": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_ulonglong)]),
     "objective_eval": (C.c_int, [C.c_void_p, C.c_int, _f64p, _P(Objective)]),
     "objective_check_host": (C.c_int, [C.c_void_p, _P(ModelState), C.c_int, _f64p, _P(Objective)]),
+    "solve_level0_all": (C.c_int, [C.c_void_p, _P(ModelState), C.c_int, C.c_int, _P(Level0Report), _f64p]),
     "solve_level1_dense_host": (C.c_int, [C.c_void_p, _P(ModelState), C.c_int, C.c_int, C.c_double, _f32p,
                                           C.c_size_t, _f64p, C.c_size_t, C.c_int, _P(C.c_int)]),
     "objective_total_ms": (C.c_double, []),
@@ -229,6 +235,11 @@ class Session:
             for q in range(desc.c0):
                 self.state.lo0[q] = -1.0
                 self.state.hi0[q] = 1.0
+        else:
+            self.state.has_palette0 = 1
+            for q in range(desc.c0):
+                for k, v in enumerate(level0_palette(desc.bits0[q], bool(desc.bc0))):
+                    self.state.palette0[q][k] = v
         self.chain = [np.ascontiguousarray(c, dtype=np.float32) for c in chain]
         ptrs = (_f32p * len(self.chain))(*[_ptr(c, C.c_float) for c in self.chain])
         self.ctx = arm.fn["device_create"](C.byref(desc), ptrs)
@@ -348,6 +359,13 @@ class Session:
             self.call(name, C.byref(self.state), k, ridge, sweeps, rep, C.byref(ms))
         return list(rep), ms.value
 
+    def solve_level0_all(self, k, sweeps):
+        """--l0 palette's block (c): the per-texel palette search on every plane."""
+        rep = (Level0Report * self.nplanes)()
+        ms = C.c_double(0)
+        self.call("solve_level0_all", C.byref(self.state), k, sweeps, rep, C.byref(ms))
+        return list(rep), ms.value
+
     def decode_plane(self, p):
         pl = self.plane(p)
         out = np.zeros(pl.w0 * pl.h0 * self.nout, dtype=np.uint8)
@@ -431,17 +449,38 @@ def plane_sizes(width: int, height: int, mips: bool = True, mip_min: int = 8) ->
     return planes
 
 
-def make_desc(textures: int, c0: int, c1: int, planes, cw=None, bits1: int = 8) -> ModelDesc:
+def replicate_byte(k: int, bits: int) -> int:
+    """The index k of `bits` bits in a byte's top bits, replicated downward (the uncompressed texel rule)."""
+    byte, sh = 0, 8 - bits
+    while sh > -bits:
+        byte |= (k << sh) if sh >= 0 else (k >> -sh)
+        sh -= bits
+    return byte & 255
+
+
+def level0_palette(bits: int, bc: bool) -> list:
+    """--l0 palette's values of one channel, what the shipped format decodes index k to: -1 + 2 k / (2^bits - 1)
+    block-compressed, -1 + 2 rep(k) / 255 uncompressed, rounded to float (src/model.h:205-222)."""
+    levels = (1 << bits) - 1
+    out = []
+    for k in range(levels + 1):
+        frac = k / levels if bc else replicate_byte(k, bits) / 255.0
+        out.append(float(np.float32(-1.0 + 2.0 * frac)))
+    return out
+
+
+def make_desc(textures: int, c0: int, c1: int, planes, cw=None, bits1: int = 8, bits0=None, bc0: int = 1) -> ModelDesc:
+    """The model descriptor; bits0 (one entry per level-0 channel, 1 to 4) selects --l0 palette."""
     d = ModelDesc()
     d.textures = textures
     d.nout = 3 * textures
     d.c0 = c0
     d.c1 = c1
     for q in range(MAX_CH):
-        d.bits0[q] = 8
+        d.bits0[q] = 8 if bits0 is None or q >= len(bits0) else bits0[q]
     d.bits1 = bits1
-    d.bc0 = 1
-    d.l0_bc8 = 1
+    d.bc0 = bc0
+    d.l0_bc8 = 1 if bits0 is None else 0
     d.init0_residual = 1
     d.nplanes = len(planes)
     for i, (w0, h0, w1, h1) in enumerate(planes):

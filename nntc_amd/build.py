@@ -23,7 +23,7 @@ HOST = os.path.join(HERE, "host")
 OBJ = os.path.join(ROOT, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
-HIP_SOURCES = ["arm", "objective", "decoder", "latent", "init", "refine", "adjoint", "twins"]
+HIP_SOURCES = ["arm", "objective", "decoder", "latent", "init", "refine", "adjoint", "twins", "palette"]
 HOST_SOURCES = ["encode", "image", "writer", "library"]
 
 # -ffp-contract=off: no fused multiply-add anywhere, as the reference builds (CMakeLists.txt:183-187, 206-208).

@@ -315,6 +315,12 @@ void cov_accumulate(Ctx& c, const float* data, long long texels, double weight, 
 void cov_read(Ctx& c, std::vector<double>& raw);
 double proj_peak_chain(Ctx& c, const float* resid, const double* e, const double* mu);
 void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const double* mu, double inv, int channel);
+// The palette mode's seed divisor: the 99th percentile of |projection| over the chain, from a 4096-bin histogram on
+// [0, peak], taken at its bin's upper edge (the peak itself when every texel falls in the first bin).
+double proj_percentile_chain(Ctx& c, const float* resid, const double* e, const double* mu, double peak);
+// palette.hip: --l0 palette
+void init_level0_palette(Ctx& c, const float* luma, int luma_channel0);
+void solve_level0_all(Ctx& c, int k, int sweeps, nntc_level0_report* rep);
 unsigned long long level1_off_grid(Ctx& c, const float* lo, const float* hi);
 // twins.hip: the --check twins (host, fp64, from the device's current planes)
 void objective_check_host(Ctx& c, const nntc_model_state* st, int k, double* acc /* nplanes * 3 */);

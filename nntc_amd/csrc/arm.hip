@@ -791,7 +791,9 @@ void create_model(Ctx& c, const nntc_model_desc* md)
             md->c1 > MAX_CH || md->nout < 3 || md->nout > MAX_NOUT)
             throw Refused { "model descriptor out of range" };
         if (!md->l0_bc8)
-            throw Refused { "the gfx950 arm encodes --l0 bc8; the palette mode is not built" };
+            for (int q = 0; q < md->c0; q++)
+                if (md->bits0[q] < 1 || md->bits0[q] > 4)
+                    throw Refused { "--l0 palette takes 1 to 4 bits per level-0 channel" };
         c.md = *md;
         c.nout = md->nout;
         c.c0 = md->c0;
@@ -1143,7 +1145,14 @@ int nntc_hip_init_level0(nntc_hip_ctx* h, nntc_model_state* st, const float luma
         c.site_target_k = -1;
         c.gather_tab_k = -1;
         upload_grids(c, st);
-        init_level0_kernel(c, luma, luma_channel0);
+        if (c.md.l0_bc8)
+            init_level0_kernel(c, luma, luma_channel0);
+        else
+        {
+            if (!st->has_palette0)
+                throw Refused { "init_level0: the palette mode needs the state's palette" };
+            init_level0_palette(c, luma, luma_channel0);
+        }
         HIP_OK(hipStreamSynchronize(c.stream));
     });
 }
@@ -1264,14 +1273,37 @@ int nntc_hip_init0_residual_channel(nntc_hip_ctx* h, nntc_model_state* st, int c
         // (4a) the divisor: the largest |projection| over the whole chain.
         const double peak = proj_peak_chain(c, c.resid.p, e.data(), mu.data());
         r.peak = peak;
-        r.divisor = peak;
-        // (4b) the channel on every plane, continuous under --l0 bc8.
-        const double inv = peak > 0.0 ? 1.0 / peak : 0.0;
+        // Under --l0 palette the divisor is the 99th percentile of |projection| instead (src/init.cu:1327-1377): the
+        // palette is fixed on [-1, 1], so the seed must use its range.
+        double divisor = peak;
+        if (!c.md.l0_bc8 && peak > 0.0)
+            divisor = proj_percentile_chain(c, c.resid.p, e.data(), mu.data(), peak);
+        r.divisor = divisor;
+        // (4b) the channel on every plane: continuous under --l0 bc8, snapped onto the palette otherwise.
+        const double inv = divisor > 0.0 ? 1.0 / divisor : 0.0;
         for (int p = 0; p < c.planes.n; p++)
             seed_channel(c, p, c.resid.p + c.g(p).off0 * nout, e.data(), mu.data(), inv, channel);
         HIP_OK(hipStreamSynchronize(c.stream));
         if (rep)
             *rep = r;
+    });
+}
+
+int nntc_hip_solve_level0_all(nntc_hip_ctx* h, const nntc_model_state* st, int k, int sweeps,
+                              nntc_level0_report* rep, double* ms)
+{
+    return guarded([&] {
+        Ctx& c = h->c;
+        if (c.md.l0_bc8)
+            throw Refused { "solve_level0_all is the palette mode's block (c); --l0 bc8 runs solve_level0_cont_all" };
+        if (!st->has_decoder || !st->has_palette0)
+            throw Refused { "solve_level0_all: the state needs its decoder and its palette" };
+        const double t0 = now_ms();
+        upload_decoder(c, st);
+        upload_grids(c, st);
+        solve_level0_all(c, k, sweeps, rep);
+        if (ms)
+            *ms = now_ms() - t0;
     });
 }
 

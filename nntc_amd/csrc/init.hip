@@ -281,14 +281,48 @@ __global__ __launch_bounds__(256) void k_proj_peak(const float* resid, long long
         out[blockIdx.x] = red[0];
 }
 
+// The seeded channel: the scaled projection clamped to [-1, 1], continuous under --l0 bc8; in the palette mode
+// snapped to index round((value + 1) / 2 levels) of the channel's palette (src/init.cu:1106-1130).
 __global__ __launch_bounds__(256) void k_seed_channel(const float* resid, long long texels, int nout, ProjDir dir,
-                                                      double inv, int c0, int channel, float* v0)
+                                                      double inv, int c0, int channel, float* v0, uint8_t* k0,
+                                                      const float* palette, int levels)
 {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= texels)
         return;
     const double p = proj_value(resid, t, nout, dir) * inv;
-    v0[t * c0 + channel] = (float)(p < -1.0 ? -1.0 : (p > 1.0 ? 1.0 : p));
+    const float value = (float)(p < -1.0 ? -1.0 : (p > 1.0 ? 1.0 : p));
+    if (palette)
+    {
+        int k = (int)lroundf((value + 1.0f) * 0.5f * (float)levels);
+        k = k < 0 ? 0 : (k > levels ? levels : k);
+        v0[t * c0 + channel] = palette[channel * 16 + k];
+        k0[t * c0 + channel] = (uint8_t)k;
+    }
+    else
+        v0[t * c0 + channel] = value;
+}
+
+// The palette mode's divisor: |projection| / peak binned over PROJ_HIST_BINS uniform bins of [0, 1] on every plane,
+// 64-bit counts (order-free), src/init.cu:1088-1101.
+__global__ __launch_bounds__(256) void k_proj_hist(const float* resid, long long texels, int nout, ProjDir dir,
+                                                   double inv_peak, int bins, unsigned long long* hist)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < texels; t += stride)
+    {
+        const double v = fabs(proj_value(resid, t, nout, dir)) * inv_peak;
+        int b = (int)(v * (double)bins);
+        b = b < 0 ? 0 : (b >= bins ? bins - 1 : b);
+        atomicAdd(hist + b, 1ull);
+    }
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n)
+        p[i] = 0ull;
 }
 
 // The fast form's covariance partials (nout <= 12): one pass over the rows, each lane accumulating the sums and the
@@ -682,9 +716,50 @@ void seed_channel(Ctx& c, int plane, const float* resid, const double* e, const 
         dir.mu[r] = r < c.nout ? mu[r] : 0.0;
     }
     const PlaneGeom& g = c.g(plane);
+    const bool palette = !c.md.l0_bc8;
     hipLaunchKernelGGL(k_seed_channel, dim3(blocks_for(g.n0, 256)), dim3(256), 0, c.stream, resid, g.n0, c.nout, dir,
-                       inv, c.c0, channel, c.v0.p + g.off0 * c.c0);
+                       inv, c.c0, channel, c.v0.p + g.off0 * c.c0, c.k0.p + g.off0 * c.c0,
+                       palette ? (const float*)(c.grid.p + 16) : nullptr, (1 << c.md.bits0[channel]) - 1);
     HIP_OK(hipGetLastError());
+}
+
+double proj_percentile_chain(Ctx& c, const float* resid, const double* e, const double* mu, double peak)
+{
+    constexpr int BINS = 4096;                 // PROJ_HIST_BINS (src/model.h:312)
+    constexpr double PERCENTILE = 0.99;        // INIT0_PERCENTILE (src/model.h:311)
+    ProjDir dir;
+    for (int r = 0; r < MAX_NOUT; r++)
+    {
+        dir.e[r] = r < c.nout ? e[r] : 0.0;
+        dir.mu[r] = r < c.nout ? mu[r] : 0.0;
+    }
+    if (c.cov_partial.n < (size_t)BINS)
+        throw Refused { "init0: the histogram does not fit the covariance partials" };
+    unsigned long long* hist = reinterpret_cast<unsigned long long*>(c.cov_partial.p);
+    hipLaunchKernelGGL(k_zero_u64, dim3(BINS / 256), dim3(256), 0, c.stream, hist, BINS);
+    unsigned long long total = 0;
+    for (int p = 0; p < c.planes.n; p++)
+    {
+        hipLaunchKernelGGL(k_proj_hist, dim3(SITE_BLOCKS), dim3(256), 0, c.stream, resid + c.g(p).off0 * c.nout,
+                           c.g(p).n0, c.nout, dir, 1.0 / peak, BINS, hist);
+        total += (unsigned long long)c.g(p).n0;
+    }
+    HIP_OK(hipGetLastError());
+    const unsigned long long* h = read_back(c, (const unsigned long long*)hist, (size_t)BINS);
+    const unsigned long long want = (unsigned long long)(PERCENTILE * (double)total);
+    unsigned long long cum = 0;
+    int bin = BINS - 1;
+    for (int b = 0; b < BINS; b++)
+    {
+        cum += h[b];
+        if (cum >= want)
+        {
+            bin = b;
+            break;
+        }
+    }
+    const double divisor = peak * (double)(bin + 1) / (double)BINS;
+    return divisor > 0.0 ? divisor : peak;
 }
 
 }   // namespace nntc

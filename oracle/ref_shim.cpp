@@ -72,6 +72,11 @@ void state_to_model(const nntc_model_state* s, Model& m)
         m.lo1.clear();
         m.hi1.clear();
     }
+    // The palette mode's palette: one list per level-0 channel, 2^bits0[c] entries.
+    m.palette0.clear();
+    if (s->has_palette0)
+        for (int c = 0; c < m.c0; c++)
+            m.palette0.emplace_back(s->palette0[c], s->palette0[c] + (1 << m.bits0[(size_t)c]));
 }
 
 void model_to_state(const Model& m, nntc_model_state* s)
@@ -369,6 +374,28 @@ void nntc_ref_decoder_ridge_tally(long long out[4])
             *ms = t;                                                                                                   \
         return 0;                                                                                                      \
     }
+int nntc_ref_solve_level0_all(nntc_hip_ctx* ctx, const nntc_model_state* st, int k, int sweeps,
+                              nntc_level0_report* rep, double* ms)
+{
+    REF_CTX;
+    state_to_model(st, m);
+    std::vector<Level0Report> lr;
+    const double t = be::solve_level0_all(r->d, m, k, sweeps, lr);
+    for (size_t i = 0; i < lr.size(); i++)
+    {
+        rep[i] = nntc_level0_report {};
+        for (int q = 0; q < 4; q++)
+            rep[i].moved[q] = lr[i].moved[q];
+        rep[i].moved_total = lr[i].moved_total;
+        rep[i].ms = lr[i].ms;
+        rep[i].joint = lr[i].joint ? 1 : 0;
+        rep[i].states = lr[i].states;
+    }
+    if (ms)
+        *ms = t;
+    return 0;
+}
+
 REF_BLOCK_CALL(solve_level1_all)
 REF_BLOCK_CALL(solve_level0_cont_all)
 REF_SWEEP_CALL(sweep_level1_all)

@@ -181,3 +181,83 @@ def steps(s: abi.Session, o: Options, per_site):
     yield "solve_decoder", {}
     shipped = s.objective(o.k, per_site)
     yield "shipped", {"e": shipped.e, "centre_mse": shipped.centre_mse}
+
+
+def steps_palette(s: abi.Session, o: Options, per_site, sweeps: int = 2):
+    """The --l0 palette schedule (src/main.cpp:1930-2440 under !l0_bc8): block (c) is the per-texel palette search,
+    each seeded channel is followed by one search and a refit, level 0 has no grid to freeze, and the run ends with a
+    last search and a last block (a)."""
+    nplanes = s.nplanes
+    s.init_level0(luma_channel0=o.luma)
+    yield "init_level0", {}
+    irep = s.init_level1(pca=o.pca)
+    yield "init_level1", {"eigenvalue": list(irep.eigenvalue), "peak": list(irep.peak)}
+    s.objective(o.k, per_site)
+    s.solve_decoder(o.k, per_site)
+    yield "solve_decoder", {}
+    after = s.objective(o.k, per_site)
+    yield "objective", {"e": after.e}
+    for c in range(0 if o.luma else s.desc.c0):
+        rep = s.init0_residual_channel(c, per_site)
+        yield "init0_residual_channel", {"eigenvalue": rep.eigenvalue, "peak": rep.peak, "divisor": rep.divisor,
+                                         "texture": rep.texture}
+        seeded = s.objective(o.k, per_site)
+        yield "objective", {"e": seeded.e}
+        s.solve_decoder(o.k, per_site)
+        yield "solve_decoder", {}
+        s.objective(o.k, per_site)
+        l0, _ = s.solve_level0_all(o.k, sweeps)
+        yield "solve_level0_all", {"moved": [list(x.moved) for x in l0]}
+        s.solve_decoder(o.k, per_site)
+        yield "solve_decoder", {}
+        after = s.objective(o.k, per_site)
+        yield "objective", {"e": after.e}
+
+    frozen = False
+    e_prev = after.e
+    tol_hits = 0
+    for r in range(1, o.rounds + 1):
+        s.solve_decoder(o.k, per_site)
+        yield "solve_decoder", {"round": r}
+        s.objective(o.k, per_site)
+        if o.q1_start > 0 and r >= o.q1_start and not frozen:
+            s.grid("freeze_level1_grid", o.q1_range)
+            yield "freeze_level1_grid", {}
+            frozen = True
+        if frozen:
+            lr, _ = s.block("sweep_level1_all", o.k, o.ridge, o.q1_sweeps)
+            yield "sweep_level1_all", {"moved": [x.moved for x in lr]}
+        else:
+            lr, _ = s.block("solve_level1_all", o.k, o.ridge)
+            yield "solve_level1_all", {"iterations": [x.iterations for x in lr]}
+        s.objective(o.k, per_site)
+        l0, _ = s.solve_level0_all(o.k, sweeps)
+        yield "solve_level0_all", {"moved": [list(x.moved) for x in l0], "joint": [x.joint for x in l0]}
+        ec = s.objective(o.k, per_site)
+        yield "objective", {"e": ec.e, "round_end": r}
+        moved1 = sum(x.moved if x.quantised else x.moved_values for x in lr)
+        moved0 = sum(x.moved_total for x in l0)
+        delta = np.sqrt(sum(x.delta2 for x in lr))
+        plane = np.sqrt(sum(x.plane2 for x in lr))
+        drop = (e_prev - ec.e) / ec.e if ec.e > 0.0 else 0.0
+        e_prev = ec.e
+        still = moved1 == 0 if frozen else delta <= 1e-7 * (plane if plane > 0.0 else 1.0)
+        if moved0 == 0 and still:
+            break
+        tol_hits = tol_hits + 1 if drop < o.tol else 0
+        if tol_hits >= 2:
+            break
+
+    if frozen:
+        s.block("sweep_level1_all", o.k, o.ridge, o.q1_sweeps)
+        yield "sweep_level1_all", {}
+    else:
+        s.grid("quantise_level1", o.q1_range)
+        yield "quantise_level1", {}
+    l0, _ = s.solve_level0_all(o.k, sweeps)
+    yield "solve_level0_all", {"moved": [list(x.moved) for x in l0]}
+    s.solve_decoder(o.k, per_site)
+    yield "solve_decoder", {}
+    shipped = s.objective(o.k, per_site)
+    yield "objective", {"e": shipped.e, "shipped": True}
+    del nplanes

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from nntc_amd import abi, chain, synth
-from schedule import Options, steps
+from schedule import Options, steps, steps_palette
 
 pytestmark = pytest.mark.gpu
 
@@ -30,11 +30,11 @@ def material(name):
     return tex, c0, c1
 
 
-def sessions(name, hip_arm, ref_arm, init0_texture=False, init0_luma=False):
+def sessions(name, hip_arm, ref_arm, init0_texture=False, init0_luma=False, bits0=None, bc0=1):
     tex, c0, c1 = material(name)
     planes, ch = chain.source_chain(tex)
     _, per_site = chain.mip_weights(planes)
-    desc = abi.make_desc(len(tex), c0, c1, planes, chain.output_weights(len(tex)))
+    desc = abi.make_desc(len(tex), c0, c1, planes, chain.output_weights(len(tex)), bits0=bits0, bc0=bc0)
     desc.init0_texture = 1 if init0_texture else 0
     desc.init0_residual = 0 if init0_luma else 1
     flat = [c.reshape(-1) for c in ch]
@@ -86,8 +86,25 @@ def test_lockstep_exact_luma_init(hip_arm, ref_arm):
     run_lockstep(h, r, per_site, Options(luma=True))
 
 
-def run_lockstep(h, r, per_site, o):
-    gh, gr = steps(h, o, per_site), steps(r, o, per_site)
+@pytest.mark.parametrize("name,bits0,bc0", [("small_2x100x76_c3c3", [3, 2, 3], 1), ("small_2x100x76_c3c3", [3, 3, 3], 1),
+                                             ("small_1x64x40_c2c4", [4, 4], 0), ("small_1x64x40_c1c2", [1], 1),
+                                             ("small_6x100x76_c4c4", [4, 4, 4, 4], 0)],
+                         ids=["8bits-joint-bc", "9bits-sweeps-bc", "8bits-joint-uncompressed", "1bit-joint-bc",
+                              "16bits-sweeps-uncompressed"])
+def test_lockstep_exact_palette(name, bits0, bc0, hip_arm, ref_arm):
+    """--l0 palette (src/solve_level0.cu, src/init.cu:389-428, 1327-1377): the per-texel palette search (the joint
+    enumeration at 8 bits or fewer, coordinate sweeps above), the palette init and the seed's percentile divisor and
+    snap, call by call against the reference, on both palettes (block-compressed k / levels, uncompressed replicated
+    bytes)."""
+    print(f"lockstep {name}, --l0 palette bits {bits0} bc0 {bc0}", flush=True)
+    h, r, per_site = sessions(name, hip_arm, ref_arm, bits0=bits0, bc0=bc0)
+    run_lockstep(h, r, per_site, Options(rounds=5), palette=True)
+
+
+def run_lockstep(h, r, per_site, o, palette=False):
+    step = steps_palette if palette else steps
+    gh, gr = step(h, o, per_site), step(r, o, per_site)
+    least = 20 if palette else 40
     i = 0
     for (nh, ih), (nr, ir) in zip(gh, gr):
         assert nh == nr
@@ -100,9 +117,10 @@ def run_lockstep(h, r, per_site, o):
             assert ih[key] == ir[key], f"{key} at step {i} ({nh}): {ih[key]!r} vs {ir[key]!r}"
         compare(h, r, nh, i)
         i += 1
-    assert i > 40
+    assert i > least
     for p in range(h.nplanes):
-        for f in range(1 if h.desc.c0 <= 2 else 2):
+        # The palette mode packs level 0 on the host when it writes the asset: no blocks live on either arm.
+        for f in range(0 if palette else 1 if h.desc.c0 <= 2 else 2):
             np.testing.assert_array_equal(h.bc_blocks(p, f), r.download_blocks(p, f, h.bc_blocks(p, f).size * 2),
                                           err_msg=f"blocks plane {p} file {f}")
         np.testing.assert_array_equal(h.decode_plane(p), r.decode_plane(p), err_msg=f"decode plane {p}")
