@@ -59,6 +59,10 @@ typedef struct nntc_encode_options
     int32_t init1_pca;           /* --init pca (1) or box (0, the default) */
     int32_t init0_texture;       /* --init0 texture (1) or residual (0, the default) */
     int32_t init0_luma;          /* --init0 luma (1): level 0's channel 0 from the luminance, no residual seed */
+    int32_t l0_palette;          /* --l0 palette (1) or bc8 (0, the default) */
+    int32_t bits0[4];            /* --bits0 under --l0 palette: bits per level-0 channel, 1..4 (default 3) */
+    int32_t bc0;                 /* --bc0 under --l0 palette: 1 BC4 / BC5 (default), 0 an uncompressed level 0 */
+    int32_t sweeps;              /* --sweeps 2: the palette search's coordinate sweeps above 8 bits a texel */
 } nntc_encode_options;
 
 void nntc_encode_default_options(nntc_encode_options* o);
@@ -100,8 +104,10 @@ typedef struct nntc_encode_report
     double psnr[6];              /* per texture, 8-bit, base plane, original extent (src/image.h:387-407) */
     double centre_psnr, sampled_psnr;
     double e_shipped;
-    double e_round[64][3];       /* per round: E after blocks (a), (b), (c') */
-    double pack_psnr, pack_psnr_seed;
+    double e_round[64][3];       /* per round: E after blocks (a), (b), (c') (under --l0 palette: (c), the search) */
+    double pack_psnr, pack_psnr_seed;  /* the refined and the seed BC pack's psnr; under --l0 palette both are the
+                                          host pack's psnr against the exact index values (100 when lossless) */
+    long long moved0_final;      /* --l0 palette: the last search's moved texels over every plane */
     double seconds;              /* the run's wall clock */
     double ms_a, ms_b, ms_c, ms_e, ms_pack, ms_refine;
     long long ridge_tally[4];
@@ -110,7 +116,8 @@ typedef struct nntc_encode_report
 
 int nntc_encoder_report(const nntc_encoder* e, nntc_encode_report* r);
 int nntc_encoder_state(const nntc_encoder* e, nntc_model_state* s);
-/* Plane `plane`'s level-1 indices (w1*h1*c1 bytes) and level-0 .dds blocks of file `file`, as the asset stores them. */
+/* Plane `plane`'s level-1 indices (w1*h1*c1 bytes) and level-0 .dds blocks of file `file`, as the asset stores them
+ * (--l0 bc8; under --l0 palette the writer packs the indices on the host and there are no device blocks). */
 int nntc_encoder_level1_indices(const nntc_encoder* e, int plane, uint8_t* out, size_t n, size_t* count);
 int nntc_encoder_level0_blocks(const nntc_encoder* e, int plane, int file, uint8_t* out, size_t n, size_t* count);
 /* The base plane decoded to bytes (w0*h0*3T). */

@@ -255,6 +255,13 @@ int nntc_hip_solve_decoder(nntc_hip_ctx* ctx, nntc_model_state* state, int k, co
  * ended on (the reference counts per process; per thread here, so encodes driven from other threads do not mix in). */
 void nntc_hip_decoder_ridge_tally(long long out[4]);
 
+/* No counterpart in the reference (its one arithmetic has no such case): how many of this host thread's fast-form
+ * block (a) calls the split-bf16 Gram could not settle - a feature whose Schur complement is below 1e-4 of its
+ * diagonal entry, or every rung refused - and were redone from the fp64 Gram of the plain form. The state that makes
+ * them is a level-0 channel holding one nonzero value everywhere (a palette seed snapped to one entry), whose features
+ * are collinear with the bias and level 1's. */
+long long nntc_hip_decoder_fast_fallbacks(void);
+
 /* Replace be::solve_level1_all, sweep_level1_all, solve_level0_cont_all, sweep_level0_cont_all
  * (src/backend.h:107-111): blocks (b) and (c'). rep has nplanes entries. */
 int nntc_hip_solve_level1_all(nntc_hip_ctx* ctx, const nntc_model_state* state, int k, double ridge,

@@ -210,6 +210,7 @@ class Arm:
         if not os.path.exists(lib_path):
             raise FileNotFoundError(f"{lib_path} is not built; run __graft_entry__.build()")
         self.lib = C.CDLL(lib_path, mode=C.RTLD_GLOBAL)
+        self.path = lib_path
         self.prefix = prefix
         self.fn = {}
         for name, (res, args) in SIGNATURES.items():

@@ -217,6 +217,8 @@ thread_local std::string g_error;
 thread_local double g_objective_ms = 0.0;
 thread_local long long g_objective_calls = 0;
 thread_local long long g_ridge_tally[4] = { 0, 0, 0, 0 };
+// The fast form's decoder solves that every rung refused on the split-bf16 Gram and were redone from the fp64 one.
+thread_local long long g_fast_fallbacks = 0;
 
 const double RIDGE_LADDER[] = { 1e-9, 1e-12, 0.0 };
 const double LS_ACCEPT_RISE = 1e-12;
@@ -1348,12 +1350,7 @@ int nntc_hip_solve_decoder(nntc_hip_ctx* h, nntc_model_state* st, int k, const d
         std::vector<double> acc;
         const bool on_device = c.form == NNTC_FORM_FAST;
         ls_accumulate(c, k, per_site, acc, !on_device);
-        const double ms = on_device ? 0.0 : c.ms_since(c.ev0);
-
-        double trace = 0.0;
-        for (int i = 0; i < mm && !on_device; i++)
-            trace += acc[(size_t)tri_index(i, i, mm)];
-        const double mean_diag = trace > 0.0 ? trace / mm : 1.0;
+        double ms = on_device ? 0.0 : c.ms_since(c.ev0);
         const bool had_previous = st->has_decoder != 0;
         std::vector<double> x_prev((size_t)nout * mm, 0.0);
         if (had_previous)
@@ -1378,9 +1375,9 @@ int nntc_hip_solve_decoder(nntc_hip_ctx* h, nntc_model_state* st, int k, const d
             bool wrote = false;
             std::vector<float> x;
             decoder_solve_device(c, st, inv_norm, norm_ok, RIDGE_LADDER, LS_ACCEPT_RISE, rung, wrote, x);
-            g_ridge_tally[rung < 3 ? rung : RIDGE_REFUSED]++;
-            if (wrote)
+            if (rung < 3)
             {
+                g_ridge_tally[rung]++;
                 st->has_decoder = 1;
                 for (int cc = 0; cc < nout; cc++)
                 {
@@ -1388,12 +1385,35 @@ int nntc_hip_solve_decoder(nntc_hip_ctx* h, nntc_model_state* st, int k, const d
                         st->w[(size_t)cc * nin + i] = x[(size_t)cc * mm + i];
                     st->b[cc] = x[(size_t)cc * mm + nin];
                 }
+                upload_decoder(c, st);
+                if (ms_out)
+                    *ms_out = c.ms_since(c.ev0);
+                return;
             }
-            upload_decoder(c, st);
-            if (ms_out)
-                *ms_out = c.ms_since(c.ev0);
-            return;
+            // The split-bf16 Gram cannot settle this system (status 4: a feature nearly a combination of the ones
+            // before it, where the 1e-9 ridge needs the Gram exact to far below its rounding - a level-0 channel
+            // that holds one nonzero palette value everywhere makes s_i a multiple of the bias and s_i c_j of c_j),
+            // or every rung was refused (3). The same call is redone from the fp64 Gram of the plain form and solved
+            // on the host as that form solves it: the answer of the exact arithmetic, at a cost paid only in such a
+            // state.
+            g_fast_fallbacks++;
+            c.form = NNTC_FORM_PLAIN;
+            try
+            {
+                ls_accumulate(c, k, per_site, acc, true);
+            }
+            catch (...)
+            {
+                c.form = NNTC_FORM_FAST;
+                throw;
+            }
+            c.form = NNTC_FORM_FAST;
+            ms = c.ms_since(c.ev0);
         }
+        double trace = 0.0;
+        for (int i = 0; i < mm; i++)
+            trace += acc[(size_t)tri_index(i, i, mm)];
+        const double mean_diag = trace > 0.0 ? trace / mm : 1.0;
         auto qform = [&](const std::vector<double>& x, double& err) {
             double q = 0.0, e = 0.0;
             for (int cc = 0; cc < nout; cc++)
@@ -1491,6 +1511,11 @@ void nntc_hip_decoder_ridge_tally(long long out[4])
 {
     for (int i = 0; i < 4; i++)
         out[i] = g_ridge_tally[i];
+}
+
+long long nntc_hip_decoder_fast_fallbacks(void)
+{
+    return g_fast_fallbacks;
 }
 
 int nntc_hip_solve_level1_all(nntc_hip_ctx* h, const nntc_model_state* st, int k, double ridge,

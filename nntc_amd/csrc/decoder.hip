@@ -566,8 +566,17 @@ void gram_accumulate(Ctx& c, int k, const double* omega, std::vector<double>& ac
 // solution once against the same fp64 system (r = rhs - (A + ridge I) x, x += (L L^T)^-1 r), which removes the
 // factorisation's own rounding. A rung whose factorisation meets a pivot that is not positive and finite is skipped,
 // as the reference skips a singular elimination.
+//
+// The split-bf16 Gram is accurate to about 1e-6 of its entries, far coarser than the first rung's ridge of 1e-9 of
+// the mean diagonal. Where a feature is (nearly) a combination of the features before it - a level-0 channel that
+// holds one nonzero value everywhere is a multiple of the bias - the part of the system the ridge is there to settle
+// is below that accuracy, and the Cholesky factors of the rounded Gram settle it arbitrarily. The first rung
+// therefore measures each feature's Schur complement (its pivot less the ridge) against its own diagonal entry, and
+// a feature below DS_RESOLVE of it hands the call back to the host (status 4), which redoes it from the fp64 Gram.
+// Exactly zero features (a level-0 plane at 0, whose products are exact zeros in any precision) are not referred.
 
 constexpr int DS_MM = NNTC_MAX_NIN + 1, DS_NOUT = NNTC_MAX_NOUT;
+constexpr double DS_RESOLVE = 1e-4;
 
 struct DecoderSolveArgs
 {
@@ -578,7 +587,8 @@ struct DecoderSolveArgs
     double inv_norm, accept_rise;
     double ladder[3];
     float* x_out;              // nout x mm: the decoder the call leaves (accepted, or the constant fallback)
-    int* status;               // rung taken (0..2), 3 when every rung was refused; [1]: 1 when x_out holds a decoder
+    int* status;               // rung taken (0..2), 3 when every rung was refused, 4 when the Gram cannot resolve the
+                               // system (see DS_RESOLVE); [1]: 1 when x_out holds a decoder
 };
 
 // sum_c cw_c Q_c(x) and its rounding bound, Q_c(x) = x_c^T A x_c - 2 x_c . rhs_c (x_c at x + c DS_MM); lanes < nout
@@ -621,7 +631,7 @@ __global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
 {
     __shared__ double sA[DS_MM * DS_MM], sL[DS_MM * DS_MM], sR[DS_NOUT * DS_MM], sX[DS_NOUT * DS_MM];
     __shared__ double sP[DS_NOUT * DS_MM], s_q[DS_NOUT], s_e[DS_NOUT];
-    __shared__ int s_ok;
+    __shared__ int s_ok, s_weak;
     const int mm = a.mm, nout = a.nout, tid = threadIdx.x;
     for (int e = tid; e < mm * mm; e += 256)
     {
@@ -653,7 +663,10 @@ __global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
             sL[i * DS_MM + j] = sA[i * DS_MM + j] + (i == j ? ridge : 0.0);
         }
         if (tid == 0)
+        {
             s_ok = 1;
+            s_weak = 0;
+        }
         __syncthreads();
         // ---- L L^T = A + ridge I in the lower triangle of sL (right-looking: column j, then the trailing update)
         for (int j = 0; j < mm; j++)
@@ -663,6 +676,9 @@ __global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
                 const double d = sL[j * DS_MM + j];
                 if (!(d > 0.0) || !isfinite(d))
                     s_ok = 0;
+                const double diag = sA[j * DS_MM + j];
+                if (step == 0 && diag > 0.0 && !(d - ridge >= DS_RESOLVE * diag))
+                    s_weak = 1;
                 sL[j * DS_MM + j] = d > 0.0 ? sqrt(d) : 1.0;
             }
             __syncthreads();
@@ -678,6 +694,11 @@ __global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
                     sL[i * DS_MM + k] -= sL[i * DS_MM + j] * sL[k * DS_MM + j];
             }
             __syncthreads();
+        }
+        if (s_weak)
+        {
+            taken = 4;
+            break;
         }
         if (!s_ok)
             continue;
@@ -741,7 +762,7 @@ __global__ __launch_bounds__(256) void k_decoder_solve(DecoderSolveArgs a)
             a.x_out[e] = (float)sX[(e / mm) * DS_MM + e % mm];
         wrote = 1;
     }
-    else if (!a.had_previous)
+    else if (taken == 3 && !a.had_previous)
     {
         // the constant fallback: every weight zero, each bias the mean target (src/solve_decoder.cu:440-452)
         const double count = sA[(mm - 1) * DS_MM + (mm - 1)];

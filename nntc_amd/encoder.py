@@ -30,6 +30,7 @@ class Options(C.Structure):
         ("sweep_lanes", C.c_int32),
         ("graphs", C.c_int32),
         ("init1_pca", C.c_int32), ("init0_texture", C.c_int32), ("init0_luma", C.c_int32),
+        ("l0_palette", C.c_int32), ("bits0", C.c_int32 * 4), ("bc0", C.c_int32), ("sweeps", C.c_int32),
     ]
 
 
@@ -39,7 +40,8 @@ class Report(C.Structure):
         ("outer_taken", C.c_int32), ("outer_tried", C.c_int32),
         ("psnr", C.c_double * 6), ("centre_psnr", C.c_double), ("sampled_psnr", C.c_double),
         ("e_shipped", C.c_double), ("e_round", (C.c_double * 3) * 64),
-        ("pack_psnr", C.c_double), ("pack_psnr_seed", C.c_double), ("seconds", C.c_double),
+        ("pack_psnr", C.c_double), ("pack_psnr_seed", C.c_double), ("moved0_final", C.c_longlong),
+        ("seconds", C.c_double),
         ("ms_a", C.c_double), ("ms_b", C.c_double), ("ms_c", C.c_double), ("ms_e", C.c_double),
         ("ms_pack", C.c_double), ("ms_refine", C.c_double), ("ridge_tally", C.c_longlong * 4),
         ("mpix", C.c_double),
@@ -94,6 +96,9 @@ def options(**kw) -> Options:
     o = Options()
     lib().nntc_encode_default_options(C.byref(o))
     for k, v in kw.items():
+        if k == "bits0":
+            v = list(v) + [v[-1]] * (4 - len(v)) if len(v) else [3] * 4
+            v = (C.c_int32 * 4)(*v)
         setattr(o, k, v)
     return o
 

@@ -124,6 +124,115 @@ using namespace nntc_host;
 namespace
 {
 
+// What every run ends with: level 1 checked on its grid, the shipped objective, the per-texture error and the report.
+struct Totals
+{
+    double ms_a = 0.0, ms_b = 0.0, ms_c = 0.0, ms_pack = 0.0, ms_refine = 0.0;
+    double pack_psnr = 0.0, pack_psnr_seed = 0.0, obj_ms0 = 0.0;
+    long long tally0[4] = {};
+    long long moved0_final = 0;
+    std::chrono::steady_clock::time_point t_start {};
+};
+
+void ship(Encoder& E, const Totals& tt)
+{
+    Seam& S = E.seam;
+    const nntc_encode_options& o = E.opts;
+    nntc_hip_ctx* d = E.ctx;
+    nntc_encode_report& rep = E.report;
+    const int k = o.k;
+    // ---- level 1 must be exactly the grid value of every index about to be written
+    E.verify_level1_on_grid();
+
+    nntc_objective shipped;
+    E.check(S.objective_eval(d, k, E.per_site.data(), &shipped), "objective_eval");
+    // The shipped decode's per-texture error, summed on the device; the bytes themselves are fetched only when asked.
+    E.check(S.texture_sse(d, E.source_w, E.source_h, E.texture_se), "texture_sse");
+    E.recon.clear();
+    rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - tt.t_start).count();
+    rep.e_shipped = shipped.e;
+    rep.centre_psnr = mse_to_db(shipped.centre_mse);
+    rep.sampled_psnr = k > 0 ? mse_to_db(shipped.sampled_mse) : 0.0;
+    rep.pack_psnr = tt.pack_psnr;
+    rep.pack_psnr_seed = tt.pack_psnr_seed;
+    rep.moved0_final = tt.moved0_final;
+    rep.ms_a = tt.ms_a;
+    rep.ms_b = tt.ms_b;
+    rep.ms_c = tt.ms_c;
+    rep.ms_e = S.objective_total_ms() - tt.obj_ms0;
+    rep.ms_pack = tt.ms_pack;
+    rep.ms_refine = tt.ms_refine;
+    long long tally[4];
+    S.decoder_ridge_tally(tally);
+    for (int i = 0; i < 4; i++)
+        rep.ridge_tally[i] = tally[i] - tt.tally0[i];
+    for (int tx = 0; tx < E.desc.textures; tx++)
+        rep.psnr[tx] = E.texture_psnr(tx);
+    E.print_call_times();
+    if (!o.quiet)
+    {
+        std::printf("\nreport\n  backend      %s\n", S.name.c_str());
+        for (int tx = 0; tx < E.desc.textures; tx++)
+            std::printf("  psnr         texture %d  %.2f dB\n", tx, rep.psnr[tx]);
+        std::printf("  centre psnr  %.2f dB\n  E shipped    %.9e\n  time         %.3f s total, (a) %.3f ms / (b) "
+                    "%.3f ms / (c) %.3f ms / E %.3f ms\n",
+                    rep.centre_psnr, rep.e_shipped, rep.seconds, tt.ms_a, tt.ms_b, tt.ms_c, rep.ms_e);
+    }
+}
+
+// --l0 palette's end (src/main.cpp:2713-2737): at 4 bits a BC4 / BC5 level 0 is a lossy pack of the indices, so the
+// plane is packed as the writer packs it and decoded back onto the device before anything is measured; at 1-3 bits
+// (and uncompressed) the device's plane is already what the file decodes to.
+void finish_palette(Encoder& E, Totals tt)
+{
+    const nntc_model_desc& dd = E.desc;
+    bool lossy = false;
+    for (int c = 0; c < dd.c0; c++)
+        lossy = lossy || dd.bits0[c] > 3;
+    tt.pack_psnr = tt.pack_psnr_seed = 100.0;
+    if (dd.bc0 && lossy)
+    {
+        double se = 0.0;
+        size_t n = 0;
+        const int stored = dd.c0 == 3 ? 4 : dd.c0;
+        for (int i = 0; i < dd.nplanes; i++)
+        {
+            const int w = dd.planes[i].w0, h = dd.planes[i].h0, bx = (w + 3) / 4;
+            const size_t texels = (size_t)w * h;
+            std::vector<uint8_t> k0(texels * dd.c0), bytes;
+            E.check(E.seam.level0_download(E.ctx, i, k0.data(), k0.size(), nullptr), "level0_download");
+            level0_index_bytes(k0.data(), texels, dd.c0, dd.bits0, bytes);
+            std::vector<float> v0(texels * dd.c0);
+            for (int f = 0; f < level0_files(dd.c0); f++)
+            {
+                const int nc = level0_file_channels(dd.c0, f), bpb = nc == 1 ? 8 : 16;
+                const std::vector<uint8_t> blocks =
+                    bc_pack_indices(bytes.data(), w, h, stored, 2 * f, nc, dd.bits0, dd.c0, se, n);
+                for (int c = 0; c < nc; c++)
+                    for (int by = 0; by < (h + 3) / 4; by++)
+                        for (int bxi = 0; bxi < bx; bxi++)
+                        {
+                            float vals[16];
+                            bc4_block_values(&blocks[((size_t)by * bx + bxi) * bpb + 8 * c], vals);
+                            for (int t = 0; t < 16; t++)
+                            {
+                                const int x = 4 * bxi + (t & 3), y = 4 * by + (t >> 2);
+                                if (x < w && y < h)
+                                    v0[((size_t)y * w + x) * dd.c0 + 2 * f + c] =
+                                        (float)(-1.0 + (double)vals[t] / 255.0 * 2.0);
+                            }
+                        }
+            }
+            E.check(E.seam.level0_upload(E.ctx, i, v0.data(), v0.size()), "level0_upload");
+        }
+        tt.pack_psnr = tt.pack_psnr_seed = se > 0.0 ? 10.0 * std::log10(255.0 * 255.0 / (se / (double)n)) : 100.0;
+        if (!E.opts.quiet)
+            std::printf("level 0 packed: the report below is of the PACKED plane (packing psnr %.2f dB against the "
+                        "exact index values), decoded back from the blocks the .dds holds\n", tt.pack_psnr);
+    }
+    ship(E, tt);
+}
+
 void run_encode(Encoder& E)
 {
     Seam& S = E.seam;
@@ -153,12 +262,36 @@ void run_encode(Encoder& E)
     st.has_decoder = 1;
     cks(S.upload_decoder_to_device(d, &st), "upload_decoder_to_device");
     st.has_decoder = 0;
-    st.has_grid0 = 1;
-    for (int q = 0; q < E.desc.c0; q++)
+    const bool palette = E.desc.l0_bc8 == 0;
+    if (palette)
     {
-        st.lo0[q] = -1.0f;
-        st.hi0[q] = 1.0f;
+        // --l0 palette: the values the shipped format decodes each index to (src/model.h:205-222)
+        st.has_palette0 = 1;
+        for (int q = 0; q < E.desc.c0; q++)
+            for (int kk = 0; kk < (1 << E.desc.bits0[q]); kk++)
+                st.palette0[q][kk] = level0_value(E.desc.bits0[q], kk, E.desc.bc0 != 0);
     }
+    else
+    {
+        st.has_grid0 = 1;
+        for (int q = 0; q < E.desc.c0; q++)
+        {
+            st.lo0[q] = -1.0f;
+            st.hi0[q] = 1.0f;
+        }
+    }
+    std::vector<nntc_level0_report> l0((size_t)np);
+    long long moved0_search = 0;
+    // --l0 palette's block (c): the per-texel palette search on every plane; returns the texels it moved.
+    auto search = [&]() {
+        double tc = 0.0;
+        cks(S.solve_level0_all(d, &st, k, o.sweeps, l0.data(), &tc), "solve_level0_all");
+        ms_c += tc;
+        long long moved = 0;
+        for (int p = 0; p < np; p++)
+            moved += l0[(size_t)p].moved_total;
+        return moved;
+    };
 
     // ---- the latents
     cks(S.init_level0(d, &st, LUMA, E.opts.init0_luma ? 1 : 0), "init_level0");
@@ -184,12 +317,24 @@ void run_encode(Encoder& E)
         cks(S.solve_decoder(d, &st, k, ps, &t), "solve_decoder");
         ms_a += t;
         obj(after);
+        const double e_refit = after.e;
+        if (palette)
+        {
+            // The snap of the seed is not a minimisation: one search and a refit follow each seeded channel.
+            search();
+            cks(S.solve_decoder(d, &st, k, ps, &t), "solve_decoder");
+            ms_a += t;
+            obj(after);
+        }
         if (!o.quiet)
         {
             std::printf("level 0 init: channel %d = the residual's leading direction, %.1f %% of its variance "
                         "(eigenvalue %.3e of a trace of %.3e, divisor %.4f of a peak of %.4f)\n",
                         c, 100.0 * cr.share, cr.eigenvalue, cr.total_variance, cr.divisor, cr.peak);
-            std::printf("              E %.9e -> %.9e across the refit\n", seeded.e, after.e);
+            std::printf("              E %.9e -> %.9e across the refit\n", seeded.e, e_refit);
+            if (palette)
+                std::printf("              E %.9e -> %.9e across the search and the refit after it\n", e_refit,
+                            after.e);
         }
     }
 
@@ -207,7 +352,8 @@ void run_encode(Encoder& E)
         if (o.q1_start > 0 && r >= o.q1_start && !frozen)
         {
             cks(S.freeze_level1_grid(d, &st, o.q1_range), "freeze_level1_grid");
-            cks(S.freeze_level0_grid(d, &st, o.q1_range), "freeze_level0_grid");
+            if (!palette)   // a palette level 0 has no grid to fit
+                cks(S.freeze_level0_grid(d, &st, o.q1_range), "freeze_level0_grid");
             frozen = true;
             frozen_round = r;
             if (!o.quiet)
@@ -215,9 +361,12 @@ void run_encode(Encoder& E)
                 std::printf("grid frozen: round %d, level 1", r);
                 for (int q = 0; q < E.desc.c1; q++)
                     std::printf("  ch%d [%.4f, %.4f]", q, (double)st.lo1[q], (double)st.hi1[q]);
-                std::printf("; level 0");
-                for (int q = 0; q < E.desc.c0; q++)
-                    std::printf("  ch%d [%.4f, %.4f]", q, (double)st.lo0[q], (double)st.hi0[q]);
+                if (!palette)
+                {
+                    std::printf("; level 0");
+                    for (int q = 0; q < E.desc.c0; q++)
+                        std::printf("  ch%d [%.4f, %.4f]", q, (double)st.lo0[q], (double)st.hi0[q]);
+                }
                 std::printf("\n");
             }
         }
@@ -240,15 +389,24 @@ void run_encode(Encoder& E)
         plane_norm = std::sqrt(plane_norm);
         obj(eb);
         double round_c = 0.0;
-        if (frozen)
-            cks(S.sweep_level0_cont_all(d, &st, k, o.ridge, o.q1_sweeps, lr0.data(), &round_c),
-                "sweep_level0_cont_all");
-        else
-            cks(S.solve_level0_cont_all(d, &st, k, o.ridge, lr0.data(), &round_c), "solve_level0_cont_all");
-        ms_c += round_c;
         long long moved0 = 0;
-        for (int p = 0; p < np; p++)
-            moved0 += lr0[(size_t)p].quantised ? lr0[(size_t)p].moved : lr0[(size_t)p].moved_values;
+        if (palette)
+        {
+            const double before_c = ms_c;
+            moved0 = search();
+            round_c = ms_c - before_c;
+        }
+        else
+        {
+            if (frozen)
+                cks(S.sweep_level0_cont_all(d, &st, k, o.ridge, o.q1_sweeps, lr0.data(), &round_c),
+                    "sweep_level0_cont_all");
+            else
+                cks(S.solve_level0_cont_all(d, &st, k, o.ridge, lr0.data(), &round_c), "solve_level0_cont_all");
+            ms_c += round_c;
+            for (int p = 0; p < np; p++)
+                moved0 += lr0[(size_t)p].quantised ? lr0[(size_t)p].moved : lr0[(size_t)p].moved_values;
+        }
         obj(ec);
 
         const bool rose_a = ea.e > e_prev * (1.0 + E_REL) + E_NOISE;
@@ -296,6 +454,23 @@ void run_encode(Encoder& E)
     }
     else
         cks(S.quantise_level1(d, &st, o.q1_range), "quantise_level1");
+    if (palette)
+    {
+        // one last search against the level 1 the asset stores, one last block (a), and the asset as it stands
+        moved0_search = search();
+        cks(S.solve_decoder(d, &st, k, ps, &t), "solve_decoder");
+        ms_a += t;
+        Totals tt;
+        tt.ms_a = ms_a;
+        tt.ms_b = ms_b;
+        tt.ms_c = ms_c;
+        tt.obj_ms0 = obj_ms0;
+        tt.t_start = t_start;
+        tt.moved0_final = moved0_search;
+        std::memcpy(tt.tally0, tally0, sizeof(tally0));
+        finish_palette(E, tt);
+        return;
+    }
     if (frozen)
     {
         cks(S.sweep_level0_cont_all(d, &st, k, o.ridge, o.q1_sweeps, lr0.data(), &t), "sweep_level0_cont_all");
@@ -413,43 +588,19 @@ void run_encode(Encoder& E)
                     "%.9e psnr %.2f -> E %.9e psnr %.2f\n",
                     e_ship, psnr_ship, probe.e, mse_to_db(probe.centre_mse));
 
-    // ---- level 1 must be exactly the grid value of every index about to be written
-    E.verify_level1_on_grid();
-
-    nntc_objective shipped;
-    obj(shipped);
-    // The shipped decode's per-texture error, summed on the device; the bytes themselves are fetched only when asked.
-    cks(S.texture_sse(d, E.source_w, E.source_h, E.texture_se), "texture_sse");
-    E.recon.clear();
-    rep.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-    rep.e_shipped = shipped.e;
-    rep.centre_psnr = mse_to_db(shipped.centre_mse);
-    rep.sampled_psnr = k > 0 ? mse_to_db(shipped.sampled_mse) : 0.0;
-    rep.pack_psnr = pack_psnr;
-    rep.pack_psnr_seed = pack_psnr_seed;
-    rep.ms_a = ms_a;
-    rep.ms_b = ms_b;
-    rep.ms_c = ms_c;
-    rep.ms_e = S.objective_total_ms() - obj_ms0;
-    rep.ms_pack = ms_pack;
-    rep.ms_refine = ms_refine;
+    Totals tt;
+    tt.ms_a = ms_a;
+    tt.ms_b = ms_b;
+    tt.ms_c = ms_c;
+    tt.ms_pack = ms_pack;
+    tt.ms_refine = ms_refine;
+    tt.pack_psnr = pack_psnr;
+    tt.pack_psnr_seed = pack_psnr_seed;
+    tt.obj_ms0 = obj_ms0;
+    tt.t_start = t_start;
+    std::memcpy(tt.tally0, tally0, sizeof(tally0));
     (void)passes0;
-    long long tally[4];
-    S.decoder_ridge_tally(tally);
-    for (int i = 0; i < 4; i++)
-        rep.ridge_tally[i] = tally[i] - tally0[i];
-    for (int tx = 0; tx < E.desc.textures; tx++)
-        rep.psnr[tx] = E.texture_psnr(tx);
-    E.print_call_times();
-    if (!o.quiet)
-    {
-        std::printf("\nreport\n  backend      %s\n", S.name.c_str());
-        for (int tx = 0; tx < E.desc.textures; tx++)
-            std::printf("  psnr         texture %d  %.2f dB\n", tx, rep.psnr[tx]);
-        std::printf("  centre psnr  %.2f dB\n  E shipped    %.9e\n  time         %.3f s total, (a) %.3f ms / (b) "
-                    "%.3f ms / (c) %.3f ms / E %.3f ms\n",
-                    rep.centre_psnr, rep.e_shipped, rep.seconds, ms_a, ms_b, ms_c, rep.ms_e);
-    }
+    ship(E, tt);
 }
 
 }   // namespace
@@ -485,6 +636,23 @@ void Encoder::describe(int textures, int width, int height)
     desc.bits1 = opts.bits1;
     desc.bc0 = 1;
     desc.l0_bc8 = 1;
+    if (opts.l0_palette)
+    {
+        // --l0 palette: an index of 1-4 bits per level-0 channel on a fixed palette, block-compressed or not
+        // (src/main.cpp:1295-1356)
+        if (opts.bc0 != 0 && opts.bc0 != 1)
+            throw Failure { "--bc0 must be 0 or 1" };
+        if (opts.sweeps < 1)
+            throw Failure { "--sweeps takes at least 1" };
+        for (int q = 0; q < opts.c0; q++)
+        {
+            if (opts.bits0[q] < 1 || opts.bits0[q] > 4)
+                throw Failure { "--bits0 " + std::to_string(opts.bits0[q]) + " is out of range (1..4)" };
+            desc.bits0[q] = opts.bits0[q];
+        }
+        desc.bc0 = opts.bc0;
+        desc.l0_bc8 = 0;
+    }
     // --init0 residual or texture both seed from the residual; luma does not (src/main.cpp:1904-1906).
     desc.init0_residual = opts.init0_luma ? 0 : 1;
     desc.init0_texture = opts.init0_texture && !opts.init0_luma ? 1 : 0;
@@ -726,6 +894,11 @@ void nntc_encode_default_options(nntc_encode_options* o)
     o->init1_pca = 0;
     o->init0_texture = 0;
     o->init0_luma = 0;
+    o->l0_palette = 0;
+    for (int q = 0; q < 4; q++)
+        o->bits0[q] = 3;
+    o->bc0 = 1;
+    o->sweeps = 2;
 }
 
 const char* nntc_encode_last_error(void)

@@ -342,4 +342,11 @@ float level1_value(float lo, float hi, int bits, int k)
     return (float)((double)lo + frac * span);
 }
 
+float level0_value(int bits, int k, bool bc)
+{
+    const int levels = (1 << bits) - 1;
+    const double frac = bc ? (double)k / (double)levels : (double)replicate_index(k, bits) / 255.0;
+    return (float)(-1.0 + 2.0 * frac);
+}
+
 }   // namespace nntc_host

@@ -5,13 +5,15 @@
 //                   [--mip-weight pixels|sqrt|uniform] [--mip-mix X] [--q1-start R0] [--q1-sweeps N]
 //                   [--q1-range minmax|pct] [--ridge X] [--bc-refine N] [--bc-refine-after N] [--bc-outer N]
 //                   [--init box|pca] [--init0 residual|texture|luma] [--init0-scope base|chain] [--weights W,..]
-//                   [--rgb-weights R,G,B] [--form exact|plain|fast] [--quiet] [-o PREFIX]
+//                   [--rgb-weights R,G,B] [--l0 bc8|palette] [--bits0 B[,B..]] [--bc0 0|1] [--sweeps N]
+//                   [--form exact|plain|fast] [--quiet] [-o PREFIX]
 //
-// Options the reference has and this path does not build (--l0 palette, named mip filters, --backend cuda|cpu|check)
-// are refused by name rather than ignored.
+// Options the reference has and this path does not build (named mip filters, --backend cuda|cpu|check, --bc0 both,
+// the recon PNGs) are refused by name rather than ignored.
 
 #include "nntc_encode.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -45,6 +47,8 @@ int main(int argc, char** argv)
     nntc_encode_default_options(&o);
     std::vector<const char*> inputs;
     std::string prefix;
+    std::vector<double> bits0;
+    bool bc0_given = false, pack_flags = false;
     for (int i = 1; i < argc; i++)
     {
         const std::string a = argv[i];
@@ -99,11 +103,20 @@ int main(int argc, char** argv)
         else if (a == "--ridge")
             o.ridge = std::atof(next());
         else if (a == "--bc-refine")
+        {
             o.bc_refine = std::atoi(next());
+            pack_flags = true;
+        }
         else if (a == "--bc-refine-after")
+        {
             o.bc_refine_after = std::atoi(next());
+            pack_flags = true;
+        }
         else if (a == "--bc-outer")
+        {
             o.bc_outer = std::atoi(next());
+            pack_flags = true;
+        }
         else if (a == "--init0-scope")
         {
             const std::string v = next();
@@ -140,9 +153,28 @@ int main(int argc, char** argv)
             prefix = next();
         else if (a == "--l0")
         {
-            if (std::string(next()) != "bc8")
-                refuse("palette is not built in the gfx950 arm; bc8, the default, is");
+            const std::string v = next();
+            if (v != "bc8" && v != "palette")
+                refuse("must be bc8 or palette");
+            o.l0_palette = v == "palette";
         }
+        else if (a == "--bits0")
+        {
+            std::vector<double> b;
+            if (!parse_list(next(), b) || b.empty() || b.size() > 4)
+                refuse("takes B or one B per level-0 channel");
+            bits0.assign(b.begin(), b.end());
+        }
+        else if (a == "--bc0")
+        {
+            const std::string v = next();
+            if (v != "0" && v != "1")
+                refuse("must be 0 or 1");
+            o.bc0 = v == "1";
+            bc0_given = true;
+        }
+        else if (a == "--sweeps")
+            o.sweeps = std::atoi(next());
         else if (a == "--init")
         {
             const std::string v = next();
@@ -178,6 +210,24 @@ int main(int argc, char** argv)
         std::fprintf(stderr, "usage: nntc_hip_encode a.png [b.png ... up to 6] [options]\n");
         return 1;
     }
+    // --l0 bc8 fixes --bits0 and --bc0, and the BC pack's flags belong to it (src/main.cpp:1310-1356)
+    if (!o.l0_palette && (!bits0.empty() || (bc0_given && o.bc0 == 0)))
+    {
+        std::fprintf(stderr, "ERROR: --bits0 and --bc0 0 apply under --l0 palette only\n");
+        return 1;
+    }
+    if (o.l0_palette && pack_flags)
+    {
+        std::fprintf(stderr, "ERROR: --bc-refine, --bc-refine-after and --bc-outer belong to --l0 bc8\n");
+        return 1;
+    }
+    if (bits0.size() > 1 && (int)bits0.size() != o.c0)
+    {
+        std::fprintf(stderr, "ERROR: --bits0 gives %zu values for %d level-0 channels\n", bits0.size(), o.c0);
+        return 1;
+    }
+    for (int q = 0; q < 4 && !bits0.empty(); q++)
+        o.bits0[q] = (int)bits0[bits0.size() == 1 ? 0 : (size_t)std::min(q, (int)bits0.size() - 1)];
     if (prefix.empty())
     {
         prefix = inputs[0];

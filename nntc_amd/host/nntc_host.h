@@ -63,6 +63,7 @@ struct Failure
     X(bc_blocks_download)                                                                                              \
     X(bc_state_save)                                                                                                   \
     X(bc_state_restore)                                                                                                \
+    X(solve_level0_all)                                                                                                \
     X(state_snapshot)                                                                                                  \
     X(state_restore)
 
@@ -125,5 +126,29 @@ void mip_weights(const std::vector<nntc_plane_size>& planes, int kind, double mi
 // level1_value (src/model.h:159-168), for the host's on-grid check.
 int replicate_index(int k, int bits);
 float level1_value(float lo, float hi, int bits, int k);
+// --l0 palette's value of index k (src/model.h:205-210): -1 + 2 k / (2^bits - 1) when level 0 is block-compressed,
+// -1 + 2 rep(k) / 255 when it is stored uncompressed - what the shipped format decodes the index to.
+float level0_value(int bits, int k, bool bc);
+
+// writer.cpp: --l0 palette's level 0 as the asset stores it (src/export.cpp:235-246, shared/bc_pack.h).
+// One plane's indices (texels x c0) as the stored bytes: each index replicated over its byte, three channels stored
+// as four with 255 in the fourth.
+void level0_index_bytes(const uint8_t* k0, size_t texels, int c0, const int* bits0, std::vector<uint8_t>& bytes);
+// The channels first .. first + nc - 1 of a w x h plane of `stored` bytes a texel into BC4 (nc 1) or BC5 (nc 2)
+// blocks, row-major, 8 or 16 bytes a block; se / n accumulate the squared byte error against the exact index values
+// over the texels inside the plane, for channels below `used`.
+std::vector<uint8_t> bc_pack_indices(const uint8_t* bytes, int w, int h, int stored, int first, int nc,
+                                     const int* bits, int used, double& se, size_t& n);
+// The sixteen values of one BC4 block, the standard palette evaluated in float.
+void bc4_block_values(const uint8_t blk[8], float out[16]);
+// How level 0's channels are spread over .dds files: BC4 for one, BC5 for two, BC5 + BC4 for three, two BC5s for four.
+inline int level0_files(int c0)
+{
+    return c0 <= 2 ? 1 : 2;
+}
+inline int level0_file_channels(int c0, int file)
+{
+    return file == 0 ? (c0 == 1 ? 1 : 2) : c0 - 2;
+}
 
 }   // namespace nntc_host

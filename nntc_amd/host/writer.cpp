@@ -12,6 +12,7 @@
 
 #include "nntc_host.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -80,7 +81,147 @@ std::string base_name(const std::string& p)
     return slash == std::string::npos ? p : p.substr(slash + 1);
 }
 
+// The standard BC4 palette of endpoints r0, r1 (the Direct3D functional specification): r0 > r1 gives eight values,
+// r0, r1 and ((8 - i) r0 + (i - 1) r1) / 7; otherwise six, ((6 - i) r0 + (i - 1) r1) / 5 between them, then 0 and 255.
+void bc4_palette(int r0, int r1, float pal[8])
+{
+    pal[0] = (float)r0;
+    pal[1] = (float)r1;
+    if (r0 > r1)
+    {
+        for (int i = 2; i < 8; i++)
+            pal[i] = (float)((8 - i) * r0 + (i - 1) * r1) / 7.0f;
+        return;
+    }
+    for (int i = 2; i < 6; i++)
+        pal[i] = (float)((6 - i) * r0 + (i - 1) * r1) / 5.0f;
+    pal[6] = 0.0f;
+    pal[7] = 255.0f;
+}
+
+// One channel's 4x4 block of index bytes (each an index of `bits` bits replicated over its byte) into a BC4 block;
+// returns the squared byte error against the exact index values k * 255 / (2^bits - 1) (shared/bc_pack.h).
+//
+// At 1-3 bits fixed endpoints make the block's palette contain the whole index grid, so the pack is a relabelling:
+//   1 bit   (0, 255), six-value mode: 0 and 255 are entries 6 and 7;
+//   2 bits  (85, 170), six-value mode: 0, 85, 170, 255 are entries 6, 0, 1, 7;
+//   3 bits  (255, 0), eight-value mode: entry 1 is 0, entry 0 is 255, and entry 8 - k is k * 255 / 7 between.
+// At 4 bits sixteen values do not fit in eight: the block's highest and lowest index values, rounded to bytes, are the
+// endpoints (the high one first, which selects the eight-value mode) and each texel takes the nearest entry, the
+// first of equals.
+double bc4_index_block(const uint8_t v[16], int bits, uint8_t blk[8])
+{
+    static const int ENDPOINTS[3][2] = { { 0, 255 }, { 85, 170 }, { 255, 0 } };
+    static const int ENTRY1[2] = { 6, 7 };
+    static const int ENTRY2[4] = { 6, 0, 1, 7 };
+    static const int ENTRY3[8] = { 1, 7, 6, 5, 4, 3, 2, 0 };
+    const int levels = (1 << bits) - 1;
+    int r0 = 0, r1 = 0, sel[16] = {};
+    double err = 0.0;
+    if (bits <= 3)
+    {
+        r0 = ENDPOINTS[bits - 1][0];
+        r1 = ENDPOINTS[bits - 1][1];
+        const int* entry = bits == 1 ? ENTRY1 : (bits == 2 ? ENTRY2 : ENTRY3);
+        for (int t = 0; t < 16; t++)
+            sel[t] = entry[v[t] >> (8 - bits)];
+    }
+    else
+    {
+        int lo = levels, hi = 0;
+        for (int t = 0; t < 16; t++)
+        {
+            lo = std::min(lo, v[t] >> (8 - bits));
+            hi = std::max(hi, v[t] >> (8 - bits));
+        }
+        r0 = (int)std::lround(hi * 255.0 / levels);
+        r1 = (int)std::lround(lo * 255.0 / levels);
+        if (r0 != r1)
+        {
+            float pal[8];
+            bc4_palette(r0, r1, pal);
+            for (int t = 0; t < 16; t++)
+            {
+                const float want = (float)(v[t] >> (8 - bits)) * 255.0f / (float)levels;
+                float best = std::fabs(pal[0] - want);
+                for (int i = 1; i < 8; i++)
+                {
+                    const float d = std::fabs(pal[i] - want);
+                    if (d < best)
+                    {
+                        best = d;
+                        sel[t] = i;
+                    }
+                }
+                err += (double)best * (double)best;
+            }
+        }
+    }
+    blk[0] = (uint8_t)r0;
+    blk[1] = (uint8_t)r1;
+    uint64_t packed = 0;
+    for (int t = 0; t < 16; t++)
+        packed |= (uint64_t)(sel[t] & 7) << (3 * t);
+    for (int i = 0; i < 6; i++)
+        blk[2 + i] = (uint8_t)(packed >> (8 * i));
+    return err;
+}
+
 }   // namespace
+
+namespace nntc_host
+{
+
+void bc4_block_values(const uint8_t blk[8], float out[16])
+{
+    float pal[8];
+    bc4_palette(blk[0], blk[1], pal);
+    uint64_t packed = 0;
+    for (int i = 0; i < 6; i++)
+        packed |= (uint64_t)blk[2 + i] << (8 * i);
+    for (int t = 0; t < 16; t++)
+        out[t] = pal[(packed >> (3 * t)) & 7];
+}
+
+void level0_index_bytes(const uint8_t* k0, size_t texels, int c0, const int* bits0, std::vector<uint8_t>& bytes)
+{
+    const int stored = c0 == 3 ? 4 : c0;
+    bytes.assign(texels * (size_t)stored, 255);
+    for (size_t t = 0; t < texels; t++)
+        for (int c = 0; c < c0; c++)
+            bytes[t * stored + c] = (uint8_t)replicate_index(k0[t * c0 + c], bits0[c]);
+}
+
+std::vector<uint8_t> bc_pack_indices(const uint8_t* bytes, int w, int h, int stored, int first, int nc,
+                                     const int* bits, int used, double& se, size_t& n)
+{
+    const int bx = (w + 3) / 4, by = (h + 3) / 4, bpb = nc == 1 ? 8 : 16;
+    std::vector<uint8_t> out((size_t)bx * by * bpb);
+    for (int y = 0; y < by; y++)
+        for (int x = 0; x < bx; x++)
+        {
+            // the texels of the block inside the plane (the rest repeat the last row and column)
+            const int inside = (std::min(w, 4 * x + 4) - 4 * x) * (std::min(h, 4 * y + 4) - 4 * y);
+            for (int c = 0; c < nc; c++)
+            {
+                uint8_t v[16];
+                for (int t = 0; t < 16; t++)
+                {
+                    const int px = std::min(w - 1, 4 * x + (t & 3)), py = std::min(h - 1, 4 * y + (t >> 2));
+                    v[t] = bytes[((size_t)py * w + px) * stored + first + c];
+                }
+                const double e = bc4_index_block(v, bits[first + c], &out[((size_t)y * bx + x) * bpb + 8 * c]);
+                if (first + c < used)
+                {
+                    se += e * inside / 16.0;
+                    n += (size_t)inside;
+                }
+            }
+        }
+    return out;
+}
+
+}   // namespace nntc_host
 
 extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
 {
@@ -128,36 +269,6 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
           "(hi - lo) per channel\",\n";
     js += "  \"textures\": [\n";
 
-    // ---- level 0: the refined BC4 / BC5 blocks, as the .dds holds them
-    std::string files_js;
-    for (int f = 0; f < files; f++)
-    {
-        const int nc = f == 0 ? (d.c0 == 1 ? 1 : 2) : d.c0 - 2;
-        const int dxgi = nc == 1 ? 80 : 83;
-        const int bpb = nc == 1 ? 8 : 16;
-        std::vector<std::vector<uint8_t>> levels((size_t)np);
-        for (int i = 0; i < np; i++)
-        {
-            const size_t nb = (size_t)((d.planes[i].w0 + 3) / 4) * ((d.planes[i].h0 + 3) / 4);
-            levels[(size_t)i].resize(nb * bpb);
-            if (e.seam.bc_blocks_download(e.ctx, i, f, levels[(size_t)i].data(), levels[(size_t)i].size(),
-                                          nullptr) != 0)
-            {
-                g_error = std::string("bc_blocks_download: ") + e.seam.last_error();
-                return 1;
-            }
-        }
-        const std::string file = prefix + "_lat0" + (files == 2 ? (f == 0 ? "a" : "b") : "") + ".dds";
-        if (!dds_write(file, d.planes[0].w0, d.planes[0].h0, dxgi, true, (uint32_t)levels[0].size(), levels))
-            return 1;
-        const std::string fname = nc == 1 ? "BC4_UNORM" : "BC5_UNORM";
-        if (files == 2)
-            files_js += std::string(f ? ", " : "") + "{ \"file\": \"" + base_name(file) + "\", \"dxgi_format\": \"" +
-                        fname + "\", \"dxgi_format_id\": " + std::to_string(dxgi) + ", \"channels_stored\": " +
-                        std::to_string(nc) + " }";
-        else
-            files_js += "\"" + base_name(file) + "\"";
-    }
     std::string sizes0 = "[", sizes1 = "[";
     for (int i = 0; i < np; i++)
     {
@@ -168,20 +279,121 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
     }
     sizes0 += "]";
     sizes1 += "]";
-    const int nc0 = d.c0 == 1 ? 1 : 2;
-    js += "    {\n      " + std::string(files == 2 ? "\"files\": [" + files_js + "]" : "\"file\": " + files_js) +
+
+    // ---- level 0. --l0 bc8: the refined BC4 / BC5 blocks, as the device holds them. --l0 palette: the indices,
+    // replicated over their bytes, packed here (block-compressed) or stored as they are (--bc0 0).
+    const bool palette = d.l0_bc8 == 0;
+    const bool compress = !palette || d.bc0 != 0;
+    const int stored0 = d.c0 == 3 ? 4 : d.c0;
+    std::vector<std::vector<uint8_t>> bytes0;
+    if (palette)
+    {
+        bytes0.resize((size_t)np);
+        for (int i = 0; i < np; i++)
+        {
+            const size_t n = (size_t)d.planes[i].w0 * d.planes[i].h0;
+            std::vector<uint8_t> k(n * d.c0);
+            if (e.seam.level0_download(e.ctx, i, k.data(), k.size(), nullptr) != 0)
+            {
+                g_error = std::string("level0_download: ") + e.seam.last_error();
+                return 1;
+            }
+            level0_index_bytes(k.data(), n, d.c0, d.bits0, bytes0[(size_t)i]);
+        }
+    }
+    std::string files_js;
+    if (compress)
+        for (int f = 0; f < files; f++)
+        {
+            const int nc = level0_file_channels(d.c0, f);
+            const int dxgi = nc == 1 ? 80 : 83;
+            const int bpb = nc == 1 ? 8 : 16;
+            std::vector<std::vector<uint8_t>> levels((size_t)np);
+            for (int i = 0; i < np; i++)
+            {
+                if (palette)
+                {
+                    double se = 0.0;
+                    size_t n = 0;
+                    levels[(size_t)i] = bc_pack_indices(bytes0[(size_t)i].data(), d.planes[i].w0, d.planes[i].h0,
+                                                        stored0, 2 * f, nc, d.bits0, d.c0, se, n);
+                    continue;
+                }
+                const size_t nb = (size_t)((d.planes[i].w0 + 3) / 4) * ((d.planes[i].h0 + 3) / 4);
+                levels[(size_t)i].resize(nb * bpb);
+                if (e.seam.bc_blocks_download(e.ctx, i, f, levels[(size_t)i].data(), levels[(size_t)i].size(),
+                                              nullptr) != 0)
+                {
+                    g_error = std::string("bc_blocks_download: ") + e.seam.last_error();
+                    return 1;
+                }
+            }
+            const std::string file = prefix + "_lat0" + (files == 2 ? (f == 0 ? "a" : "b") : "") + ".dds";
+            if (!dds_write(file, d.planes[0].w0, d.planes[0].h0, dxgi, true, (uint32_t)levels[0].size(), levels))
+                return 1;
+            const std::string fname = nc == 1 ? "BC4_UNORM" : "BC5_UNORM";
+            if (files == 2)
+                files_js += std::string(f ? ", " : "") + "{ \"file\": \"" + base_name(file) + "\", \"dxgi_format\": \"" +
+                            fname + "\", \"dxgi_format_id\": " + std::to_string(dxgi) + ", \"channels_stored\": " +
+                            std::to_string(nc) + " }";
+            else
+                files_js += "\"" + base_name(file) + "\"";
+        }
+    int dxgi0 = 0, stored_out0 = 0;
+    std::string format0;
+    if (compress)
+    {
+        stored_out0 = level0_file_channels(d.c0, 0);
+        dxgi0 = stored_out0 == 1 ? 80 : 83;
+        format0 = stored_out0 == 1 ? "BC4_UNORM" : "BC5_UNORM";
+    }
+    else
+    {
+        stored_out0 = stored0;
+        dxgi0 = d.c0 == 1 ? 61 : (d.c0 == 2 ? 49 : 28);
+        format0 = d.c0 == 1 ? "R8_UNORM" : (d.c0 == 2 ? "R8G8_UNORM" : "R8G8B8A8_UNORM");
+        const std::string file = prefix + "_lat0.dds";
+        if (!dds_write(file, d.planes[0].w0, d.planes[0].h0, dxgi0, false, (uint32_t)(d.planes[0].w0 * stored0),
+                       bytes0))
+            return 1;
+        files_js = "\"" + base_name(file) + "\"";
+    }
+    const bool two_files = compress && files == 2;
+    js += "    {\n      " + std::string(two_files ? "\"files\": [" + files_js + "]" : "\"file\": " + files_js) +
           ", \"level\": 0, \"width\": " + std::to_string(d.planes[0].w0) + ", \"height\": " +
           std::to_string(d.planes[0].h0) + ",\n";
-    js += "      \"dxgi_format\": \"" + std::string(nc0 == 1 ? "BC4_UNORM" : "BC5_UNORM") +
-          "\", \"dxgi_format_id\": " + std::to_string(nc0 == 1 ? 80 : 83) + ", \"channels_used\": " +
-          std::to_string(d.c0) + ", \"channels_stored\": " + std::to_string(nc0) + ",\n";
-    js += "      \"bc_palette\": \"standard\",\n      \"bits_per_channel\": [";
+    js += "      \"dxgi_format\": \"" + format0 + "\", \"dxgi_format_id\": " + std::to_string(dxgi0) +
+          ", \"channels_used\": " + std::to_string(d.c0) + ", \"channels_stored\": " + std::to_string(stored_out0) +
+          ",\n";
+    if (compress)
+        js += "      \"bc_palette\": \"standard\",\n";
+    js += "      \"bits_per_channel\": [";
     for (int c = 0; c < d.c0; c++)
-        js += std::string("8") + (c + 1 < d.c0 ? "," : "");
+        js += std::to_string(palette ? d.bits0[c] : 8) + (c + 1 < d.c0 ? "," : "");
     js += "],\n      \"mip_count\": " + std::to_string(np) + ", \"mip_sizes\": " + sizes0 + ",\n";
-    js += "      \"texel\": \"an 8-bit continuous value, BC4 / BC5 encoded\",\n";
-    js += "      \"dequantise\": { \"kind\": \"range\", \"lo\": " + nums(st.lo0, (size_t)d.c0) + ", \"hi\": " +
-          nums(st.hi0, (size_t)d.c0) + ", \"levels\": 255 }\n    },\n";
+    if (!palette)
+    {
+        js += "      \"texel\": \"an 8-bit continuous value, BC4 / BC5 encoded\",\n";
+        js += "      \"dequantise\": { \"kind\": \"range\", \"lo\": " + nums(st.lo0, (size_t)d.c0) + ", \"hi\": " +
+              nums(st.hi0, (size_t)d.c0) + ", \"levels\": 255 }\n    },\n";
+    }
+    else
+    {
+        // The palette each index decodes to under THIS file's format, then its ends and its number of steps.
+        js += std::string("      \"texel\": \"") +
+              (compress ? "BC4 / BC5 blocks of the index, lossless at 1-3 bits" : "the index replicated over its byte") +
+              "\",\n      \"dequantise\": { \"kind\": \"palette\", \"palette\": [";
+        std::string lo = "[", hi = "[", lv = "[";
+        for (int c = 0; c < d.c0; c++)
+        {
+            const int levels = (1 << d.bits0[c]) - 1;
+            js += nums(st.palette0[c], (size_t)levels + 1) + (c + 1 < d.c0 ? "," : "");
+            lo += num(st.palette0[c][0]) + (c + 1 < d.c0 ? "," : "");
+            hi += num(st.palette0[c][levels]) + (c + 1 < d.c0 ? "," : "");
+            lv += std::to_string(levels) + (c + 1 < d.c0 ? "," : "");
+        }
+        js += "], \"lo\": " + lo + "], \"hi\": " + hi + "], \"levels\": " + lv + "] }\n    },\n";
+    }
 
     // ---- level 1: the indices replicated over their bytes, three channels padded to four
     {

@@ -147,3 +147,47 @@ def test_driver_inits_match_reference_binary(name, inits, tmp_path):
         a = hashlib.sha256(open(os.path.join(out, f), "rb").read()).hexdigest()
         b = hashlib.sha256(open(str(tmp_path / "drv") + suffix, "rb").read()).hexdigest()
         assert a == b, f"{suffix} differs from the reference's"
+
+
+@pytest.mark.parametrize("bits0,bc0", [("3", 1), ("4", 1), ("2", 0)], ids=["3bit-bc", "4bit-bc-lossy", "2bit-uncompressed"])
+@pytest.mark.parametrize("name", ["smoke_1x64x40", "small_2x100x76"])
+def test_driver_palette_matches_reference_binary(name, bits0, bc0, tmp_path):
+    """--l0 palette: the driver over the reference CPU backend, with the writer's own host pack of the indices (lossless
+    BC4 / BC5 at 1-3 bits, the nearest-entry pack at 4, the replicated bytes under --bc0 0), writes the .dds bytes the
+    reference encoder writes, and the same palette in the descriptor."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "nntc_encode")
+    if not (os.path.exists(REF_LIB) and os.path.exists(ref_bin)):
+        pytest.skip("oracle/_ref is not built")
+    spec = synth.WORKLOADS[name]
+    paths = synth.write_pngs(synth.workload(name), str(tmp_path / "m"))
+    out = str(tmp_path / "ref") + os.sep
+    os.makedirs(out)
+    cmd = [ref_bin, *paths, "--c0", str(spec["c0"]), "--c1", str(spec["c1"]), "--rounds", "4", "--tol", "0",
+           "--png", "0", "--backend", "cpu", "-j", "8", "--l0", "palette", "--bits0", bits0, "--bc0",
+           str(bc0), "-o", out]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_GLOBAL).nntc_ref_set_threads(8)
+    e = encoder.Encoder(files=paths, arm_lib=REF_LIB, arm_prefix="nntc_ref_", c0=spec["c0"], c1=spec["c1"], rounds=4,
+                        tol=0.0, quiet=1, l0_palette=1, bits0=[int(bits0)], bc0=bc0)
+    rep = e.run()
+    e.write(str(tmp_path / "drv"))
+    ref_files = sorted(f for f in os.listdir(out) if f.endswith(".dds"))
+    assert ref_files
+    for f in ref_files:
+        suffix = f[len("m_t0"):]
+        a = hashlib.sha256(open(os.path.join(out, f), "rb").read()).hexdigest()
+        b = hashlib.sha256(open(str(tmp_path / "drv") + suffix, "rb").read()).hexdigest()
+        assert a == b, f"{suffix} differs from the reference's"
+    jref = json.load(open(os.path.join(out, [f for f in os.listdir(out) if f.endswith("_nntc.json")][0])))
+    jdrv = json.load(open(str(tmp_path / "drv") + "_nntc.json"))
+    for key in ("palette", "lo", "hi", "levels", "kind"):
+        assert jdrv["textures"][0]["dequantise"][key] == jref["textures"][0]["dequantise"][key], key
+    for key in ("dxgi_format_id", "channels_stored", "bits_per_channel", "mip_sizes"):
+        assert jdrv["textures"][0][key] == jref["textures"][0][key], key
+    assert (rep.pack_psnr == 100.0) == (int(bits0) <= 3 or bc0 == 0)
+    # The report: at 4 bits block-compressed it is of the packed plane, decoded back from the blocks.
+    shipped = [ln for ln in r.stdout.splitlines() if ln.strip().startswith("E shipped")]
+    assert shipped and shipped[0].split()[2] == f"{rep.e_shipped:.9e}"
+    psnr = [ln.split()[3] for ln in r.stdout.splitlines() if ln.strip().startswith("psnr         texture")]
+    assert psnr == [f"{rep.psnr[t]:.2f}" for t in range(len(psnr))] and psnr

@@ -240,3 +240,70 @@ def test_device_resident_channel_copies(hip_arm, tmp_path):
     lib.nntc_hip_source_copies.argtypes = [C.c_void_p, C.POINTER(C.c_uint32)]
     assert lib.nntc_hip_source_copies(s.ctx, m) == 0
     assert [m[i] for i in range(9)] == [0, 0, 0, 1, 2, 4, 0, 0, 0]
+
+
+@pytest.mark.parametrize("name,bits0,bc0", [("small_2x100x76", [4], 1), ("plain_cpu_4x256", [3], 1),
+                                             ("plain_cpu_4x256", [4, 2], 0), ("plain_gpu_4x1024", [4], 1)],
+                         ids=["small-4bit-bc", "256-3bit-bc", "256-4+2bit-uncompressed", "1024-4bit-bc"])
+def test_encode_palette_matches_reference(name, bits0, bc0, ref_arm, tmp_path):
+    """--l0 palette through the driver on the gfx950 arm (exact form) against the same driver over the reference CPU
+    backend: every round's E, the shipped E (of the packed plane when the pack is lossy), the per-texture psnr and
+    the .dds bytes."""
+    spec = synth.WORKLOADS[name]
+    tex = synth.workload(name)
+    print(f"palette encode {name} bits {bits0} bc0 {bc0}", flush=True)
+    kw = dict(c0=spec["c0"], c1=spec["c1"], rounds=6, tol=0.0, quiet=1, l0_palette=1, bits0=bits0, bc0=bc0)
+    h = encoder.Encoder(tex, **kw)
+    rh = h.run()
+    h.write(str(tmp_path / "h"))
+    r = encoder.Encoder(tex, arm_lib=ref_arm.path, arm_prefix="nntc_ref_", **kw)
+    rr = r.run()
+    r.write(str(tmp_path / "r"))
+    print(f"  {rh.seconds:.3f} s on the arm, {rr.seconds:.3f} s on the reference backend", flush=True)
+    assert rh.rounds_used == rr.rounds_used
+    for i in range(rh.rounds_used):
+        assert list(rh.e_round[i]) == list(rr.e_round[i]), f"round {i + 1}"
+    assert rh.e_shipped == rr.e_shipped
+    assert rh.pack_psnr == rr.pack_psnr
+    assert [rh.psnr[t] for t in range(len(tex))] == [rr.psnr[t] for t in range(len(tex))]
+    files = sorted(f for f in os.listdir(tmp_path) if f.startswith("r_"))
+    assert any(f.endswith(".dds") for f in files)
+    for f in files:
+        a = open(tmp_path / f, "rb").read()
+        b = open(tmp_path / ("h_" + f[2:]), "rb").read()
+        if f.endswith(".json"):
+            a = a.replace(b'"r_lat', b'"h_lat')   # the descriptor names its own prefix's files
+        assert a == b, f"{f[1:]} differs between the arm and the reference backend"
+
+
+@pytest.mark.parametrize("bits0", [[3], [4]], ids=["3bit", "4bit"])
+def test_encode_palette_fast_form_within_psnr_bar(bits0, hip_arm, tmp_path):
+    """--l0 palette under the fast form against the exact form: decoded psnr within 0.05 dB on every texture and the
+    shipped E within 0.5 %. The palette seed leaves each level-0 channel at one nonzero value until its turn, which
+    makes the decoder's features collinear; the fast form's split-bf16 Gram cannot settle that system and hands those
+    solves to the fp64 Gram (counted by nntc_hip_decoder_fast_fallbacks), which is what keeps the run on the exact
+    form's track."""
+    spec = synth.WORKLOADS["plain_gpu_4x1024"]
+    tex = synth.workload("plain_gpu_4x1024")
+    kw = dict(c0=spec["c0"], c1=spec["c1"], rounds=6, tol=0.0, quiet=1, l0_palette=1, bits0=bits0)
+    counter = hip_arm.lib.nntc_hip_decoder_fast_fallbacks
+    counter.restype = C.c_longlong
+    rx = encoder.Encoder(tex, **kw).run()
+    before = counter()
+    rf = encoder.Encoder(tex, form=1, **kw).run()
+    assert counter() > before
+    for t in range(len(tex)):
+        assert abs(rf.psnr[t] - rx.psnr[t]) <= 0.05, f"texture {t}: {rf.psnr[t]:.4f} vs {rx.psnr[t]:.4f} dB"
+    assert abs(rf.e_shipped - rx.e_shipped) <= 0.005 * rx.e_shipped
+
+
+def test_fast_decoder_solve_keeps_the_split_gram_on_a_full_rank_state(hip_arm):
+    """The referral is for degenerate systems only: a --l0 bc8 encode under the fast form settles every block (a) on
+    the split-bf16 Gram (no call redone in fp64)."""
+    spec = synth.WORKLOADS["plain_gpu_4x1024"]
+    counter = hip_arm.lib.nntc_hip_decoder_fast_fallbacks
+    counter.restype = C.c_longlong
+    before = counter()
+    encoder.Encoder(synth.workload("plain_gpu_4x1024"), c0=spec["c0"], c1=spec["c1"], rounds=6, tol=0.0, quiet=1,
+                    form=1).run()
+    assert counter() == before
