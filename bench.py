@@ -55,7 +55,7 @@ HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16_mfma": 2500.0}
 # The arithmetic each kernel's flops run on (every other kernel is priced against HBM only).
 KERNEL_UNIT = {"k_gram_tiles": "bf16_mfma", "k_objective_tiles": "fp32", "k_objective": "fp64",
-               "k_ls_accumulate": "fp64", "k_adjoint_matvec0": "fp32"}
+               "k_ls_accumulate": "fp64", "k_adjoint_matvec0": "fp32", "k_level0_search": "fp32"}
 WORKLOAD = "differentiation_4x4096"
 EXTRA = "plain_gpu_4x1024"
 FORMS = {"exact": 0, "chosen": 1, "plain": 2}
@@ -185,13 +185,13 @@ def cpu_baseline(textures, spec):
 class Job:
     """One form's encode jobs on one workload, from device-resident textures."""
 
-    def __init__(self, hip, textures, spec, form, tmp):
+    def __init__(self, hip, textures, spec, form, tmp, **opts):
         from nntc_amd import abi
 
         self.hip, self.spec, self.form, self.tmp = hip, spec, form, tmp
         self.h, self.w = textures[0].shape[:2]
         self.dev = [abi.DeviceBytes(t) for t in textures]
-        self.kw = dict(c0=spec["c0"], c1=spec["c1"], rounds=8, tol=0.0, quiet=1, form=form)
+        self.kw = dict(c0=spec["c0"], c1=spec["c1"], rounds=8, tol=0.0, quiet=1, form=form, **opts)
         self.mpix = self.w * self.h * len(textures) / 1e6
 
     def step(self, keep=False):
@@ -340,6 +340,9 @@ def main():
     ap.add_argument("--forms", choices=["all", "chosen", "plain", "exact"], default="all",
                     help="time every form (default) or one; a single form is for profiler runs")
     ap.add_argument("--workload", default=WORKLOAD)
+    ap.add_argument("--l0", choices=["bc8", "palette"], default="bc8",
+                    help="level 0's mode; palette (with --forms one form) times the --l0 palette encode instead")
+    ap.add_argument("--bits0", type=int, default=3, help="bits per level-0 channel under --l0 palette")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB device to device (the known-size kernel rocprofv3 counters are scaled by)")
     a = ap.parse_args()
@@ -367,15 +370,21 @@ def main():
     ntex = len(textures)
     tmp = tempfile.mkdtemp(prefix="nntc_bench_")
     try:
+        if a.l0 == "palette" and a.forms == "all":
+            print("--l0 palette times one form: name it with --forms", file=sys.stderr)
+            sys.exit(2)
         if a.forms != "all":
-            job = Job(hip, textures, spec, FORMS[a.forms], tmp)
+            opts = dict(l0_palette=1, bits0=[a.bits0]) if a.l0 == "palette" else {}
+            job = Job(hip, textures, spec, FORMS[a.forms], tmp, **opts)
             e, rep, prefix, sec, table, dom, live = job.timed(a.steps, a.warmup)
             ps = psnr_of(e, textures)
             e.close()
             job.close()
-            print(json.dumps({"form": a.forms, "workload": a.workload, "mpix_s": job.mpix / sec,
+            print(json.dumps({"form": a.forms, "workload": a.workload, "l0": a.l0,
+                              "bits0": a.bits0 if a.l0 == "palette" else None, "mpix_s": job.mpix / sec,
                               "ms_per_step": sec * 1e3, "psnr": [round(x, 5) for x in ps],
-                              "psnr_gap_db": max(abs(ps[t] - ref_psnr[t]) for t in range(ntex)) if ref_psnr else None,
+                              "psnr_gap_db": (max(abs(ps[t] - ref_psnr[t]) for t in range(ntex))
+                                              if ref_psnr and a.l0 == "bc8" else None),
                               "roofline": roofline(dom, live, table, a.workload),
                               "kernels_per_step": kernels_brief(table)}))
             return

@@ -136,6 +136,32 @@ def _file_entries(tex: dict) -> list:
              "channels_stored": tex.get("channels_stored")}]
 
 
+def _replicate(k: int, bits: int) -> int:
+    """Index k of `bits` bits replicated over a byte: its bits from the top down, repeated."""
+    byte, sh = 0, 8 - bits
+    while sh > -bits:
+        byte |= (k << sh) if sh >= 0 else (k >> -sh)
+        sh -= bits
+    return byte & 255
+
+
+def _check_palette(lvl: int, dq: dict, bits: list, lo: np.ndarray, hi: np.ndarray, compressed: bool):
+    """A --l0 palette level publishes palette[c][k], the value index k decodes to; the format promises it equals the
+    range rule on the byte a sampler returns, k / (2^bits - 1) of full scale when block-compressed and the replicated
+    index over 255 when not, so a palette that disagrees (beyond float rounding) is refused."""
+    pal = dq.get("palette")
+    if pal is None or len(pal) != len(bits):
+        raise AssetError(f"level {lvl}: a palette per channel is required")
+    for c, (p, b) in enumerate(zip(pal, bits)):
+        levels = (1 << b) - 1
+        if len(p) != levels + 1:
+            raise AssetError(f"level {lvl}: channel {c}'s palette has {len(p)} entries for {b} bits")
+        for k, v in enumerate(p):
+            frac = k / levels if compressed else _replicate(k, b) / 255.0
+            if abs(v - (lo[c] + frac * (hi[c] - lo[c]))) > 1e-6:
+                raise AssetError(f"level {lvl}: palette entry {k} of channel {c} is not what its byte decodes to")
+
+
 def load(path: str) -> Asset:
     """The descriptor (PREFIX_nntc.json, or a prefix) and the .dds files it names, checked as the format requires."""
     if not path.endswith(".json"):
@@ -190,6 +216,8 @@ def load(path: str) -> Asset:
         dq = tex["dequantise"]
         lo.append(np.asarray(dq["lo"], dtype=np.float64)[:used])
         hi.append(np.asarray(dq["hi"], dtype=np.float64)[:used])
+        if dq.get("kind") == "palette":
+            _check_palette(lvl, dq, bits, lo[-1], hi[-1], "bc_palette" in tex)
         planes.append(mips)
     src = meta.get("source", {})
     return Asset(meta, c0, c1, nout, wts, bias, lo, hi, planes, int(meta["decode"]["width"]),

@@ -50,18 +50,27 @@ struct SearchArgs
     int bits[MAX_CH];
 };
 
-__device__ __forceinline__ double search_energy(double a0, const double* a1, const double* a2, const float* x, int c0)
+template <int C0>
+__device__ __forceinline__ double search_energy(double a0, const double (&a1)[C0], const double (&a2)[C0 * C0],
+                                                const float (&x)[C0])
 {
     double e = a0;
-    for (int i = 0; i < c0; i++)
+#pragma unroll
+    for (int i = 0; i < C0; i++)
     {
         e += 2.0 * (double)x[i] * a1[i];
-        for (int j = 0; j < c0; j++)
-            e += (double)x[i] * (double)x[j] * a2[i * c0 + j];
+#pragma unroll
+        for (int j = 0; j < C0; j++)
+            e += (double)x[i] * (double)x[j] * a2[i * C0 + j];
     }
     return e;
 }
 
+// One lane per texel of the colour. The channel counts are compile-time so that the quadratic's fp64 terms (A1, A2 and
+// a site's partial sums g, S) stay in registers; the outputs are visited in an outer loop - for each output its
+// decode, its two slopes and its residual, then its term of A0, g and S - so no per-site array of outputs is kept.
+// Each of those sums still receives its terms in increasing output order, the reference's order.
+template <int C0, int C1>
 __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
 {
     const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -74,16 +83,21 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
     const long long ci = gi - a.colour.start[p];
     const int cw0 = (w0 - cx + 1) / 2;
     const int tx = cx + 2 * (int)(ci % cw0), ty = cy + 2 * (int)(ci / cw0);
-    const int c0 = a.c0, c1 = a.c1, nin = a.nin, nout = a.nout;
-    const float* v0 = a.v0 + g.off0 * c0;
-    const float* v1 = a.v1 + g.off1 * c1;
-    const float* src = a.src + g.off0 * nout;
+    const int nin = a.nin, nout = a.nout;
+    const float* __restrict__ v0 = a.v0 + g.off0 * C0;
+    const float* __restrict__ v1 = a.v1 + g.off1 * C1;
+    const float* __restrict__ src = a.src + g.off0 * nout;
+    const float* __restrict__ wts = a.weights;
+    const float* __restrict__ bias = a.bias;
+    const float* __restrict__ cwt = a.cw;
     const long long t = (long long)ty * w0 + tx;
 
-    double a0 = 0.0, a1[MAX_CH], a2[MAX_CH * MAX_CH];
-    for (int i = 0; i < c0; i++)
+    double a0 = 0.0, a1[C0], a2[C0 * C0];
+#pragma unroll
+    for (int i = 0; i < C0; i++)
         a1[i] = 0.0;
-    for (int i = 0; i < c0 * c0; i++)
+#pragma unroll
+    for (int i = 0; i < C0 * C0; i++)
         a2[i] = 0.0;
     for (int oy = -1; oy <= 1; oy++)
         for (int ox = -1; ox <= 1; ox++)
@@ -96,97 +110,140 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
             {
                 float sdx, sdy;
                 site_delta(a.k, j, px, py, sdx, sdy);
-                float alpha = 0.0f, beta[MAX_CH], cbar[MAX_CH], target[MAX_NOUT];
-                sample_bilinear(v1, w1, c1, tap_pixel(px, py, sdx, sdy, w0, h0, w1, h1), cbar);
-                for (int i = 0; i < c0; i++)
+                float alpha = 1.0f, beta[C0];
+#pragma unroll
+                for (int i = 0; i < C0; i++)
                     beta[i] = 0.0f;
-                if (j == 0)
-                {
-                    alpha = 1.0f;
-                    for (int r = 0; r < nout; r++)
-                        target[r] = src[t * nout + r];
-                }
-                else
+                // The source taps of a fractional site: the texel's own weight among them decides whether the site
+                // sees the texel at all; a site that does not is the same constant in every state and is skipped.
+                const float* s00 = src + t * nout;
+                const float* s10 = s00;
+                const float* s01 = s00;
+                const float* s11 = s00;
+                float q00 = 1.0f, q10 = 0.0f, q01 = 0.0f, q11 = 0.0f;
+                if (j > 0)
                 {
                     const Tap t0 = tap_pixel(px, py, sdx, sdy, w0, h0, w0, h0);
-                    sample_bilinear(src, w0, nout, t0, target);
                     const int qx[4] = { t0.x0, t0.x1, t0.x0, t0.x1 };
                     const int qy[4] = { t0.y0, t0.y0, t0.y1, t0.y1 };
                     const float qw[4] = { (1.0f - t0.fx) * (1.0f - t0.fy), t0.fx * (1.0f - t0.fy),
                                           (1.0f - t0.fx) * t0.fy, t0.fx * t0.fy };
+                    alpha = 0.0f;
+#pragma unroll
                     for (int q = 0; q < 4; q++)
                         if (qx[q] == tx && qy[q] == ty)
                             alpha += qw[q];
                     if (!(alpha > 0.0f))
-                        continue;   // the site does not see the texel: the same constant in every state
+                        continue;
+                    // Every neighbour value is read before any is summed; the texel's own taps are not read, so a
+                    // lane never reads a texel of its own colour, which another lane of the pass may be writing.
+#pragma unroll
                     for (int q = 0; q < 4; q++)
                         if (!(qx[q] == tx && qy[q] == ty))
                         {
-                            const float* nv = v0 + ((long long)qy[q] * w0 + qx[q]) * c0;
-                            for (int i = 0; i < c0; i++)
+                            const float* nv = v0 + ((long long)qy[q] * w0 + qx[q]) * C0;
+#pragma unroll
+                            for (int i = 0; i < C0; i++)
                                 beta[i] += qw[q] * nv[i];
                         }
+                    s00 = src + ((long long)t0.y0 * w0 + t0.x0) * nout;
+                    s10 = src + ((long long)t0.y0 * w0 + t0.x1) * nout;
+                    s01 = src + ((long long)t0.y1 * w0 + t0.x0) * nout;
+                    s11 = src + ((long long)t0.y1 * w0 + t0.x1) * nout;
+                    q00 = qw[0];
+                    q10 = qw[1];
+                    q01 = qw[2];
+                    q11 = qw[3];
                 }
-                // p (the decode with the texel's channels at zero) and Q's columns at the site's level-1 sample.
-                float qcol[MAX_NOUT * MAX_CH], resid[MAX_NOUT];
+                float cbar[C1];
+                {
+                    const Tap t1 = tap_pixel(px, py, sdx, sdy, w0, h0, w1, h1);
+                    const float* b00 = v1 + ((long long)t1.y0 * w1 + t1.x0) * C1;
+                    const float* b10 = v1 + ((long long)t1.y0 * w1 + t1.x1) * C1;
+                    const float* b01 = v1 + ((long long)t1.y1 * w1 + t1.x0) * C1;
+                    const float* b11 = v1 + ((long long)t1.y1 * w1 + t1.x1) * C1;
+                    const float w00 = (1.0f - t1.fx) * (1.0f - t1.fy), w10 = t1.fx * (1.0f - t1.fy);
+                    const float w01 = (1.0f - t1.fx) * t1.fy, w11 = t1.fx * t1.fy;
+#pragma unroll
+                    for (int q = 0; q < C1; q++)
+                        cbar[q] = w00 * b00[q] + w10 * b10[q] + w01 * b01[q] + w11 * b11[q];
+                }
+                double gs[C0], ss[C0 * C0];
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                    gs[i] = 0.0;
+#pragma unroll
+                for (int i = 0; i < C0 * C0; i++)
+                    ss[i] = 0.0;
                 for (int r = 0; r < nout; r++)
                 {
-                    const float* row = a.weights + (size_t)r * nin;
-                    float out = a.bias[r];
-                    for (int q = 0; q < c1; q++)
+                    // p (the decode with the texel's channels at zero) and Q's column for output r
+                    const float* row = wts + (size_t)r * nin;
+                    float out = bias[r];
+#pragma unroll
+                    for (int q = 0; q < C1; q++)
                         out += row[q] * cbar[q];
-                    for (int i = 0; i < c0; i++)
+                    float qc[C0];
+#pragma unroll
+                    for (int i = 0; i < C0; i++)
                     {
-                        float slope = row[c1 + i];
-                        for (int q = 0; q < c1; q++)
-                            slope += row[c1 + c0 + i * c1 + q] * cbar[q];
-                        qcol[r * c0 + i] = slope;
+                        float slope = row[C1 + i];
+#pragma unroll
+                        for (int q = 0; q < C1; q++)
+                            slope += row[C1 + C0 + i * C1 + q] * cbar[q];
+                        qc[i] = slope;
                         out += slope * beta[i];
                     }
-                    resid[r] = out - target[r];
-                }
-                for (int r = 0; r < nout; r++)
-                    a0 += (double)a.cw[r] * (double)resid[r] * (double)resid[r];
-                for (int i = 0; i < c0; i++)
-                {
-                    double gsum = 0.0;
-                    for (int r = 0; r < nout; r++)
-                        gsum += (double)a.cw[r] * (double)qcol[r * c0 + i] * (double)resid[r];
-                    a1[i] += (double)alpha * gsum;
-                    for (int i2 = 0; i2 < c0; i2++)
+                    const float target = j == 0 ? s00[r] : q00 * s00[r] + q10 * s10[r] + q01 * s01[r] + q11 * s11[r];
+                    const float resid = out - target;
+                    const double cr = (double)cwt[r];
+                    a0 += cr * (double)resid * (double)resid;
+#pragma unroll
+                    for (int i = 0; i < C0; i++)
                     {
-                        double s = 0.0;
-                        for (int r = 0; r < nout; r++)
-                            s += (double)a.cw[r] * (double)qcol[r * c0 + i] * (double)qcol[r * c0 + i2];
-                        a2[i * c0 + i2] += (double)alpha * (double)alpha * s;
+                        gs[i] += cr * (double)qc[i] * (double)resid;
+#pragma unroll
+                        for (int i2 = 0; i2 < C0; i2++)
+                            ss[i * C0 + i2] += cr * (double)qc[i] * (double)qc[i2];
                     }
+                }
+#pragma unroll
+                for (int i = 0; i < C0; i++)
+                {
+                    a1[i] += (double)alpha * gs[i];
+#pragma unroll
+                    for (int i2 = 0; i2 < C0; i2++)
+                        a2[i * C0 + i2] += (double)alpha * (double)alpha * ss[i * C0 + i2];
                 }
             }
         }
 
     // The argmin over the palette from the current state (a tie keeps it).
-    int cur[MAX_CH], best[MAX_CH], levels[MAX_CH];
-    float x[MAX_CH];
-    uint8_t* k0 = a.k0 + g.off0 * c0;
-    for (int i = 0; i < c0; i++)
+    int cur[C0], best[C0], levels[C0];
+    float x[C0];
+    uint8_t* k0 = a.k0 + g.off0 * C0;
+#pragma unroll
+    for (int i = 0; i < C0; i++)
     {
         levels[i] = (1 << a.bits[i]) - 1;
-        cur[i] = (int)k0[t * c0 + i];
+        cur[i] = (int)k0[t * C0 + i];
         best[i] = cur[i];
         x[i] = a.palette[i * 16 + cur[i]];
     }
-    double best_e = search_energy(a0, a1, a2, x, c0);
+    double best_e = search_energy<C0>(a0, a1, a2, x);
     if (a.joint)
     {
         int total = 1;
-        for (int i = 0; i < c0; i++)
+#pragma unroll
+        for (int i = 0; i < C0; i++)
             total *= levels[i] + 1;
         for (int n = 0; n < total; n++)
         {
             int rest = n;
             bool same = true;
-            int state[MAX_CH];
-            for (int i = 0; i < c0; i++)
+            int state[C0];
+#pragma unroll
+            for (int i = 0; i < C0; i++)
             {
                 state[i] = rest % (levels[i] + 1);
                 rest /= levels[i] + 1;
@@ -195,11 +252,12 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
             }
             if (same)
                 continue;
-            const double e = search_energy(a0, a1, a2, x, c0);
+            const double e = search_energy<C0>(a0, a1, a2, x);
             if (e < best_e)
             {
                 best_e = e;
-                for (int i = 0; i < c0; i++)
+#pragma unroll
+                for (int i = 0; i < C0; i++)
                     best[i] = state[i];
             }
         }
@@ -209,9 +267,11 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
         for (int sw = 0; sw < a.sweeps; sw++)
         {
             bool changed = false;
-            for (int i = 0; i < c0; i++)
+#pragma unroll
+            for (int i = 0; i < C0; i++)
             {
-                for (int q = 0; q < c0; q++)
+#pragma unroll
+                for (int q = 0; q < C0; q++)
                     x[q] = a.palette[q * 16 + best[q]];
                 const int hold = best[i];
                 for (int kk = 0; kk <= levels[i]; kk++)
@@ -219,7 +279,7 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
                     if (kk == best[i])
                         continue;
                     x[i] = a.palette[i * 16 + kk];
-                    const double e = search_energy(a0, a1, a2, x, c0);
+                    const double e = search_energy<C0>(a0, a1, a2, x);
                     if (e < best_e)
                     {
                         best_e = e;
@@ -233,17 +293,42 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
         }
     }
     bool moved = false;
-    for (int i = 0; i < c0; i++)
+#pragma unroll
+    for (int i = 0; i < C0; i++)
         moved = moved || best[i] != cur[i];
     if (moved)
     {
-        float* v0w = a.v0 + g.off0 * c0;
-        for (int i = 0; i < c0; i++)
+        float* v0w = a.v0 + g.off0 * C0;
+#pragma unroll
+        for (int i = 0; i < C0; i++)
         {
-            v0w[t * c0 + i] = a.palette[i * 16 + best[i]];
-            k0[t * c0 + i] = (uint8_t)best[i];
+            v0w[t * C0 + i] = a.palette[i * 16 + best[i]];
+            k0[t * C0 + i] = (uint8_t)best[i];
         }
         atomicAdd(a.moved + p * 4 + a.colour_index, 1u);
+    }
+}
+
+template <int C0>
+void launch_search_c1(int c1, unsigned grid, hipStream_t s, const SearchArgs& a)
+{
+    switch (c1)
+    {
+    case 1: hipLaunchKernelGGL((k_level0_search<C0, 1>), dim3(grid), dim3(SEARCH_THREADS), 0, s, a); break;
+    case 2: hipLaunchKernelGGL((k_level0_search<C0, 2>), dim3(grid), dim3(SEARCH_THREADS), 0, s, a); break;
+    case 3: hipLaunchKernelGGL((k_level0_search<C0, 3>), dim3(grid), dim3(SEARCH_THREADS), 0, s, a); break;
+    default: hipLaunchKernelGGL((k_level0_search<C0, 4>), dim3(grid), dim3(SEARCH_THREADS), 0, s, a); break;
+    }
+}
+
+void launch_search(int c0, int c1, unsigned grid, hipStream_t s, const SearchArgs& a)
+{
+    switch (c0)
+    {
+    case 1: launch_search_c1<1>(c1, grid, s, a); break;
+    case 2: launch_search_c1<2>(c1, grid, s, a); break;
+    case 3: launch_search_c1<3>(c1, grid, s, a); break;
+    default: launch_search_c1<4>(c1, grid, s, a); break;
     }
 }
 
@@ -346,9 +431,15 @@ void solve_level0_all(Ctx& c, int k, int sweeps, nntc_level0_report* rep)
         a.colour_index = col;
         if (acc == 0)
             continue;
-        KernelTimer kt(c.stream, "k_level0_search", 0.0, 0.0);
-        hipLaunchKernelGGL(k_level0_search, dim3(blocks_for(acc, SEARCH_THREADS)), dim3(SEARCH_THREADS), 0, c.stream,
-                           a);
+        // Algorithmic bytes: each texel's own source, values and indices read once and its values and indices
+        // written once, and the level-1 planes read once. Flops, nominal: every candidate site of the texel's 3 x 3
+        // footprint (9K + 1 of them) decoded in fp32 (nout x nin multiply-adds) and folded into the quadratic in fp64
+        // (nout (1 + C0 + C0^2) multiply-adds); a site whose taps miss the texel stops before its decode.
+        const double n_col = (double)acc;
+        const double bytes = n_col * (4.0 * c.nout + 2.0 * (4.0 * c.c0 + c.c0)) + 4.0 * (double)c.n1_total * c.c1;
+        const double flops = n_col * (9.0 * k + 1.0) * 2.0 * c.nout * (c.nin + 1.0 + c.c0 + (double)c.c0 * c.c0);
+        KernelTimer kt(c.stream, "k_level0_search", bytes, flops);
+        launch_search(c.c0, c.c1, (unsigned)blocks_for(acc, SEARCH_THREADS), c.stream, a);
     }
     HIP_OK(hipGetLastError());
     const float ms = c.ms_since(c.ev0);

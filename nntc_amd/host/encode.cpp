@@ -208,20 +208,22 @@ void finish_palette(Encoder& E, Totals tt)
                 const int nc = level0_file_channels(dd.c0, f), bpb = nc == 1 ? 8 : 16;
                 const std::vector<uint8_t> blocks =
                     bc_pack_indices(bytes.data(), w, h, stored, 2 * f, nc, dd.bits0, dd.c0, se, n);
-                for (int c = 0; c < nc; c++)
-                    for (int by = 0; by < (h + 3) / 4; by++)
-                        for (int bxi = 0; bxi < bx; bxi++)
-                        {
-                            float vals[16];
-                            bc4_block_values(&blocks[((size_t)by * bx + bxi) * bpb + 8 * c], vals);
-                            for (int t = 0; t < 16; t++)
+                parallel_rows((h + 3) / 4, [&](int b0, int b1) {
+                    for (int c = 0; c < nc; c++)
+                        for (int by = b0; by < b1; by++)
+                            for (int bxi = 0; bxi < bx; bxi++)
                             {
-                                const int x = 4 * bxi + (t & 3), y = 4 * by + (t >> 2);
-                                if (x < w && y < h)
-                                    v0[((size_t)y * w + x) * dd.c0 + 2 * f + c] =
-                                        (float)(-1.0 + (double)vals[t] / 255.0 * 2.0);
+                                float vals[16];
+                                bc4_block_values(&blocks[((size_t)by * bx + bxi) * bpb + 8 * c], vals);
+                                for (int t = 0; t < 16; t++)
+                                {
+                                    const int x = 4 * bxi + (t & 3), y = 4 * by + (t >> 2);
+                                    if (x < w && y < h)
+                                        v0[((size_t)y * w + x) * dd.c0 + 2 * f + c] =
+                                            (float)(-1.0 + (double)vals[t] / 255.0 * 2.0);
+                                }
                             }
-                        }
+                });
             }
             E.check(E.seam.level0_upload(E.ctx, i, v0.data(), v0.size()), "level0_upload");
         }

@@ -187,9 +187,17 @@ void level0_index_bytes(const uint8_t* k0, size_t texels, int c0, const int* bit
 {
     const int stored = c0 == 3 ? 4 : c0;
     bytes.assign(texels * (size_t)stored, 255);
-    for (size_t t = 0; t < texels; t++)
-        for (int c = 0; c < c0; c++)
-            bytes[t * stored + c] = (uint8_t)replicate_index(k0[t * c0 + c], bits0[c]);
+    uint8_t rep[4][16];
+    for (int c = 0; c < c0; c++)
+        for (int k = 0; k < (1 << bits0[c]); k++)
+            rep[c][k] = (uint8_t)replicate_index(k, bits0[c]);
+    const size_t rows = (texels + 4095) / 4096;
+    parallel_rows((int)rows, [&](int r0, int r1) {
+        const size_t end = std::min(texels, (size_t)r1 * 4096);
+        for (size_t t = (size_t)r0 * 4096; t < end; t++)
+            for (int c = 0; c < c0; c++)
+                bytes[t * stored + c] = rep[c][k0[t * c0 + c] & ((1 << bits0[c]) - 1)];
+    });
 }
 
 std::vector<uint8_t> bc_pack_indices(const uint8_t* bytes, int w, int h, int stored, int first, int nc,
@@ -197,26 +205,35 @@ std::vector<uint8_t> bc_pack_indices(const uint8_t* bytes, int w, int h, int sto
 {
     const int bx = (w + 3) / 4, by = (h + 3) / 4, bpb = nc == 1 ? 8 : 16;
     std::vector<uint8_t> out((size_t)bx * by * bpb);
+    // Block rows packed side by side on host threads; each block's error is kept and the errors are added afterwards
+    // in block order, so the sums do not depend on the split.
+    std::vector<double> err((size_t)bx * by * nc);
+    parallel_rows(by, [&](int y0, int y1) {
+        for (int y = y0; y < y1; y++)
+            for (int x = 0; x < bx; x++)
+                for (int c = 0; c < nc; c++)
+                {
+                    uint8_t v[16];
+                    for (int t = 0; t < 16; t++)
+                    {
+                        const int px = std::min(w - 1, 4 * x + (t & 3)), py = std::min(h - 1, 4 * y + (t >> 2));
+                        v[t] = bytes[((size_t)py * w + px) * stored + first + c];
+                    }
+                    err[((size_t)y * bx + x) * nc + c] =
+                        bc4_index_block(v, bits[first + c], &out[((size_t)y * bx + x) * bpb + 8 * c]);
+                }
+    });
     for (int y = 0; y < by; y++)
         for (int x = 0; x < bx; x++)
         {
             // the texels of the block inside the plane (the rest repeat the last row and column)
             const int inside = (std::min(w, 4 * x + 4) - 4 * x) * (std::min(h, 4 * y + 4) - 4 * y);
             for (int c = 0; c < nc; c++)
-            {
-                uint8_t v[16];
-                for (int t = 0; t < 16; t++)
-                {
-                    const int px = std::min(w - 1, 4 * x + (t & 3)), py = std::min(h - 1, 4 * y + (t >> 2));
-                    v[t] = bytes[((size_t)py * w + px) * stored + first + c];
-                }
-                const double e = bc4_index_block(v, bits[first + c], &out[((size_t)y * bx + x) * bpb + 8 * c]);
                 if (first + c < used)
                 {
-                    se += e * inside / 16.0;
+                    se += err[((size_t)y * bx + x) * nc + c] * inside / 16.0;
                     n += (size_t)inside;
                 }
-            }
         }
     return out;
 }

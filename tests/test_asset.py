@@ -11,6 +11,7 @@ import ctypes
 import json
 import os
 import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -109,21 +110,27 @@ def test_reader_refuses_what_the_format_does_not_define(tmp_path):
             asset.load(p)
 
 
-def ref_encode(name, rounds, files=None):
+def ref_encode(name, rounds, files=None, **opts):
     if not os.path.exists(REF_LIB):
         pytest.skip("oracle/_ref is not built")
     ctypes.CDLL(REF_LIB, mode=ctypes.RTLD_GLOBAL).nntc_ref_set_threads(min(8, os.cpu_count() or 1))
     spec = synth.WORKLOADS[name]
-    kw = dict(arm_lib=REF_LIB, arm_prefix="nntc_ref_", c0=spec["c0"], c1=spec["c1"], rounds=rounds, tol=0.0, quiet=1)
+    kw = dict(arm_lib=REF_LIB, arm_prefix="nntc_ref_", c0=spec["c0"], c1=spec["c1"], rounds=rounds, tol=0.0, quiet=1,
+              **opts)
     e = encoder.Encoder(files=files, **kw) if files else encoder.Encoder(synth.workload(name), **kw)
     return e, e.run()
 
 
-@pytest.mark.parametrize("name,rounds", [("smoke_1x64x40", 4), ("small_2x100x76", 4)])
-def test_written_asset_reads_back(name, rounds, tmp_path):
-    """The driver's asset over the reference CPU backend (one BC4 file at one level-0 channel; a BC5 + BC4 pair at
-    three): the reader's decode is within one step of the encoder's own, its PSNR the report's to 0.01 dB."""
-    e, rep = ref_encode(name, rounds)
+@pytest.mark.parametrize("name,rounds,opts", [
+    ("smoke_1x64x40", 4, {}), ("small_2x100x76", 4, {}),
+    ("small_2x100x76", 4, dict(l0_palette=1, bits0=[3])), ("small_2x100x76", 4, dict(l0_palette=1, bits0=[4])),
+    ("smoke_1x64x40", 4, dict(l0_palette=1, bits0=[2], bc0=0))],
+    ids=["bc8-bc4", "bc8-bc5+bc4", "palette-3bit", "palette-4bit-lossy", "palette-2bit-uncompressed"])
+def test_written_asset_reads_back(name, rounds, opts, tmp_path):
+    """The driver's asset over the reference CPU backend (--l0 bc8: one BC4 file at one level-0 channel, a BC5 + BC4
+    pair at three; --l0 palette: the indices block-compressed at 3 and 4 bits and stored as bytes at 2): the reader's
+    decode is within one step of the encoder's own, its PSNR the report's to 0.01 dB."""
+    e, rep = ref_encode(name, rounds, **opts)
     e.write(str(tmp_path / "m"))
     a = asset.load(str(tmp_path / "m"))
     rgb = asset.decode(a)
@@ -155,3 +162,36 @@ def test_descriptor_matches_the_reference(tmp_path):
         assert ours["textures"][lvl].pop("file") == f"out_lat{lvl}.dds"
         assert ref["textures"][lvl].pop("file") == f"m_t0_lat{lvl}.dds"
     assert ours == ref
+
+
+@pytest.mark.parametrize("bits0,bc0", [("4", 1), ("2", 0)], ids=["4bit-bc", "2bit-uncompressed"])
+def test_palette_descriptor_matches_the_reference(bits0, bc0, tmp_path):
+    """--l0 palette's _nntc.json from the driver over the reference CPU backend equals the reference encoder's own,
+    run here on the same PNG files, key by key (prose aside, and the .dds names, which follow the prefix)."""
+    ref_bin = os.path.join(ROOT, "oracle", "_ref", "nntc_encode")
+    if not (os.path.exists(REF_LIB) and os.path.exists(ref_bin)):
+        pytest.skip("oracle/_ref is not built")
+    paths = synth.write_pngs(synth.workload("small_2x100x76"), str(tmp_path / "m"))
+    out = str(tmp_path / "ref") + os.sep
+    os.makedirs(out)
+    spec = synth.WORKLOADS["small_2x100x76"]
+    r = subprocess.run([ref_bin, *paths, "--c0", str(spec["c0"]), "--c1", str(spec["c1"]), "--rounds", "4", "--tol",
+                        "0", "--png", "0", "--backend", "cpu", "-j", "8", "--quiet", "--l0", "palette", "--bits0",
+                        bits0, "--bc0", str(bc0), "-o", out], stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    e, _ = ref_encode("small_2x100x76", 4, files=paths, l0_palette=1, bits0=[int(bits0)], bc0=bc0)
+    e.write(str(tmp_path / "out"))
+    ours = strip_prose(json.load(open(str(tmp_path / "out") + "_nntc.json")))
+    ref = strip_prose(json.load(open(os.path.join(out, "m_t0_nntc.json"))))
+    for lvl in range(2):
+        for key in ("file", "files"):
+            a, b = ours["textures"][lvl].pop(key, None), ref["textures"][lvl].pop(key, None)
+            assert json.dumps(a).replace("out_lat", "m_t0_lat") == json.dumps(b)
+    assert ours == ref
+    # The reader takes the asset, and refuses it once a palette entry is not what its byte decodes to.
+    asset.load(str(tmp_path / "out"))
+    bad = json.load(open(str(tmp_path / "out") + "_nntc.json"))
+    bad["textures"][0]["dequantise"]["palette"][0][1] += 0.01
+    json.dump(bad, open(str(tmp_path / "out") + "_nntc.json", "w"))
+    with pytest.raises(asset.AssetError):
+        asset.load(str(tmp_path / "out"))
