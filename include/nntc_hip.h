@@ -185,6 +185,13 @@ int nntc_hip_source_download(nntc_hip_ctx* ctx, int plane, float* out, size_t n,
 /* Replaces be::device_destroy (src/backend.h:88); NULL is ignored. */
 void nntc_hip_device_destroy(nntc_hip_ctx* ctx);
 
+/* No counterpart in the reference (which allocates and frees per run): device_destroy gives a drained context's device
+ * buffers and pinned host areas to a cache of this process (at most 96 GiB), and the contexts created after it take
+ * their memory from there before they ask the runtime; this returns everything cached to the runtime now and says
+ * how many bytes that was. An allocation the runtime refuses for lack of memory empties the cache and is tried
+ * again. */
+size_t nntc_hip_release_cached_memory(void);
+
 /* Replaces be::device_memory (src/backend.h:89): bytes of device memory the context holds. */
 size_t nntc_hip_device_memory(const nntc_hip_ctx* ctx);
 

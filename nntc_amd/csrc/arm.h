@@ -43,11 +43,24 @@ struct DevBuf
 {
     T* p = nullptr;
     size_t n = 0;
+    size_t cap = 0;        // the bytes the block holds (at least n * sizeof(T); more when it came from the cache)
+    int device = -1;
     void alloc(size_t count);
     void release();
     ~DevBuf() { release(); }
     size_t bytes() const { return n * sizeof(T); }
 };
+
+// Memory a destroyed context gives back, kept for the contexts this process creates after it (device buffers by device
+// and size, pinned host areas by size). A context gives its memory back only once its stream has drained, so a block
+// handed out again is never still in use; cache_take returns null when nothing cached fits.
+void* cache_take(size_t bytes, int device, bool host, size_t& got);
+bool cache_put(void* p, size_t bytes, int device, bool host);
+size_t cache_flush();
+void* host_alloc(size_t bytes, size_t& got);   // pinned host memory, from the cache when it can
+void host_release(void* p, size_t bytes);
+// Set while a drained context releases its memory: releases then go to the cache instead of the runtime.
+extern thread_local bool g_recycling;
 
 // Scalar row per plane (the reference's PlaneScalar slots, src/cpu/cpu_constants.h:82-101).
 enum Scalar
@@ -191,6 +204,7 @@ struct Ctx
 
     // Pinned host staging for the small per-call readbacks.
     double* host_scalars = nullptr;
+    size_t host_scalars_bytes = 0;
     // Pinned host staging: a copy between device memory and pageable host memory is staged by the runtime through
     // buffers of its own, in pieces, each a copy kernel and a wait. Readbacks use one area (each waits for its copy);
     // uploads take successive pieces of a second area, so a new upload never waits for an earlier one still in flight,

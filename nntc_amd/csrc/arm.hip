@@ -36,19 +36,124 @@ void DevBuf<T>::alloc(size_t count)
     release();
     if (count == 0)
         count = 1;
-    void* ptr = nullptr;
-    HIP_OK(hipMalloc(&ptr, count * sizeof(T)));
+    const size_t want = count * sizeof(T);
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    size_t got = 0;
+    void* ptr = cache_take(want, dev, false, got);
+    if (!ptr)
+    {
+        hipError_t e = hipMalloc(&ptr, want);
+        if (e == hipErrorOutOfMemory && cache_flush() > 0)
+        {
+            (void)hipGetLastError();
+            e = hipMalloc(&ptr, want);
+        }
+        HIP_OK(e);
+        got = want;
+    }
     p = static_cast<T*>(ptr);
     n = count;
+    cap = got;
+    device = dev;
 }
 
 template <class T>
 void DevBuf<T>::release()
 {
-    if (p)
+    if (p && !(g_recycling && cache_put(p, cap, device, false)))
         (void)hipFree(p);
     p = nullptr;
     n = 0;
+    cap = 0;
+}
+
+namespace
+{
+
+struct Cached
+{
+    void* p;
+    size_t bytes;
+    int device;
+    bool host;
+};
+
+std::mutex g_cache_lock;
+std::vector<Cached> g_cache;
+size_t g_cache_bytes = 0;
+// At most this much is held for later contexts; beyond it a released block goes back to the runtime.
+constexpr size_t CACHE_CAP = (size_t)96 << 30;
+
+}   // namespace
+
+thread_local bool g_recycling = false;
+
+void* cache_take(size_t bytes, int device, bool host, size_t& got)
+{
+    std::lock_guard<std::mutex> g(g_cache_lock);
+    // The smallest cached block that holds the request and wastes at most a quarter of itself.
+    int best = -1;
+    for (int i = 0; i < (int)g_cache.size(); i++)
+    {
+        const Cached& c = g_cache[(size_t)i];
+        if (c.host == host && c.device == device && c.bytes >= bytes && c.bytes - bytes <= c.bytes / 4 &&
+            (best < 0 || c.bytes < g_cache[(size_t)best].bytes))
+            best = i;
+    }
+    if (best < 0)
+        return nullptr;
+    const Cached c = g_cache[(size_t)best];
+    g_cache.erase(g_cache.begin() + best);
+    g_cache_bytes -= c.bytes;
+    got = c.bytes;
+    return c.p;
+}
+
+bool cache_put(void* p, size_t bytes, int device, bool host)
+{
+    std::lock_guard<std::mutex> g(g_cache_lock);
+    if (g_cache_bytes + bytes > CACHE_CAP)
+        return false;
+    g_cache.push_back(Cached { p, bytes, device, host });
+    g_cache_bytes += bytes;
+    return true;
+}
+
+size_t cache_flush()
+{
+    std::vector<Cached> all;
+    {
+        std::lock_guard<std::mutex> g(g_cache_lock);
+        all.swap(g_cache);
+        g_cache_bytes = 0;
+    }
+    size_t freed = 0;
+    for (const Cached& c : all)
+    {
+        if (c.host)
+            (void)hipHostFree(c.p);
+        else
+            (void)hipFree(c.p);
+        freed += c.bytes;
+    }
+    return freed;
+}
+
+void* host_alloc(size_t bytes, size_t& got)
+{
+    void* p = cache_take(bytes, -1, true, got);
+    if (p)
+        return p;
+    HIP_OK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+    got = bytes;
+    return p;
+}
+
+void host_release(void* p, size_t bytes)
+{
+    if (p && !(g_recycling && cache_put(p, bytes, -1, true)))
+        (void)hipHostFree(p);
 }
 
 template struct DevBuf<float>;
@@ -160,13 +265,11 @@ void* Ctx::stage(size_t bytes)
 {
     if (bytes > host_stage_bytes)
     {
-        if (host_stage)
-            HIP_OK(hipHostFree(host_stage));
+        host_release(host_stage, host_stage_bytes);
         host_stage = nullptr;
         host_stage_bytes = 0;
         const size_t want = bytes < ((size_t)1 << 20) ? ((size_t)1 << 20) : bytes;
-        HIP_OK(hipHostMalloc(&host_stage, want, hipHostMallocDefault));
-        host_stage_bytes = want;
+        host_stage = host_alloc(want, host_stage_bytes);
     }
     return host_stage;
 }
@@ -183,15 +286,11 @@ void* Ctx::upload_slot(size_t bytes)
         }
         if (need > host_up_bytes)
         {
-            if (host_up)
-                HIP_OK(hipHostFree(host_up));
+            host_release(host_up, host_up_bytes);
             host_up = nullptr;
             host_up_bytes = 0;
             const size_t want = need < ((size_t)1 << 20) ? ((size_t)1 << 20) : need;
-            void* ptr = nullptr;
-            HIP_OK(hipHostMalloc(&ptr, want, hipHostMallocDefault));
-            host_up = static_cast<char*>(ptr);
-            host_up_bytes = want;
+            host_up = static_cast<char*>(host_alloc(want, host_up_bytes));
         }
     }
     void* slot = host_up + up_head;
@@ -900,7 +999,11 @@ void create_model(Ctx& c, const nntc_model_desc* md)
         // The assembly's site buffer: every site of the chain at the largest K the encoder takes (4).
         c.site_f.alloc((size_t)o0 * 5 * std::max({ c0, c1, c0 * (c0 + 1) / 2 }));
         c.site_d.alloc((size_t)o0 * 5 * std::max(c0, c1));
-        HIP_OK(hipHostMalloc((void**)&c.host_scalars, sizeof(double) * SC_COUNT * MAX_PLANES, hipHostMallocDefault));
+        {
+            size_t got = 0;
+            c.host_scalars = static_cast<double*>(host_alloc(sizeof(double) * SC_COUNT * MAX_PLANES, got));
+            c.host_scalars_bytes = got;
+        }
 
         c.bytes = c.src.bytes() + c.v0.bytes() + c.v1.bytes() + c.k0.bytes() + c.k1.bytes() + c.weights.bytes() +
                   c.bias.bytes() + c.cw.bytes() + c.grid.bytes() + c.scalars.bytes() + c.partial.bytes() +
@@ -1019,14 +1122,13 @@ void nntc_hip_device_destroy(nntc_hip_ctx* h)
     if (!h)
         return;
     Ctx& c = h->c;
-    if (c.stream)
-        (void)hipStreamSynchronize(c.stream);
-    if (c.host_scalars)
-        (void)hipHostFree(c.host_scalars);
-    if (c.host_stage)
-        (void)hipHostFree(c.host_stage);
-    if (c.host_up)
-        (void)hipHostFree(c.host_up);
+    // Once the stream has drained nothing of the context is in use: its memory goes to the process's cache, for the
+    // next context to take (a stream that failed to drain returns its memory to the runtime instead).
+    const bool drained = !c.stream || hipStreamSynchronize(c.stream) == hipSuccess;
+    g_recycling = drained;
+    host_release(c.host_scalars, c.host_scalars_bytes);
+    host_release(c.host_stage, c.host_stage_bytes);
+    host_release(c.host_up, c.host_up_bytes);
     if (c.ev0)
         (void)hipEventDestroy(c.ev0);
     if (c.ev1)
@@ -1040,8 +1142,14 @@ void nntc_hip_device_destroy(nntc_hip_ctx* h)
             (void)hipGraphExecDestroy(g.exec);
     hipStream_t s = c.stream;
     delete h;
+    g_recycling = false;
     if (s)
         (void)hipStreamDestroy(s);
+}
+
+size_t nntc_hip_release_cached_memory(void)
+{
+    return cache_flush();
 }
 
 size_t nntc_hip_device_memory(const nntc_hip_ctx* h)

@@ -307,3 +307,24 @@ def test_fast_decoder_solve_keeps_the_split_gram_on_a_full_rank_state(hip_arm):
     encoder.Encoder(synth.workload("plain_gpu_4x1024"), c0=spec["c0"], c1=spec["c1"], rounds=6, tol=0.0, quiet=1,
                     form=1).run()
     assert counter() == before
+
+
+def test_recycled_memory_gives_the_same_bits(hip_arm):
+    """A destroyed context's device buffers and pinned areas go to the process's cache and the next context takes them
+    (holding the first encode's data): the second encode's outputs are the first's bits, and releasing the cache
+    returns what the destroy gave back."""
+    rel = hip_arm.lib.nntc_hip_release_cached_memory
+    rel.restype = C.c_size_t
+    rel()
+    spec = synth.WORKLOADS["small_2x100x76"]
+    tex = synth.workload("small_2x100x76")
+    out = []
+    for run in range(2):
+        e = encoder.Encoder(tex, c0=spec["c0"], c1=spec["c1"], rounds=4, tol=0.0, quiet=1, form=abi.FORM_FAST)
+        r = e.run()
+        blocks = [e.level0_blocks(p, f).tobytes() for p in range(e.desc.nplanes) for f in range(2)]
+        out.append((r.e_shipped, hashlib.sha256(b"".join(blocks)).hexdigest()))
+        e.close()
+    assert out[0] == out[1]
+    assert rel() > 0
+    assert rel() == 0
