@@ -3680,7 +3680,10 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
         if (!fold_in_update)
             hipLaunchKernelGGL(k_fold_alpha_fast, dim3(np), dim3(256), 0, c.stream, fc);
         {
-            KernelTimer kt(c.stream, "k_cg_update", (double)n * 4.0 * cc * cc + 44.0 * nv, nv * (4.0 + 2.0 * cc));
+            // Per element: x (fp64) read and written, p and Ap read, r read and written, z written (36 bytes); per
+            // texel its C x C block of the preconditioner.
+            KernelTimer kt(c.stream, level0 ? "k_cg_update0" : "k_cg_update1", (double)n * 4.0 * cc * cc + 36.0 * nv,
+                           nv * (4.0 + 2.0 * cc));
             if (fold_in_update)
                 hipLaunchKernelGGL(k_cg_update_fast<true>, dim3(fgrid), dim3(256), 0, c.stream, fc);
             else
@@ -3699,7 +3702,7 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
         if (!fold_in_update)
             hipLaunchKernelGGL(k_fold_beta_fast, dim3(np), dim3(256), 0, c.stream, fc);
         {
-            KernelTimer kt(c.stream, "k_xpby", 12.0 * nv, 2.0 * nv);
+            KernelTimer kt(c.stream, level0 ? "k_xpby0" : "k_xpby1", 12.0 * nv, 2.0 * nv);
             if (fold_in_update)
                 hipLaunchKernelGGL(k_xpby_fold_fast, dim3(fgrid), dim3(256), 0, c.stream, fc);
             else
@@ -3712,7 +3715,7 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
     // period and call. Not while the profiler times one of its kernels (its events are host-side per launch).
     hipGraphExec_t period = nullptr;
     if (c.form == NNTC_FORM_FAST && c.graphs != 0 && !profiled(mf ? "k_adjoint_matvec0" : (level0 ? "k_matvec0" : "k_matvec1")) &&
-        !profiled("k_cg_update") && !profiled("k_xpby"))
+        !profiled(level0 ? "k_cg_update0" : "k_cg_update1") && !profiled(level0 ? "k_xpby0" : "k_xpby1"))
     {
         GraphKey key;
         key.add(level0).add(mf).add(fold_in_update).add(fgrid).add(tb).add(mf_grid).add(mfa.k).add(np);
@@ -3753,13 +3756,13 @@ void cg_solve(Ctx& c, bool level0, std::vector<int>& iterations)
             dot(L.p, L.ap, nullptr, nullptr, 16.0 * nv);
             hipLaunchKernelGGL(k_fold_alpha, dim3(1), dim3(64), 0, c.stream, ch);
             {
-                KernelTimer kt(c.stream, "k_cg_update", b_update, nv * (4.0 + 2.0 * cc));
+                KernelTimer kt(c.stream, level0 ? "k_cg_update0" : "k_cg_update1", b_update, nv * (4.0 + 2.0 * cc));
                 hipLaunchKernelGGL(k_cg_update, dim3(tb), dim3(256), 0, c.stream, L);
             }
             dot(L.r, L.r, L.r, L.z, 16.0 * nv);
             hipLaunchKernelGGL(k_fold_beta, dim3(1), dim3(64), 0, c.stream, ch, 0);
             {
-                KernelTimer kt(c.stream, "k_xpby", 24.0 * nv, 2.0 * nv);
+                KernelTimer kt(c.stream, level0 ? "k_xpby0" : "k_xpby1", 24.0 * nv, 2.0 * nv);
                 hipLaunchKernelGGL(k_xpby, dim3(tb), dim3(256), 0, c.stream, L);
             }
             HIP_OK(hipGetLastError());

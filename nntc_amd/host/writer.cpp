@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace nntc_host;
@@ -259,6 +260,17 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
         }
     const std::string prefix = prefix_c;
     const int np = d.nplanes;
+    // The .dds files, written side by side on host threads once every level is on the host (the context is driven
+    // from this thread only, so the downloads stay here).
+    struct PendingFile
+    {
+        std::string name;
+        int w, h, dxgi;
+        bool compressed;
+        uint32_t size_field;
+        std::vector<std::vector<uint8_t>> levels;
+    };
+    std::vector<PendingFile> pending;
     const int files = d.c0 <= 2 ? 1 : 2;
     std::string js = "{\n  \"format\": \"nntc-dds-1\",\n";
     js += "  \"source\": { \"width\": " + std::to_string(e.source_w) + ", \"height\": " + std::to_string(e.source_h);
@@ -346,8 +358,8 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
                 }
             }
             const std::string file = prefix + "_lat0" + (files == 2 ? (f == 0 ? "a" : "b") : "") + ".dds";
-            if (!dds_write(file, d.planes[0].w0, d.planes[0].h0, dxgi, true, (uint32_t)levels[0].size(), levels))
-                return 1;
+            const uint32_t linear = (uint32_t)levels[0].size();
+            pending.push_back({ file, d.planes[0].w0, d.planes[0].h0, dxgi, true, linear, std::move(levels) });
             const std::string fname = nc == 1 ? "BC4_UNORM" : "BC5_UNORM";
             if (files == 2)
                 files_js += std::string(f ? ", " : "") + "{ \"file\": \"" + base_name(file) + "\", \"dxgi_format\": \"" +
@@ -370,9 +382,8 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
         dxgi0 = d.c0 == 1 ? 61 : (d.c0 == 2 ? 49 : 28);
         format0 = d.c0 == 1 ? "R8_UNORM" : (d.c0 == 2 ? "R8G8_UNORM" : "R8G8B8A8_UNORM");
         const std::string file = prefix + "_lat0.dds";
-        if (!dds_write(file, d.planes[0].w0, d.planes[0].h0, dxgi0, false, (uint32_t)(d.planes[0].w0 * stored0),
-                       bytes0))
-            return 1;
+        pending.push_back({ file, d.planes[0].w0, d.planes[0].h0, dxgi0, false, (uint32_t)(d.planes[0].w0 * stored0),
+                            std::move(bytes0) });
         files_js = "\"" + base_name(file) + "\"";
     }
     const bool two_files = compress && files == 2;
@@ -432,9 +443,8 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
                     levels[(size_t)i][t * stored + c] = (uint8_t)replicate_index(k[t * ch + c], d.bits1);
         }
         const std::string file = prefix + "_lat1.dds";
-        if (!dds_write(file, d.planes[0].w1, d.planes[0].h1, dxgi, false, (uint32_t)(d.planes[0].w1 * stored),
-                       levels))
-            return 1;
+        pending.push_back({ file, d.planes[0].w1, d.planes[0].h1, dxgi, false, (uint32_t)(d.planes[0].w1 * stored),
+                            std::move(levels) });
         const std::string fname = stored == 1 ? "R8_UNORM" : (stored == 2 ? "R8G8_UNORM" : "R8G8B8A8_UNORM");
         js += "    {\n      \"file\": \"" + base_name(file) + "\", \"level\": 1, \"width\": " +
               std::to_string(d.planes[0].w1) + ", \"height\": " + std::to_string(d.planes[0].h1) + ",\n";
@@ -456,6 +466,22 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
     js += "    \"layers\": [\n      { \"rows\": " + std::to_string(d.nout) + ", \"cols\": " + std::to_string(nin) +
           ", \"weights\": " + nums(st.w, (size_t)d.nout * nin) + ", \"bias\": " + nums(st.b, (size_t)d.nout) +
           " }\n    ]\n  }\n}\n";
+    std::vector<std::string> errors(pending.size());
+    std::vector<std::thread> writers;
+    for (size_t i = 0; i < pending.size(); i++)
+        writers.emplace_back([&, i] {
+            const PendingFile& pf = pending[i];
+            if (!dds_write(pf.name, pf.w, pf.h, pf.dxgi, pf.compressed, pf.size_field, pf.levels))
+                errors[i] = g_error;
+        });
+    for (std::thread& t : writers)
+        t.join();
+    for (const std::string& e : errors)
+        if (!e.empty())
+        {
+            g_error = e;
+            return 1;
+        }
     const std::string jname = prefix + "_nntc.json";
     FILE* f = std::fopen(jname.c_str(), "wb");
     if (!f)
