@@ -177,6 +177,7 @@ struct Ctx
     DevBuf<float> resid;                   // base texels * nout
     DevBuf<double> cov_partial;            // SITE_BLOCKS * entries
     DevBuf<unsigned int> counters;         // per plane, MOVED_SLOTS counters MOVED_STRIDE apart
+    DevBuf<unsigned int> pal_counters;     // the palette search's: the same per colour, then the folded counts
     DevBuf<double> cg_partial;             // the fast form's CG: three partials per chunk of 256 texels
     DevBuf<double> gather_tab;             // the fast level-0 gather's interior site lists (4 x 45 entries of 7)
     int gather_tab_k = -1;                 // the K the list was built for

@@ -45,7 +45,8 @@ struct SearchArgs
     const float* palette;   // 4 x 16, channel-major
     float* v0;
     uint8_t* k0;
-    unsigned int* moved;    // per plane, four colour counters
+    unsigned int* moved;    // per colour, MOVED_SLOTS counters a plane MOVED_STRIDE apart, colours moved_stride apart
+    size_t moved_stride;
     int c0, c1, nin, nout, k, joint, sweeps;
     int bits[MAX_CH];
 };
@@ -71,12 +72,8 @@ __device__ __forceinline__ double search_energy(double a0, const double (&a1)[C0
 // decode, its two slopes and its residual, then its term of A0, g and S - so no per-site array of outputs is kept.
 // Each of those sums still receives its terms in increasing output order, the reference's order.
 template <int C0, int C1>
-__global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
+__device__ __forceinline__ bool search_texel(const SearchArgs& a, long long gi, int p)
 {
-    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (gi >= a.colour.start[a.colour.n])
-        return;
-    const int p = range_plane(a.colour, gi);
     const PlaneGeom& g = a.planes.p[p];
     const int w0 = g.w0, h0 = g.h0, w1 = g.w1, h1 = g.h1;
     const int cx = a.colour_index & 1, cy = a.colour_index >> 1;
@@ -305,8 +302,54 @@ __global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
             v0w[t * C0 + i] = a.palette[i * 16 + best[i]];
             k0[t * C0 + i] = (uint8_t)best[i];
         }
-        atomicAdd(a.moved + p * 4 + a.colour_index, 1u);
     }
+    return moved;
+}
+
+// The moved texels are counted as the quantised sweeps count theirs: the lanes of the workgroup's first plane summed
+// by shuffles and LDS into one atomic a workgroup, a lane of another plane (a workgroup across a plane boundary)
+// adding its own, the workgroups spread over MOVED_SLOTS counters (plane, colour) on separate cache lines - a count
+// per moved texel on one address had the launches queue on it.
+template <int C0, int C1>
+__global__ __launch_bounds__(SEARCH_THREADS) void k_level0_search(SearchArgs a)
+{
+    const long long gi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = a.colour.start[a.colour.n];
+    const bool live = gi < total;
+    const int p = range_plane(a.colour, live ? gi : total - 1);
+    const unsigned int moved = live && search_texel<C0, C1>(a, gi, p) ? 1u : 0u;
+    __shared__ unsigned int s_wave[SEARCH_THREADS / 64];
+    const int pw = range_plane(a.colour, (long long)blockIdx.x * blockDim.x);
+    unsigned int same = p == pw ? moved : 0u;
+    for (int off = 32; off > 0; off >>= 1)
+        same += __shfl_xor(same, off);
+    if (threadIdx.x % 64 == 0)
+        s_wave[threadIdx.x / 64] = same;
+    unsigned int* group = a.moved + (size_t)a.colour_index * a.moved_stride;
+    if (p != pw && moved)
+        atomicAdd(group + ((size_t)p * MOVED_SLOTS + blockIdx.x % MOVED_SLOTS) * MOVED_STRIDE, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0)
+    {
+        unsigned int sum = 0;
+        for (int w = 0; w < SEARCH_THREADS / 64; w++)
+            sum += s_wave[w];
+        if (sum)
+            atomicAdd(group + ((size_t)pw * MOVED_SLOTS + blockIdx.x % MOVED_SLOTS) * MOVED_STRIDE, sum);
+    }
+}
+
+// Each (plane, colour)'s slots summed into out[plane * 4 + colour].
+__global__ void k_fold_moved(const unsigned int* slots, size_t stride, int planes, unsigned int* out)
+{
+    const int i = (int)threadIdx.x;
+    if (i >= planes * 4)
+        return;
+    const int p = i / 4, col = i % 4;
+    unsigned int sum = 0;
+    for (int sl = 0; sl < MOVED_SLOTS; sl++)
+        sum += slots[(size_t)col * stride + ((size_t)p * MOVED_SLOTS + sl) * MOVED_STRIDE];
+    out[i] = sum;
 }
 
 template <int C0>
@@ -410,12 +453,15 @@ void solve_level0_all(Ctx& c, int k, int sweeps, nntc_level0_report* rep)
         total_bits += c.md.bits0[q];
     }
     a.joint = total_bits <= 8 ? 1 : 0;
-    // Four counters a plane, in the head of the sweeps' moved-index counters (the two are never live together).
-    a.moved = c.counters.p;
+    // The slots of the four colours, then the folded counts (four a plane) after them.
+    const size_t stride = (size_t)c.planes.n * MOVED_SLOTS * MOVED_STRIDE;
     const int ncount = c.planes.n * 4;
-    if ((size_t)ncount > c.counters.n)
-        throw Refused { "solve_level0_all: the counters hold too few planes" };
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c.stream, a.moved, ncount);
+    if (c.pal_counters.n < 4 * stride + (size_t)ncount)
+        c.pal_counters.alloc(4 * stride + (size_t)ncount);
+    a.moved = c.pal_counters.p;
+    a.moved_stride = stride;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks_for((long long)(4 * stride), 256)), dim3(256), 0, c.stream,
+                       a.moved, (int)(4 * stride));
     HIP_OK(hipEventRecord(c.ev0, c.stream));
     for (int col = 0; col < 4; col++)
     {
@@ -442,8 +488,12 @@ void solve_level0_all(Ctx& c, int k, int sweeps, nntc_level0_report* rep)
         launch_search(c.c0, c.c1, (unsigned)blocks_for(acc, SEARCH_THREADS), c.stream, a);
     }
     HIP_OK(hipGetLastError());
+    unsigned int* folded = c.pal_counters.p + 4 * stride;
+    hipLaunchKernelGGL(k_fold_moved, dim3(1), dim3(4 * MAX_PLANES), 0, c.stream, (const unsigned int*)a.moved, stride,
+                       c.planes.n, folded);
+    HIP_OK(hipGetLastError());
     const float ms = c.ms_since(c.ev0);
-    const unsigned int* h = read_back(c, (const unsigned int*)a.moved, (size_t)ncount);
+    const unsigned int* h = read_back(c, (const unsigned int*)folded, (size_t)ncount);
     // One launch per colour covers every plane, so the block's time is reported on the base plane.
     for (int p = 0; p < c.planes.n; p++)
     {
