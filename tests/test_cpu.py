@@ -191,3 +191,24 @@ def test_driver_palette_matches_reference_binary(name, bits0, bc0, tmp_path):
     assert shipped and shipped[0].split()[2] == f"{rep.e_shipped:.9e}"
     psnr = [ln.split()[3] for ln in r.stdout.splitlines() if ln.strip().startswith("psnr         texture")]
     assert psnr == [f"{rep.psnr[t]:.2f}" for t in range(len(psnr))] and psnr
+
+
+@pytest.mark.parametrize("args,message", [
+    (["--l0", "palette", "--bc-refine", "2"], "belong to --l0 bc8"),
+    (["--bits0", "3"], "apply under --l0 palette only"),
+    (["--bc0", "0"], "apply under --l0 palette only"),
+    (["--l0", "palette", "--c0", "3", "--bits0", "3,3"], "gives 2 values for 3 level-0 channels"),
+    (["--l0", "rgb"], "must be bc8 or palette"),
+    (["--mip-filter", "mitchell"], "is not an option of this encoder"),
+])
+def test_cli_refuses_what_it_does_not_build(args, message, tmp_path):
+    """The command line refuses, by name and before touching a device, the options that do not apply (--bits0 and
+    --bc0 0 under --l0 bc8, the BC pack flags under --l0 palette, src/main.cpp:1310-1356) and the reference options
+    this encoder does not build."""
+    exe = os.path.join(ROOT, "nntc_amd", "bin", "nntc_hip_encode")
+    if not os.path.exists(exe):
+        pytest.skip("the CLI is not built")
+    png = synth.write_pngs(synth.workload("smoke_1x64x40"), str(tmp_path / "m"))[0]
+    r = subprocess.run([exe, png, *args], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=60)
+    assert r.returncode != 0
+    assert message in r.stderr, r.stderr
