@@ -961,18 +961,21 @@ __global__ __launch_bounds__(256) void k_gather_tile0(AssembleArgs a, Ranges til
     const int row_sites = bw * pp;
     const long long site0 = a.sites.start[p];
     {
-        // Box row r holds sites [((by0 + r) w0 + bx0) pp, + row_sites) of the plane, at LDS site r row_sites. Eight
-        // loads in flight per lane before their LDS stores.
+        // Box row r holds sites [((by0 + r) w0 + bx0) pp, + row_sites) of the plane, at LDS site r row_sites, each
+        // sum in its own array (component q of site s at q BOX + s): the lanes of a wavefront walk sites ten apart
+        // (two texels of pp = 5 sites), which the site-major layout (six floats a site) put on a quarter of the
+        // banks. Sixteen loads in flight per lane before their LDS stores.
+        constexpr int U = 16;
         const Div16 by_rowf(row_sites * NH), by_rowd(row_sites * C);
         const float* gf = a.site_f + (site0 + ((long long)by0 * w0 + bx0) * pp) * NH;
         const float* gd = a.site_d32 + (site0 + ((long long)by0 * w0 + bx0) * pp) * C;
         const long long stride_f = (long long)w0 * pp * NH, stride_d = (long long)w0 * pp * C;
         const int nf = bh * row_sites * NH, nd = bh * row_sites * C;
-        for (int i0 = threadIdx.x; i0 < nf || i0 < nd; i0 += 8 * 256)
+        for (int i0 = threadIdx.x; i0 < nf || i0 < nd; i0 += U * 256)
         {
-            float vf[8], vd[8];
+            float vf[U], vd[U];
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+            for (int u = 0; u < U; u++)
             {
                 const int i = i0 + u * 256;
                 if (i < nf)
@@ -987,13 +990,13 @@ __global__ __launch_bounds__(256) void k_gather_tile0(AssembleArgs a, Ranges til
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 8; u++)
+            for (int u = 0; u < U; u++)
             {
                 const int i = i0 + u * 256;
                 if (i < nf)
-                    s_f[i] = vf[u];
+                    s_f[(i % NH) * BOX + i / NH] = vf[u];
                 if (i < nd)
-                    s_d[i] = vd[u];
+                    s_d[(i % C) * BOX + i / C] = vd[u];
             }
         }
     }
@@ -1016,16 +1019,16 @@ __global__ __launch_bounds__(256) void k_gather_tile0(AssembleArgs a, Ranges til
     for (int i = 0; i < C; i++)
         gacc[i] = 0.0;
     auto add = [&](int site, double ws, const double* scale) {
-        const float* f = s_f + site * NH;
-        const float* d = s_d + site * C;
+        const float* f = s_f + site;
+        const float* d = s_d + site;
 #pragma unroll
         for (int aa = 0; aa < C; aa++)
-            gacc[aa] = fma(ws, (double)d[aa], gacc[aa]);
+            gacc[aa] = fma(ws, (double)d[aa * BOX], gacc[aa]);
 #pragma unroll
         for (int sl = 0; sl < STENCIL_BLOCKS; sl++)
 #pragma unroll
             for (int q = 0; q < NH; q++)
-                hs[sl][q] = fma(scale[sl], (double)f[q], hs[sl][q]);
+                hs[sl][q] = fma(scale[sl], (double)f[q * BOX], hs[sl][q]);
     };
     if (tx >= 2 && tx <= w0 - 3 && ty >= 2 && ty <= h0 - 3)
     {
