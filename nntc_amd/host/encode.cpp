@@ -195,6 +195,7 @@ void finish_palette(Encoder& E, Totals tt)
         double se = 0.0;
         size_t n = 0;
         const int stored = dd.c0 == 3 ? 4 : dd.c0;
+        E.packed0.assign((size_t)level0_files(dd.c0), std::vector<std::vector<uint8_t>>((size_t)dd.nplanes));
         for (int i = 0; i < dd.nplanes; i++)
         {
             const int w = dd.planes[i].w0, h = dd.planes[i].h0, bx = (w + 3) / 4;
@@ -206,7 +207,7 @@ void finish_palette(Encoder& E, Totals tt)
             for (int f = 0; f < level0_files(dd.c0); f++)
             {
                 const int nc = level0_file_channels(dd.c0, f), bpb = nc == 1 ? 8 : 16;
-                const std::vector<uint8_t> blocks =
+                std::vector<uint8_t> blocks =
                     bc_pack_indices(bytes.data(), w, h, stored, 2 * f, nc, dd.bits0, dd.c0, se, n);
                 parallel_rows((h + 3) / 4, [&](int b0, int b1) {
                     for (int c = 0; c < nc; c++)
@@ -224,6 +225,7 @@ void finish_palette(Encoder& E, Totals tt)
                                 }
                             }
                 });
+                E.packed0[(size_t)f][(size_t)i] = std::move(blocks);
             }
             E.check(E.seam.level0_upload(E.ctx, i, v0.data(), v0.size()), "level0_upload");
         }
@@ -1014,6 +1016,7 @@ int nntc_encoder_run(nntc_encoder* h)
         e.report.textures = e.desc.textures;
         e.report.nplanes = e.desc.nplanes;
         e.report.mpix = (double)e.source_w * e.source_h * e.desc.textures / 1e6;
+        e.packed0.clear();
         run_encode(e);
         e.ran = true;
         return 0;

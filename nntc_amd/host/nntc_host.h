@@ -91,6 +91,9 @@ struct Encoder
     int source_w = 0, source_h = 0;
     nntc_encode_report report {};
     bool ran = false;
+    // --l0 palette with lossy BC4 indices (a channel above 3 bits): the blocks packed for the report, [file][plane].
+    // The report's plane is decoded from them and the writer stores them; a run clears them.
+    std::vector<std::vector<std::vector<uint8_t>>> packed0;
 
     // Wall time per seam call name, when NNTC_CALL_TIMES is set in the environment: each check() charges the time
     // since the previous one (the call itself and the host work that prepared it) to the call's name.

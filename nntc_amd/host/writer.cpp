@@ -314,8 +314,11 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
     const bool palette = d.l0_bc8 == 0;
     const bool compress = !palette || d.bc0 != 0;
     const int stored0 = d.c0 == 3 ? 4 : d.c0;
+    // The blocks the encoder already packed for its report, when it did (lossy indices): the same indices give the
+    // same blocks, so they are not packed a second time.
+    const bool packed = palette && compress && e.packed0.size() == (size_t)files;
     std::vector<std::vector<uint8_t>> bytes0;
-    if (palette)
+    if (palette && !packed)
     {
         bytes0.resize((size_t)np);
         for (int i = 0; i < np; i++)
@@ -340,6 +343,11 @@ extern "C" int nntc_encoder_write(const nntc_encoder* h, const char* prefix_c)
             std::vector<std::vector<uint8_t>> levels((size_t)np);
             for (int i = 0; i < np; i++)
             {
+                if (packed)
+                {
+                    levels[(size_t)i] = e.packed0[(size_t)f][(size_t)i];
+                    continue;
+                }
                 if (palette)
                 {
                     double se = 0.0;
